@@ -1,0 +1,250 @@
+#!/usr/bin/env python3
+"""Benchmark of the Stieltjes-transform hot path on one MI355X.
+
+Workload (the metric's configuration): p = 50 000 eigenvalues, Marchenko-
+Pastur c = 0.25, sigma^2 = 1, drawn by inverse-CDF sampling of seeded
+uniforms, plus outliers at the forward-BBP images of spikes 12/7/4;
+eta = 1/sqrt(p).  One step = one all-points transform s(z_i), z_i = lam_i +
+i*eta, for all p eigenvalues (p^2 pair evaluations), with the eigenvalues
+already resident in HBM.
+
+Prints ONE JSON line: the FP32-compensated kernel's pair-evals/s as
+``value``, the FP64 exact kernel beside it, their relative error against the
+FP64 CPU oracle, a roofline object for the dominant kernel (timed with HIP
+events on the stream it runs on) and the CPU baseline (the oracle's
+restatement of the reference's parallel tiled kernel on this host's cores).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Stieltjes pair-evals/sec at p=50k: FP32-comp (≤1e-6) vs FP64 exact, 1 GPU"
+FLOPS_PER_PAIR = 7          # sub, fma(2), rcp, fma(2), add
+PEAK_FP32_TFLOPS = 157.3    # MI355X FP32 vector (published)
+PEAK_FP64_TFLOPS = 78.6     # MI355X FP64 vector (published)
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def time_steps(fn, steps: int, warmup: int, sync) -> float:
+    for _ in range(warmup):
+        fn()
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    sync()
+    return time.perf_counter() - t0
+
+
+def kernel_avg_ms(shk, fn, launches: int, sync) -> float:
+    """Average duration of the library's dominant kernel, from HIP events that
+    the library records around it on the stream it is launched on."""
+    from shrinkers_amd import stieltjes as st
+    st.profile_reset()
+    st.profile_enable(True)
+    for _ in range(launches):
+        fn()
+    sync()
+    st.profile_enable(False)
+    ms, n = st.profile_read()
+    st.profile_reset()
+    return ms / max(n, 1)
+
+
+def cpu_baseline(ev: np.ndarray, eta: float, budget_s: float):
+    """The oracle's C restatement of the reference parallel tiled kernel
+    (cacheblock.rs:1454-1513) on all host cores, timed with the reference's
+    bench_ms policy (benches/support/mod.rs:90-112), plus one core on a
+    bounded sample."""
+    import oracle
+    oracle.lib()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    p = ev.size
+
+    def once():
+        return oracle.tiled_parallel(ev, eta, threads)
+
+    out = once()  # warm-up (also the FP64 CPU result used for the error figures)
+    t0 = time.perf_counter()
+    once()
+    probe = time.perf_counter() - t0
+    reps = 15 if probe < 2e-3 else (7 if probe < 20e-3 else 3)
+    reps = max(1, min(reps, int(budget_s / max(probe, 1e-6))))
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        once()
+        ts.append(time.perf_counter() - t0)
+    med = float(np.median(ts))
+    # one core: sample of rows against all p sources
+    rows = max(1, min(p, int(2.0 / max(med * threads / p, 1e-9))))
+    sample = ev[:rows].copy()
+    t0 = time.perf_counter()
+    for x in sample:
+        oracle.autovec_sum(x, ev, eta)
+    t1 = time.perf_counter() - t0
+    return out, {
+        "value": p * p / med,
+        "unit": "pair-evals/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"full p={p} transform (oracle C/OpenMP restatement of the reference's "
+                  f"parallel tiled kernel, x86-64-v3 + FMA), median of {reps}",
+        "ms_per_transform": med * 1e3,
+        "single_core": {"value": rows * p / t1, "unit": "pair-evals/s",
+                        "sample": f"{rows} rows x {p} sources, 1 thread"},
+        "cpu_model": _cpu_model(),
+    }
+
+
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--p", type=int, default=50_000)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU timing")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on exactly one GPU (asked for --gpus {args.gpus}); "
+              "the path shards by query rows across GPUs but no multi-GPU run is part of it.",
+              file=sys.stderr)
+        return 2
+
+    import torch
+    import shrinkers_amd as shk
+    from shrinkers_amd import spectra
+
+    p = args.p
+    ev = spectra.mp_inverse_cdf_spectrum(p, c=0.25, sigma2=1.0, seed=0)
+    eta = 1.0 / np.sqrt(p)
+    dev = torch.device("cuda:0")
+    d_ev = torch.from_numpy(ev).to(dev)
+    out32 = (torch.empty_like(d_ev), torch.empty_like(d_ev))
+    out64 = (torch.empty_like(d_ev), torch.empty_like(d_ev))
+    sync = torch.cuda.synchronize
+
+    def step32():
+        shk.all_points_dev(d_ev, eta, "blocked", precision="f32c", scaled=True, out=out32)
+
+    def step64():
+        shk.all_points_dev(d_ev, eta, "blocked", precision="f64", scaled=True, out=out64)
+
+    log(f"p={p} eta={eta:.6g}: timing FP32-compensated ({args.steps} steps)")
+    t32 = time_steps(step32, args.steps, args.warmup, sync)
+    log("timing FP64 exact")
+    steps64 = max(1, args.steps // 4)
+    t64 = time_steps(step64, steps64, max(1, args.warmup // 4), sync)
+    log("kernel event timing")
+    k32 = kernel_avg_ms(shk, step32, 20, sync)
+    k64 = kernel_avg_ms(shk, step64, 10, sync)
+
+    # ChebCode FAST (build + evaluate) and DEFAULT at the same p, for reference
+    def cheb(m):
+        shk.all_points_dev(d_ev, eta, m, scaled=True, out=out64)
+    tcf = time_steps(lambda: cheb("chebcode_fast"), 10, 2, sync) / 10
+    kcf = kernel_avg_ms(shk, lambda: cheb("chebcode_fast"), 5, sync)
+
+    r32 = (out32[0].cpu().numpy(), out32[1].cpu().numpy())
+    r64 = (out64[0].cpu().numpy(), out64[1].cpu().numpy())
+    step64()
+    sync()
+    r64 = (out64[0].cpu().numpy(), out64[1].cpu().numpy())
+
+    def rel(a, b):
+        num = np.sum((a[0] - b[0]) ** 2 + (a[1] - b[1]) ** 2)
+        return float(np.sqrt(num / np.sum(b[0] ** 2 + b[1] ** 2)))
+
+    pairs = float(p) * float(p)
+    ms32 = t32 / args.steps * 1e3
+    ms64 = t64 / steps64 * 1e3
+    value = pairs * args.steps / t32
+    achieved = FLOPS_PER_PAIR * pairs / (k32 * 1e-3) / 1e12
+
+    cpu = None
+    err_oracle_32 = err_oracle_64 = None
+    if not args.no_cpu:
+        log("CPU baseline (oracle restatement, all host cores)")
+        ref, cpu = cpu_baseline(ev, eta, args.cpu_budget)
+        ref = (ref[0] / p, ref[1] / p)
+        err_oracle_32 = rel(r32, ref)
+        err_oracle_64 = rel(r64, ref)
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "pair-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms32,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "exact all-points Stieltjes transform, MP inverse-CDF spectrum "
+                        "c=0.25 sigma2=1 seed 0 + forward-BBP outliers of spikes 12/7/4, "
+                        "eta=1/sqrt(p), FP32-compensated kernel",
+            "p": p,
+            "eta": eta,
+            "pair_evals_per_step": pairs,
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": None,
+            "kernel": "exact_f32c_kernel",
+            "kernel_avg_ms": k32,
+            "flops_per_launch": FLOPS_PER_PAIR * pairs,
+        },
+        "cpu_baseline": cpu,
+        "fp64_exact": {
+            "value": pairs / (ms64 * 1e-3),
+            "unit": "pair-evals/s",
+            "ms_per_step": ms64,
+            "kernel_avg_ms": k64,
+            "achieved_tflops": FLOPS_PER_PAIR * pairs / (k64 * 1e-3) / 1e12,
+            "peak_tflops": PEAK_FP64_TFLOPS,
+            "speedup_f32c_over_f64": ms64 / ms32,
+        },
+        "rel_l2_f32c_vs_fp64_gpu": rel(r32, r64),
+        "rel_l2_f32c_vs_fp64_oracle": err_oracle_32,
+        "rel_l2_fp64_gpu_vs_fp64_oracle": err_oracle_64,
+        "chebcode_fast": {"ms_end_to_end": tcf * 1e3, "eval_kernel_ms": kcf},
+    }
+    if cpu:
+        line["cpu_baseline"]["gpu_over_cpu"] = value / cpu["value"]
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,221 @@
+/*
+ * shrinkers_stieltjes.h -- C ABI of the MI355X Stieltjes-transform library.
+ *
+ * Drop-in boundary for the `src/stieltjes` method API of alexisrosuel/shrinkers
+ * (Rust).  Every entry point takes plain pointers and sizes; none takes a
+ * torch or HIP type (streams are passed as `void*` and may be NULL).
+ *
+ * Conventions, as in the reference (src/stieltjes/mod.rs:22-34,
+ * src/stieltjes/term.rs:3-7):
+ *   S_i = sum_j 1/((x_i - lam_j) - i*eta), diagonal term included,
+ *   Re S = sum d/(d^2+eta^2), Im S = eta * sum 1/(d^2+eta^2) > 0.
+ *   "raw" results are these sums; "scaled" results are multiplied by 1/p.
+ *   Outputs are caller-owned structure-of-arrays (re[], im[]).
+ *
+ * Error convention: every function returns an shk_status and never aborts.
+ * The reference's Rust API returns empty output for p = 0 and panics on NaN;
+ * its Python boundary raises ValueError (src/python.rs:138-318).  Here p = 0
+ * returns SHK_OK with nothing written (like the Rust API), except where a
+ * function says otherwise; the text of the last error of the calling thread
+ * is available from shk_last_error().
+ *
+ * Host-pointer entry points copy inputs to the GPU, run on an internal
+ * stream of the current device and return after the results are back in
+ * host memory.  Device-pointer entry points (suffix _dev) enqueue work on the
+ * given stream and return without synchronising; the caller keeps the
+ * buffers alive until the stream has passed the work.
+ *
+ * Thread safety: calls on distinct streams may run concurrently; scratch
+ * memory is stream-ordered (hipMallocAsync) per call.
+ */
+#ifndef SHRINKERS_STIELTJES_H
+#define SHRINKERS_STIELTJES_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum shk_status {
+    SHK_OK = 0,
+    SHK_ERR_EMPTY = 1,            /* empty input where one is required        */
+    SHK_ERR_NONFINITE = 2,        /* NaN or infinity in the input             */
+    SHK_ERR_BAD_ETA = 3,          /* eta not a positive finite number         */
+    SHK_ERR_BAD_METHOD = 4,       /* unknown method id or name                */
+    SHK_ERR_UNSUPPORTED = 5,      /* method known to the reference but not
+                                     built for the GPU (FFT/NUFFT/HODLR/...) */
+    SHK_ERR_BAD_ARG = 6,          /* NULL pointer, bad size or bad option     */
+    SHK_ERR_HIP = 7,              /* a HIP runtime call failed                */
+    SHK_ERR_NO_DEVICE = 8         /* no usable gfx950 device                  */
+} shk_status;
+
+/* Method vocabulary, in the order of StieltjesMethod
+ * (src/config.rs:19-89); names as in StieltjesMethod::name (:92-118) and the
+ * Python aliases of parse_method (src/python.rs:321-350). */
+typedef enum shk_method {
+    SHK_NAIVE = 0,
+    SHK_AUTOVEC = 1,
+    SHK_BLOCKED = 2,
+    SHK_BLOCKED_AUTOVEC = 3,
+    SHK_BLOCKED_TILED = 4,
+    SHK_BLOCKED_WINDOWED = 5,
+    SHK_BLOCKED_HYBRID = 6,
+    SHK_ADAPTIVE = 7,
+    SHK_FFT5 = 8,
+    SHK_FFT3 = 9,
+    SHK_FFT2 = 10,
+    SHK_NUFFT = 11,
+    SHK_TREECODE = 12,
+    SHK_CHEBCODE = 13,
+    SHK_CHEBCODE_FAST = 14,
+    SHK_CHEBCODE_XTREME = 15,
+    SHK_CHEBCODE_BALANCED = 16,
+    SHK_EWALD = 17,
+    SHK_DST = 18,
+    SHK_AUTO = 19,
+    SHK_HODLR = 20,
+    SHK_SPEED_AUTO = 21,
+    SHK_ACCURACY_AUTO = 22
+} shk_method;
+
+/* Arithmetic of the exact family.
+ *   SHK_PREC_F64  : FP64 everywhere (the correctness anchor).
+ *   SHK_PREC_F32C : FP32 pair arithmetic on eigenvalues split into f32
+ *                   hi/lo parts relative to a per-tile centre, FP32 partial
+ *                   sums over short runs folded into FP64 accumulators, FP64
+ *                   cross-block reduction.  Normwise relative error ~1e-7.
+ *   SHK_PREC_F32  : the reference's f32 arm (input and output float,
+ *                   src/stieltjes/mod.rs:805-851); runs the F32C kernel. */
+typedef enum shk_precision {
+    SHK_PREC_F64 = 0,
+    SHK_PREC_F32C = 1,
+    SHK_PREC_F32 = 2
+} shk_precision;
+
+/* ---- library ------------------------------------------------------------ */
+
+/* Text of the last error raised on the calling thread ("" if none). */
+const char *shk_last_error(void);
+
+/* Library version string. */
+const char *shk_version(void);
+
+/* Resolve a Python method name or alias (src/python.rs:321-350) to its id.
+ * Returns SHK_ERR_BAD_METHOD for an unknown name. */
+shk_status shk_method_from_name(const char *name, int *method_out);
+
+/* Canonical name of a method id (src/config.rs:92-118), NULL if invalid. */
+const char *shk_method_name(int method);
+
+/* The reference's default eta = 0.1/sqrt(p) (src/stieltjes/mod.rs:105-108). */
+double shk_default_eta(size_t p);
+
+/* Resolve the auto presets (src/config.rs:543-555) for this library:
+ * Auto/SpeedAuto -> CHEBCODE_FAST, AccuracyAuto -> BLOCKED; a concrete
+ * method is returned unchanged. */
+int shk_resolve_auto_method(int method, int parallel, size_t p);
+
+/* ---- exact and treecode all-points transform ---------------------------- */
+
+/* compute_all_stieltjes (src/stieltjes/mod.rs:601-790): transform at every
+ * eigenvalue, SCALED by 1/p.  cutoff_ratio <= 0 disables the far-field
+ * cutoff (CutoffConfig::Disabled).  `parallel` is accepted for interface
+ * parity; the GPU path is the same for both values.  precision is an
+ * shk_precision (SHK_PREC_F32 is rejected here: use the _f32 entry). */
+shk_status shk_compute_all_stieltjes(const double *eigenvalues, size_t p, double eta,
+                                     int method, double cutoff_ratio, int parallel,
+                                     int precision, double *out_re, double *out_im);
+
+/* compute_all_stieltjes_f32 (src/stieltjes/mod.rs:805-851): float input,
+ * float output, scaled by 1/p. */
+shk_status shk_compute_all_stieltjes_f32(const float *eigenvalues, size_t p, float eta,
+                                         int method, double cutoff_ratio, float *out_re,
+                                         float *out_im);
+
+/* compute_stieltjes_at_points (src/stieltjes/mod.rs:394-494): RAW sums at
+ * arbitrary query points. */
+shk_status shk_compute_stieltjes_at_points(const double *query_points, size_t nq,
+                                           const double *eigenvalues, size_t p, double eta,
+                                           int method, double cutoff_ratio, int parallel,
+                                           double *out_re, double *out_im);
+
+/* compute_all_stieltjes_with_deriv (src/stieltjes/autovec.rs:63-76): RAW
+ * values and their derivative w.r.t. the real query point,
+ * S'(x) = -sum 1/(x - lam_j - i*eta)^2. */
+shk_status shk_compute_all_stieltjes_with_deriv(const double *eigenvalues, size_t p, double eta,
+                                                double *val_re, double *val_im,
+                                                double *der_re, double *der_im);
+
+/* compute_stieltjes_scaled_ray (src/stieltjes/mod.rs:304-319): RAW sums at
+ * z_i = lam_i (1 + i*scale), i.e. eta_i = scale*lam_i per query. */
+shk_status shk_compute_stieltjes_scaled_ray(const double *eigenvalues, size_t p, double scale,
+                                            double *out_re, double *out_im);
+
+/* ---- device-pointer variants --------------------------------------------- */
+
+/* All-points transform on device buffers.  scaled != 0 multiplies by 1/p.
+ * method must be a concrete exact-family or ChebCode id (autos resolved). */
+shk_status shk_all_points_dev(const double *d_eigenvalues, size_t p, double eta, int method,
+                              double cutoff_ratio, int precision, int scaled,
+                              double *d_out_re, double *d_out_im, void *stream);
+
+/* Raw sums at device query points (exact kernel or ChebCode per method). */
+shk_status shk_at_points_dev(const double *d_query, size_t nq, const double *d_eigenvalues,
+                             size_t p, double eta, int method, double cutoff_ratio,
+                             double *d_out_re, double *d_out_im, void *stream);
+
+/* ---- ChebCodeBatch (src/stieltjes/chebcode.rs:1148-1362) ----------------- */
+
+typedef struct shk_chebcode shk_chebcode;
+
+/* ChebCodeBatch::build_mode: build the tree once on the GPU.  Input may be
+ * unsorted (a permutation is kept, :1186-1217).  f32_far != 0 selects the
+ * FAST f32 far field (FastMode::F32). */
+shk_status shk_chebcode_build(const double *eigenvalues, size_t p, double theta, int n,
+                              size_t leaf_cap, int f32_far, shk_chebcode **out);
+
+/* ChebCodeBatch::build_preset with a ChebCode method id (13..16). */
+shk_status shk_chebcode_build_preset(const double *eigenvalues, size_t p, int method,
+                                     shk_chebcode **out);
+
+/* Number of tree nodes, and the eigenvalue count. */
+shk_status shk_chebcode_info(const shk_chebcode *h, size_t *node_count, size_t *p);
+
+/* Copy the tree out (pre-order; any pointer may be NULL): node intervals,
+ * source index ranges, right-child links (-1 = leaf), Chebyshev nodes and
+ * weights (node_count*n each). */
+shk_status shk_chebcode_export(const shk_chebcode *h, double *lo, double *hi, int64_t *lo_idx,
+                               int64_t *hi_idx, int32_t *right, double *nodes, double *w);
+
+/* ChebCodeBatch::evaluate (:1280-1306): RAW sums at the eigenvalues, in the
+ * order they were given to the build. */
+shk_status shk_chebcode_evaluate(const shk_chebcode *h, double eta, double *out_re,
+                                 double *out_im);
+
+/* ChebCodeBatch::evaluate_points (:1241-1277): RAW sums at query points. */
+shk_status shk_chebcode_evaluate_points(const shk_chebcode *h, const double *query, size_t nq,
+                                        double eta, double *out_re, double *out_im);
+
+/* ChebCodeBatch::evaluate_many (:1312-1361): one RAW result row per eta,
+ * out_re/out_im are n_eta*p row-major. */
+shk_status shk_chebcode_evaluate_many(const shk_chebcode *h, const double *etas, size_t n_eta,
+                                      double *out_re, double *out_im);
+
+shk_status shk_chebcode_destroy(shk_chebcode *h);
+
+/* ---- measurement ---------------------------------------------------------- */
+
+/* When enabled, the library brackets the dominant kernel of each exact or
+ * treecode evaluation with HIP events on the stream it is launched on and
+ * accumulates the elapsed time (read after the stream has synchronised). */
+shk_status shk_profile_enable(int enable);
+shk_status shk_profile_read(double *kernel_ms_total, int64_t *launches);
+shk_status shk_profile_reset(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SHRINKERS_STIELTJES_H */

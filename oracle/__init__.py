@@ -1,0 +1,291 @@
+"""CPU oracle for the Stieltjes hot path -- TEST INFRASTRUCTURE ONLY.
+
+Thin numpy/ctypes wrappers over ``liboracle.so`` (built from
+``stieltjes_oracle.c`` by ``oracle/Makefile``), a plain-C restatement of the
+reference kernels of alexisrosuel/shrinkers.  Each C function cites the
+reference file:line it follows.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this package, and only as the checker or as the
+timed CPU baseline.  The product package ``shrinkers_amd`` never imports it.
+
+Every wrapper returns RAW sums (not scaled by 1/p), as the reference kernels
+do (src/stieltjes/mod.rs:29-31).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i64 = ctypes.c_int64
+_d = ctypes.c_double
+_dp = ctypes.POINTER(ctypes.c_double)
+_lp = ctypes.POINTER(ctypes.c_int64)
+
+_lib = None
+
+
+def build() -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build()
+    L = ctypes.CDLL(_LIB_PATH)
+    L.orc_lcg_fill.argtypes = [ctypes.c_uint64, _i64, _f64p]
+    L.orc_log_spectrum.argtypes = [_i64, _f64p]
+    L.orc_harness_spectrum.argtypes = [_i64, _f64p]
+    L.orc_shuffled.argtypes = [_f64p, _i64, ctypes.c_uint64, _f64p]
+    L.orc_term.argtypes = [_d, _d, _d, _dp, _dp]
+    for nm in ("orc_naive_sum", "orc_autovec_sum"):
+        getattr(L, nm).argtypes = [_d, _f64p, _i64, _d, _dp, _dp]
+    L.orc_sum_cutoff.argtypes = [_d, _f64p, _i64, _d, _d, _dp, _dp]
+    L.orc_exact_stieltjes.argtypes = [_f64p, _i64, _d, _f64p, _f64p]
+    L.orc_symmetric_sweep.argtypes = [_f64p, _i64, _d, _f64p, _f64p]
+    L.orc_tiled_parallel.argtypes = [_f64p, _i64, _d, ctypes.c_int, _f64p, _f64p]
+    L.orc_tiled_f32.argtypes = [_f32p, _i64, ctypes.c_float, _f32p, _f32p]
+    L.orc_at_points.argtypes = [_f64p, _i64, _f64p, _i64, _d, _f64p, _f64p]
+    L.orc_with_deriv.argtypes = [_f64p, _i64, _d, _f64p, _f64p, _f64p, _f64p]
+    L.orc_scaled_ray.argtypes = [_f64p, _i64, _d, _f64p, _f64p]
+    L.orc_cheb_build.argtypes = [_f64p, _i64, ctypes.c_int, _d, _i64]
+    L.orc_cheb_build.restype = ctypes.c_void_p
+    L.orc_cheb_free.argtypes = [ctypes.c_void_p]
+    L.orc_cheb_node_count.argtypes = [ctypes.c_void_p]
+    L.orc_cheb_node_count.restype = _i64
+    L.orc_cheb_export.argtypes = [ctypes.c_void_p, _f64p, _f64p, _i64p, _i64p, _i32p, _f64p, _f64p]
+    for nm in ("orc_cheb_eval_f64", "orc_cheb_eval_fast32"):
+        getattr(L, nm).argtypes = [ctypes.c_void_p, _d, _d, _dp, _dp, _lp, _lp]
+    L.orc_cheb_eval_x2.argtypes = [ctypes.c_void_p, _d, _d, _d, _dp, _dp, _dp, _dp]
+    L.orc_chebcode_all.argtypes = [_f64p, _i64, _d, _d, ctypes.c_int, _i64, ctypes.c_int,
+                                   ctypes.c_int, _f64p, _f64p]
+    L.orc_cheb_eval_points.argtypes = [ctypes.c_void_p, _f64p, _i64, _d, ctypes.c_int,
+                                       _f64p, _f64p]
+    L.orc_grid_leaf_cap.argtypes = [_i64, _i64, _i64, _i64]
+    L.orc_grid_leaf_cap.restype = _i64
+    L.orc_max_threads.restype = ctypes.c_int
+    L.orc_set_outside_theta2.argtypes = [ctypes.c_float]
+    _lib = L
+    return L
+
+
+def _c(a, dtype=np.float64):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+# ---- generators -----------------------------------------------------------
+
+def lcg(seed: int, n: int) -> np.ndarray:
+    out = np.empty(n, np.float64)
+    lib().orc_lcg_fill(seed, n, out)
+    return out
+
+
+def log_spectrum(p: int) -> np.ndarray:
+    out = np.empty(p, np.float64)
+    lib().orc_log_spectrum(p, out)
+    return out
+
+
+def harness_spectrum(p: int) -> np.ndarray:
+    out = np.empty(p, np.float64)
+    lib().orc_harness_spectrum(p, out)
+    return out
+
+
+def shuffled(v, seed: int) -> np.ndarray:
+    v = _c(v)
+    out = np.empty_like(v)
+    lib().orc_shuffled(v, v.size, seed, out)
+    return out
+
+
+# ---- exact kernels --------------------------------------------------------
+
+def term(li: float, lj: float, eta: float):
+    r, i = _d(), _d()
+    lib().orc_term(li, lj, eta, ctypes.byref(r), ctypes.byref(i))
+    return r.value, i.value
+
+
+def _single(fn, x, ev, eta):
+    ev = _c(ev)
+    r, i = _d(), _d()
+    fn(float(x), ev, ev.size, float(eta), ctypes.byref(r), ctypes.byref(i))
+    return r.value, i.value
+
+
+def naive_sum(x, ev, eta):
+    return _single(lib().orc_naive_sum, x, ev, eta)
+
+
+def autovec_sum(x, ev, eta):
+    return _single(lib().orc_autovec_sum, x, ev, eta)
+
+
+def sum_cutoff(x, ev, eta, cutoff):
+    ev = _c(ev)
+    r, i = _d(), _d()
+    lib().orc_sum_cutoff(float(x), ev, ev.size, float(eta), float(cutoff or 0.0),
+                         ctypes.byref(r), ctypes.byref(i))
+    return r.value, i.value
+
+
+def _allpoints(fn, ev, eta, *extra):
+    ev = _c(ev)
+    re = np.empty_like(ev)
+    im = np.empty_like(ev)
+    fn(ev, ev.size, float(eta), *extra, re, im)
+    return re, im
+
+
+def exact_stieltjes(ev, eta):
+    return _allpoints(lib().orc_exact_stieltjes, ev, eta)
+
+
+def symmetric_sweep(ev, eta):
+    return _allpoints(lib().orc_symmetric_sweep, ev, eta)
+
+
+def tiled_parallel(ev, eta, threads: int = 0):
+    return _allpoints(lib().orc_tiled_parallel, ev, eta, int(threads))
+
+
+def tiled_f32(ev32, eta32):
+    ev = _c(ev32, np.float32)
+    re = np.empty_like(ev)
+    im = np.empty_like(ev)
+    lib().orc_tiled_f32(ev, ev.size, float(np.float32(eta32)), re, im)
+    return re, im
+
+
+def at_points(q, ev, eta):
+    q = _c(q)
+    ev = _c(ev)
+    re = np.empty_like(q)
+    im = np.empty_like(q)
+    lib().orc_at_points(q, q.size, ev, ev.size, float(eta), re, im)
+    return re, im
+
+
+def with_deriv(ev, eta):
+    ev = _c(ev)
+    outs = [np.empty_like(ev) for _ in range(4)]
+    lib().orc_with_deriv(ev, ev.size, float(eta), *outs)
+    return tuple(outs)
+
+
+def scaled_ray(ev, scale):
+    return _allpoints(lib().orc_scaled_ray, ev, scale)
+
+
+# ---- ChebCode -------------------------------------------------------------
+
+# (theta, n, leaf_cap, fast32): src/stieltjes/chebcode.rs:942-973
+PRESETS = {
+    "chebcode": (0.5, 11, 32, 0),
+    "chebcode_fast": (1.0, 8, 32, 1),
+    "chebcode_xtreme": (0.25, 11, 16, 0),
+    "chebcode_balanced": (0.55, 11, 32, 0),
+}
+
+
+class ChebTree:
+    """Owns an oracle tree built over ascending ``sorted_ev``."""
+
+    def __init__(self, sorted_ev, n: int, theta: float, leaf_cap: int):
+        self.sorted = _c(sorted_ev)
+        self.n = int(n)
+        self._h = lib().orc_cheb_build(self.sorted, self.sorted.size, self.n, float(theta),
+                                       int(leaf_cap))
+        if not self._h:
+            raise ValueError("oracle tree build failed")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_cheb_free(self._h)
+            self._h = None
+
+    @property
+    def count(self) -> int:
+        return int(lib().orc_cheb_node_count(self._h))
+
+    def export(self) -> dict:
+        c, n = self.count, self.n
+        out = dict(lo=np.empty(c), hi=np.empty(c), lo_idx=np.empty(c, np.int64),
+                   hi_idx=np.empty(c, np.int64), right=np.empty(c, np.int32),
+                   nodes=np.empty(c * n), w=np.empty(c * n))
+        lib().orc_cheb_export(self._h, out["lo"], out["hi"], out["lo_idx"], out["hi_idx"],
+                              out["right"], out["nodes"], out["w"])
+        return out
+
+    def eval_point(self, x: float, eta: float, fast32: bool = False):
+        r, i = _d(), _d()
+        npairs, npan = ctypes.c_int64(), ctypes.c_int64()
+        fn = lib().orc_cheb_eval_fast32 if fast32 else lib().orc_cheb_eval_f64
+        fn(self._h, float(x), float(eta), ctypes.byref(r), ctypes.byref(i),
+           ctypes.byref(npairs), ctypes.byref(npan))
+        return r.value, i.value, npairs.value, npan.value
+
+    def eval_points(self, q, eta: float, fast32: bool = False):
+        q = _c(q)
+        re = np.empty_like(q)
+        im = np.empty_like(q)
+        lib().orc_cheb_eval_points(self._h, q, q.size, float(eta), int(bool(fast32)), re, im)
+        return re, im
+
+    def eval_x2(self, x: float, e0: float, e1: float):
+        v = [_d() for _ in range(4)]
+        lib().orc_cheb_eval_x2(self._h, float(x), float(e0), float(e1), *[ctypes.byref(a) for a in v])
+        return (v[0].value, v[1].value), (v[2].value, v[3].value)
+
+
+def chebcode_all(ev, eta, theta, n, leaf_cap, fast32=False, threads: int = 0):
+    ev = _c(ev)
+    re = np.empty_like(ev)
+    im = np.empty_like(ev)
+    lib().orc_chebcode_all(ev, ev.size, float(eta), float(theta), int(n), int(leaf_cap),
+                           int(bool(fast32)), int(threads), re, im)
+    return re, im
+
+
+def chebcode_preset(ev, eta, preset: str, threads: int = 0):
+    theta, n, leaf, fast = PRESETS[preset]
+    return chebcode_all(ev, eta, theta, n, leaf, bool(fast), threads)
+
+
+def grid_leaf_cap(n, p, nq, preset_leaf):
+    return int(lib().orc_grid_leaf_cap(n, p, nq, preset_leaf))
+
+
+def set_outside_theta2(v: float) -> None:
+    """FAST traversal's opening multiplier for panels not containing the
+    query (1.69 in the reference, chebcode.rs:108; 1.0 reproduces the FAST
+    rows recorded in docs/pareto/bench_after.json)."""
+    lib().orc_set_outside_theta2(float(v))
+
+
+def max_threads() -> int:
+    return int(lib().orc_max_threads())
+
+
+def rel_l2(got_re, got_im, ref_re, ref_im) -> float:
+    """Normwise relative error of (re, im) pairs: benches/support/mod.rs:114-123."""
+    num = np.sum((np.asarray(got_re) - ref_re) ** 2 + (np.asarray(got_im) - ref_im) ** 2)
+    den = np.sum(np.asarray(ref_re) ** 2 + np.asarray(ref_im) ** 2)
+    return float(np.sqrt(num / den))

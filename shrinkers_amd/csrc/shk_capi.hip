@@ -1,0 +1,739 @@
+// C ABI of libshrinkers_hip.so: validation, method dispatch, device memory
+// and the measurement hooks.  Declarations and contracts are in
+// include/shrinkers_stieltjes.h; each entry point cites the reference
+// function it replaces.
+#include "shk_internal.hpp"
+#include "shrinkers_stieltjes.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace shk {
+
+namespace {
+thread_local std::string g_err;
+
+std::atomic<int> g_prof_on{0};
+std::mutex g_prof_mu;
+double g_prof_ms = 0.0;
+int64_t g_prof_n = 0;
+std::vector<std::pair<hipEvent_t, hipEvent_t>> g_prof_pending;
+}  // namespace
+
+void set_error(const std::string &msg) { g_err = msg; }
+const char *last_error() { return g_err.c_str(); }
+
+ProfileMark profile_begin(hipStream_t stream) {
+    ProfileMark m;
+    if (!g_prof_on.load(std::memory_order_relaxed)) return m;
+    if (hipEventCreate(&m.a) != hipSuccess || hipEventCreate(&m.b) != hipSuccess) {
+        if (m.a) (void)hipEventDestroy(m.a);
+        return ProfileMark{};
+    }
+    (void)hipEventRecord(m.a, stream);
+    return m;
+}
+
+void profile_end(const ProfileMark &m, hipStream_t stream) {
+    if (!m.a) return;
+    (void)hipEventRecord(m.b, stream);
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof_pending.emplace_back(m.a, m.b);
+}
+
+}  // namespace shk
+
+using namespace shk;
+
+namespace {
+
+struct MethodName {
+    const char *name;
+    int id;
+};
+
+// Python names and aliases: src/python.rs:321-350.
+const MethodName kNames[] = {
+    {"naive", SHK_NAIVE},
+    {"autovec", SHK_AUTOVEC},
+    {"blocked", SHK_BLOCKED},
+    {"blocked_autovec", SHK_BLOCKED_AUTOVEC},
+    {"blocked_tiled", SHK_BLOCKED_TILED},
+    {"blocked_windowed", SHK_BLOCKED_WINDOWED},
+    {"blocked_hybrid", SHK_BLOCKED_HYBRID},
+    {"adaptive", SHK_ADAPTIVE},
+    {"fft5", SHK_FFT5},
+    {"fft3", SHK_FFT3},
+    {"fft2", SHK_FFT2},
+    {"nufft", SHK_NUFFT},
+    {"fft_fast", SHK_NUFFT},
+    {"fmm", SHK_TREECODE},
+    {"treecode", SHK_TREECODE},
+    {"chebcode", SHK_CHEBCODE},
+    {"chebyshev", SHK_CHEBCODE},
+    {"chebcode_fast", SHK_CHEBCODE_FAST},
+    {"chebf", SHK_CHEBCODE_FAST},
+    {"chebcode_xtreme", SHK_CHEBCODE_XTREME},
+    {"chebx", SHK_CHEBCODE_XTREME},
+    {"chebcode_balanced", SHK_CHEBCODE_BALANCED},
+    {"chebb", SHK_CHEBCODE_BALANCED},
+    {"ewald", SHK_EWALD},
+    {"dst", SHK_DST},
+    {"auto", SHK_AUTO},
+    {"hodlr", SHK_HODLR},
+    {"speed_auto", SHK_SPEED_AUTO},
+    {"speed", SHK_SPEED_AUTO},
+    {"accuracy_auto", SHK_ACCURACY_AUTO},
+    {"accuracy", SHK_ACCURACY_AUTO},
+};
+
+// StieltjesMethod::name: src/config.rs:92-118.
+const char *kCanonical[] = {"naive",         "autovec",         "blocked",
+                            "blocked_autovec", "blocked_tiled", "blocked_windowed",
+                            "blocked_hybrid", "adaptive",       "fft5",
+                            "fft3",          "fft2",            "nufft",
+                            "fmm",           "chebcode",        "chebcode_fast",
+                            "chebcode_xtreme", "chebcode_balanced", "ewald",
+                            "dst",           "auto",            "hodlr",
+                            "speed_auto",    "accuracy_auto"};
+constexpr int kNumMethods = 23;
+
+enum Family { FAM_EXACT, FAM_CHEB, FAM_UNSUPPORTED, FAM_BAD };
+
+Family family_of(int m) {
+    switch (m) {
+        case SHK_NAIVE:
+        case SHK_AUTOVEC:
+        case SHK_BLOCKED:
+        case SHK_BLOCKED_AUTOVEC:
+        case SHK_BLOCKED_TILED:
+        case SHK_BLOCKED_WINDOWED:
+        case SHK_BLOCKED_HYBRID:
+            return FAM_EXACT;
+        case SHK_CHEBCODE:
+        case SHK_CHEBCODE_FAST:
+        case SHK_CHEBCODE_XTREME:
+        case SHK_CHEBCODE_BALANCED:
+            return FAM_CHEB;
+        case SHK_ADAPTIVE:
+        case SHK_FFT5:
+        case SHK_FFT3:
+        case SHK_FFT2:
+        case SHK_NUFFT:
+        case SHK_TREECODE:
+        case SHK_EWALD:
+        case SHK_DST:
+        case SHK_HODLR:
+            return FAM_UNSUPPORTED;
+        default:
+            return FAM_BAD;
+    }
+}
+
+// ChebPreset constants: src/stieltjes/chebcode.rs:942-973.
+ChebParams preset_of(int m) {
+    switch (m) {
+        case SHK_CHEBCODE_FAST:
+            return ChebParams{1.0, 8, 32, 1};
+        case SHK_CHEBCODE_XTREME:
+            return ChebParams{0.25, 11, 16, 0};
+        case SHK_CHEBCODE_BALANCED:
+            return ChebParams{0.55, 11, 32, 0};
+        case SHK_CHEBCODE:
+        default:
+            return ChebParams{0.5, 11, 32, 0};
+    }
+}
+
+// grid_leaf_cap: src/stieltjes/mod.rs:361-370.
+int64_t grid_leaf_cap(int64_t n, int64_t p, int64_t nq, int64_t preset_leaf) {
+    const int64_t kMaxGridLeaf = 8192;
+    if (nq == 0 || p == 0) return preset_leaf;
+    const double optimal = (double)n * std::sqrt(2.0 * (double)p / (double)nq);
+    int64_t o = (int64_t)std::llround(optimal);
+    const int64_t hi = std::max(kMaxGridLeaf, preset_leaf);
+    return std::min(std::max(o, preset_leaf), hi);
+}
+
+shk_status fail(shk_status s, const std::string &msg) {
+    set_error(msg);
+    return s;
+}
+
+bool all_finite(const double *v, size_t n) {
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(v[i])) return false;
+    return true;
+}
+
+shk_status check_eta(double eta) {
+    if (!(std::isfinite(eta) && eta > 0.0))
+        return fail(SHK_ERR_BAD_ETA, "eta must be a positive finite number, got " +
+                                         std::to_string(eta));
+    return SHK_OK;
+}
+
+std::string method_label(int m) {
+    if (m >= 0 && m < kNumMethods) return kCanonical[m];
+    return std::to_string(m);
+}
+
+shk_status check_method(int m) {
+    switch (family_of(m)) {
+        case FAM_EXACT:
+        case FAM_CHEB:
+            return SHK_OK;
+        case FAM_UNSUPPORTED:
+            return fail(SHK_ERR_UNSUPPORTED,
+                        "method '" + method_label(m) +
+                            "' has no GPU implementation in this library (supported: the "
+                            "exact family and the chebcode presets)");
+        default:
+            return fail(SHK_ERR_BAD_METHOD, "unknown method id " + std::to_string(m));
+    }
+}
+
+// Device readiness: a gfx950 device must be current.
+shk_status ensure_device() {
+    static std::once_flag once;
+    static shk_status state = SHK_OK;
+    static std::string msg;
+    std::call_once(once, [] {
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+            state = SHK_ERR_NO_DEVICE;
+            msg = "no HIP device is visible";
+            return;
+        }
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+            state = SHK_ERR_NO_DEVICE;
+            msg = "hipGetDeviceProperties failed";
+            return;
+        }
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+            state = SHK_ERR_NO_DEVICE;
+            msg = std::string("libshrinkers_hip is built for gfx950, device is ") +
+                  prop.gcnArchName;
+        }
+    });
+    if (state != SHK_OK) set_error(msg);
+    return state;
+}
+
+// Per-thread stream for the host entry points (default priority).
+hipStream_t host_stream() {
+    thread_local hipStream_t s = nullptr;
+    if (!s) {
+        if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) s = nullptr;
+    }
+    return s;
+}
+
+// RAII device buffer freed on the stream.
+struct DevBuf {
+    void *p = nullptr;
+    hipStream_t s = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFreeAsync(p, s);
+    }
+    template <class T>
+    T *as() const {
+        return static_cast<T *>(p);
+    }
+};
+
+int dev_alloc(DevBuf &b, size_t bytes, hipStream_t s) {
+    b.s = s;
+    SHK_HIP(hipMallocAsync(&b.p, bytes ? bytes : 8, s));
+    return 0;
+}
+
+// Exact-family kind for an all-points call (src/stieltjes/mod.rs:709-790,
+// dispatch_exact_blocked :562-594).
+ExactKind allpoints_kind(int m, double cutoff_ratio) {
+    const bool cut = cutoff_ratio > 0.0;
+    switch (m) {
+        case SHK_NAIVE:
+        case SHK_AUTOVEC:
+            return EXACT_PLAIN;  // the per-point kernels ignore the cutoff
+        case SHK_BLOCKED_HYBRID:
+            return cut ? EXACT_HYBRID : EXACT_PLAIN;
+        default:
+            return cut ? EXACT_CUTOFF : EXACT_PLAIN;
+    }
+}
+
+// Exact-family kind at arbitrary query points (mod.rs:425-446, 480-491).
+ExactKind atpoints_kind(int m, double cutoff_ratio) {
+    const bool cut = cutoff_ratio > 0.0;
+    if (m == SHK_NAIVE || m == SHK_AUTOVEC) return EXACT_PLAIN;
+    return cut ? EXACT_CUTOFF : EXACT_PLAIN;
+}
+
+int run_allpoints(const double *d_ev, size_t p, double eta, int method, double cutoff_ratio,
+                  int precision, double scale, double *d_re, double *d_im, hipStream_t s) {
+    const int fam = family_of(method);
+    if (fam == FAM_CHEB) {
+        ChebTree *t = nullptr;
+        SHK_TRY(cheb_build(d_ev, p, preset_of(method), &t, s));
+        int st = cheb_eval(t, nullptr, p, eta, scale, 1, d_re, d_im, s);
+        if (!st) {
+            hipError_t e = hipStreamSynchronize(s);
+            if (e != hipSuccess) {
+                set_error(std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+                st = SHK_ERR_HIP;
+            }
+        }
+        cheb_destroy(t);
+        return st;
+    }
+    const ExactKind kind = allpoints_kind(method, cutoff_ratio);
+    if (precision == SHK_PREC_F32C && kind == EXACT_PLAIN)
+        return exact_f32c(d_ev, p, d_ev, p, eta, scale, d_re, d_im, 0, s);
+    double *outs[2] = {d_re, d_im};
+    return exact_f64(d_ev, p, d_ev, p, eta, kind, cutoff_ratio * eta, scale, outs, s);
+}
+
+int run_atpoints(const double *d_q, size_t nq, const double *d_ev, size_t p, double eta,
+                 int method, double cutoff_ratio, double *d_re, double *d_im, hipStream_t s) {
+    if (family_of(method) == FAM_CHEB) {
+        ChebParams prm = preset_of(method);
+        prm.leaf_cap = grid_leaf_cap(prm.n, (int64_t)p, (int64_t)nq, prm.leaf_cap);
+        ChebTree *t = nullptr;
+        SHK_TRY(cheb_build(d_ev, p, prm, &t, s));
+        int st = cheb_eval(t, d_q, nq, eta, 1.0, 0, d_re, d_im, s);
+        if (!st) {
+            hipError_t e = hipStreamSynchronize(s);
+            if (e != hipSuccess) {
+                set_error(std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+                st = SHK_ERR_HIP;
+            }
+        }
+        cheb_destroy(t);
+        return st;
+    }
+    double *outs[2] = {d_re, d_im};
+    return exact_f64(d_q, nq, d_ev, p, eta, atpoints_kind(method, cutoff_ratio),
+                     cutoff_ratio * eta, 1.0, outs, s);
+}
+
+#define SHK_CHECK(expr)                     \
+    do {                                    \
+        int s_ = (expr);                    \
+        if (s_ != 0) return (shk_status)s_; \
+    } while (0)
+
+#define SHK_HIP_S(expr)                                                         \
+    do {                                                                        \
+        hipError_t e_ = (expr);                                                 \
+        if (e_ != hipSuccess) {                                                 \
+            set_error(std::string(#expr) + ": " + hipGetErrorString(e_));       \
+            return SHK_ERR_HIP;                                                 \
+        }                                                                       \
+    } while (0)
+
+}  // namespace
+
+struct shk_chebcode {
+    ChebTree *tree = nullptr;
+};
+
+extern "C" {
+
+const char *shk_last_error(void) { return last_error(); }
+
+const char *shk_version(void) { return "shrinkers_amd 0.1.0 (gfx950)"; }
+
+shk_status shk_method_from_name(const char *name, int *method_out) {
+    if (!name || !method_out) return fail(SHK_ERR_BAD_ARG, "NULL argument");
+    for (const auto &e : kNames) {
+        if (std::strcmp(e.name, name) == 0) {
+            *method_out = e.id;
+            return SHK_OK;
+        }
+    }
+    return fail(SHK_ERR_BAD_METHOD, std::string("unknown method '") + name + "'");
+}
+
+const char *shk_method_name(int method) {
+    if (method < 0 || method >= kNumMethods) return nullptr;
+    return kCanonical[method];
+}
+
+double shk_default_eta(size_t p) { return 0.1 / std::sqrt((double)p); }
+
+int shk_resolve_auto_method(int method, int parallel, size_t p) {
+    (void)parallel;
+    (void)p;
+    switch (method) {
+        case SHK_AUTO:
+        case SHK_SPEED_AUTO:
+            return SHK_CHEBCODE_FAST;
+        case SHK_ACCURACY_AUTO:
+            return SHK_BLOCKED;
+        default:
+            return method;
+    }
+}
+
+shk_status shk_compute_all_stieltjes(const double *eigenvalues, size_t p, double eta, int method,
+                                     double cutoff_ratio, int parallel, int precision,
+                                     double *out_re, double *out_im) {
+    if (p == 0) return SHK_OK;  // mod.rs:611-613
+    if (!eigenvalues || !out_re || !out_im) return fail(SHK_ERR_BAD_ARG, "NULL pointer");
+    if (!all_finite(eigenvalues, p))
+        return fail(SHK_ERR_NONFINITE, "eigenvalues must be finite (found NaN or infinity)");
+    SHK_CHECK(check_eta(eta));
+    if (precision != SHK_PREC_F64 && precision != SHK_PREC_F32C)
+        return fail(SHK_ERR_BAD_ARG, "precision must be SHK_PREC_F64 or SHK_PREC_F32C here");
+    method = shk_resolve_auto_method(method, parallel, p);
+    SHK_CHECK(check_method(method));
+    SHK_CHECK(ensure_device());
+    hipStream_t s = host_stream();
+    if (!s) return fail(SHK_ERR_HIP, "could not create a HIP stream");
+    DevBuf ev, re, im;
+    SHK_CHECK(dev_alloc(ev, sizeof(double) * p, s));
+    SHK_CHECK(dev_alloc(re, sizeof(double) * p, s));
+    SHK_CHECK(dev_alloc(im, sizeof(double) * p, s));
+    SHK_HIP_S(hipMemcpyAsync(ev.p, eigenvalues, sizeof(double) * p, hipMemcpyHostToDevice, s));
+    SHK_CHECK(run_allpoints(ev.as<double>(), p, eta, method, cutoff_ratio, precision,
+                            1.0 / (double)p, re.as<double>(), im.as<double>(), s));
+    SHK_HIP_S(hipMemcpyAsync(out_re, re.p, sizeof(double) * p, hipMemcpyDeviceToHost, s));
+    SHK_HIP_S(hipMemcpyAsync(out_im, im.p, sizeof(double) * p, hipMemcpyDeviceToHost, s));
+    SHK_HIP_S(hipStreamSynchronize(s));
+    return SHK_OK;
+}
+
+shk_status shk_compute_all_stieltjes_f32(const float *eigenvalues, size_t p, float eta,
+                                         int method, double cutoff_ratio, float *out_re,
+                                         float *out_im) {
+    if (p == 0) return SHK_OK;  // mod.rs:811-814
+    if (!eigenvalues || !out_re || !out_im) return fail(SHK_ERR_BAD_ARG, "NULL pointer");
+    for (size_t i = 0; i < p; ++i)
+        if (!std::isfinite(eigenvalues[i]))
+            return fail(SHK_ERR_NONFINITE, "eigenvalues must be finite (found NaN or infinity)");
+    SHK_CHECK(check_eta((double)eta));
+    method = shk_resolve_auto_method(method, 0, p);
+    SHK_CHECK(check_method(method));
+    SHK_CHECK(ensure_device());
+    hipStream_t s = host_stream();
+    if (!s) return fail(SHK_ERR_HIP, "could not create a HIP stream");
+    DevBuf ev32, ev, re, im, re32, im32;
+    SHK_CHECK(dev_alloc(ev32, sizeof(float) * p, s));
+    SHK_CHECK(dev_alloc(ev, sizeof(double) * p, s));
+    SHK_HIP_S(hipMemcpyAsync(ev32.p, eigenvalues, sizeof(float) * p, hipMemcpyHostToDevice, s));
+    SHK_CHECK(widen_f32(ev32.as<float>(), ev.as<double>(), p, s));
+    const double eta_d = (double)eta;
+    const double scale = 1.0 / (double)p;
+    const ExactKind kind = allpoints_kind(method, cutoff_ratio);
+    if (family_of(method) == FAM_EXACT && kind == EXACT_PLAIN) {
+        // Blocked/BlockedTiled/Auto arm (mod.rs:817-830): compensated f32.
+        SHK_CHECK(dev_alloc(re32, sizeof(float) * p, s));
+        SHK_CHECK(dev_alloc(im32, sizeof(float) * p, s));
+        SHK_CHECK(exact_f32c(ev.as<double>(), p, ev.as<double>(), p, eta_d, scale, re32.p,
+                             im32.p, 1, s));
+        SHK_HIP_S(hipMemcpyAsync(out_re, re32.p, sizeof(float) * p, hipMemcpyDeviceToHost, s));
+        SHK_HIP_S(hipMemcpyAsync(out_im, im32.p, sizeof(float) * p, hipMemcpyDeviceToHost, s));
+        SHK_HIP_S(hipStreamSynchronize(s));
+        return SHK_OK;
+    }
+    // Other methods: the f64 path, converted (mod.rs:832-849).
+    SHK_CHECK(dev_alloc(re, sizeof(double) * p, s));
+    SHK_CHECK(dev_alloc(im, sizeof(double) * p, s));
+    SHK_CHECK(run_allpoints(ev.as<double>(), p, eta_d, method, cutoff_ratio, SHK_PREC_F64, scale,
+                            re.as<double>(), im.as<double>(), s));
+    std::vector<double> hr(p), hi(p);
+    SHK_HIP_S(hipMemcpyAsync(hr.data(), re.p, sizeof(double) * p, hipMemcpyDeviceToHost, s));
+    SHK_HIP_S(hipMemcpyAsync(hi.data(), im.p, sizeof(double) * p, hipMemcpyDeviceToHost, s));
+    SHK_HIP_S(hipStreamSynchronize(s));
+    for (size_t i = 0; i < p; ++i) {
+        out_re[i] = (float)hr[i];
+        out_im[i] = (float)hi[i];
+    }
+    return SHK_OK;
+}
+
+shk_status shk_compute_stieltjes_at_points(const double *query_points, size_t nq,
+                                           const double *eigenvalues, size_t p, double eta,
+                                           int method, double cutoff_ratio, int parallel,
+                                           double *out_re, double *out_im) {
+    if (nq == 0) return SHK_OK;
+    if (!query_points || !out_re || !out_im || (p && !eigenvalues))
+        return fail(SHK_ERR_BAD_ARG, "NULL pointer");
+    if (p == 0) {  // an empty sum
+        for (size_t i = 0; i < nq; ++i) out_re[i] = out_im[i] = 0.0;
+        return SHK_OK;
+    }
+    if (!all_finite(eigenvalues, p) || !all_finite(query_points, nq))
+        return fail(SHK_ERR_NONFINITE, "eigenvalues must be finite (found NaN or infinity)");
+    SHK_CHECK(check_eta(eta));
+    method = shk_resolve_auto_method(method, parallel, p);
+    SHK_CHECK(check_method(method));
+    SHK_CHECK(ensure_device());
+    hipStream_t s = host_stream();
+    if (!s) return fail(SHK_ERR_HIP, "could not create a HIP stream");
+    DevBuf q, ev, re, im;
+    SHK_CHECK(dev_alloc(q, sizeof(double) * nq, s));
+    SHK_CHECK(dev_alloc(ev, sizeof(double) * p, s));
+    SHK_CHECK(dev_alloc(re, sizeof(double) * nq, s));
+    SHK_CHECK(dev_alloc(im, sizeof(double) * nq, s));
+    SHK_HIP_S(hipMemcpyAsync(q.p, query_points, sizeof(double) * nq, hipMemcpyHostToDevice, s));
+    SHK_HIP_S(hipMemcpyAsync(ev.p, eigenvalues, sizeof(double) * p, hipMemcpyHostToDevice, s));
+    SHK_CHECK(run_atpoints(q.as<double>(), nq, ev.as<double>(), p, eta, method, cutoff_ratio,
+                           re.as<double>(), im.as<double>(), s));
+    SHK_HIP_S(hipMemcpyAsync(out_re, re.p, sizeof(double) * nq, hipMemcpyDeviceToHost, s));
+    SHK_HIP_S(hipMemcpyAsync(out_im, im.p, sizeof(double) * nq, hipMemcpyDeviceToHost, s));
+    SHK_HIP_S(hipStreamSynchronize(s));
+    return SHK_OK;
+}
+
+shk_status shk_compute_all_stieltjes_with_deriv(const double *eigenvalues, size_t p, double eta,
+                                                double *val_re, double *val_im, double *der_re,
+                                                double *der_im) {
+    if (p == 0) return SHK_OK;
+    if (!eigenvalues || !val_re || !val_im || !der_re || !der_im)
+        return fail(SHK_ERR_BAD_ARG, "NULL pointer");
+    if (!all_finite(eigenvalues, p))
+        return fail(SHK_ERR_NONFINITE, "eigenvalues must be finite (found NaN or infinity)");
+    SHK_CHECK(check_eta(eta));
+    SHK_CHECK(ensure_device());
+    hipStream_t s = host_stream();
+    if (!s) return fail(SHK_ERR_HIP, "could not create a HIP stream");
+    DevBuf ev, o;
+    SHK_CHECK(dev_alloc(ev, sizeof(double) * p, s));
+    SHK_CHECK(dev_alloc(o, sizeof(double) * 4 * p, s));
+    SHK_HIP_S(hipMemcpyAsync(ev.p, eigenvalues, sizeof(double) * p, hipMemcpyHostToDevice, s));
+    double *base = o.as<double>();
+    double *outs[4] = {base, base + p, base + 2 * p, base + 3 * p};
+    SHK_CHECK(exact_f64(ev.as<double>(), p, ev.as<double>(), p, eta, EXACT_DERIV, 0.0, 1.0, outs,
+                        s));
+    double *host[4] = {val_re, val_im, der_re, der_im};
+    for (int k = 0; k < 4; ++k)
+        SHK_HIP_S(hipMemcpyAsync(host[k], outs[k], sizeof(double) * p, hipMemcpyDeviceToHost, s));
+    SHK_HIP_S(hipStreamSynchronize(s));
+    return SHK_OK;
+}
+
+shk_status shk_compute_stieltjes_scaled_ray(const double *eigenvalues, size_t p, double scale,
+                                            double *out_re, double *out_im) {
+    if (p == 0) return SHK_OK;
+    if (!eigenvalues || !out_re || !out_im) return fail(SHK_ERR_BAD_ARG, "NULL pointer");
+    if (!all_finite(eigenvalues, p))
+        return fail(SHK_ERR_NONFINITE, "eigenvalues must be finite (found NaN or infinity)");
+    if (!std::isfinite(scale)) return fail(SHK_ERR_BAD_ARG, "scale must be finite");
+    SHK_CHECK(ensure_device());
+    hipStream_t s = host_stream();
+    if (!s) return fail(SHK_ERR_HIP, "could not create a HIP stream");
+    DevBuf ev, re, im;
+    SHK_CHECK(dev_alloc(ev, sizeof(double) * p, s));
+    SHK_CHECK(dev_alloc(re, sizeof(double) * p, s));
+    SHK_CHECK(dev_alloc(im, sizeof(double) * p, s));
+    SHK_HIP_S(hipMemcpyAsync(ev.p, eigenvalues, sizeof(double) * p, hipMemcpyHostToDevice, s));
+    double *outs[2] = {re.as<double>(), im.as<double>()};
+    SHK_CHECK(exact_f64(ev.as<double>(), p, ev.as<double>(), p, scale, EXACT_RAY, 0.0, 1.0, outs,
+                        s));
+    SHK_HIP_S(hipMemcpyAsync(out_re, re.p, sizeof(double) * p, hipMemcpyDeviceToHost, s));
+    SHK_HIP_S(hipMemcpyAsync(out_im, im.p, sizeof(double) * p, hipMemcpyDeviceToHost, s));
+    SHK_HIP_S(hipStreamSynchronize(s));
+    return SHK_OK;
+}
+
+shk_status shk_all_points_dev(const double *d_eigenvalues, size_t p, double eta, int method,
+                              double cutoff_ratio, int precision, int scaled, double *d_out_re,
+                              double *d_out_im, void *stream) {
+    if (p == 0) return SHK_OK;
+    if (!d_eigenvalues || !d_out_re || !d_out_im) return fail(SHK_ERR_BAD_ARG, "NULL pointer");
+    SHK_CHECK(check_eta(eta));
+    if (precision != SHK_PREC_F64 && precision != SHK_PREC_F32C)
+        return fail(SHK_ERR_BAD_ARG, "precision must be SHK_PREC_F64 or SHK_PREC_F32C");
+    method = shk_resolve_auto_method(method, 0, p);
+    SHK_CHECK(check_method(method));
+    SHK_CHECK(ensure_device());
+    const double scale = scaled ? 1.0 / (double)p : 1.0;
+    return (shk_status)run_allpoints(d_eigenvalues, p, eta, method, cutoff_ratio, precision,
+                                     scale, d_out_re, d_out_im, (hipStream_t)stream);
+}
+
+shk_status shk_at_points_dev(const double *d_query, size_t nq, const double *d_eigenvalues,
+                             size_t p, double eta, int method, double cutoff_ratio,
+                             double *d_out_re, double *d_out_im, void *stream) {
+    if (nq == 0 || p == 0) return SHK_OK;
+    if (!d_query || !d_eigenvalues || !d_out_re || !d_out_im)
+        return fail(SHK_ERR_BAD_ARG, "NULL pointer");
+    SHK_CHECK(check_eta(eta));
+    method = shk_resolve_auto_method(method, 0, p);
+    SHK_CHECK(check_method(method));
+    SHK_CHECK(ensure_device());
+    return (shk_status)run_atpoints(d_query, nq, d_eigenvalues, p, eta, method, cutoff_ratio,
+                                    d_out_re, d_out_im, (hipStream_t)stream);
+}
+
+shk_status shk_chebcode_build(const double *eigenvalues, size_t p, double theta, int n,
+                              size_t leaf_cap, int f32_far, shk_chebcode **out) {
+    if (!out) return fail(SHK_ERR_BAD_ARG, "NULL handle pointer");
+    *out = nullptr;
+    if (p == 0) return fail(SHK_ERR_EMPTY, "eigenvalues must be non-empty");
+    if (!eigenvalues) return fail(SHK_ERR_BAD_ARG, "NULL pointer");
+    if (!all_finite(eigenvalues, p))
+        return fail(SHK_ERR_NONFINITE, "eigenvalues must be finite (found NaN or infinity)");
+    if (n < 2) return fail(SHK_ERR_BAD_ARG, "ChebCode needs at least 2 Chebyshev nodes");
+    SHK_CHECK(ensure_device());
+    hipStream_t s = host_stream();
+    if (!s) return fail(SHK_ERR_HIP, "could not create a HIP stream");
+    DevBuf ev;
+    SHK_CHECK(dev_alloc(ev, sizeof(double) * p, s));
+    SHK_HIP_S(hipMemcpyAsync(ev.p, eigenvalues, sizeof(double) * p, hipMemcpyHostToDevice, s));
+    ChebParams prm{theta, n, (int64_t)leaf_cap, f32_far ? 1 : 0};
+    ChebTree *t = nullptr;
+    SHK_CHECK(cheb_build(ev.as<double>(), p, prm, &t, s));
+    *out = new shk_chebcode{t};
+    return SHK_OK;
+}
+
+shk_status shk_chebcode_build_preset(const double *eigenvalues, size_t p, int method,
+                                     shk_chebcode **out) {
+    if (family_of(method) != FAM_CHEB)
+        return fail(SHK_ERR_BAD_METHOD, "not a chebcode preset: " + method_label(method));
+    const ChebParams prm = preset_of(method);
+    return shk_chebcode_build(eigenvalues, p, prm.theta, prm.n, (size_t)prm.leaf_cap, prm.f32_far,
+                              out);
+}
+
+shk_status shk_chebcode_info(const shk_chebcode *h, size_t *node_count, size_t *p) {
+    if (!h || !h->tree) return fail(SHK_ERR_BAD_ARG, "NULL handle");
+    if (node_count) *node_count = cheb_node_count(h->tree);
+    if (p) *p = cheb_p(h->tree);
+    return SHK_OK;
+}
+
+shk_status shk_chebcode_export(const shk_chebcode *h, double *lo, double *hi, int64_t *lo_idx,
+                               int64_t *hi_idx, int32_t *right, double *nodes, double *w) {
+    if (!h || !h->tree) return fail(SHK_ERR_BAD_ARG, "NULL handle");
+    hipStream_t s = host_stream();
+    if (!s) return fail(SHK_ERR_HIP, "could not create a HIP stream");
+    return (shk_status)cheb_export(h->tree, lo, hi, lo_idx, hi_idx, right, nodes, w, s);
+}
+
+shk_status shk_chebcode_evaluate(const shk_chebcode *h, double eta, double *out_re,
+                                 double *out_im) {
+    if (!h || !h->tree || !out_re || !out_im) return fail(SHK_ERR_BAD_ARG, "NULL pointer");
+    SHK_CHECK(check_eta(eta));
+    hipStream_t s = host_stream();
+    if (!s) return fail(SHK_ERR_HIP, "could not create a HIP stream");
+    const size_t p = cheb_p(h->tree);
+    DevBuf re, im;
+    SHK_CHECK(dev_alloc(re, sizeof(double) * p, s));
+    SHK_CHECK(dev_alloc(im, sizeof(double) * p, s));
+    SHK_CHECK(cheb_eval(h->tree, nullptr, p, eta, 1.0, 1, re.as<double>(), im.as<double>(), s));
+    SHK_HIP_S(hipMemcpyAsync(out_re, re.p, sizeof(double) * p, hipMemcpyDeviceToHost, s));
+    SHK_HIP_S(hipMemcpyAsync(out_im, im.p, sizeof(double) * p, hipMemcpyDeviceToHost, s));
+    SHK_HIP_S(hipStreamSynchronize(s));
+    return SHK_OK;
+}
+
+shk_status shk_chebcode_evaluate_points(const shk_chebcode *h, const double *query, size_t nq,
+                                        double eta, double *out_re, double *out_im) {
+    if (!h || !h->tree) return fail(SHK_ERR_BAD_ARG, "NULL handle");
+    if (nq == 0) return SHK_OK;
+    if (!query || !out_re || !out_im) return fail(SHK_ERR_BAD_ARG, "NULL pointer");
+    if (!all_finite(query, nq))
+        return fail(SHK_ERR_NONFINITE, "query points must be finite (found NaN or infinity)");
+    SHK_CHECK(check_eta(eta));
+    hipStream_t s = host_stream();
+    if (!s) return fail(SHK_ERR_HIP, "could not create a HIP stream");
+    DevBuf q, re, im;
+    SHK_CHECK(dev_alloc(q, sizeof(double) * nq, s));
+    SHK_CHECK(dev_alloc(re, sizeof(double) * nq, s));
+    SHK_CHECK(dev_alloc(im, sizeof(double) * nq, s));
+    SHK_HIP_S(hipMemcpyAsync(q.p, query, sizeof(double) * nq, hipMemcpyHostToDevice, s));
+    SHK_CHECK(cheb_eval(h->tree, q.as<double>(), nq, eta, 1.0, 0, re.as<double>(),
+                        im.as<double>(), s));
+    SHK_HIP_S(hipMemcpyAsync(out_re, re.p, sizeof(double) * nq, hipMemcpyDeviceToHost, s));
+    SHK_HIP_S(hipMemcpyAsync(out_im, im.p, sizeof(double) * nq, hipMemcpyDeviceToHost, s));
+    SHK_HIP_S(hipStreamSynchronize(s));
+    return SHK_OK;
+}
+
+shk_status shk_chebcode_evaluate_many(const shk_chebcode *h, const double *etas, size_t n_eta,
+                                      double *out_re, double *out_im) {
+    if (!h || !h->tree) return fail(SHK_ERR_BAD_ARG, "NULL handle");
+    if (n_eta == 0) return SHK_OK;
+    if (!etas || !out_re || !out_im) return fail(SHK_ERR_BAD_ARG, "NULL pointer");
+    for (size_t k = 0; k < n_eta; ++k) SHK_CHECK(check_eta(etas[k]));
+    hipStream_t s = host_stream();
+    if (!s) return fail(SHK_ERR_HIP, "could not create a HIP stream");
+    const size_t p = cheb_p(h->tree);
+    DevBuf buf;
+    SHK_CHECK(dev_alloc(buf, sizeof(double) * 4 * p, s));
+    double *b = buf.as<double>();
+    for (size_t k = 0; k < n_eta; k += 2) {
+        const double e0 = etas[k];
+        const double e1 = (k + 1 < n_eta) ? etas[k + 1] : e0;
+        SHK_CHECK(cheb_eval_x2(h->tree, e0, e1, b, b + p, b + 2 * p, b + 3 * p, s));
+        SHK_HIP_S(hipMemcpyAsync(out_re + k * p, b, sizeof(double) * p, hipMemcpyDeviceToHost, s));
+        SHK_HIP_S(hipMemcpyAsync(out_im + k * p, b + p, sizeof(double) * p, hipMemcpyDeviceToHost,
+                                 s));
+        if (k + 1 < n_eta) {
+            SHK_HIP_S(hipMemcpyAsync(out_re + (k + 1) * p, b + 2 * p, sizeof(double) * p,
+                                     hipMemcpyDeviceToHost, s));
+            SHK_HIP_S(hipMemcpyAsync(out_im + (k + 1) * p, b + 3 * p, sizeof(double) * p,
+                                     hipMemcpyDeviceToHost, s));
+        }
+        SHK_HIP_S(hipStreamSynchronize(s));
+    }
+    return SHK_OK;
+}
+
+shk_status shk_chebcode_destroy(shk_chebcode *h) {
+    if (!h) return SHK_OK;
+    cheb_destroy(h->tree);
+    delete h;
+    return SHK_OK;
+}
+
+shk_status shk_profile_enable(int enable) {
+    g_prof_on.store(enable ? 1 : 0);
+    return SHK_OK;
+}
+
+shk_status shk_profile_read(double *kernel_ms_total, int64_t *launches) {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    for (auto &ab : g_prof_pending) {
+        float ms = 0.f;
+        if (hipEventSynchronize(ab.second) == hipSuccess &&
+            hipEventElapsedTime(&ms, ab.first, ab.second) == hipSuccess) {
+            g_prof_ms += ms;
+            g_prof_n += 1;
+        }
+        (void)hipEventDestroy(ab.first);
+        (void)hipEventDestroy(ab.second);
+    }
+    g_prof_pending.clear();
+    if (kernel_ms_total) *kernel_ms_total = g_prof_ms;
+    if (launches) *launches = g_prof_n;
+    return SHK_OK;
+}
+
+shk_status shk_profile_reset(void) {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    for (auto &ab : g_prof_pending) {
+        (void)hipEventSynchronize(ab.second);
+        (void)hipEventDestroy(ab.first);
+        (void)hipEventDestroy(ab.second);
+    }
+    g_prof_pending.clear();
+    g_prof_ms = 0.0;
+    g_prof_n = 0;
+    return SHK_OK;
+}
+
+}  // extern "C"

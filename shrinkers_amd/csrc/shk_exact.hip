@@ -1,0 +1,450 @@
+// Exact O(p^2) Stieltjes kernels for gfx950 (CDNA4).
+//
+// Reference arithmetic: src/stieltjes/term.rs:26-45 (term),
+// src/stieltjes/cacheblock.rs:1286-1424 (full-square tiled body) and
+// :1454-1513 (disjoint output spans).  The GPU decomposition keeps the
+// reference's "query tile owns its outputs, sweeps the sources" shape:
+//
+//   grid.x = query tiles of kQT = kWG*kR queries (kR per thread, strided so
+//            loads and stores coalesce), grid.y = source splits.
+//   Each workgroup stages its source split through LDS in pieces of kPiece
+//   values; every wavefront then reads the same LDS word (a broadcast, no
+//   bank conflicts) and updates kR independent query accumulators, so one
+//   LDS read feeds kR pair evaluations and every reciprocal chain is
+//   independent of its neighbours.
+//   Splits write FP64 partial sums to scratch ([acc][split][query]); a second
+//   kernel adds them in split order (deterministic) and applies eta and the
+//   1/p scale once per output, as the reference's eta-hoisted form does
+//   (term.rs:33-45, cacheblock.rs:218-221).
+//
+// FP32-compensated variant (exact_f32c): the pair arithmetic is FP32 on
+// eigenvalues re-expressed relative to a per-tile centre c (computed in FP64
+// from the tile's query range).  Per workgroup one of three forms is chosen
+// from the tile half-width M:
+//   M <= 2*eta  : d = qh - sh                  (rounding of x_i - c costs at
+//                                               most eps/2*M <= eps*eta)
+//   M <= 32*eta : d = (qh - sh) + ql           (query residual restored)
+//   otherwise   : d = (qh - sh) + (ql - sl)    (both residuals restored)
+// where qh/ql and sh/sl are the f32 hi/lo parts of x_i - c and lam_j - c.
+// For the near pairs that dominate Im S, qh - sh is exact (Sterbenz), so the
+// pair difference carries at most one f32 rounding.  The reciprocal is
+// v_rcp_f32 (1 ulp).  Partial sums run over kRun sources in FP32 and are
+// folded into FP64 accumulators; the cross-split reduction is FP64.
+#include "shk_internal.hpp"
+
+#include <algorithm>
+#include <vector>
+
+namespace shk {
+namespace {
+
+constexpr int kWG = 256;        // threads per workgroup (4 wavefronts)
+constexpr int kR = 4;           // queries per thread
+constexpr int kQT = kWG * kR;   // queries per tile
+constexpr int kPiece = 2048;    // sources staged in LDS per pass
+constexpr int kRun = 32;        // f32 terms per FP64 fold
+constexpr int kTargetWG = 2048; // workgroups to aim for (8 per CU)
+
+// 1/x to ~1 ulp: v_rcp_f64 seed refined by two Newton-Raphson steps
+// r <- r + r*(1 - x*r) (each step squares the relative error).
+__device__ __forceinline__ double rcp_f64(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
+template <int KIND>
+struct NAcc {
+    static constexpr int value = (KIND == EXACT_DERIV) ? 4 : 2;
+};
+
+template <int KIND>
+__global__ __launch_bounds__(kWG) void exact_f64_kernel(
+    const double *__restrict__ q, int nq, const double *__restrict__ src, int p, double eta,
+    double cut, int src_per_split, double *__restrict__ part) {
+    constexpr int NA = NAcc<KIND>::value;
+    __shared__ double s_src[kPiece];
+    const int tid = threadIdx.x;
+    const int split = blockIdx.y;
+    const int n_split = gridDim.y;
+    const int j_begin = split * src_per_split;
+    const int j_end = min(j_begin + src_per_split, p);
+
+    double x[kR], e2[kR], acc[NA][kR];
+#pragma unroll
+    for (int k = 0; k < kR; ++k) {
+        const int i = blockIdx.x * kQT + k * kWG + tid;
+        x[k] = (i < nq) ? q[i] : 0.0;
+        const double ek = (KIND == EXACT_RAY) ? eta * x[k] : eta;
+        e2[k] = ek * ek;
+#pragma unroll
+        for (int a = 0; a < NA; ++a) acc[a][k] = 0.0;
+    }
+
+    for (int base = j_begin; base < j_end; base += kPiece) {
+        const int cnt = min(kPiece, j_end - base);
+        __syncthreads();
+        for (int j = tid; j < cnt; j += kWG) s_src[j] = src[base + j];
+        __syncthreads();
+#pragma unroll 2
+        for (int j = 0; j < cnt; ++j) {
+            const double s = s_src[j];
+#pragma unroll
+            for (int k = 0; k < kR; ++k) {
+                const double d = x[k] - s;
+                const double r = rcp_f64(__builtin_fma(d, d, e2[k]));
+                if constexpr (KIND == EXACT_CUTOFF) {
+                    const double rm = (__builtin_fabs(d) <= cut) ? r : 0.0;
+                    acc[0][k] = __builtin_fma(d, rm, acc[0][k]);
+                    acc[1][k] += rm;
+                } else if constexpr (KIND == EXACT_HYBRID) {
+                    acc[0][k] = __builtin_fma(d, r, acc[0][k]);
+                    acc[1][k] += (__builtin_fabs(d) <= cut) ? r : 0.0;
+                } else if constexpr (KIND == EXACT_DERIV) {
+                    // S' = -sum (d^2 - eta^2) inv^2  +  i * (-sum 2 d eta inv^2)
+                    acc[0][k] = __builtin_fma(d, r, acc[0][k]);
+                    acc[1][k] += r;
+                    const double r2 = r * r;
+                    acc[2][k] = __builtin_fma(-__builtin_fma(d, d, -e2[k]), r2, acc[2][k]);
+                    acc[3][k] = __builtin_fma(-2.0 * d * eta, r2, acc[3][k]);
+                } else {
+                    acc[0][k] = __builtin_fma(d, r, acc[0][k]);
+                    acc[1][k] += r;
+                }
+            }
+        }
+    }
+
+#pragma unroll
+    for (int k = 0; k < kR; ++k) {
+        const int i = blockIdx.x * kQT + k * kWG + tid;
+        if (i < nq) {
+#pragma unroll
+            for (int a = 0; a < NA; ++a)
+                part[((size_t)a * n_split + split) * nq + i] = acc[a][k];
+        }
+    }
+}
+
+// ---- FP32-compensated pair kernel ------------------------------------------
+
+template <int FORM>
+__device__ __forceinline__ float pair_diff(float qh, float ql, float sh, float sl) {
+    if constexpr (FORM == 0) return qh - sh;
+    else if constexpr (FORM == 1) return (qh - sh) + ql;
+    else return (qh - sh) + (ql - sl);
+}
+
+// Sweeps `cnt` staged sources for this thread's kR queries.
+template <int FORM>
+__device__ __forceinline__ void f32c_sweep(const float *__restrict__ sh_lds,
+                                           const float *__restrict__ sl_lds, int cnt,
+                                           const float (&qh)[kR], const float (&ql)[kR],
+                                           float eta2f, double (&re)[kR], double (&iv)[kR]) {
+    int j = 0;
+    for (; j + kRun <= cnt; j += kRun) {
+        float ar[kR], ai[kR];
+#pragma unroll
+        for (int k = 0; k < kR; ++k) ar[k] = ai[k] = 0.0f;
+#pragma unroll 8
+        for (int u = 0; u < kRun; ++u) {
+            const float sh = sh_lds[j + u];
+            const float sl = (FORM == 2) ? sl_lds[j + u] : 0.0f;
+#pragma unroll
+            for (int k = 0; k < kR; ++k) {
+                const float d = pair_diff<FORM>(qh[k], ql[k], sh, sl);
+                const float r = __builtin_amdgcn_rcpf(__builtin_fmaf(d, d, eta2f));
+                ar[k] = __builtin_fmaf(d, r, ar[k]);
+                ai[k] += r;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kR; ++k) {
+            re[k] += (double)ar[k];
+            iv[k] += (double)ai[k];
+        }
+    }
+    if (j < cnt) {
+        float ar[kR], ai[kR];
+#pragma unroll
+        for (int k = 0; k < kR; ++k) ar[k] = ai[k] = 0.0f;
+        for (; j < cnt; ++j) {
+            const float sh = sh_lds[j];
+            const float sl = (FORM == 2) ? sl_lds[j] : 0.0f;
+#pragma unroll
+            for (int k = 0; k < kR; ++k) {
+                const float d = pair_diff<FORM>(qh[k], ql[k], sh, sl);
+                const float r = __builtin_amdgcn_rcpf(__builtin_fmaf(d, d, eta2f));
+                ar[k] = __builtin_fmaf(d, r, ar[k]);
+                ai[k] += r;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kR; ++k) {
+            re[k] += (double)ar[k];
+            iv[k] += (double)ai[k];
+        }
+    }
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__global__ __launch_bounds__(kWG) void exact_f32c_kernel(const double *__restrict__ q, int nq,
+                                                         const double *__restrict__ src, int p,
+                                                         double eta, int src_per_split,
+                                                         double *__restrict__ part) {
+    __shared__ float s_hi[kPiece];
+    __shared__ float s_lo[kPiece];
+    __shared__ double s_red[2][kWG / 64];
+    const int tid = threadIdx.x;
+    const int split = blockIdx.y;
+    const int n_split = gridDim.y;
+    const int j_begin = split * src_per_split;
+    const int j_end = min(j_begin + src_per_split, p);
+
+    double x[kR];
+    double mn = __builtin_inf(), mx = -__builtin_inf();
+#pragma unroll
+    for (int k = 0; k < kR; ++k) {
+        const int i = blockIdx.x * kQT + k * kWG + tid;
+        x[k] = (i < nq) ? q[i] : 0.0;
+        if (i < nq) {
+            mn = fmin(mn, x[k]);
+            mx = fmax(mx, x[k]);
+        }
+    }
+    mn = wave_min(mn);
+    mx = wave_max(mx);
+    if ((tid & 63) == 0) {
+        s_red[0][tid >> 6] = mn;
+        s_red[1][tid >> 6] = mx;
+    }
+    __syncthreads();
+    mn = s_red[0][0];
+    mx = s_red[1][0];
+#pragma unroll
+    for (int w = 1; w < kWG / 64; ++w) {
+        mn = fmin(mn, s_red[0][w]);
+        mx = fmax(mx, s_red[1][w]);
+    }
+    const double c = 0.5 * (mn + mx);
+    const double half_w = 0.5 * (mx - mn);
+    const int form = (half_w <= 2.0 * eta) ? 0 : ((half_w <= 32.0 * eta) ? 1 : 2);
+
+    float qh[kR], ql[kR];
+    double re[kR], iv[kR];
+#pragma unroll
+    for (int k = 0; k < kR; ++k) {
+        const double t = x[k] - c;
+        qh[k] = (float)t;
+        ql[k] = (float)(t - (double)qh[k]);
+        re[k] = iv[k] = 0.0;
+    }
+    const float eta2f = (float)(eta * eta);
+
+    for (int base = j_begin; base < j_end; base += kPiece) {
+        const int cnt = min(kPiece, j_end - base);
+        __syncthreads();
+        for (int j = tid; j < cnt; j += kWG) {
+            const double t = src[base + j] - c;
+            const float h = (float)t;
+            s_hi[j] = h;
+            s_lo[j] = (float)(t - (double)h);
+        }
+        __syncthreads();
+        if (form == 0)
+            f32c_sweep<0>(s_hi, s_lo, cnt, qh, ql, eta2f, re, iv);
+        else if (form == 1)
+            f32c_sweep<1>(s_hi, s_lo, cnt, qh, ql, eta2f, re, iv);
+        else
+            f32c_sweep<2>(s_hi, s_lo, cnt, qh, ql, eta2f, re, iv);
+    }
+
+#pragma unroll
+    for (int k = 0; k < kR; ++k) {
+        const int i = blockIdx.x * kQT + k * kWG + tid;
+        if (i < nq) {
+            part[((size_t)0 * n_split + split) * nq + i] = re[k];
+            part[((size_t)1 * n_split + split) * nq + i] = iv[k];
+        }
+    }
+}
+
+// ---- deterministic cross-split reduction ------------------------------------
+
+template <int KIND, typename OutT>
+__global__ __launch_bounds__(256) void reduce_kernel(const double *__restrict__ part, int n_split,
+                                                     int nq, double eta, double scale,
+                                                     const double *__restrict__ q,
+                                                     OutT *__restrict__ o0, OutT *__restrict__ o1,
+                                                     OutT *__restrict__ o2, OutT *__restrict__ o3) {
+    constexpr int NA = NAcc<KIND>::value;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    double a[NA];
+#pragma unroll
+    for (int t = 0; t < NA; ++t) a[t] = 0.0;
+    for (int s = 0; s < n_split; ++s) {
+#pragma unroll
+        for (int t = 0; t < NA; ++t) a[t] += part[((size_t)t * n_split + s) * nq + i];
+    }
+    const double eta_i = (KIND == EXACT_RAY) ? eta * q[i] : eta;
+    o0[i] = (OutT)(a[0] * scale);
+    o1[i] = (OutT)((eta_i * a[1]) * scale);
+    if constexpr (NA == 4) {
+        o2[i] = (OutT)(a[2] * scale);
+        o3[i] = (OutT)(a[3] * scale);
+    }
+}
+
+__global__ void widen_kernel(const float *__restrict__ in, double *__restrict__ out, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (double)in[i];
+}
+
+// Source splits so that about kTargetWG workgroups are in flight.
+void plan_splits(size_t nq, size_t p, int *n_qtiles, int *n_split, int *src_per_split) {
+    const int qt = (int)((nq + kQT - 1) / kQT);
+    int ns = std::max(1, (kTargetWG + qt - 1) / qt);
+    const int max_split = (int)std::max<size_t>(1, (p + 63) / 64);  // >= 64 sources each
+    ns = std::min(ns, max_split);
+    int per = (int)((p + ns - 1) / ns);
+    per = (per + 31) / 32 * 32;
+    ns = (int)((p + per - 1) / per);
+    *n_qtiles = qt;
+    *n_split = std::max(ns, 1);
+    *src_per_split = per;
+}
+
+template <int KIND>
+int launch_f64(const double *q, size_t nq, const double *src, size_t p, double eta, double cut,
+               int n_qtiles, int n_split, int per, double *part, hipStream_t stream) {
+    dim3 grid(n_qtiles, n_split);
+    ProfileMark m = profile_begin(stream);
+    hipLaunchKernelGGL(exact_f64_kernel<KIND>, grid, dim3(kWG), 0, stream, q, (int)nq, src,
+                       (int)p, eta, cut, per, part);
+    profile_end(m, stream);
+    SHK_HIP(hipGetLastError());
+    return 0;
+}
+
+template <int KIND, typename OutT>
+int launch_reduce(const double *part, int n_split, size_t nq, double eta, double scale,
+                  const double *q, OutT *o0, OutT *o1, OutT *o2, OutT *o3, hipStream_t stream) {
+    const int blocks = (int)((nq + 255) / 256);
+    hipLaunchKernelGGL((reduce_kernel<KIND, OutT>), dim3(blocks), dim3(256), 0, stream, part,
+                       n_split, (int)nq, eta, scale, q, o0, o1, o2, o3);
+    SHK_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+int exact_f64(const double *q, size_t nq, const double *src, size_t p, double eta,
+              ExactKind kind, double cut, double scale, double *const *out,
+              hipStream_t stream) {
+    if (nq == 0 || p == 0) return 0;
+    if (nq > (size_t)INT32_MAX || p > (size_t)INT32_MAX) {
+        set_error("exact kernel: sizes above 2^31-1 are not supported");
+        return 6;
+    }
+    int qt, ns, per;
+    plan_splits(nq, p, &qt, &ns, &per);
+    const int na = (kind == EXACT_DERIV) ? 4 : 2;
+    double *part = nullptr;
+    SHK_HIP(hipMallocAsync((void **)&part, sizeof(double) * na * (size_t)ns * nq, stream));
+    int st = 0;
+    switch (kind) {
+        case EXACT_PLAIN:
+            st = launch_f64<EXACT_PLAIN>(q, nq, src, p, eta, cut, qt, ns, per, part, stream);
+            if (!st) st = launch_reduce<EXACT_PLAIN, double>(part, ns, nq, eta, scale, q, out[0], out[1], nullptr, nullptr, stream);
+            break;
+        case EXACT_CUTOFF:
+            st = launch_f64<EXACT_CUTOFF>(q, nq, src, p, eta, cut, qt, ns, per, part, stream);
+            if (!st) st = launch_reduce<EXACT_CUTOFF, double>(part, ns, nq, eta, scale, q, out[0], out[1], nullptr, nullptr, stream);
+            break;
+        case EXACT_HYBRID:
+            st = launch_f64<EXACT_HYBRID>(q, nq, src, p, eta, cut, qt, ns, per, part, stream);
+            if (!st) st = launch_reduce<EXACT_HYBRID, double>(part, ns, nq, eta, scale, q, out[0], out[1], nullptr, nullptr, stream);
+            break;
+        case EXACT_DERIV:
+            st = launch_f64<EXACT_DERIV>(q, nq, src, p, eta, cut, qt, ns, per, part, stream);
+            if (!st) st = launch_reduce<EXACT_DERIV, double>(part, ns, nq, eta, scale, q, out[0], out[1], out[2], out[3], stream);
+            break;
+        case EXACT_RAY:
+            st = launch_f64<EXACT_RAY>(q, nq, src, p, eta, cut, qt, ns, per, part, stream);
+            if (!st) st = launch_reduce<EXACT_RAY, double>(part, ns, nq, eta, scale, q, out[0], out[1], nullptr, nullptr, stream);
+            break;
+        default:
+            st = 6;
+            set_error("exact kernel: unknown kind");
+    }
+    hipError_t fe = hipFreeAsync(part, stream);
+    if (st) return st;
+    if (fe != hipSuccess) {
+        set_error(std::string("hipFreeAsync: ") + hipGetErrorString(fe));
+        return 7;
+    }
+    return 0;
+}
+
+int exact_f32c(const double *q, size_t nq, const double *src, size_t p, double eta,
+               double scale, void *out_re, void *out_im, int out_f32, hipStream_t stream) {
+    if (nq == 0 || p == 0) return 0;
+    if (nq > (size_t)INT32_MAX || p > (size_t)INT32_MAX) {
+        set_error("exact kernel: sizes above 2^31-1 are not supported");
+        return 6;
+    }
+    int qt, ns, per;
+    plan_splits(nq, p, &qt, &ns, &per);
+    double *part = nullptr;
+    SHK_HIP(hipMallocAsync((void **)&part, sizeof(double) * 2 * (size_t)ns * nq, stream));
+    dim3 grid(qt, ns);
+    ProfileMark m = profile_begin(stream);
+    hipLaunchKernelGGL(exact_f32c_kernel, grid, dim3(kWG), 0, stream, q, (int)nq, src, (int)p,
+                       eta, per, part);
+    profile_end(m, stream);
+    int st = 0;
+    hipError_t le = hipGetLastError();
+    if (le != hipSuccess) {
+        set_error(std::string("exact_f32c_kernel launch: ") + hipGetErrorString(le));
+        st = 7;
+    }
+    if (!st) {
+        if (out_f32)
+            st = launch_reduce<EXACT_PLAIN, float>(part, ns, nq, eta, scale, q, (float *)out_re,
+                                                   (float *)out_im, nullptr, nullptr, stream);
+        else
+            st = launch_reduce<EXACT_PLAIN, double>(part, ns, nq, eta, scale, q,
+                                                    (double *)out_re, (double *)out_im, nullptr,
+                                                    nullptr, stream);
+    }
+    hipError_t fe = hipFreeAsync(part, stream);
+    if (st) return st;
+    if (fe != hipSuccess) {
+        set_error(std::string("hipFreeAsync: ") + hipGetErrorString(fe));
+        return 7;
+    }
+    return 0;
+}
+
+int widen_f32(const float *in, double *out, size_t n, hipStream_t stream) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(widen_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, in,
+                       out, n);
+    SHK_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace shk

@@ -1,0 +1,105 @@
+// Internal declarations shared by the HIP translation units of
+// libshrinkers_hip.so.  Not part of the C ABI (see include/).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+namespace shk {
+
+// ---- error plumbing --------------------------------------------------------
+
+void set_error(const std::string &msg);
+const char *last_error();
+
+// Evaluates a HIP call; on failure records the message and returns
+// SHK_ERR_HIP (7) from the enclosing function.
+#define SHK_HIP(expr)                                                           \
+    do {                                                                        \
+        hipError_t e_ = (expr);                                                 \
+        if (e_ != hipSuccess) {                                                 \
+            ::shk::set_error(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+            return 7;                                                           \
+        }                                                                       \
+    } while (0)
+
+// Propagates a non-zero status.
+#define SHK_TRY(expr)                  \
+    do {                               \
+        int s_ = (expr);               \
+        if (s_ != 0) return s_;        \
+    } while (0)
+
+// ---- measurement -----------------------------------------------------------
+
+// Brackets the dominant kernel of an evaluation with events on `stream` when
+// profiling is enabled.  begin() returns an opaque token consumed by end().
+struct ProfileMark {
+    hipEvent_t a = nullptr, b = nullptr;
+};
+ProfileMark profile_begin(hipStream_t stream);
+void profile_end(const ProfileMark &m, hipStream_t stream);
+
+// ---- exact family (shk_exact.hip) -------------------------------------------
+
+// Which accumulation the exact kernels perform.
+enum ExactKind : int {
+    EXACT_PLAIN = 0,   // re = sum d*inv, im = eta * sum inv
+    EXACT_CUTOFF = 1,  // both parts skip |d| > cut
+    EXACT_HYBRID = 2,  // re from every term, im skips |d| > cut
+    EXACT_DERIV = 3,   // values + d/dx (autovec.rs:38-59)
+    EXACT_RAY = 4,     // eta_i = scale * x_i per query (mod.rs:304-319)
+};
+
+// Raw or scaled exact sums at nq query points against p sources, FP64.
+//   eta      : eta (EXACT_RAY: the ray scale h)
+//   cut      : absolute cutoff distance (ratio*eta) for CUTOFF/HYBRID
+//   scale    : multiplier applied to every output (1 or 1/p)
+//   out[0..n_out) : re, im (+ deriv re, deriv im for EXACT_DERIV)
+int exact_f64(const double *q, size_t nq, const double *src, size_t p, double eta,
+              ExactKind kind, double cut, double scale, double *const *out,
+              hipStream_t stream);
+
+// FP32-compensated exact sums (no cutoff).  out_re/out_im are double, or
+// float when out_f32 != 0.
+int exact_f32c(const double *q, size_t nq, const double *src, size_t p, double eta,
+               double scale, void *out_re, void *out_im, int out_f32, hipStream_t stream);
+
+// float -> double widening on device (for the f32 entry points).
+int widen_f32(const float *in, double *out, size_t n, hipStream_t stream);
+
+// ---- ChebCode (shk_chebcode.hip) --------------------------------------------
+
+struct ChebTree;  // device-resident tree
+
+struct ChebParams {
+    double theta;
+    int n;
+    int64_t leaf_cap;
+    int f32_far;
+};
+
+// Builds the tree on the GPU from device eigenvalues (any order).
+int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **out,
+               hipStream_t stream);
+void cheb_destroy(ChebTree *t);
+size_t cheb_node_count(const ChebTree *t);
+size_t cheb_p(const ChebTree *t);
+int cheb_params(const ChebTree *t, ChebParams *prm);
+// Copies the pre-order tree arrays to host memory (NULL pointers skipped).
+int cheb_export(const ChebTree *t, double *lo, double *hi, int64_t *lo_idx, int64_t *hi_idx,
+                int32_t *right, double *nodes, double *w, hipStream_t stream);
+// Raw sums at nq device query points; order_by_perm != 0 means the queries
+// are the tree's own sorted eigenvalues and the outputs are scattered back
+// to the build input order.  scale multiplies the outputs.
+int cheb_eval(const ChebTree *t, const double *d_query, size_t nq, double eta, double scale,
+              int queries_are_sources, double *d_out_re, double *d_out_im,
+              hipStream_t stream);
+// Two-eta traversal for evaluate_many: acceptance at min(eta0, eta1).
+int cheb_eval_x2(const ChebTree *t, double eta0, double eta1, double *d_re0, double *d_im0,
+                 double *d_re1, double *d_im1, hipStream_t stream);
+
+}  // namespace shk

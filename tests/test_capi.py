@@ -1,0 +1,96 @@
+"""C ABI checks that need no GPU: the library loads, exports every function
+declared in include/shrinkers_stieltjes.h, and rejects bad arguments before
+touching a device."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "shrinkers_stieltjes.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(shk_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_boundary():
+    names = declared_functions()
+    for must in ("shk_compute_all_stieltjes", "shk_compute_stieltjes_at_points",
+                 "shk_compute_all_stieltjes_f32", "shk_chebcode_build",
+                 "shk_chebcode_evaluate_points", "shk_chebcode_evaluate_many",
+                 "shk_all_points_dev"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol(shk):
+    from shrinkers_amd import _lib
+    lib = _lib.load()
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+        assert name in _lib.SIGNATURES, f"{name} has no ctypes signature"
+
+
+def test_method_names_and_aliases(shk):
+    from shrinkers_amd import stieltjes as st
+    # src/python.rs:321-350
+    aliases = {"fft_fast": "nufft", "treecode": "fmm", "chebyshev": "chebcode",
+               "chebf": "chebcode_fast", "chebx": "chebcode_xtreme", "chebb": "chebcode_balanced",
+               "speed": "speed_auto", "accuracy": "accuracy_auto"}
+    for k, name in enumerate(st.METHODS):
+        assert st.parse_method(name) == k
+        assert st.method_name(k) == name
+    for a, name in aliases.items():
+        assert st.method_name(st.parse_method(a)) == name
+    with pytest.raises(ValueError, match="unknown method 'bogus'"):
+        st.parse_method("bogus")
+
+
+def test_status_codes_before_device(shk):
+    from shrinkers_amd import _lib
+    lib = _lib.load()
+    ev = np.array([1.0, np.nan, 2.0])
+    out = np.empty(3)
+    st = lib.shk_compute_all_stieltjes(_lib.dptr(ev), 3, 0.1, 2, 0.0, 0, 0, _lib.dptr(out),
+                                       _lib.dptr(out))
+    assert st == _lib.SHK_ERR_NONFINITE
+    assert "finite" in _lib.last_error()
+    ev = np.array([1.0, 2.0, 3.0])
+    st = lib.shk_compute_all_stieltjes(_lib.dptr(ev), 3, -1.0, 2, 0.0, 0, 0, _lib.dptr(out),
+                                       _lib.dptr(out))
+    assert st == _lib.SHK_ERR_BAD_ETA
+    st = lib.shk_compute_all_stieltjes(_lib.dptr(ev), 3, 0.1, 8, 0.0, 0, 0, _lib.dptr(out),
+                                       _lib.dptr(out))
+    assert st == _lib.SHK_ERR_UNSUPPORTED and "fft5" in _lib.last_error()
+    st = lib.shk_compute_all_stieltjes(_lib.dptr(ev), 3, 0.1, 99, 0.0, 0, 0, _lib.dptr(out),
+                                       _lib.dptr(out))
+    assert st == _lib.SHK_ERR_BAD_METHOD
+    # p = 0 returns OK and writes nothing (src/stieltjes/mod.rs:611-613)
+    assert lib.shk_compute_all_stieltjes(None, 0, 0.1, 2, 0.0, 0, 0, None, None) == _lib.SHK_OK
+    h = ctypes.c_void_p()
+    assert lib.shk_chebcode_build(None, 0, 0.5, 11, 32, 0, ctypes.byref(h)) == _lib.SHK_ERR_EMPTY
+
+
+def test_auto_resolution(shk):
+    from shrinkers_amd import _lib
+    lib = _lib.load()
+    assert lib.shk_resolve_auto_method(19, 0, 50000) == 14   # auto -> chebcode_fast
+    assert lib.shk_resolve_auto_method(21, 1, 10) == 14      # speed_auto
+    assert lib.shk_resolve_auto_method(22, 0, 10) == 2       # accuracy_auto -> blocked
+    assert lib.shk_resolve_auto_method(4, 0, 10) == 4
+    assert abs(lib.shk_default_eta(400) - 0.005) < 1e-15
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU may be present")
+def test_no_device_is_an_error_not_a_fallback(shk):
+    from shrinkers_amd import _lib
+    lib = _lib.load()
+    ev = np.array([1.0, 2.0, 3.0])
+    out = np.empty(3)
+    st = lib.shk_compute_all_stieltjes(_lib.dptr(ev), 3, 0.1, 2, 0.0, 0, 0, _lib.dptr(out),
+                                       _lib.dptr(out))
+    assert st == _lib.SHK_ERR_NO_DEVICE

@@ -1,0 +1,220 @@
+"""GPU parity of the exact O(p^2) family against the CPU oracle.
+
+Bars: FP64 kernel within 1e-13 (normwise and per element against ||S||_inf)
+of the oracle's restatement of the reference tiled kernel; FP32-compensated
+kernel within 1e-6 normwise of the FP64 oracle; golden master at 1e-14.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "reference_vectors.json")
+
+
+def spectra(p):
+    from shrinkers_amd import spectra as S
+    out = {"log": S.log_spectrum(p), "harness": S.harness_spectrum(p)}
+    if p >= 8:
+        out["mp"] = S.mp_inverse_cdf_spectrum(p)
+    return out
+
+
+def assert_close(got_re, got_im, ref_re, ref_im, tol, what=""):
+    err = rel_l2(got_re, got_im, ref_re, ref_im)
+    scale = max(np.max(np.abs(ref_re)), np.max(np.abs(ref_im)))
+    emax = max(np.max(np.abs(got_re - ref_re)), np.max(np.abs(got_im - ref_im))) / scale
+    print(f"{what}: rel-L2 {err:.3e}  max/||S||inf {emax:.3e}", flush=True)
+    assert err <= tol, f"{what}: rel-L2 {err:.3e} > {tol:.1e}"
+    assert emax <= tol, f"{what}: max err {emax:.3e} > {tol:.1e}"
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 7, 50, 64, 65, 300, 1000, 1024, 2001, 2002, 2003, 4001])
+def test_f64_blocked_matches_oracle(shk, orc, p):
+    for name, ev in spectra(p).items():
+        eta = 1.0 / np.sqrt(p)
+        r, i = shk.compute_all_stieltjes(ev, eta, "blocked")
+        rr, ri = orc.tiled_parallel(ev, eta)
+        assert_close(r, i, rr / p, ri / p, 1e-13, f"blocked f64 {name} p={p}")
+
+
+def test_p1_is_pure_diagonal(shk):
+    # cacheblock.rs:1604-1607: only the diagonal 1/(0 - i eta) = i/eta survives.
+    r, i = shk.compute_all_stieltjes(np.array([2.0]), 0.5, "blocked")
+    assert r[0] == 0.0
+    assert abs(i[0] - 2.0) <= 4e-16
+
+
+def test_all_exact_methods_agree(shk, orc):
+    # mod.rs:859-913: every exact method vs naive/p, 1e-13 absolute.
+    evals = np.log1p((np.arange(50) + 0.5) * 0.2)
+    eta = 0.05
+    ref = [orc.naive_sum(x, evals, eta) for x in evals]
+    ref_r = np.array([a for a, _ in ref]) / 50
+    ref_i = np.array([b for _, b in ref]) / 50
+    for m in ("naive", "autovec", "blocked", "blocked_autovec", "blocked_tiled",
+              "blocked_windowed", "blocked_hybrid", "accuracy_auto"):
+        r, i = shk.compute_all_stieltjes(evals, eta, m)
+        assert np.max(np.abs(r - ref_r)) < 1e-13, m
+        assert np.max(np.abs(i - ref_i)) < 1e-13, m
+
+
+def test_unsorted_input(shk, orc):
+    # cacheblock.rs:1580-1601
+    evals = np.array([3.0, -1.5, 2.25, 0.7, 4.2, -0.3, 1.1])
+    eta = 0.37
+    r, i = shk.compute_all_stieltjes(evals, eta, "blocked")
+    for k, x in enumerate(evals):
+        a, b = orc.autovec_sum(x, evals, eta)
+        assert abs(r[k] * 7 - a) < 1e-12 and abs(i[k] * 7 - b) < 1e-12
+
+
+def test_stieltjes_transform_binding(shk, orc):
+    rng = np.random.default_rng(0)
+    ev = np.sort(0.5 + rng.uniform(0, 1.5, 200))
+    res = shk.stieltjes_transform(ev)
+    assert set(res) == {"real", "imag"}
+    assert np.all(res["imag"] > 0)
+    eta = 0.1 / np.sqrt(200)
+    rr, ri = orc.tiled_parallel(ev, eta)
+    assert_close(res["real"], res["imag"], rr / 200, ri / 200, 1e-13, "binding")
+    a = shk.stieltjes_transform(ev[:80].copy(), method="naive")
+    b = shk.stieltjes_transform(ev[:80].copy(), method="blocked_tiled")
+    np.testing.assert_allclose(a["real"], b["real"], atol=1e-11)
+    f32 = shk.stieltjes_transform(ev[:100].copy(), precision="f32")
+    f64 = shk.stieltjes_transform(ev[:100].copy(), precision="f64")
+    assert f32["real"].dtype == np.float32
+    np.testing.assert_allclose(f32["real"], f64["real"], rtol=5e-2)
+    serial = shk.stieltjes_transform(ev, parallel=False)
+    multi = shk.stieltjes_transform(ev, parallel=True)
+    np.testing.assert_allclose(serial["real"], multi["real"], atol=1e-12)
+
+
+@pytest.mark.parametrize("eta_scale", [1.0, 0.1])
+@pytest.mark.parametrize("p", [1000, 4001, 20000])
+def test_f32c_within_1e6(shk, orc, p, eta_scale):
+    for name, ev in spectra(p).items():
+        eta = eta_scale / np.sqrt(p)
+        r, i = shk.compute_all_stieltjes(ev, eta, "blocked", precision="f32c")
+        rr, ri = orc.tiled_parallel(ev, eta)
+        err = rel_l2(r, i, rr / p, ri / p)
+        print(f"f32c {name} p={p} eta={eta_scale}/sqrt(p): rel-L2 {err:.3e}", flush=True)
+        assert err <= 1e-6
+
+
+def test_f32c_unsorted(shk, orc):
+    p = 3000
+    from shrinkers_amd import spectra as S
+    ev = orc.shuffled(S.mp_inverse_cdf_spectrum(p), 11)
+    eta = 1.0 / np.sqrt(p)
+    r, i = shk.compute_all_stieltjes(ev, eta, "blocked", precision="f32c")
+    rr, ri = orc.tiled_parallel(ev, eta)
+    assert rel_l2(r, i, rr / p, ri / p) <= 1e-6
+
+
+def test_f32_entry_point(shk, orc):
+    # mod.rs:805-851: float in, float out, scaled.  The kernel is the
+    # compensated one, so it is checked against f64 on the rounded input.
+    p = 5000
+    ev32 = np.sort(np.random.default_rng(3).uniform(0.3, 2.5, p)).astype(np.float32)
+    eta = np.float32(1.0 / np.sqrt(p))
+    r, i = shk.compute_all_stieltjes_f32(ev32, eta, "blocked")
+    assert r.dtype == np.float32
+    rr, ri = orc.tiled_parallel(ev32.astype(np.float64), float(eta))
+    assert rel_l2(r, i, rr / p, ri / p) <= 1e-6
+
+
+def test_cutoff_and_windowed(shk, orc):
+    # cacheblock.rs:1779-1884: windowed == cutoff, hybrid = exact re + windowed im.
+    p = 2000
+    ev = np.sort(orc.log_spectrum(p))
+    eta = 0.1 / np.sqrt(p)
+    cut = 10.0
+    ref = np.array([orc.sum_cutoff(x, ev, eta, cut) for x in ev])
+    for m in ("blocked", "blocked_tiled", "blocked_windowed"):
+        r, i = shk.compute_all_stieltjes(ev, eta, m, cutoff=cut)
+        assert_close(r, i, ref[:, 0] / p, ref[:, 1] / p, 1e-12, f"{m} cutoff")
+    r, i = shk.compute_all_stieltjes(ev, eta, "blocked_hybrid", cutoff=cut)
+    er, ei = orc.tiled_parallel(ev, eta)
+    assert np.max(np.abs(r - er / p)) <= 1e-12 * np.max(np.abs(er / p))
+    assert np.max(np.abs(i - ref[:, 1] / p)) <= 1e-12 * np.max(np.abs(ref[:, 1] / p))
+
+
+def test_at_points_matches_single_point(shk, orc):
+    # cacheblock.rs:1651-1688
+    p = 300
+    ev = np.sort(orc.log_spectrum(p))
+    eta = 0.1 / np.sqrt(p)
+    q = np.linspace(ev[0] - 0.5, ev[-1] + 0.5, 200)
+    for cut in (None, 10.0):
+        r, i = shk.compute_stieltjes_at_points(q, ev, eta, "blocked", cutoff=cut)
+        for k, x in enumerate(q):
+            a, b = orc.sum_cutoff(x, ev, eta, cut)
+            assert abs(r[k] - a) < 1e-12 and abs(i[k] - b) < 1e-12
+
+
+def test_golden_master_through_gpu(shk):
+    from shrinkers_amd.deconvolution import spectral_deconvolution
+    g = json.load(open(GOLDEN))["golden_master"]
+    res = spectral_deconvolution(np.array(g["evals"]), g["c"], g["n_points"], eta=g["eta"])
+    for key in ("density", "w_re", "sample_stieltjes_imag", "population_stieltjes_imag"):
+        got = getattr(res, key)
+        exp = np.array(g[key])
+        # approx::assert_relative_eq!(epsilon = 1e-14): absolute or relative
+        ok = np.abs(got - exp) <= np.maximum(1e-14, 1e-14 * np.abs(exp))
+        assert ok.all(), (key, np.max(np.abs(got - exp)))
+
+
+def test_with_deriv(shk, orc):
+    p = 300
+    ev = orc.log_spectrum(p)
+    eta = 0.05 / np.sqrt(p)
+    got = shk.compute_all_stieltjes_with_deriv(ev, eta)
+    ref = orc.with_deriv(ev, eta)
+    for g, r in zip(got, ref):
+        assert np.max(np.abs(g - r)) <= 1e-12 * np.max(np.abs(r))
+    res = shk.stieltjes_transform_with_deriv(ev, eta=eta)
+    assert set(res) == {"real", "imag", "deriv_real", "deriv_imag"}
+
+
+def test_scaled_ray(shk, orc):
+    ev = 0.4 + (np.arange(61) + 1.0) * 0.05  # inverse.rs:301-303
+    h = 0.13
+    r, i = shk.compute_stieltjes_scaled_ray(ev, h)
+    rr, ri = orc.scaled_ray(ev, h)
+    assert np.max(np.abs(r - rr)) <= 1e-12 * np.max(np.abs(rr))
+    assert np.max(np.abs(i - ri)) <= 1e-12 * np.max(np.abs(ri))
+
+
+def test_large_p_properties(shk, orc):
+    """p = 200 000: rows spot-checked against the oracle's single-point sum,
+    antisymmetry sum_i Re S_i = 0, Im S_i >= 1/eta, FP32C vs FP64."""
+    from shrinkers_amd import spectra as S
+    p = 200_000
+    ev = S.mp_inverse_cdf_spectrum(p, c=0.1)
+    eta = 1.0 / np.sqrt(p)
+    r, i = shk.compute_all_stieltjes(ev, eta, "blocked")
+    rows = np.linspace(0, p - 1, 64).astype(int)
+    for k in rows:
+        a, b = orc.autovec_sum(ev[k], ev, eta)
+        assert abs(r[k] * p - a) <= 1e-12 * abs(b) and abs(i[k] * p - b) <= 1e-12 * abs(b)
+    assert abs(np.sum(r)) <= 1e-12 * np.sum(np.abs(r)) * np.sqrt(p)
+    assert np.all(i * p >= (1.0 / eta) * (1 - 1e-12))
+    r2, i2 = shk.compute_all_stieltjes(ev, eta, "blocked", precision="f32c")
+    err = rel_l2(r2, i2, r, i)
+    print(f"f32c vs f64 at p={p}: {err:.3e}", flush=True)
+    assert err <= 1e-6
+
+
+def test_errors_are_raised(shk):
+    with pytest.raises(NotImplementedError):
+        shk.compute_all_stieltjes(np.array([1.0, 2.0]), 0.1, "fft5")
+    with pytest.raises(shk.ShkError):
+        shk.compute_all_stieltjes(np.array([1.0, np.nan]), 0.1, "blocked")
+    with pytest.raises(shk.ShkError):
+        shk.compute_all_stieltjes(np.array([1.0, 2.0]), -0.1, "blocked")
