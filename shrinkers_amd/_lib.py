@@ -82,6 +82,15 @@ def load() -> ctypes.CDLL:
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `python -m shrinkers_amd.build` "
             "(hipcc, gfx950). shrinkers_amd has no CPU implementation.")
+    # PyTorch-ROCm ships its own HIP runtime, which its libraries load under the
+    # name libamdhip64.so while this library asks for libamdhip64.so.7.  If this
+    # library were loaded first, a later torch import would bring in a second
+    # HIP/HSA runtime and neither would share device memory with the other.
+    # Importing torch first makes its runtime the one this library binds to.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)
