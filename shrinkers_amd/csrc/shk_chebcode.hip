@@ -43,6 +43,7 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <cmath>
+#include <memory>
 #include <vector>
 
 namespace shk {
@@ -64,13 +65,16 @@ struct ChebTree {
     float4 *rec32 = nullptr;                // {lo, hi, acc, unused} per node
     int4 *link = nullptr;                   // {right, esc, lo_idx, hi_idx} per node
     float *panel32 = nullptr;               // N*2n: nodes then weights
+    void *arena = nullptr;   // sources (sorted, perm, hi/lo)
+    void *arena2 = nullptr;  // node records and panels
+    hipEvent_t ready = nullptr;  // recorded when the build's last kernel is enqueued
 };
 
 namespace {
 
 constexpr int kMaxDepth = 48;    // chebcode.rs:95
 constexpr int kTopoThreads = 1024;
-constexpr int kMaxN = 128;       // largest supported interpolation order
+constexpr int kMaxN = 120;       // largest interpolation order (LDS of the build)
 constexpr float kOutsideTheta2 = 1.69f;  // chebcode.rs:108
 constexpr double kPi = 3.14159265358979323846;  // std::f64::consts::PI
 
@@ -236,65 +240,128 @@ __global__ void finalize_kernel(const double *__restrict__ sorted, int N, int n,
     }
 }
 
-// One barycentric row (chebcode.rs:309-337) added into w[0..n) with `mass`.
-// The quotients lam_j/(x - t_j) are recomputed in the second pass instead
-// of being kept in a scratch array; the values are identical.
-__device__ void barycentric_row(double x, const double *__restrict__ t,
-                                const double *__restrict__ lam, double *__restrict__ w,
-                                double mass, int n) {
-    double s = 0.0;
-    for (int j = 0; j < n; ++j) {
-        const double d = x - t[j];
-        if (d == 0.0) {
-            w[j] += mass;
-            return;
-        }
-        s += lam[j] / d;
-    }
-    const double m = mass * (1.0 / s);
-    for (int j = 0; j < n; ++j) {
-        const double d = x - t[j];
-        w[j] = __builtin_fma(m, lam[j] / d, w[j]);
-    }
-}
+// Barycentric accumulation of `nrows` rows (x_r, mass_r) into one panel's
+// weights w[0..n) (chebcode.rs:309-337), one wavefront per panel.
+//
+// Lanes first evaluate 64 rows at a time: the quotients lam_j/(x_r - t_j)
+// go to LDS together with m_r = mass_r * (1/s_r), or the node index hit by
+// an exact coincidence.  Then lane j adds the rows into w_j in row order,
+// w_j = fma(m_r, q_rj, w_j), which is the reference's per-row update in the
+// reference's order, so the weights equal a sequential build bit for bit.
+// Rows with zero mass are skipped (merge_all_weights, chebcode.rs:496-499).
+// `row(r, x, mass)` supplies row r; w must hold zeros on entry.
+constexpr int kBaryLanes = 64;
+constexpr int kWaccPerLane = (kMaxN + kBaryLanes - 1) / kBaryLanes;
 
-__global__ void leaf_fill_kernel(const double *__restrict__ sorted, int N, int n,
-                                 const double *__restrict__ lam, TreeArrays t) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= N || t.right[k] >= 0) return;
-    double *w = t.w + (size_t)k * n;
-    const double *tn = t.nodes + (size_t)k * n;
-    for (int s = t.lo_idx[k]; s < t.hi_idx[k]; ++s) barycentric_row(sorted[s], tn, lam, w, 1.0, n);
-}
-
-// Parents from children, deepest level first (one workgroup; levels are
-// BFS ranges level_off[d]..level_off[d+1]).
-__global__ __launch_bounds__(kTopoThreads) void merge_kernel(int n, int n_levels,
-                                                             const int *__restrict__ info,
-                                                             const int *__restrict__ b_left,
-                                                             const int *__restrict__ b_pre,
-                                                             const double *__restrict__ lam,
-                                                             TreeArrays t) {
-    const int *level = info + 4;
-    for (int d = n_levels - 2; d >= 0; --d) {
-        for (int v = level[d] + threadIdx.x; v < level[d + 1]; v += kTopoThreads) {
-            if (b_left[v] < 0) continue;
-            const int k = b_pre[v];
-            double *pw = t.w + (size_t)k * n;
-            const double *tn = t.nodes + (size_t)k * n;
-            for (int j = 0; j < n; ++j) pw[j] = 0.0;
-            const int kids[2] = {k + 1, t.right[k]};
-            for (int c = 0; c < 2; ++c) {
-                const size_t cb = (size_t)kids[c] * n;
-                for (int ti = 0; ti < n; ++ti) {
-                    const double m = t.w[cb + ti];
-                    if (m == 0.0) continue;
-                    barycentric_row(t.nodes[cb + ti], tn, lam, pw, m, n);
+template <class RowFn>
+__device__ void bary_accumulate(RowFn row, int nrows, const double *__restrict__ t,
+                                const double *__restrict__ lam, double *__restrict__ w, int n,
+                                double *__restrict__ sq, double *__restrict__ sm,
+                                int *__restrict__ shit) {
+    const int lane = threadIdx.x;
+    double wacc[kWaccPerLane];
+#pragma unroll
+    for (int u = 0; u < kWaccPerLane; ++u) wacc[u] = 0.0;
+    for (int base = 0; base < nrows; base += kBaryLanes) {
+        const int r = base + lane;
+        int hit = -2;  // -2: no row or zero mass; -1: regular row; >= 0: exact hit
+        double m = 0.0;
+        if (r < nrows) {
+            double x, mass;
+            row(r, x, mass);
+            if (mass != 0.0) {
+                double s = 0.0;
+                hit = -1;
+                for (int j = 0; j < n; ++j) {
+                    const double d = x - t[j];
+                    if (d == 0.0) {
+                        hit = j;
+                        break;
+                    }
+                    const double q = lam[j] / d;
+                    sq[lane * n + j] = q;
+                    s += q;
                 }
+                m = (hit >= 0) ? mass : mass * (1.0 / s);
+            }
+        }
+        sm[lane] = m;
+        shit[lane] = hit;
+        __syncthreads();
+        const int rows = min(kBaryLanes, nrows - base);
+#pragma unroll
+        for (int u = 0; u < kWaccPerLane; ++u) {
+            const int j = lane + u * kBaryLanes;
+            if (j < n) {
+                double acc = wacc[u];
+                for (int rr = 0; rr < rows; ++rr) {
+                    const int h = shit[rr];
+                    if (h == -1) acc = __builtin_fma(sm[rr], sq[rr * n + j], acc);
+                    else if (h == j) acc += sm[rr];
+                }
+                wacc[u] = acc;
             }
         }
         __syncthreads();
     }
+#pragma unroll
+    for (int u = 0; u < kWaccPerLane; ++u) {
+        const int j = lane + u * kBaryLanes;
+        if (j < n) w[j] = wacc[u];
+    }
+}
+
+// LDS layout of the two build kernels below: sq[64*n] doubles, sm[64]
+// doubles, shit[64] ints.
+inline size_t bary_lds_bytes(int n) {
+    return sizeof(double) * ((size_t)kBaryLanes * n + kBaryLanes) + sizeof(int) * kBaryLanes;
+}
+
+// One wavefront per node; internal nodes exit at once.
+__global__ __launch_bounds__(kBaryLanes) void leaf_fill_kernel(const double *__restrict__ sorted,
+                                                               int n,
+                                                               const double *__restrict__ lam,
+                                                               TreeArrays t) {
+    extern __shared__ double lds[];
+    const int k = blockIdx.x;
+    if (t.right[k] >= 0) return;
+    double *sq = lds;
+    double *sm = lds + kBaryLanes * n;
+    int *shit = reinterpret_cast<int *>(sm + kBaryLanes);
+    const double *xs = sorted + t.lo_idx[k];
+    auto row = [xs](int r, double &x, double &mass) {
+        x = xs[r];
+        mass = 1.0;
+    };
+    bary_accumulate(row, t.hi_idx[k] - t.lo_idx[k], t.nodes + (size_t)k * n, lam,
+                    t.w + (size_t)k * n, n, sq, sm, shit);
+}
+
+// Parents of one tree level (BFS ids [v0, v0 + gridDim.x)), one wavefront
+// each.  The 2n rows are the children's Chebyshev nodes with their weights
+// as masses, left child first (merge_all_weights, chebcode.rs:474-506).
+__global__ __launch_bounds__(kBaryLanes) void merge_level_kernel(int n, int v0,
+                                                                 const int *__restrict__ b_left,
+                                                                 const int *__restrict__ b_pre,
+                                                                 const double *__restrict__ lam,
+                                                                 TreeArrays t) {
+    extern __shared__ double lds[];
+    const int v = v0 + blockIdx.x;
+    if (b_left[v] < 0) return;
+    const int k = b_pre[v];
+    double *sq = lds;
+    double *sm = lds + kBaryLanes * n;
+    int *shit = reinterpret_cast<int *>(sm + kBaryLanes);
+    const size_t lb = (size_t)(k + 1) * n, rb = (size_t)t.right[k] * n;
+    const double *nodes = t.nodes;
+    const double *w = t.w;
+    auto row = [=](int r, double &x, double &mass) {
+        const size_t o = (r < n) ? lb + r : rb + (r - n);
+        x = nodes[o];
+        mass = w[o];
+    };
+    bary_accumulate(row, 2 * n, t.nodes + (size_t)k * n, lam, t.w + (size_t)k * n, n, sq, sm,
+                    shit);
 }
 
 __global__ void mirror_kernel(int N, int n, TreeArrays t, float4 *__restrict__ rec32,
@@ -564,13 +631,38 @@ inline unsigned blocks_for(size_t n, int bs) { return (unsigned)((n + bs - 1) / 
 
 void cheb_destroy(ChebTree *t) {
     if (!t) return;
-    void *ptrs[] = {t->sorted, t->perm, t->s_hi, t->s_lo, t->lo,    t->hi,   t->acc,
-                    t->lo_idx, t->hi_idx, t->right, t->esc, t->nodes, t->w, t->rec32,
-                    t->link,   t->panel32};
-    for (void *p : ptrs)
-        if (p) (void)hipFree(p);
+    if (t->arena) (void)hipFree(t->arena);
+    if (t->arena2) (void)hipFree(t->arena2);
+    if (t->ready) (void)hipEventDestroy(t->ready);
     delete t;
 }
+
+namespace {
+
+// Carves 256-byte aligned sub-buffers out of one allocation.
+struct Carver {
+    size_t off = 0;
+    char *base = nullptr;
+    template <class T>
+    T *take(size_t count) {
+        off = (off + 255) / 256 * 256;
+        T *p = reinterpret_cast<T *>(base ? base + off : nullptr);
+        off += sizeof(T) * count;
+        return p;
+    }
+};
+
+// Stream-ordered scratch freed when it goes out of scope.
+struct Scratch {
+    void *p = nullptr;
+    hipStream_t s;
+    explicit Scratch(hipStream_t st) : s(st) {}
+    ~Scratch() {
+        if (p) (void)hipFreeAsync(p, s);
+    }
+};
+
+}  // namespace
 
 int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **out,
                hipStream_t stream) {
@@ -580,14 +672,14 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
         return 6;
     }
     if (prm.n < 2 || prm.n > kMaxN) {
-        set_error("chebcode build: n must be in [2, 128]");
+        set_error("chebcode build: n must be in [2, 120]");
         return 6;
     }
     if (!(prm.theta > 0.0) || prm.leaf_cap < 1) {
         set_error("chebcode build: theta must be > 0 and leaf_cap >= 1");
         return 6;
     }
-    ChebTree *t = new ChebTree();
+    std::unique_ptr<ChebTree, void (*)(ChebTree *)> t(new ChebTree(), &cheb_destroy);
     t->p = p;
     t->n = prm.n;
     t->theta = prm.theta;
@@ -595,135 +687,163 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
     t->leaf_cap = prm.leaf_cap;
     t->f32_far = prm.f32_far;
     const int ip = (int)p;
+    const int n = prm.n;
     const int leaf_cap = (int)std::min<int64_t>(prm.leaf_cap, INT32_MAX);
-    int st = 0;
-    int *d_info = nullptr;
-    int *b_buf = nullptr;
-    double *d_tab = nullptr;
-    void *d_tmp = nullptr;
-    int32_t *d_iota = nullptr;
-    std::vector<int> info(4 + kMaxDepth + 3, 0);
-    auto fail = [&](int code) {
-        if (d_info) (void)hipFree(d_info);
-        if (b_buf) (void)hipFree(b_buf);
-        if (d_tab) (void)hipFree(d_tab);
-        if (d_tmp) (void)hipFree(d_tmp);
-        if (d_iota) (void)hipFree(d_iota);
-        cheb_destroy(t);
-        return code;
-    };
-#define SHK_BHIP(expr)                                                          \
-    do {                                                                        \
-        hipError_t e_ = (expr);                                                 \
-        if (e_ != hipSuccess) {                                                 \
-            set_error(std::string(#expr) + ": " + hipGetErrorString(e_));       \
-            return fail(7);                                                     \
-        }                                                                       \
-    } while (0)
-
-    // 1. sorted copy (+ permutation when the input is not ascending)
-    SHK_BHIP(hipMalloc(&d_info, sizeof(int) * info.size()));
-    SHK_BHIP(hipMemsetAsync(d_info, 0, sizeof(int) * info.size(), stream));
-    SHK_BHIP(hipMalloc(&t->sorted, sizeof(double) * p));
-    if (p > 1) {
-        hipLaunchKernelGGL(check_sorted_kernel, dim3(blocks_for(p, 256)), dim3(256), 0, stream,
-                           d_ev, ip, d_info + 3);
-        SHK_BHIP(hipGetLastError());
-    }
-    SHK_BHIP(hipMemcpyAsync(info.data(), d_info, sizeof(int) * 4, hipMemcpyDeviceToHost, stream));
-    SHK_BHIP(hipStreamSynchronize(stream));
-    if (info[3] == 0) {
-        SHK_BHIP(hipMemcpyAsync(t->sorted, d_ev, sizeof(double) * p, hipMemcpyDeviceToDevice,
-                                stream));
-    } else {
-        SHK_BHIP(hipMalloc(&t->perm, sizeof(int32_t) * p));
-        SHK_BHIP(hipMalloc(&d_iota, sizeof(int32_t) * p));
-        hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(p, 256)), dim3(256), 0, stream, d_iota, ip);
-        SHK_BHIP(hipGetLastError());
-        size_t tmp_bytes = 0;
-        SHK_BHIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, d_ev, t->sorted, d_iota, t->perm, p,
-                                           0, 64, stream));
-        SHK_BHIP(hipMalloc(&d_tmp, tmp_bytes));
-        SHK_BHIP(rocprim::radix_sort_pairs(d_tmp, tmp_bytes, d_ev, t->sorted, d_iota, t->perm, p,
-                                           0, 64, stream));
-    }
-
-    // 2. topology
     const size_t cap_a = 2 * p - 1;
     const size_t cap_b = 2 * (size_t)(kMaxDepth + 1) * ((p + leaf_cap) / (leaf_cap + 1)) + 1;
     const int cap = (int)std::min(cap_a, cap_b);
-    SHK_BHIP(hipMalloc(&b_buf, sizeof(int) * 5 * (size_t)cap));
+
+    // Scratch: info words, BFS arrays, node/weight tables, sort temporaries.
+    const int n_info = 4 + kMaxDepth + 3;
+    Scratch scratch(stream);
+    Carver sc;
+    (void)sc.take<int>(n_info);
+    (void)sc.take<int>(5 * (size_t)cap);
+    (void)sc.take<double>(2 * (size_t)n);
+    (void)sc.take<int32_t>(p);
+    size_t sort_bytes = 0;
+    SHK_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, d_ev, (double *)nullptr,
+                                      (const int32_t *)nullptr, (int32_t *)nullptr, p, 0, 64,
+                                      stream));
+    (void)sc.take<char>(sort_bytes);
+    SHK_HIP(hipMallocAsync(&scratch.p, sc.off, stream));
+    sc.base = static_cast<char *>(scratch.p);
+    sc.off = 0;
+    int *d_info = sc.take<int>(n_info);
+    int *b_buf = sc.take<int>(5 * (size_t)cap);
+    double *d_tab = sc.take<double>(2 * (size_t)n);
+    int32_t *d_iota = sc.take<int32_t>(p);
+    void *d_sort_tmp = sc.take<char>(sort_bytes);
     int *b_lo = b_buf, *b_hi = b_buf + cap, *b_left = b_buf + 2 * (size_t)cap,
         *b_size = b_buf + 3 * (size_t)cap, *b_pre = b_buf + 4 * (size_t)cap;
+
+    // Arena part 1 (sizes known before the topology): sources.
+    Carver ar;
+    (void)ar.take<double>(p);
+    (void)ar.take<int32_t>(p);
+    (void)ar.take<float>(p);
+    (void)ar.take<float>(p);
+    const size_t src_bytes = ar.off;
+
+    // 1. sortedness, then the sorted copy (+ permutation)
+    std::vector<int> info(n_info, 0);
+    // The topology needs the sorted values; its node count sizes the rest of
+    // the arena, so the arena is allocated after a first pass with the
+    // sources in scratch-independent storage: allocate the source part now.
+    void *src_arena = nullptr;
+    SHK_HIP(hipMalloc(&src_arena, src_bytes));
+    ar.base = static_cast<char *>(src_arena);
+    ar.off = 0;
+    t->sorted = ar.take<double>(p);
+    int32_t *perm_buf = ar.take<int32_t>(p);
+    t->s_hi = ar.take<float>(p);
+    t->s_lo = ar.take<float>(p);
+    t->arena = src_arena;  // owned from here on (freed by cheb_destroy)
+
+    int zero4[4] = {0, 0, 0, 0};
+    SHK_HIP(hipMemcpyAsync(d_info, zero4, sizeof(zero4), hipMemcpyHostToDevice, stream));
+    if (p > 1) {
+        hipLaunchKernelGGL(check_sorted_kernel, dim3(blocks_for(p, 256)), dim3(256), 0, stream,
+                           d_ev, ip, d_info + 3);
+        SHK_HIP(hipGetLastError());
+    }
+    SHK_HIP(hipMemcpyAsync(info.data(), d_info, sizeof(int) * 4, hipMemcpyDeviceToHost, stream));
+    SHK_HIP(hipStreamSynchronize(stream));
+    if (info[3] == 0) {
+        SHK_HIP(hipMemcpyAsync(t->sorted, d_ev, sizeof(double) * p, hipMemcpyDeviceToDevice,
+                               stream));
+    } else {
+        t->perm = perm_buf;
+        hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(p, 256)), dim3(256), 0, stream, d_iota, ip);
+        SHK_HIP(hipGetLastError());
+        SHK_HIP(rocprim::radix_sort_pairs(d_sort_tmp, sort_bytes, d_ev, t->sorted, d_iota, t->perm,
+                                          p, 0, 64, stream));
+    }
+
+    // 2. topology
     hipLaunchKernelGGL(topo_kernel, dim3(1), dim3(kTopoThreads), 0, stream, t->sorted, ip, leaf_cap,
                        cap, b_lo, b_hi, b_left, b_size, b_pre, d_info);
-    SHK_BHIP(hipGetLastError());
-    SHK_BHIP(hipMemcpyAsync(info.data(), d_info, sizeof(int) * info.size(), hipMemcpyDeviceToHost,
-                            stream));
-    SHK_BHIP(hipStreamSynchronize(stream));
+    SHK_HIP(hipGetLastError());
+    SHK_HIP(hipMemcpyAsync(info.data(), d_info, sizeof(int) * n_info, hipMemcpyDeviceToHost,
+                           stream));
+    SHK_HIP(hipStreamSynchronize(stream));
     if (info[2]) {
         set_error("chebcode build: node capacity exceeded");
-        return fail(6);
+        return 6;
     }
     const int N = info[0];
     t->N = (size_t)N;
     t->n_levels = info[1];
-    const int n = prm.n;
 
-    // 3. pre-order arrays and positions
-    SHK_BHIP(hipMalloc(&t->lo, sizeof(double) * N));
-    SHK_BHIP(hipMalloc(&t->hi, sizeof(double) * N));
-    SHK_BHIP(hipMalloc(&t->acc, sizeof(double) * N));
-    SHK_BHIP(hipMalloc(&t->lo_idx, sizeof(int32_t) * N));
-    SHK_BHIP(hipMalloc(&t->hi_idx, sizeof(int32_t) * N));
-    SHK_BHIP(hipMalloc(&t->right, sizeof(int32_t) * N));
-    SHK_BHIP(hipMalloc(&t->esc, sizeof(int32_t) * N));
-    SHK_BHIP(hipMalloc(&t->nodes, sizeof(double) * (size_t)N * n));
-    SHK_BHIP(hipMalloc(&t->w, sizeof(double) * (size_t)N * n));
-    SHK_BHIP(hipMalloc(&t->rec32, sizeof(float4) * N));
-    SHK_BHIP(hipMalloc(&t->link, sizeof(int4) * N));
-    SHK_BHIP(hipMalloc(&t->panel32, sizeof(float) * (size_t)N * 2 * n));
-    SHK_BHIP(hipMalloc(&t->s_hi, sizeof(float) * p));
-    SHK_BHIP(hipMalloc(&t->s_lo, sizeof(float) * p));
-    // Chebyshev nodes of the second kind s_j = cos(j*pi/(n-1)) and barycentric
-    // weights (-1)^j, halved at both ends (chebcode.rs:242-261).
+    // Arena part 2: node records and panels.
+    Carver nr;
+    (void)nr.take<double>(N);
+    (void)nr.take<double>(N);
+    (void)nr.take<double>(N);
+    for (int k = 0; k < 4; ++k) (void)nr.take<int32_t>(N);
+    (void)nr.take<double>((size_t)N * n);
+    (void)nr.take<double>((size_t)N * n);
+    (void)nr.take<float4>(N);
+    (void)nr.take<int4>(N);
+    (void)nr.take<float>((size_t)N * 2 * n);
+    void *node_arena = nullptr;
+    SHK_HIP(hipMalloc(&node_arena, nr.off));
+    t->arena2 = node_arena;
+    nr.base = static_cast<char *>(node_arena);
+    nr.off = 0;
+    t->lo = nr.take<double>(N);
+    t->hi = nr.take<double>(N);
+    t->acc = nr.take<double>(N);
+    t->lo_idx = nr.take<int32_t>(N);
+    t->hi_idx = nr.take<int32_t>(N);
+    t->right = nr.take<int32_t>(N);
+    t->esc = nr.take<int32_t>(N);
+    t->nodes = nr.take<double>((size_t)N * n);
+    t->w = nr.take<double>((size_t)N * n);
+    t->rec32 = nr.take<float4>(N);
+    t->link = nr.take<int4>(N);
+    t->panel32 = nr.take<float>((size_t)N * 2 * n);
+
+    // 3. pre-order arrays and positions.  Chebyshev nodes of the second kind
+    // s_j = cos(j*pi/(n-1)) and barycentric weights (-1)^j halved at both
+    // ends (chebcode.rs:242-261).
     std::vector<double> tab(2 * n);
     for (int j = 0; j < n; ++j) {
         tab[j] = std::cos((double)j * kPi / (double)(n - 1));
         const double sign = (j % 2 == 0) ? 1.0 : -1.0;
         tab[n + j] = (j == 0 || j == n - 1) ? 0.5 * sign : sign;
     }
-    SHK_BHIP(hipMalloc(&d_tab, sizeof(double) * 2 * n));
-    SHK_BHIP(hipMemcpyAsync(d_tab, tab.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice,
-                            stream));
+    SHK_HIP(hipMemcpyAsync(d_tab, tab.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice,
+                           stream));
     TreeArrays ta{t->lo, t->hi, t->acc, t->nodes, t->w, t->lo_idx, t->hi_idx, t->right, t->esc};
     hipLaunchKernelGGL(finalize_kernel, dim3(blocks_for(N, 256)), dim3(256), 0, stream, t->sorted, N,
                        n, t->theta_sq, b_lo, b_hi, b_left, b_size, b_pre, d_tab, ta);
-    SHK_BHIP(hipGetLastError());
-    // 4. leaves
-    hipLaunchKernelGGL(leaf_fill_kernel, dim3(blocks_for(N, 64)), dim3(64), 0, stream, t->sorted, N,
-                       n, d_tab + n, ta);
-    SHK_BHIP(hipGetLastError());
-    // 5. parents
-    hipLaunchKernelGGL(merge_kernel, dim3(1), dim3(kTopoThreads), 0, stream, n, t->n_levels, d_info,
-                       b_left, b_pre, d_tab + n, ta);
-    SHK_BHIP(hipGetLastError());
+    SHK_HIP(hipGetLastError());
+    // 4. leaves: one wavefront per node
+    const size_t lds = bary_lds_bytes(n);
+    hipLaunchKernelGGL(leaf_fill_kernel, dim3(N), dim3(kBaryLanes), lds, stream, t->sorted, n,
+                       d_tab + n, ta);
+    SHK_HIP(hipGetLastError());
+    // 5. parents, deepest level first: one launch per level, one wavefront per node
+    for (int d = t->n_levels - 2; d >= 0; --d) {
+        const int v0 = info[4 + d], v1 = info[4 + d + 1];
+        if (v1 <= v0) continue;
+        hipLaunchKernelGGL(merge_level_kernel, dim3(v1 - v0), dim3(kBaryLanes), lds, stream, n, v0,
+                           b_left, b_pre, d_tab + n, ta);
+        SHK_HIP(hipGetLastError());
+    }
     // 6. mirrors
     hipLaunchKernelGGL(mirror_kernel, dim3(blocks_for(N, 256)), dim3(256), 0, stream, N, n, ta,
                        t->rec32, t->link, t->panel32);
-    SHK_BHIP(hipGetLastError());
+    SHK_HIP(hipGetLastError());
     hipLaunchKernelGGL(split_kernel, dim3(blocks_for(p, 256)), dim3(256), 0, stream, t->sorted, ip,
                        t->s_hi, t->s_lo);
-    SHK_BHIP(hipGetLastError());
-    SHK_BHIP(hipStreamSynchronize(stream));
-#undef SHK_BHIP
-    (void)hipFree(d_info);
-    (void)hipFree(b_buf);
-    (void)hipFree(d_tab);
-    if (d_tmp) (void)hipFree(d_tmp);
-    if (d_iota) (void)hipFree(d_iota);
-    *out = t;
-    return st;
+    SHK_HIP(hipGetLastError());
+    // Evaluations on any stream wait for this event (cheb_eval*).
+    SHK_HIP(hipEventCreateWithFlags(&t->ready, hipEventDisableTiming));
+    SHK_HIP(hipEventRecord(t->ready, stream));
+    *out = t.release();
+    return 0;
 }
 
 size_t cheb_node_count(const ChebTree *t) { return t ? t->N : 0; }
@@ -740,6 +860,7 @@ int cheb_params(const ChebTree *t, ChebParams *prm) {
 int cheb_export(const ChebTree *t, double *lo, double *hi, int64_t *lo_idx, int64_t *hi_idx,
                 int32_t *right, double *nodes, double *w, hipStream_t stream) {
     const size_t N = t->N, n = t->n;
+    SHK_HIP(hipStreamWaitEvent(stream, t->ready, 0));
     if (lo) SHK_HIP(hipMemcpyAsync(lo, t->lo, sizeof(double) * N, hipMemcpyDeviceToHost, stream));
     if (hi) SHK_HIP(hipMemcpyAsync(hi, t->hi, sizeof(double) * N, hipMemcpyDeviceToHost, stream));
     if (right)
@@ -771,6 +892,7 @@ int cheb_eval(const ChebTree *t, const double *d_query, size_t nq, double eta, d
         set_error("chebcode eval: too many query points");
         return 6;
     }
+    SHK_HIP(hipStreamWaitEvent(stream, t->ready, 0));
     EvalView v = view_of(t);
     ProfileMark m = profile_begin(stream);
     if (t->f32_far)
@@ -786,6 +908,7 @@ int cheb_eval(const ChebTree *t, const double *d_query, size_t nq, double eta, d
 
 int cheb_eval_x2(const ChebTree *t, double eta0, double eta1, double *d_re0, double *d_im0,
                  double *d_re1, double *d_im1, hipStream_t stream) {
+    SHK_HIP(hipStreamWaitEvent(stream, t->ready, 0));
     EvalView v = view_of(t);
     const size_t nq = t->p;
     hipLaunchKernelGGL(eval_x2_kernel, dim3(blocks_for(nq, 256)), dim3(256), 0, stream, v, (int)nq,

@@ -6,7 +6,8 @@
 // reference's "query tile owns its outputs, sweeps the sources" shape:
 //
 //   grid.x = query tiles of kQT = kWG*kR queries (kR per thread, strided so
-//            loads and stores coalesce), grid.y = source splits.
+//            loads and stores coalesce), grid.y = source splits (about
+//            kTargetWG workgroups in flight).
 //   Each workgroup stages its source split through LDS in pieces of kPiece
 //   values; every wavefront then reads the same LDS word (a broadcast, no
 //   bank conflicts) and updates kR independent query accumulators, so one
@@ -38,15 +39,40 @@
 namespace shk {
 namespace {
 
-constexpr int kWG = 256;        // threads per workgroup (4 wavefronts)
-constexpr int kR = 4;           // queries per thread
-constexpr int kQT = kWG * kR;   // queries per tile
-constexpr int kPiece = 2048;    // sources staged in LDS per pass
-constexpr int kRun = 32;        // f32 terms per FP64 fold
-constexpr int kTargetWG = 2048; // workgroups to aim for (8 per CU)
+// Tile shape; the defaults are the measured choice on MI355X (one wavefront
+// per workgroup, 2 queries per lane: 128-query tiles, narrow enough that
+// the one-subtraction pair difference applies to most tiles) and can be
+// overridden at compile time for tuning sweeps.
+#ifndef SHK_EXACT_WG
+#define SHK_EXACT_WG 64
+#endif
+#ifndef SHK_EXACT_R
+#define SHK_EXACT_R 2
+#endif
+#ifndef SHK_EXACT_PIECE
+#define SHK_EXACT_PIECE 512
+#endif
+#ifndef SHK_EXACT_TARGET_WG
+#define SHK_EXACT_TARGET_WG 32768
+#endif
+constexpr int kWG = SHK_EXACT_WG;          // threads per workgroup
+constexpr int kR = SHK_EXACT_R;            // queries per thread
+constexpr int kQT = kWG * kR;              // queries per tile
+constexpr int kPiece = SHK_EXACT_PIECE;    // sources staged in LDS per pass
+constexpr int kRun = 32;                   // f32 terms per FP64 fold
+constexpr int kTargetWG = SHK_EXACT_TARGET_WG;  // workgroups to aim for
 
-// 1/x to ~1 ulp: v_rcp_f64 seed refined by two Newton-Raphson steps
-// r <- r + r*(1 - x*r) (each step squares the relative error).
+// 1/x to ~1 ulp from the v_rcp_f64 seed r0.  With e = 1 - x*r0,
+// 1/x = r0 (1 + e + e^2 + ...): the default takes the cubic correction
+// r0 + r0*(e + e^2) (relative error e^3, three FMAs); the alternative takes
+// two Newton-Raphson steps r <- r + r*(1 - x*r) (error e^4, four FMAs).
+#ifndef SHK_F64_RCP_NEWTON2
+__device__ __forceinline__ double rcp_f64(double x) {
+    const double r = __builtin_amdgcn_rcp(x);
+    const double e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, __builtin_fma(e, e, e), r);
+}
+#else
 __device__ __forceinline__ double rcp_f64(double x) {
     double r = __builtin_amdgcn_rcp(x);
     double e = __builtin_fma(-x, r, 1.0);
@@ -54,6 +80,7 @@ __device__ __forceinline__ double rcp_f64(double x) {
     e = __builtin_fma(-x, r, 1.0);
     return __builtin_fma(r, e, r);
 }
+#endif
 
 template <int KIND>
 struct NAcc {
