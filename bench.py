@@ -109,6 +109,66 @@ def cpu_baseline(ev: np.ndarray, eta: float, budget_s: float):
     }
 
 
+def chebcode_leg(shk, sync) -> dict:
+    """ChebCode at p = 2e5 and 1e6 (FAST, DEFAULT and the f64 n = 12 point),
+    error against the GPU FP64 exact kernel, build+evaluate and evaluate-only times."""
+    import torch
+    from shrinkers_amd import spectra
+    res = {}
+    for p, c in ((200_000, 0.1), (200_000, 0.5), (1_000_000, 0.1)):
+        ev = spectra.mp_inverse_cdf_spectrum(p, c=c)
+        eta = 1.0 / np.sqrt(p)
+        d = torch.from_numpy(ev).cuda()
+        er, ei = shk.all_points_dev(d, eta, "blocked", precision="f64", scaled=False)
+        sync()
+        er, ei = er.cpu().numpy(), ei.cpu().numpy()
+        den = np.sum(er ** 2 + ei ** 2)
+        row = {}
+        for name, (theta, n, leaf, f32) in (("chebcode_fast", (1.0, 8, 32, True)),
+                                            ("chebcode", (0.5, 11, 32, False)),
+                                            ("chebcode_n12", (0.5, 12, 32, False))):
+            ts = []
+            for _ in range(3):
+                sync()
+                t0 = time.perf_counter()
+                b = shk.ChebCodeBatch.build_dev(d, theta, n, leaf, f32)
+                r, i = b.evaluate_dev(eta)
+                sync()
+                ts.append(time.perf_counter() - t0)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(5):
+                b.evaluate_dev(eta, out=(r, i))
+            e1.record()
+            sync()
+            err = float(np.sqrt(np.sum((r.cpu().numpy() - er) ** 2 + (i.cpu().numpy() - ei) ** 2) / den))
+            row[name] = {"ms_build_plus_eval": 1e3 * float(np.median(ts)),
+                         "ms_eval": e0.elapsed_time(e1) / 5, "rel_l2_vs_fp64_exact": err,
+                         "nodes": b.node_count}
+            del b
+        res[f"p={p},c={c}"] = row
+    return res
+
+
+def grid_leg(shk, sync) -> dict:
+    """At-points path of the El Karoui consumer: 200 grid points, p = 50 000."""
+    import torch
+    from shrinkers_amd import spectra
+    ev = spectra.mp_inverse_cdf_spectrum(50_000)
+    d = torch.from_numpy(ev).cuda()
+    q = torch.linspace(float(ev[0]) - 0.4, float(ev[-1]) + 0.4, 200, dtype=torch.float64).cuda()
+    out = {}
+    for m in ("blocked", "chebcode_fast", "chebcode_balanced"):
+        shk.at_points_dev(q, d, 0.1 / np.sqrt(ev.size), m)
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(10):
+            shk.at_points_dev(q, d, 0.1 / np.sqrt(ev.size), m)
+        sync()
+        out[m] = {"ms_end_to_end": (time.perf_counter() - t0) * 100.0}
+    return out
+
+
 def _cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -127,6 +187,8 @@ def main() -> int:
     ap.add_argument("--p", type=int, default=50_000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU timing")
+    ap.add_argument("--quick", action="store_true",
+                    help="only the headline workload (no batched / ChebCode / grid legs)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -242,6 +304,14 @@ def main() -> int:
     }
     if cpu:
         line["cpu_baseline"]["gpu_over_cpu"] = value / cpu["value"]
+    if not args.quick:
+        # The batched launch is switched off in the library (shk_batch_create
+        # refuses); its leg is not run.
+        line["batched"] = {"status": "switched off: first GPU run did not finish, cause not found"}
+        log("ChebCode at p = 2e5 / 1e6")
+        line["chebcode_large"] = chebcode_leg(shk, sync)
+        log("grid (at-points) leg")
+        line["grid_at_points_p50000_nq200"] = grid_leg(shk, sync)
     print(json.dumps(line), flush=True)
     return 0
 

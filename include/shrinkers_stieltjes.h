@@ -205,6 +205,49 @@ shk_status shk_chebcode_evaluate_many(const shk_chebcode *h, const double *etas,
 
 shk_status shk_chebcode_destroy(shk_chebcode *h);
 
+/* Device-resident ChebCode: build from device eigenvalues on `stream`
+ * (returns once the tree's node count is known; later work on any stream
+ * waits for the build), and evaluate at the tree's own eigenvalues (input
+ * order) into device buffers, multiplied by `scale`.  Same semantics as
+ * shk_chebcode_build / shk_chebcode_evaluate. */
+shk_status shk_chebcode_build_dev(const double *d_eigenvalues, size_t p, double theta, int n,
+                                  size_t leaf_cap, int f32_far, void *stream, shk_chebcode **out);
+shk_status shk_chebcode_evaluate_dev(const shk_chebcode *h, double eta, double scale,
+                                     double *d_out_re, double *d_out_im, void *stream);
+
+/* ---- batched spectra (no counterpart in the reference) ------------------- */
+
+/* Many independent spectra in one persistent launch captured in a hipGraph.
+ * The reference batches only across eta on one spectrum
+ * (ChebCodeBatch::evaluate_many, chebcode.rs:1312); each spectrum's result
+ * here is its own all-points transform (compute_all_stieltjes with the exact
+ * family), in FP32-compensated arithmetic, scaled by 1/p_s when scaled != 0.
+ *
+ * d_eigenvalues: all spectra concatenated on the device (kept alive by the
+ * caller); offsets: host array of n_spectra+1 starts (offsets[0] = 0);
+ * etas: host array of one eta per spectrum.  Outputs share the layout of
+ * d_eigenvalues.
+ *
+ * SWITCHED OFF: unless the library is built with -DSHK_ENABLE_BATCH,
+ * shk_batch_create checks its arguments and then returns SHK_ERR_UNSUPPORTED
+ * (the first GPU run of this path did not finish and no cause has been
+ * shown); the other batch calls then only see a NULL handle. */
+typedef struct shk_batch shk_batch;
+
+shk_status shk_batch_create(const double *d_eigenvalues, const int64_t *offsets, size_t n_spectra,
+                            const double *etas, int scaled, shk_batch **out);
+/* Enqueue the graph on `stream` (re-captured when the outputs change). */
+shk_status shk_batch_run(shk_batch *b, double *d_out_re, double *d_out_im, void *stream);
+/* Time every (queries per lane, workgroups per CU) candidate on this batch
+ * and keep the fastest; the chosen values and time are returned. */
+shk_status shk_batch_tune(shk_batch *b, double *d_out_re, double *d_out_im, void *stream,
+                          int *best_r, int *best_workers_per_cu, double *best_ms);
+shk_status shk_batch_set_config(shk_batch *b, int r, int workers_per_cu);
+/* Per-spectrum start/end of the last completed run in microseconds from the
+ * run's first item (device wall clock); the caller synchronises first. */
+shk_status shk_batch_spectrum_times(const shk_batch *b, double *start_us, double *end_us);
+shk_status shk_batch_destroy(shk_batch *b);
+
 /* ---- measurement ---------------------------------------------------------- */
 
 /* When enabled, the library brackets the dominant kernel of each exact or

@@ -65,6 +65,14 @@ SIGNATURES = {
     "shk_chebcode_evaluate_points": (_i, [_vp, _dp, _sz, _d, _dp, _dp]),
     "shk_chebcode_evaluate_many": (_i, [_vp, _dp, _sz, _dp, _dp]),
     "shk_chebcode_destroy": (_i, [_vp]),
+    "shk_chebcode_build_dev": (_i, [_vp, _sz, _d, _i, _sz, _i, _vp, ctypes.POINTER(_vp)]),
+    "shk_chebcode_evaluate_dev": (_i, [_vp, _d, _d, _vp, _vp, _vp]),
+    "shk_batch_create": (_i, [_vp, _i64p, _sz, _dp, _i, ctypes.POINTER(_vp)]),
+    "shk_batch_run": (_i, [_vp, _vp, _vp, _vp]),
+    "shk_batch_tune": (_i, [_vp, _vp, _vp, _vp, _ip, _ip, _dp]),
+    "shk_batch_set_config": (_i, [_vp, _i, _i]),
+    "shk_batch_spectrum_times": (_i, [_vp, _dp, _dp]),
+    "shk_batch_destroy": (_i, [_vp]),
     "shk_profile_enable": (_i, [_i]),
     "shk_profile_read": (_i, [_dp, _i64p]),
     "shk_profile_reset": (_i, []),

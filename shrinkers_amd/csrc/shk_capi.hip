@@ -693,6 +693,29 @@ shk_status shk_chebcode_evaluate_many(const shk_chebcode *h, const double *etas,
     return SHK_OK;
 }
 
+shk_status shk_chebcode_build_dev(const double *d_eigenvalues, size_t p, double theta, int n,
+                                  size_t leaf_cap, int f32_far, void *stream, shk_chebcode **out) {
+    if (!out) return fail(SHK_ERR_BAD_ARG, "NULL handle pointer");
+    *out = nullptr;
+    if (p == 0) return fail(SHK_ERR_EMPTY, "eigenvalues must be non-empty");
+    if (!d_eigenvalues) return fail(SHK_ERR_BAD_ARG, "NULL pointer");
+    if (n < 2) return fail(SHK_ERR_BAD_ARG, "ChebCode needs at least 2 Chebyshev nodes");
+    SHK_CHECK(ensure_device());
+    ChebParams prm{theta, n, (int64_t)leaf_cap, f32_far ? 1 : 0};
+    ChebTree *t = nullptr;
+    SHK_CHECK(cheb_build(d_eigenvalues, p, prm, &t, (hipStream_t)stream));
+    *out = new shk_chebcode{t};
+    return SHK_OK;
+}
+
+shk_status shk_chebcode_evaluate_dev(const shk_chebcode *h, double eta, double scale,
+                                     double *d_out_re, double *d_out_im, void *stream) {
+    if (!h || !h->tree || !d_out_re || !d_out_im) return fail(SHK_ERR_BAD_ARG, "NULL pointer");
+    SHK_CHECK(check_eta(eta));
+    return (shk_status)cheb_eval(h->tree, nullptr, cheb_p(h->tree), eta, scale, 1, d_out_re,
+                                 d_out_im, (hipStream_t)stream);
+}
+
 shk_status shk_chebcode_destroy(shk_chebcode *h) {
     if (!h) return SHK_OK;
     cheb_destroy(h->tree);

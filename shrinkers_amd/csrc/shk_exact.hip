@@ -29,9 +29,10 @@
 // where qh/ql and sh/sl are the f32 hi/lo parts of x_i - c and lam_j - c.
 // For the near pairs that dominate Im S, qh - sh is exact (Sterbenz), so the
 // pair difference carries at most one f32 rounding.  The reciprocal is
-// v_rcp_f32 (1 ulp).  Partial sums run over kRun sources in FP32 and are
+// v_rcp_f32 (1 ulp).  Partial sums run over kF32Run sources in FP32 and are
 // folded into FP64 accumulators; the cross-split reduction is FP64.
 #include "shk_internal.hpp"
+#include "shk_f32c.hpp"
 
 #include <algorithm>
 #include <vector>
@@ -59,7 +60,6 @@ constexpr int kWG = SHK_EXACT_WG;          // threads per workgroup
 constexpr int kR = SHK_EXACT_R;            // queries per thread
 constexpr int kQT = kWG * kR;              // queries per tile
 constexpr int kPiece = SHK_EXACT_PIECE;    // sources staged in LDS per pass
-constexpr int kRun = 32;                   // f32 terms per FP64 fold
 constexpr int kTargetWG = SHK_EXACT_TARGET_WG;  // workgroups to aim for
 
 // 1/x to ~1 ulp from the v_rcp_f64 seed r0.  With e = 1 - x*r0,
@@ -157,76 +157,6 @@ __global__ __launch_bounds__(kWG) void exact_f64_kernel(
 
 // ---- FP32-compensated pair kernel ------------------------------------------
 
-template <int FORM>
-__device__ __forceinline__ float pair_diff(float qh, float ql, float sh, float sl) {
-    if constexpr (FORM == 0) return qh - sh;
-    else if constexpr (FORM == 1) return (qh - sh) + ql;
-    else return (qh - sh) + (ql - sl);
-}
-
-// Sweeps `cnt` staged sources for this thread's kR queries.
-template <int FORM>
-__device__ __forceinline__ void f32c_sweep(const float *__restrict__ sh_lds,
-                                           const float *__restrict__ sl_lds, int cnt,
-                                           const float (&qh)[kR], const float (&ql)[kR],
-                                           float eta2f, double (&re)[kR], double (&iv)[kR]) {
-    int j = 0;
-    for (; j + kRun <= cnt; j += kRun) {
-        float ar[kR], ai[kR];
-#pragma unroll
-        for (int k = 0; k < kR; ++k) ar[k] = ai[k] = 0.0f;
-#pragma unroll 8
-        for (int u = 0; u < kRun; ++u) {
-            const float sh = sh_lds[j + u];
-            const float sl = (FORM == 2) ? sl_lds[j + u] : 0.0f;
-#pragma unroll
-            for (int k = 0; k < kR; ++k) {
-                const float d = pair_diff<FORM>(qh[k], ql[k], sh, sl);
-                const float r = __builtin_amdgcn_rcpf(__builtin_fmaf(d, d, eta2f));
-                ar[k] = __builtin_fmaf(d, r, ar[k]);
-                ai[k] += r;
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < kR; ++k) {
-            re[k] += (double)ar[k];
-            iv[k] += (double)ai[k];
-        }
-    }
-    if (j < cnt) {
-        float ar[kR], ai[kR];
-#pragma unroll
-        for (int k = 0; k < kR; ++k) ar[k] = ai[k] = 0.0f;
-        for (; j < cnt; ++j) {
-            const float sh = sh_lds[j];
-            const float sl = (FORM == 2) ? sl_lds[j] : 0.0f;
-#pragma unroll
-            for (int k = 0; k < kR; ++k) {
-                const float d = pair_diff<FORM>(qh[k], ql[k], sh, sl);
-                const float r = __builtin_amdgcn_rcpf(__builtin_fmaf(d, d, eta2f));
-                ar[k] = __builtin_fmaf(d, r, ar[k]);
-                ai[k] += r;
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < kR; ++k) {
-            re[k] += (double)ar[k];
-            iv[k] += (double)ai[k];
-        }
-    }
-}
-
-__device__ __forceinline__ double wave_min(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
 __global__ __launch_bounds__(kWG) void exact_f32c_kernel(const double *__restrict__ q, int nq,
                                                          const double *__restrict__ src, int p,
                                                          double eta, int src_per_split,
@@ -267,7 +197,7 @@ __global__ __launch_bounds__(kWG) void exact_f32c_kernel(const double *__restric
     }
     const double c = 0.5 * (mn + mx);
     const double half_w = 0.5 * (mx - mn);
-    const int form = (half_w <= 2.0 * eta) ? 0 : ((half_w <= 32.0 * eta) ? 1 : 2);
+    const int form = f32c_form(half_w, eta);
 
     float qh[kR], ql[kR];
     double re[kR], iv[kR];
@@ -291,11 +221,11 @@ __global__ __launch_bounds__(kWG) void exact_f32c_kernel(const double *__restric
         }
         __syncthreads();
         if (form == 0)
-            f32c_sweep<0>(s_hi, s_lo, cnt, qh, ql, eta2f, re, iv);
+            f32c_sweep<kR, 0>(s_hi, s_lo, cnt, qh, ql, eta2f, re, iv);
         else if (form == 1)
-            f32c_sweep<1>(s_hi, s_lo, cnt, qh, ql, eta2f, re, iv);
+            f32c_sweep<kR, 1>(s_hi, s_lo, cnt, qh, ql, eta2f, re, iv);
         else
-            f32c_sweep<2>(s_hi, s_lo, cnt, qh, ql, eta2f, re, iv);
+            f32c_sweep<kR, 2>(s_hi, s_lo, cnt, qh, ql, eta2f, re, iv);
     }
 
 #pragma unroll

@@ -1,0 +1,92 @@
+// FP32-compensated pair arithmetic shared by the exact kernels
+// (shk_exact.hip) and the batched launch (shk_batch.hip).
+//
+// Eigenvalues are re-expressed relative to a tile centre c and split into
+// f32 hi/lo parts; FORM selects how much of the residual enters the pair
+// difference (see shk_exact.hip).  Terms are summed in f32 over runs of
+// kF32Run sources and folded into f64 accumulators.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace shk {
+
+constexpr int kF32Run = 32;  // f32 terms per FP64 fold
+
+template <int FORM>
+__device__ __forceinline__ float pair_diff(float qh, float ql, float sh, float sl) {
+    if constexpr (FORM == 0) return qh - sh;
+    else if constexpr (FORM == 1) return (qh - sh) + ql;
+    else return (qh - sh) + (ql - sl);
+}
+
+// Sweeps `cnt` staged sources for this thread's R queries.
+template <int R, int FORM>
+__device__ __forceinline__ void f32c_sweep(const float *__restrict__ sh_lds,
+                                           const float *__restrict__ sl_lds, int cnt,
+                                           const float (&qh)[R], const float (&ql)[R],
+                                           float eta2f, double (&re)[R], double (&iv)[R]) {
+    int j = 0;
+    for (; j + kF32Run <= cnt; j += kF32Run) {
+        float ar[R], ai[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) ar[k] = ai[k] = 0.0f;
+#pragma unroll 8
+        for (int u = 0; u < kF32Run; ++u) {
+            const float sh = sh_lds[j + u];
+            const float sl = (FORM == 2) ? sl_lds[j + u] : 0.0f;
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                const float d = pair_diff<FORM>(qh[k], ql[k], sh, sl);
+                const float r = __builtin_amdgcn_rcpf(__builtin_fmaf(d, d, eta2f));
+                ar[k] = __builtin_fmaf(d, r, ar[k]);
+                ai[k] += r;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            re[k] += (double)ar[k];
+            iv[k] += (double)ai[k];
+        }
+    }
+    if (j < cnt) {
+        float ar[R], ai[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) ar[k] = ai[k] = 0.0f;
+        for (; j < cnt; ++j) {
+            const float sh = sh_lds[j];
+            const float sl = (FORM == 2) ? sl_lds[j] : 0.0f;
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+                const float d = pair_diff<FORM>(qh[k], ql[k], sh, sl);
+                const float r = __builtin_amdgcn_rcpf(__builtin_fmaf(d, d, eta2f));
+                ar[k] = __builtin_fmaf(d, r, ar[k]);
+                ai[k] += r;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            re[k] += (double)ar[k];
+            iv[k] += (double)ai[k];
+        }
+    }
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+
+// Pair-difference form for a tile of half-width half_w (see shk_exact.hip).
+__device__ __forceinline__ int f32c_form(double half_w, double eta) {
+    return (half_w <= 2.0 * eta) ? 0 : ((half_w <= 32.0 * eta) ? 1 : 2);
+}
+
+}  // namespace shk

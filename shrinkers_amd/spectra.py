@@ -72,18 +72,37 @@ def _mp_cdf_theta(theta: np.ndarray, c: float) -> np.ndarray:
 
 
 def mp_inverse_cdf(u: np.ndarray, c: float, sigma2: float = 1.0) -> np.ndarray:
-    """Inverse MP CDF by bisection on theta (64 halvings, f64)."""
+    """Inverse MP CDF in theta: a bracket from a 4097-point table of F, then
+    safeguarded Newton steps (F'(theta) = r^2 sin^2(theta) / (2 pi c x)),
+    falling back to bisection whenever a step leaves the bracket."""
     u = np.asarray(u, dtype=np.float64)
-    lo = np.zeros_like(u)
-    hi = np.full_like(u, math.pi)
-    for _ in range(64):
-        mid = 0.5 * (lo + hi)
-        below = _mp_cdf_theta(mid, c) < u
-        lo = np.where(below, mid, lo)
-        hi = np.where(below, hi, mid)
-    theta = 0.5 * (lo + hi)
     m = 1.0 + c
     r = 2.0 * math.sqrt(c)
+    grid = np.linspace(0.0, math.pi, 4097)
+    table = _mp_cdf_theta(grid, c)
+    k = np.clip(np.searchsorted(table, u, side="right") - 1, 0, grid.size - 2)
+    lo = grid[k]
+    hi = grid[k + 1]
+    theta = lo + (u - table[k]) / np.maximum(table[k + 1] - table[k], 1e-300) * (hi - lo)
+    theta = np.clip(theta, lo, hi)
+    act = np.arange(u.size)
+    for _ in range(80):  # converged elements leave the active set
+        t, a, b, uu = theta[act], lo[act], hi[act], u[act]
+        f = _mp_cdf_theta(t, c) - uu
+        a = np.where(f < 0.0, t, a)
+        b = np.where(f < 0.0, b, t)
+        x = m - r * np.cos(t)
+        fp = r * r * np.sin(t) ** 2 / (2.0 * math.pi * c * x)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            step = t - f / fp
+        conv = (f == 0.0) | (np.abs(step - t) <= 4e-16 * np.maximum(t, 1e-300))
+        inside = (step >= a) & (step <= b) & np.isfinite(step)
+        tn = np.where(conv, t, np.where(inside, step, 0.5 * (a + b)))
+        theta[act], lo[act], hi[act] = tn, a, b
+        done = conv | (b - a <= 1e-15)
+        act = act[~done]
+        if act.size == 0:
+            break
     return sigma2 * (m - r * np.cos(theta))
 
 
@@ -107,3 +126,25 @@ def spiked_wishart_spectrum(n: int = 4096, p: int = 1024, spikes=(12.0, 7.0, 4.0
     sig[: len(spikes)] = spikes
     x = rng.standard_normal((n, p)) * np.sqrt(sig)[None, :]
     return np.linalg.eigvalsh(x.T @ x / n)
+
+
+def batched_workload(n: int = 512, seed: int = 0, p_range=(500, 20000), c_range=(0.05, 0.5),
+                     sigma2_range=(0.5, 2.0), max_spikes: int = 5, spike_range=(2.0, 20.0)):
+    """Independent spectra of uneven size: p ~ U[p_range], c ~ U[c_range],
+    sigma^2 log-uniform on sigma2_range, 0..max_spikes population spikes
+    l ~ U[spike_range]*sigma^2, each placed at its forward-BBP image when
+    above the detection threshold sigma^2 (1 + sqrt(c)) (otherwise it stays in
+    the bulk).  Returns (list of sorted spectra, list of (p, c, sigma2, k))."""
+    rng = np.random.default_rng(seed)
+    spectra, meta = [], []
+    for s in range(n):
+        p = int(rng.integers(p_range[0], p_range[1] + 1))
+        c = float(rng.uniform(*c_range))
+        sigma2 = float(np.exp(rng.uniform(np.log(sigma2_range[0]), np.log(sigma2_range[1]))))
+        k = int(rng.integers(0, max_spikes + 1))
+        ells = rng.uniform(*spike_range, size=k) * sigma2
+        outl = [forward_bbp(l, c, sigma2) for l in ells if l / sigma2 > 1.0 + math.sqrt(c)]
+        bulk = mp_inverse_cdf(rng.random(p - len(outl)), c, sigma2)
+        spectra.append(np.sort(np.concatenate([bulk, np.asarray(outl, np.float64)])))
+        meta.append((p, c, sigma2, len(outl)))
+    return spectra, meta

@@ -48,6 +48,12 @@ CHEB_PRESETS = {
     "chebcode_balanced": (0.55, 11, 32, False),
 }
 
+#: Additional operating point with no reference counterpart: the DEFAULT
+#: opening angle with interpolation order 12 and an f64 far field.
+EXTRA_CHEB_PRESETS = {
+    "chebcode_n12": (0.5, 12, 32, False),
+}
+
 _PRECISIONS = {"f64": _lib.PREC_F64, "f32c": _lib.PREC_F32C, "f32": _lib.PREC_F32}
 
 
@@ -299,8 +305,33 @@ class ChebCodeBatch:
 
     @classmethod
     def build_preset(cls, eigenvalues, preset: str) -> "ChebCodeBatch":
-        theta, n, leaf, f32 = CHEB_PRESETS[preset]
+        theta, n, leaf, f32 = {**CHEB_PRESETS, **EXTRA_CHEB_PRESETS}[preset]
         return cls.build_mode(eigenvalues, theta, n, leaf, f32)
+
+    @classmethod
+    def build_dev(cls, ev, theta: float, n: int, leaf_cap: int, f32_far: bool = False,
+                  stream=None) -> "ChebCodeBatch":
+        """Build from a CUDA float64 tensor on ``stream`` (default: torch's current)."""
+        import torch
+        if not (ev.is_cuda and ev.dtype == torch.float64 and ev.dim() == 1 and ev.is_contiguous()):
+            raise ValueError("ev must be a contiguous 1-D float64 CUDA tensor")
+        h = ctypes.c_void_p()
+        check(_lib.load().shk_chebcode_build_dev(ev.data_ptr(), ev.numel(), float(theta), int(n),
+                                                 int(leaf_cap), int(bool(f32_far)),
+                                                 _stream_handle(stream), ctypes.byref(h)))
+        return cls(h, ev.numel(), bool(f32_far))
+
+    def evaluate_dev(self, eta: float, scale: float = 1.0, out=None, stream=None):
+        """Evaluate at the tree's own eigenvalues (build order) into CUDA
+        float64 tensors, enqueued on ``stream``; returns ``(re, im)``."""
+        import torch
+        if out is None:
+            out = (torch.empty(self.p, dtype=torch.float64, device="cuda"),
+                   torch.empty(self.p, dtype=torch.float64, device="cuda"))
+        check(_lib.load().shk_chebcode_evaluate_dev(self._h, float(eta), float(scale),
+                                                    out[0].data_ptr(), out[1].data_ptr(),
+                                                    _stream_handle(stream)))
+        return out
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -94,3 +94,27 @@ def test_no_device_is_an_error_not_a_fallback(shk):
     st = lib.shk_compute_all_stieltjes(_lib.dptr(ev), 3, 0.1, 2, 0.0, 0, 0, _lib.dptr(out),
                                        _lib.dptr(out))
     assert st == _lib.SHK_ERR_NO_DEVICE
+
+
+def test_batch_argument_checks(shk):
+    from shrinkers_amd import _lib
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    off = np.array([0, 3, 5], dtype=np.int64)
+    eta = np.array([0.1, 0.1])
+    offp = off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    dummy = ctypes.c_void_p(16)  # never dereferenced: the checks fail first
+    assert lib.shk_batch_create(dummy, offp, 0, _lib.dptr(eta), 1, ctypes.byref(h)) == _lib.SHK_ERR_EMPTY
+    bad = np.array([1, 3, 5], dtype=np.int64)
+    assert lib.shk_batch_create(dummy, bad.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), 2,
+                                _lib.dptr(eta), 1, ctypes.byref(h)) == _lib.SHK_ERR_BAD_ARG
+    empty = np.array([0, 3, 3], dtype=np.int64)
+    assert lib.shk_batch_create(dummy, empty.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), 2,
+                                _lib.dptr(eta), 1, ctypes.byref(h)) == _lib.SHK_ERR_EMPTY
+    neg = np.array([0.1, -1.0])
+    assert lib.shk_batch_create(dummy, offp, 2, _lib.dptr(neg), 1, ctypes.byref(h)) == _lib.SHK_ERR_BAD_ETA
+    assert lib.shk_batch_set_config(None, 2, 16) == _lib.SHK_ERR_BAD_ARG
+    # valid arguments: the batched launch is switched off and says so
+    assert lib.shk_batch_create(dummy, offp, 2, _lib.dptr(eta), 1, ctypes.byref(h)) == \
+        _lib.SHK_ERR_UNSUPPORTED
+    assert "switched off" in _lib.last_error() and not h.value
