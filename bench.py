@@ -150,6 +150,75 @@ def chebcode_leg(shk, sync) -> dict:
     return res
 
 
+def adjoint_leg(shk, sync, d_ev, eta) -> dict:
+    """Reverse mode: (1) the differentiated El Karoui bulk fit at p = 10 000,
+    c = 0.25, 256 atoms, 512 grid points (forward + backward per step, FP32-
+    compensated and FP64, gradient agreement between them); (2) the adjoint
+    kernels on the headline all-points workload (gradient of a linear
+    functional of S with respect to every eigenvalue and eta: an eigenvalue
+    sweep and a query sweep, 2 p^2 pair evaluations)."""
+    import torch
+    from shrinkers_amd import autograd as ag
+    from shrinkers_amd import stieltjes as st
+    ev, grid, atoms, eta_f = ag.bulk_fit_problem()
+    logits = 0.5 * np.random.default_rng(0).standard_normal(atoms.size)
+    t_ev = torch.from_numpy(ev).cuda()
+    t_grid = torch.from_numpy(grid).cuda()
+    t_atoms = torch.from_numpy(atoms).cuda()
+    out = {"workload": "bulk fit p=10000 c=0.25 MP (no outliers), K=256 atoms log-spaced "
+                       "[0.2,5], softmax weights (seeded logits), M=512 grid points spanning "
+                       "the bulk, eta=1/sqrt(p)"}
+    grads = {}
+    for prec in ("f32c", "f64"):
+        def step():
+            tev = t_ev.clone().requires_grad_(True)
+            te = torch.tensor(eta_f, dtype=torch.float64, device="cuda", requires_grad=True)
+            tl = torch.from_numpy(logits).cuda().requires_grad_(True)
+            loss = ag.el_karoui_fit_loss(tev, te, tl, t_atoms, 0.25, t_grid, prec)
+            loss.backward()
+            return loss, tev.grad, te.grad, tl.grad
+        for _ in range(3):
+            step()
+        sync()
+        n = 20
+        t0 = time.perf_counter()
+        for _ in range(n):
+            res = step()
+        sync()
+        ms = (time.perf_counter() - t0) / n * 1e3
+        grads[prec] = [r.detach().cpu().numpy() for r in res]
+        out[f"fit_{prec}"] = {"ms_forward_backward": ms, "loss": float(grads[prec][0])}
+
+    def rel(a, b):
+        return float(np.linalg.norm(np.ravel(a - b)) / np.linalg.norm(np.ravel(b)))
+    out["f32c_vs_f64_grad_rel"] = {"eigenvalues": rel(grads["f32c"][1], grads["f64"][1]),
+                                   "eta": rel(grads["f32c"][2], grads["f64"][2]),
+                                   "logits": rel(grads["f32c"][3], grads["f64"][3])}
+
+    # adjoint kernels at the headline size: L = sum G_re Re S + G_im Im S
+    p = d_ev.numel()
+    rng = np.random.default_rng(5)
+    g_re = torch.from_numpy(rng.standard_normal(p)).cuda()
+    g_im = torch.from_numpy(rng.standard_normal(p)).cuda()
+    gev = torch.empty_like(d_ev)
+    ge = torch.empty((), dtype=torch.float64, device="cuda")
+    lib = shk._lib.load()
+    for prec_name, prec in (("f32c", 1), ("f64", 0)):
+        def adj():
+            shk._lib.check(lib.shk_exact_adjoint_dev(
+                d_ev.data_ptr(), p, d_ev.data_ptr(), p, eta, g_re.data_ptr(), g_im.data_ptr(),
+                prec, gev.data_ptr(), None, ge.data_ptr(), st._stream_handle(None)))
+        steps = 20 if prec_name == "f32c" else 8
+        t = time_steps(adj, steps, 2, sync) / steps
+        k_ms = kernel_avg_ms(shk, adj, 5, sync)   # average of the two sweep kernels
+        out[f"adjoint_all_points_p{p}_{prec_name}"] = {
+            "ms_per_backward": t * 1e3,
+            "pair_evals_per_s": 2.0 * p * p / t,
+            "sweep_kernel_avg_ms": k_ms,
+        }
+    return out
+
+
 def grid_leg(shk, sync) -> dict:
     """At-points path of the El Karoui consumer: 200 grid points, p = 50 000."""
     import torch
@@ -188,7 +257,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU timing")
     ap.add_argument("--quick", action="store_true",
-                    help="only the headline workload (no batched / ChebCode / grid legs)")
+                    help="only the headline workload (no ChebCode / grid / reverse-mode legs)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -312,6 +381,8 @@ def main() -> int:
         line["chebcode_large"] = chebcode_leg(shk, sync)
         log("grid (at-points) leg")
         line["grid_at_points_p50000_nq200"] = grid_leg(shk, sync)
+        log("reverse mode (bulk fit, adjoint kernels)")
+        line["reverse_mode"] = adjoint_leg(shk, sync, d_ev, eta)
     print(json.dumps(line), flush=True)
     return 0
 

@@ -248,6 +248,55 @@ shk_status shk_batch_set_config(shk_batch *b, int r, int workers_per_cu);
 shk_status shk_batch_spectrum_times(const shk_batch *b, double *start_us, double *end_us);
 shk_status shk_batch_destroy(shk_batch *b);
 
+/* ---- reverse mode (no counterpart in the reference) ---------------------- */
+
+/* The reference differentiates the transform only with respect to the query
+ * point (stieltjes_with_deriv_sum, src/stieltjes/autovec.rs:38-59, exposed
+ * as stieltjes_transform_with_deriv, src/python.rs:758-781).  These entries
+ * give a differentiable pair on device data:
+ *
+ * shk_exact_at_points_dev: raw exact sums S_m = sum_j 1/((x_m - lam_j) - i eta)
+ * at device query points, in SHK_PREC_F64 or SHK_PREC_F32C (the at-points
+ * counterpart of compute_stieltjes_at_points with the exact family,
+ * src/stieltjes/mod.rs:394-494, with a precision choice). */
+shk_status shk_exact_at_points_dev(const double *d_query, size_t nq, const double *d_eigenvalues,
+                                   size_t p, double eta, int precision, double *d_out_re,
+                                   double *d_out_im, void *stream);
+
+/* Adjoint of shk_exact_at_points_dev.  Given d_grad_re[m], d_grad_im[m] =
+ * dL/dRe S_m, dL/dIm S_m of a real loss L, writes dL/dlam_j (p values),
+ * dL/dx_m (nq values) and dL/deta (one device double); each output pointer
+ * may be NULL to skip it.  With d = x_m - lam_j, D = d^2 + eta^2:
+ *   dL/dlam_j = sum_m (G_re (d^2 - eta^2) - 2 G_im eta d) / D^2
+ *   dS_m/dx   = sum_j ((eta^2 - d^2) - 2 i eta d) / D^2   (autovec.rs:38-59)
+ *   dL/dx_m   = G_re Re dS/dx + G_im Im dS/dx
+ *   dL/deta   = sum_m G_re Im dS/dx - G_im Re dS/dx       (dS/deta = -i dS/dx)
+ * For the all-points transform (queries = eigenvalues) the caller adds the
+ * query and eigenvalue gradients. */
+shk_status shk_exact_adjoint_dev(const double *d_query, size_t nq, const double *d_eigenvalues,
+                                 size_t p, double eta, const double *d_grad_re,
+                                 const double *d_grad_im, int precision,
+                                 double *d_grad_eigenvalues, double *d_grad_query,
+                                 double *d_grad_eta, void *stream);
+
+/* El Karoui bulk fit, FP64.  The residual of the Marchenko-Pastur equation
+ * as written in src/deconvolution/mod.rs:6 for a discrete population
+ * H = sum_k w_k delta_{t_k}:
+ *   a_m = 1 - c - c z_m g_m,  r_m = g_m - sum_k w_k / (t_k a_m - z_m),
+ *   loss = (1/M) sum_m |r_m|^2,  z_m = x_m + i eta,
+ * where g_m is the sample Stieltjes transform in the reference inversion's
+ * convention (Im g > 0, scaled by 1/p: g = -conj(S)/p for the raw sums S,
+ * src/deconvolution/mod.rs:210-213).  Writes the loss and the gradients
+ * with respect to Re g, Im g (M values each), the weights (K values) and eta
+ * through z only (one device double; the part through g is the adjoint's).
+ * Every output pointer may be NULL.  The reference inverts the equation
+ * pointwise (mod.rs:116-259) and has no fit. */
+shk_status shk_el_karoui_fit_dev(const double *d_x, const double *d_g_re, const double *d_g_im,
+                                 size_t m, double eta, double c, const double *d_atoms,
+                                 const double *d_weights, size_t k, double *d_loss,
+                                 double *d_grad_g_re, double *d_grad_g_im,
+                                 double *d_grad_weights, double *d_grad_eta, void *stream);
+
 /* ---- measurement ---------------------------------------------------------- */
 
 /* When enabled, the library brackets the dominant kernel of each exact or

@@ -80,6 +80,12 @@ def lib() -> ctypes.CDLL:
     L.orc_grid_leaf_cap.restype = _i64
     L.orc_max_threads.restype = ctypes.c_int
     L.orc_set_outside_theta2.argtypes = [ctypes.c_float]
+    L.orc_adjoint_at_points.argtypes = [_f64p, _i64, _f64p, _i64, _d, _f64p, _f64p,
+                                        _f64p, _f64p, _dp]
+    L.orc_ek_residual.argtypes = [_f64p, _f64p, _f64p, _i64, _d, _d, _f64p, _f64p, _i64,
+                                  _f64p, _f64p, _f64p, _f64p]
+    L.orc_ek_fit.argtypes = [_f64p, _f64p, _f64p, _i64, _d, _d, _f64p, _f64p, _i64,
+                             _dp, _f64p, _f64p, _f64p, _dp]
     _lib = L
     return L
 
@@ -192,6 +198,38 @@ def with_deriv(ev, eta):
 
 def scaled_ray(ev, scale):
     return _allpoints(lib().orc_scaled_ray, ev, scale)
+
+
+# ---- adjoint and El Karoui fit (FP64 reference gradients) -----------------
+
+def adjoint_at_points(q, ev, eta, g_re, g_im):
+    """Gradients of sum_m g_re[m] Re S_m + g_im[m] Im S_m (raw at-points sums)
+    with respect to the eigenvalues, the query points and eta."""
+    q, ev, g_re, g_im = _c(q), _c(ev), _c(g_re), _c(g_im)
+    gev = np.empty_like(ev)
+    gq = np.empty_like(q)
+    ge = _d()
+    lib().orc_adjoint_at_points(q, q.size, ev, ev.size, float(eta), g_re, g_im, gev, gq,
+                                ctypes.byref(ge))
+    return gev, gq, ge.value
+
+
+def ek_residual(x, g_re, g_im, eta, c, atoms, weights):
+    """MP-equation residual r and a(z) = 1 - c - c z g per grid point."""
+    x, g_re, g_im, t, w = _c(x), _c(g_re), _c(g_im), _c(atoms), _c(weights)
+    out = [np.empty_like(x) for _ in range(4)]
+    lib().orc_ek_residual(x, g_re, g_im, x.size, float(eta), float(c), t, w, t.size, *out)
+    return out[0] + 1j * out[1], out[2] + 1j * out[3]
+
+
+def ek_fit(x, g_re, g_im, eta, c, atoms, weights):
+    """(loss, dL/dRe g, dL/dIm g, dL/dw, direct dL/deta) of the El Karoui fit."""
+    x, g_re, g_im, t, w = _c(x), _c(g_re), _c(g_im), _c(atoms), _c(weights)
+    loss, ge = _d(), _d()
+    ggr, ggi, gw = np.empty_like(x), np.empty_like(x), np.empty_like(t)
+    lib().orc_ek_fit(x, g_re, g_im, x.size, float(eta), float(c), t, w, t.size,
+                     ctypes.byref(loss), ggr, ggi, gw, ctypes.byref(ge))
+    return loss.value, ggr, ggi, gw, ge.value
 
 
 # ---- ChebCode -------------------------------------------------------------

@@ -39,7 +39,19 @@
  *                               (docs/pareto/bench_after.json), the
  *                               reference's own bounds (chebcode.rs:1402-1487)
  *   orc_harness_spectrum etc. integer LCG recomputed with Python ints
+ *   orc_adjoint_at_points     central FD of orc_at_points in lam, x and eta;
+ *                               translation invariance (sum of lam and x
+ *                               gradients is 0); homogeneity of degree -1
+ *                               (lam.grad + x.grad + eta.grad = -L); eta part
+ *                               against orc_with_deriv (dS/deta = -i dS/dx)
+ *   orc_ek_residual           a(z) and z/a against the reference golden master
+ *                               (population Im m = Im(a g), w_re = Re(z/a));
+ *                               r = 0 on the Marchenko-Pastur law (H = delta_1,
+ *                               g from the closed-form quadratic)
+ *   orc_ek_fit                loss = mean |orc_ek_residual|^2; central FD of the
+ *                               loss in Re g, Im g, w and eta
  */
+#include <complex.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -374,6 +386,134 @@ EXPORT void orc_at_points(const double *q, int64_t nq, const double *ev, int64_t
         out_r[i] = r;
         out_i[i] = s * eta;
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Adjoint of the raw at-points sums                                         */
+/* ------------------------------------------------------------------------ */
+
+/* No reference counterpart: the reference differentiates only with respect
+ * to the query (autovec.rs:38-59).  With d = x_m - lam_j, D = d^2 + eta^2,
+ * u = d/D, v = eta/D, the per-pair derivatives of (Re, Im) of
+ * 1/(d - i*eta) = (d + i*eta)/D are
+ *   d/dx   : (v^2 - u^2, -2uv)      (the reference's dS/dx, autovec.rs:38-59)
+ *   d/dlam : -(d/dx)
+ *   d/deta : (-2uv, u^2 - v^2)      (S depends on x - i*eta only: dS/deta = -i dS/dx)
+ * For upstream gradients G_re[m], G_im[m] of a real function of (Re S_m, Im S_m):
+ *   grad_ev[j] = sum_m G_re (u^2 - v^2) + 2 G_im u v
+ *   grad_q[m]  = sum_j G_re (v^2 - u^2) - 2 G_im u v
+ *   grad_eta   = sum_m sum_j -2 G_re u v + G_im (u^2 - v^2)
+ * Plain loops with IEEE divisions; any output pointer may be NULL. */
+EXPORT void orc_adjoint_at_points(const double *q, int64_t nq, const double *ev, int64_t p,
+                                  double eta, const double *gre, const double *gim,
+                                  double *grad_ev, double *grad_q, double *grad_eta) {
+    const double e2 = eta * eta;
+    if (grad_ev) {
+#pragma omp parallel for schedule(static)
+        for (int64_t j = 0; j < p; j++) {
+            double acc = 0.0;
+            for (int64_t m = 0; m < nq; m++) {
+                const double d = q[m] - ev[j];
+                const double den = d * d + e2;
+                const double u = d / den, v = eta / den;
+                acc += gre[m] * (u * u - v * v) + 2.0 * gim[m] * u * v;
+            }
+            grad_ev[j] = acc;
+        }
+    }
+    if (grad_q || grad_eta) {
+        double ge = 0.0;
+        for (int64_t m = 0; m < nq; m++) {
+            double a = 0.0, b = 0.0; /* a = sum (v^2 - u^2), b = sum u v */
+            for (int64_t j = 0; j < p; j++) {
+                const double d = q[m] - ev[j];
+                const double den = d * d + e2;
+                const double u = d / den, v = eta / den;
+                a += v * v - u * u;
+                b += u * v;
+            }
+            const double dre = a, dim = -2.0 * b;
+            if (grad_q) grad_q[m] = gre[m] * dre + gim[m] * dim;
+            ge += gre[m] * dim - gim[m] * dre;
+        }
+        if (grad_eta) *grad_eta = ge;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* El Karoui bulk fit: residual of the Marchenko-Pastur equation             */
+/* ------------------------------------------------------------------------ */
+
+/* The MP equation as written in deconvolution/mod.rs:6,
+ *   g(z) = int dH(t) / (t (1 - c - c z g(z)) - z),
+ * for a population H = sum_k w_k delta_{t_k} and the sample Stieltjes
+ * transform g (convention Im g > 0 for Im z > 0, scaled by 1/p, as the
+ * reference's inversion uses it, deconvolution/mod.rs:206-248).  With
+ * a_m = 1 - c - c z_m g_m and h_mk = 1/(t_k a_m - z_m), z_m = x_m + i*eta:
+ *   r_m = g_m - sum_k w_k h_mk,      loss = (1/M) sum_m |r_m|^2.
+ * The fit itself has no reference counterpart (the reference inverts the
+ * equation pointwise, deconvolution/mod.rs:116-259).  Outputs: r and a per
+ * point (either may be NULL). */
+EXPORT void orc_ek_residual(const double *x, const double *g_re, const double *g_im, int64_t m,
+                            double eta, double c, const double *t, const double *w, int64_t k,
+                            double *r_re, double *r_im, double *a_re, double *a_im) {
+    for (int64_t i = 0; i < m; i++) {
+        const double complex z = x[i] + eta * I;
+        const double complex g = g_re[i] + g_im[i] * I;
+        const double complex a = 1.0 - c - c * z * g;
+        double complex s = 0.0;
+        for (int64_t q = 0; q < k; q++) s += w[q] / (t[q] * a - z);
+        const double complex r = g - s;
+        if (r_re) r_re[i] = creal(r);
+        if (r_im) r_im[i] = cimag(r);
+        if (a_re) a_re[i] = creal(a);
+        if (a_im) a_im[i] = cimag(a);
+    }
+}
+
+/* Loss and its gradients.  r is holomorphic in g and in z, so for the real
+ * pairs (Re g, Im g) the gradient is (2/M) r conj(dr/dg), and eta enters
+ * through z only as d/deta = i d/dz:
+ *   dr/dg    = 1 - c z sum_k w_k t_k h^2
+ *   dr/dz|g  = -sum_k w_k h^2 (1 + c t_k g)
+ *   dL/dw_k  = -(2/M) sum_m Re(conj(r_m) h_mk)
+ *   dL/deta  = (2/M) sum_m Re(conj(r_m) i dr/dz)       (the direct part; the
+ *                                                        part through g is the
+ *                                                        Stieltjes adjoint's)
+ * Any gradient pointer may be NULL. */
+EXPORT void orc_ek_fit(const double *x, const double *g_re, const double *g_im, int64_t m,
+                       double eta, double c, const double *t, const double *w, int64_t k,
+                       double *loss, double *grad_g_re, double *grad_g_im, double *grad_w,
+                       double *grad_eta) {
+    double l = 0.0, ge = 0.0;
+    if (grad_w)
+        for (int64_t q = 0; q < k; q++) grad_w[q] = 0.0;
+    for (int64_t i = 0; i < m; i++) {
+        const double complex z = x[i] + eta * I;
+        const double complex g = g_re[i] + g_im[i] * I;
+        const double complex a = 1.0 - c - c * z * g;
+        double complex s = 0.0, sd = 0.0, sz = 0.0;
+        for (int64_t q = 0; q < k; q++) {
+            const double complex h = 1.0 / (t[q] * a - z);
+            s += w[q] * h;
+            sd += w[q] * t[q] * h * h;
+            sz += w[q] * h * h * (1.0 + c * t[q] * g);
+        }
+        const double complex r = g - s;
+        l += creal(r) * creal(r) + cimag(r) * cimag(r);
+        const double complex jg = 1.0 - c * z * sd;
+        const double complex gg = (2.0 / (double)m) * r * conj(jg);
+        if (grad_g_re) grad_g_re[i] = creal(gg);
+        if (grad_g_im) grad_g_im[i] = cimag(gg);
+        ge += (2.0 / (double)m) * creal(conj(r) * I * (-sz));
+        if (grad_w)
+            for (int64_t q = 0; q < k; q++) {
+                const double complex h = 1.0 / (t[q] * a - z);
+                grad_w[q] -= (2.0 / (double)m) * creal(conj(r) * h);
+            }
+    }
+    if (loss) *loss = l / (double)m;
+    if (grad_eta) *grad_eta = ge;
 }
 
 /* ------------------------------------------------------------------------ */

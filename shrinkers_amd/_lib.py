@@ -73,6 +73,10 @@ SIGNATURES = {
     "shk_batch_set_config": (_i, [_vp, _i, _i]),
     "shk_batch_spectrum_times": (_i, [_vp, _dp, _dp]),
     "shk_batch_destroy": (_i, [_vp]),
+    "shk_exact_at_points_dev": (_i, [_vp, _sz, _vp, _sz, _d, _i, _vp, _vp, _vp]),
+    "shk_exact_adjoint_dev": (_i, [_vp, _sz, _vp, _sz, _d, _vp, _vp, _i, _vp, _vp, _vp, _vp]),
+    "shk_el_karoui_fit_dev": (_i, [_vp, _vp, _vp, _sz, _d, _d, _vp, _vp, _sz, _vp, _vp, _vp,
+                                   _vp, _vp, _vp]),
     "shk_profile_enable": (_i, [_i]),
     "shk_profile_read": (_i, [_dp, _i64p]),
     "shk_profile_reset": (_i, []),

@@ -1,0 +1,633 @@
+// Reverse-mode kernels for the exact at-points sums and the El Karoui fit.
+//
+// The reference differentiates the transform only with respect to the query
+// point (stieltjes_with_deriv_sum, src/stieltjes/autovec.rs:38-59); these
+// kernels supply the rest of the chain for a real loss of the raw sums
+// S_m = sum_j 1/((x_m - lam_j) - i*eta) with upstream gradients
+// (G_re[m], G_im[m]) = (dL/dRe S_m, dL/dIm S_m).  With d = x_m - lam_j and
+// D = d^2 + eta^2:
+//
+//   grad_lam[j] = sum_m (G_re (d^2 - eta^2) - 2 G_im eta d) / D^2        (A)
+//   dS_m/dx     = sum_j ((eta^2 - d^2) - i 2 eta d) / D^2                 (B)
+//   grad_x[m]   = G_re Re dS/dx + G_im Im dS/dx
+//   grad_eta    = sum_m G_re Im dS/dx - G_im Re dS/dx     (dS/deta = -i dS/dx)
+//
+// (A) is the transpose sweep: a workgroup owns a tile of eigenvalues and
+// sweeps the query points (with their upstream gradients) staged in LDS.  (B)
+// is the forward sweep with two derivative accumulators.  Both use the tile
+// layout of the exact kernels (one wavefront per workgroup, two outputs per
+// lane, query/source splits to fill the chip, a deterministic FP64 split
+// reduction) and come in FP64 and FP32-compensated form; the compensated
+// form re-expresses both operands relative to the owned tile's centre in f32
+// hi/lo parts exactly as the forward kernel does, sums runs of 32 terms in
+// f32 and folds them into FP64.  Each term is evaluated as r * (r * poly)
+// with r = 1/D, so no intermediate exceeds the range of the term itself.
+//
+// The El Karoui fit (no reference counterpart; the reference inverts the MP
+// equation pointwise, src/deconvolution/mod.rs:116-259) is O(M K) FP64 work:
+// one thread per grid point forms a = 1 - c z g, the residual
+// r = g - sum_k w_k / (t_k a - z) and its gradients; one thread per atom
+// forms dL/dw_k; a one-workgroup tree sum gives the loss and the direct eta
+// term.  All sums run in a fixed order, so results are bitwise reproducible.
+#include "shk_internal.hpp"
+#include "shk_pair.hpp"
+
+#include <algorithm>
+
+namespace shk {
+namespace {
+
+constexpr int kAWG = 64;           // one wavefront per workgroup
+constexpr int kAR = 2;             // owned outputs per lane
+constexpr int kAT = kAWG * kAR;    // owned outputs per tile
+constexpr int kAPiece = 512;       // swept values staged in LDS per pass
+constexpr int kATargetWG = 32768;  // workgroups to aim for
+constexpr int kSumThreads = 256;
+
+void plan(size_t n_own, size_t n_sweep, int *tiles, int *n_split, int *per) {
+    const int qt = (int)((n_own + kAT - 1) / kAT);
+    int ns = std::max(1, (kATargetWG + qt - 1) / qt);
+    const int max_split = (int)std::max<size_t>(1, (n_sweep + 63) / 64);
+    ns = std::min(ns, max_split);
+    int pp = (int)((n_sweep + ns - 1) / ns);
+    pp = (pp + 31) / 32 * 32;
+    ns = (int)((n_sweep + pp - 1) / pp);
+    *tiles = qt;
+    *n_split = std::max(ns, 1);
+    *per = pp;
+}
+
+// Tile centre and pair-difference form from the owned values (one wavefront).
+__device__ __forceinline__ void tile_centre(const double (&v)[kAR], const bool (&ok)[kAR],
+                                            double eta, double *c, int *form) {
+    double mn = __builtin_inf(), mx = -__builtin_inf();
+#pragma unroll
+    for (int k = 0; k < kAR; ++k)
+        if (ok[k]) {
+            mn = fmin(mn, v[k]);
+            mx = fmax(mx, v[k]);
+        }
+    mn = wave_min(mn);
+    mx = wave_max(mx);
+    *c = 0.5 * (mn + mx);
+    *form = f32c_form(0.5 * (mx - mn), eta);
+}
+
+// ---- (A) gradient with respect to the eigenvalues ---------------------------
+
+// Staged per query m: x_m relative to the tile centre (hi, lo) and the term
+// coefficients ga = G_re, gb = -2 eta G_im, gc = -eta^2 G_re, so that with
+// dd = lam - x the term is (ga dd^2 + gb dd + gc) / D^2.
+template <int FORM>
+__device__ __forceinline__ void src_sweep_f32c(const float *__restrict__ xh,
+                                               const float *__restrict__ xl,
+                                               const float *__restrict__ ga,
+                                               const float *__restrict__ gb,
+                                               const float *__restrict__ gc, int cnt,
+                                               const float (&lh)[kAR], const float (&ll)[kAR],
+                                               float eta2f, double (&acc)[kAR]) {
+    int j = 0;
+    while (j < cnt) {
+        const int run = min(kF32Run, cnt - j);
+        float a[kAR];
+#pragma unroll
+        for (int k = 0; k < kAR; ++k) a[k] = 0.0f;
+        if (run == kF32Run) {
+#pragma unroll 8
+            for (int u = 0; u < kF32Run; ++u) {
+                const float sh = xh[j + u], sl = (FORM == 2) ? xl[j + u] : 0.0f;
+                const float A = ga[j + u], B = gb[j + u], C = gc[j + u];
+#pragma unroll
+                for (int k = 0; k < kAR; ++k) {
+                    const float dd = pair_diff<FORM>(lh[k], ll[k], sh, sl);
+                    const float r = __builtin_amdgcn_rcpf(__builtin_fmaf(dd, dd, eta2f));
+                    const float poly = __builtin_fmaf(__builtin_fmaf(A, dd, B), dd, C);
+                    a[k] = __builtin_fmaf(r, r * poly, a[k]);
+                }
+            }
+        } else {
+            for (int u = 0; u < run; ++u) {
+                const float sh = xh[j + u], sl = (FORM == 2) ? xl[j + u] : 0.0f;
+                const float A = ga[j + u], B = gb[j + u], C = gc[j + u];
+#pragma unroll
+                for (int k = 0; k < kAR; ++k) {
+                    const float dd = pair_diff<FORM>(lh[k], ll[k], sh, sl);
+                    const float r = __builtin_amdgcn_rcpf(__builtin_fmaf(dd, dd, eta2f));
+                    const float poly = __builtin_fmaf(__builtin_fmaf(A, dd, B), dd, C);
+                    a[k] = __builtin_fmaf(r, r * poly, a[k]);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kAR; ++k) acc[k] += (double)a[k];
+        j += run;
+    }
+}
+
+__global__ __launch_bounds__(kAWG) void adj_src_f32c_kernel(
+    const double *__restrict__ q, int nq, const double *__restrict__ gre,
+    const double *__restrict__ gim, const double *__restrict__ ev, int p, double eta, int per,
+    double *__restrict__ part) {
+    __shared__ float s_xh[kAPiece], s_xl[kAPiece], s_a[kAPiece], s_b[kAPiece], s_c[kAPiece];
+    const int lane = threadIdx.x;
+    const int split = blockIdx.y;
+    const int m_begin = split * per;
+    const int m_end = min(m_begin + per, nq);
+    double lam[kAR];
+    bool ok[kAR];
+#pragma unroll
+    for (int k = 0; k < kAR; ++k) {
+        const int j = blockIdx.x * kAT + k * kAWG + lane;
+        ok[k] = j < p;
+        lam[k] = ok[k] ? ev[j] : 0.0;
+    }
+    double c;
+    int form;
+    tile_centre(lam, ok, eta, &c, &form);
+    float lh[kAR], ll[kAR];
+    double acc[kAR];
+#pragma unroll
+    for (int k = 0; k < kAR; ++k) {
+        const double t = lam[k] - c;
+        lh[k] = (float)t;
+        ll[k] = (float)(t - (double)lh[k]);
+        acc[k] = 0.0;
+    }
+    const float eta2f = (float)(eta * eta);
+    for (int base = m_begin; base < m_end; base += kAPiece) {
+        const int cnt = min(kAPiece, m_end - base);
+        __syncthreads();
+        for (int u = lane; u < cnt; u += kAWG) {
+            const double t = q[base + u] - c;
+            const float h = (float)t;
+            s_xh[u] = h;
+            s_xl[u] = (float)(t - (double)h);
+            const double a = gre[base + u], b = gim[base + u];
+            s_a[u] = (float)a;
+            s_b[u] = (float)(-2.0 * eta * b);
+            s_c[u] = (float)(-(eta * eta) * a);
+        }
+        __syncthreads();
+        if (form == 0)
+            src_sweep_f32c<0>(s_xh, s_xl, s_a, s_b, s_c, cnt, lh, ll, eta2f, acc);
+        else if (form == 1)
+            src_sweep_f32c<1>(s_xh, s_xl, s_a, s_b, s_c, cnt, lh, ll, eta2f, acc);
+        else
+            src_sweep_f32c<2>(s_xh, s_xl, s_a, s_b, s_c, cnt, lh, ll, eta2f, acc);
+    }
+#pragma unroll
+    for (int k = 0; k < kAR; ++k) {
+        const int j = blockIdx.x * kAT + k * kAWG + lane;
+        if (j < p) part[(size_t)split * p + j] = acc[k];
+    }
+}
+
+__global__ __launch_bounds__(kAWG) void adj_src_f64_kernel(
+    const double *__restrict__ q, int nq, const double *__restrict__ gre,
+    const double *__restrict__ gim, const double *__restrict__ ev, int p, double eta, int per,
+    double *__restrict__ part) {
+    __shared__ double s_x[kAPiece], s_a[kAPiece], s_b[kAPiece], s_c[kAPiece];
+    const int lane = threadIdx.x;
+    const int split = blockIdx.y;
+    const int m_begin = split * per;
+    const int m_end = min(m_begin + per, nq);
+    const double e2 = eta * eta;
+    double lam[kAR], acc[kAR];
+#pragma unroll
+    for (int k = 0; k < kAR; ++k) {
+        const int j = blockIdx.x * kAT + k * kAWG + lane;
+        lam[k] = (j < p) ? ev[j] : 0.0;
+        acc[k] = 0.0;
+    }
+    for (int base = m_begin; base < m_end; base += kAPiece) {
+        const int cnt = min(kAPiece, m_end - base);
+        __syncthreads();
+        for (int u = lane; u < cnt; u += kAWG) {
+            const double a = gre[base + u], b = gim[base + u];
+            s_x[u] = q[base + u];
+            s_a[u] = a;
+            s_b[u] = -2.0 * eta * b;
+            s_c[u] = -e2 * a;
+        }
+        __syncthreads();
+#pragma unroll 2
+        for (int u = 0; u < cnt; ++u) {
+            const double x = s_x[u], A = s_a[u], B = s_b[u], C = s_c[u];
+#pragma unroll
+            for (int k = 0; k < kAR; ++k) {
+                const double dd = lam[k] - x;
+                const double r = rcp_f64(__builtin_fma(dd, dd, e2));
+                const double poly = __builtin_fma(__builtin_fma(A, dd, B), dd, C);
+                acc[k] = __builtin_fma(r, r * poly, acc[k]);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kAR; ++k) {
+        const int j = blockIdx.x * kAT + k * kAWG + lane;
+        if (j < p) part[(size_t)split * p + j] = acc[k];
+    }
+}
+
+__global__ __launch_bounds__(256) void adj_src_reduce_kernel(const double *__restrict__ part,
+                                                             int n_split, int p,
+                                                             double *__restrict__ out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= p) return;
+    double a = 0.0;
+    for (int s = 0; s < n_split; ++s) a += part[(size_t)s * p + j];
+    out[j] = a;
+}
+
+// ---- (B) derivative sums at the query points ---------------------------------
+
+// Per query: P = sum (eta^2 - d^2)/D^2 and Q = sum d/D^2; dS/dx = P - 2 i eta Q.
+template <int FORM>
+__device__ __forceinline__ void qry_sweep_f32c(const float *__restrict__ sh_lds,
+                                               const float *__restrict__ sl_lds, int cnt,
+                                               const float (&qh)[kAR], const float (&ql)[kAR],
+                                               float eta2f, double (&P)[kAR], double (&Q)[kAR]) {
+    int j = 0;
+    while (j < cnt) {
+        const int run = min(kF32Run, cnt - j);
+        float a[kAR], b[kAR];
+#pragma unroll
+        for (int k = 0; k < kAR; ++k) a[k] = b[k] = 0.0f;
+        if (run == kF32Run) {
+#pragma unroll 8
+            for (int u = 0; u < kF32Run; ++u) {
+                const float sh = sh_lds[j + u], sl = (FORM == 2) ? sl_lds[j + u] : 0.0f;
+#pragma unroll
+                for (int k = 0; k < kAR; ++k) {
+                    const float d = pair_diff<FORM>(qh[k], ql[k], sh, sl);
+                    const float r = __builtin_amdgcn_rcpf(__builtin_fmaf(d, d, eta2f));
+                    const float e = __builtin_fmaf(-d, d, eta2f);
+                    a[k] = __builtin_fmaf(r, r * e, a[k]);
+                    b[k] = __builtin_fmaf(r, r * d, b[k]);
+                }
+            }
+        } else {
+            for (int u = 0; u < run; ++u) {
+                const float sh = sh_lds[j + u], sl = (FORM == 2) ? sl_lds[j + u] : 0.0f;
+#pragma unroll
+                for (int k = 0; k < kAR; ++k) {
+                    const float d = pair_diff<FORM>(qh[k], ql[k], sh, sl);
+                    const float r = __builtin_amdgcn_rcpf(__builtin_fmaf(d, d, eta2f));
+                    const float e = __builtin_fmaf(-d, d, eta2f);
+                    a[k] = __builtin_fmaf(r, r * e, a[k]);
+                    b[k] = __builtin_fmaf(r, r * d, b[k]);
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kAR; ++k) {
+            P[k] += (double)a[k];
+            Q[k] += (double)b[k];
+        }
+        j += run;
+    }
+}
+
+__global__ __launch_bounds__(kAWG) void adj_qry_f32c_kernel(const double *__restrict__ q, int nq,
+                                                            const double *__restrict__ ev, int p,
+                                                            double eta, int per,
+                                                            double *__restrict__ part) {
+    __shared__ float s_hi[kAPiece], s_lo[kAPiece];
+    const int lane = threadIdx.x;
+    const int split = blockIdx.y;
+    const int n_split = gridDim.y;
+    const int j_begin = split * per;
+    const int j_end = min(j_begin + per, p);
+    double x[kAR];
+    bool ok[kAR];
+#pragma unroll
+    for (int k = 0; k < kAR; ++k) {
+        const int i = blockIdx.x * kAT + k * kAWG + lane;
+        ok[k] = i < nq;
+        x[k] = ok[k] ? q[i] : 0.0;
+    }
+    double c;
+    int form;
+    tile_centre(x, ok, eta, &c, &form);
+    float qh[kAR], ql[kAR];
+    double P[kAR], Q[kAR];
+#pragma unroll
+    for (int k = 0; k < kAR; ++k) {
+        const double t = x[k] - c;
+        qh[k] = (float)t;
+        ql[k] = (float)(t - (double)qh[k]);
+        P[k] = Q[k] = 0.0;
+    }
+    const float eta2f = (float)(eta * eta);
+    for (int base = j_begin; base < j_end; base += kAPiece) {
+        const int cnt = min(kAPiece, j_end - base);
+        __syncthreads();
+        for (int u = lane; u < cnt; u += kAWG) {
+            const double t = ev[base + u] - c;
+            const float h = (float)t;
+            s_hi[u] = h;
+            s_lo[u] = (float)(t - (double)h);
+        }
+        __syncthreads();
+        if (form == 0)
+            qry_sweep_f32c<0>(s_hi, s_lo, cnt, qh, ql, eta2f, P, Q);
+        else if (form == 1)
+            qry_sweep_f32c<1>(s_hi, s_lo, cnt, qh, ql, eta2f, P, Q);
+        else
+            qry_sweep_f32c<2>(s_hi, s_lo, cnt, qh, ql, eta2f, P, Q);
+    }
+#pragma unroll
+    for (int k = 0; k < kAR; ++k) {
+        const int i = blockIdx.x * kAT + k * kAWG + lane;
+        if (i < nq) {
+            part[((size_t)0 * n_split + split) * nq + i] = P[k];
+            part[((size_t)1 * n_split + split) * nq + i] = Q[k];
+        }
+    }
+}
+
+__global__ __launch_bounds__(kAWG) void adj_qry_f64_kernel(const double *__restrict__ q, int nq,
+                                                           const double *__restrict__ ev, int p,
+                                                           double eta, int per,
+                                                           double *__restrict__ part) {
+    __shared__ double s_src[kAPiece];
+    const int lane = threadIdx.x;
+    const int split = blockIdx.y;
+    const int n_split = gridDim.y;
+    const int j_begin = split * per;
+    const int j_end = min(j_begin + per, p);
+    const double e2 = eta * eta;
+    double x[kAR], P[kAR], Q[kAR];
+#pragma unroll
+    for (int k = 0; k < kAR; ++k) {
+        const int i = blockIdx.x * kAT + k * kAWG + lane;
+        x[k] = (i < nq) ? q[i] : 0.0;
+        P[k] = Q[k] = 0.0;
+    }
+    for (int base = j_begin; base < j_end; base += kAPiece) {
+        const int cnt = min(kAPiece, j_end - base);
+        __syncthreads();
+        for (int u = lane; u < cnt; u += kAWG) s_src[u] = ev[base + u];
+        __syncthreads();
+#pragma unroll 2
+        for (int u = 0; u < cnt; ++u) {
+            const double s = s_src[u];
+#pragma unroll
+            for (int k = 0; k < kAR; ++k) {
+                const double d = x[k] - s;
+                const double r = rcp_f64(__builtin_fma(d, d, e2));
+                const double e = __builtin_fma(-d, d, e2);
+                P[k] = __builtin_fma(r, r * e, P[k]);
+                Q[k] = __builtin_fma(r, r * d, Q[k]);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kAR; ++k) {
+        const int i = blockIdx.x * kAT + k * kAWG + lane;
+        if (i < nq) {
+            part[((size_t)0 * n_split + split) * nq + i] = P[k];
+            part[((size_t)1 * n_split + split) * nq + i] = Q[k];
+        }
+    }
+}
+
+// Split reduction of (P, Q), then grad_x and the per-query eta term.
+__global__ __launch_bounds__(256) void adj_qry_reduce_kernel(
+    const double *__restrict__ part, int n_split, int nq, double eta,
+    const double *__restrict__ gre, const double *__restrict__ gim,
+    double *__restrict__ grad_q, double *__restrict__ eta_term) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    double P = 0.0, Q = 0.0;
+    for (int s = 0; s < n_split; ++s) {
+        P += part[((size_t)0 * n_split + s) * nq + i];
+        Q += part[((size_t)1 * n_split + s) * nq + i];
+    }
+    const double dre = P, dim = -2.0 * eta * Q;
+    const double a = gre[i], b = gim[i];
+    if (grad_q) grad_q[i] = a * dre + b * dim;
+    if (eta_term) eta_term[i] = a * dim - b * dre;
+}
+
+// One workgroup: out = scale * sum(v) in a fixed order (strided per-thread
+// sums, then a tree).
+__global__ __launch_bounds__(kSumThreads) void ordered_sum_kernel(const double *__restrict__ v,
+                                                                  int n, double scale,
+                                                                  double *__restrict__ out) {
+    __shared__ double s[kSumThreads];
+    const int tid = threadIdx.x;
+    double a = 0.0;
+    for (int i = tid; i < n; i += kSumThreads) a += v[i];
+    s[tid] = a;
+    __syncthreads();
+    for (int o = kSumThreads / 2; o > 0; o >>= 1) {
+        if (tid < o) s[tid] += s[tid + o];
+        __syncthreads();
+    }
+    if (tid == 0) *out = s[0] * scale;
+}
+
+// ---- El Karoui fit ---------------------------------------------------------
+
+struct cplx {
+    double re, im;
+};
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ __forceinline__ cplx crecip(cplx a) {
+    const double n = a.re * a.re + a.im * a.im;
+    return {a.re / n, -a.im / n};
+}
+
+// Per grid point: a = 1 - c - c z g, h_k = 1/(t_k a - z),
+// r = g - sum w_k h_k, dr/dg = 1 - c z sum w_k t_k h_k^2,
+// dr/dz = -sum w_k h_k^2 (1 + c t_k g).
+__global__ __launch_bounds__(256) void ek_point_kernel(
+    const double *__restrict__ x, const double *__restrict__ g_re, const double *__restrict__ g_im,
+    int m, double eta, double c, const double *__restrict__ t, const double *__restrict__ w, int k,
+    double *__restrict__ a_out, double *__restrict__ r_out, double *__restrict__ sq_out,
+    double *__restrict__ eta_out, double *__restrict__ grad_g_re,
+    double *__restrict__ grad_g_im) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const cplx z = {x[i], eta};
+    const cplx g = {g_re[i], g_im[i]};
+    const cplx zg = cmul(z, g);
+    const cplx a = {1.0 - c - c * zg.re, -c * zg.im};
+    cplx s = {0.0, 0.0}, sd = {0.0, 0.0}, sz = {0.0, 0.0};
+    for (int q = 0; q < k; ++q) {
+        const double tq = t[q], wq = w[q];
+        const cplx h = crecip({tq * a.re - z.re, tq * a.im - z.im});
+        const cplx h2 = cmul(h, h);
+        s.re += wq * h.re;
+        s.im += wq * h.im;
+        sd.re += wq * tq * h2.re;
+        sd.im += wq * tq * h2.im;
+        const cplx f = cmul(h2, {1.0 + c * tq * g.re, c * tq * g.im});
+        sz.re += wq * f.re;
+        sz.im += wq * f.im;
+    }
+    const cplx r = {g.re - s.re, g.im - s.im};
+    const cplx zsd = cmul(z, sd);
+    const cplx jg = {1.0 - c * zsd.re, -c * zsd.im};
+    const double two_m = 2.0 / (double)m;
+    // (2/M) r conj(jg)
+    if (grad_g_re) grad_g_re[i] = two_m * (r.re * jg.re + r.im * jg.im);
+    if (grad_g_im) grad_g_im[i] = two_m * (r.im * jg.re - r.re * jg.im);
+    // (2/M) Re(conj(r) i (-sz)) = (2/M) Im(conj(r) sz)
+    eta_out[i] = two_m * (r.re * sz.im - r.im * sz.re);
+    sq_out[i] = r.re * r.re + r.im * r.im;
+    a_out[2 * i] = a.re;
+    a_out[2 * i + 1] = a.im;
+    r_out[2 * i] = r.re;
+    r_out[2 * i + 1] = r.im;
+}
+
+// Per atom: dL/dw_k = -(2/M) sum_m Re(conj(r_m) h_mk).
+__global__ __launch_bounds__(256) void ek_weight_kernel(const double *__restrict__ x, int m,
+                                                        double eta, const double *__restrict__ t,
+                                                        int k, const double *__restrict__ a_in,
+                                                        const double *__restrict__ r_in,
+                                                        double *__restrict__ grad_w) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= k) return;
+    const double tq = t[q];
+    double acc = 0.0;
+    for (int i = 0; i < m; ++i) {
+        const cplx h = crecip({tq * a_in[2 * i] - x[i], tq * a_in[2 * i + 1] - eta});
+        acc += r_in[2 * i] * h.re + r_in[2 * i + 1] * h.im;
+    }
+    grad_w[q] = -(2.0 / (double)m) * acc;
+}
+
+__global__ void zero_kernel(double *__restrict__ p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+}  // namespace
+
+int zero_fill(double *p, size_t n, hipStream_t stream) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, p, n);
+    SHK_HIP(hipGetLastError());
+    return 0;
+}
+
+int exact_adjoint(const double *q, size_t nq, const double *ev, size_t p, double eta,
+                  const double *gre, const double *gim, int f32c, double *grad_ev,
+                  double *grad_q, double *grad_eta, hipStream_t stream) {
+    if (nq == 0 || p == 0) return 0;
+    if (nq > (size_t)INT32_MAX || p > (size_t)INT32_MAX) {
+        set_error("adjoint kernel: sizes above 2^31-1 are not supported");
+        return 6;
+    }
+    if (grad_ev) {
+        int tiles, ns, per;
+        plan(p, nq, &tiles, &ns, &per);
+        double *part = nullptr;
+        SHK_HIP(hipMallocAsync((void **)&part, sizeof(double) * (size_t)ns * p, stream));
+        ProfileMark mk = profile_begin(stream);
+        if (f32c)
+            hipLaunchKernelGGL(adj_src_f32c_kernel, dim3(tiles, ns), dim3(kAWG), 0, stream, q,
+                               (int)nq, gre, gim, ev, (int)p, eta, per, part);
+        else
+            hipLaunchKernelGGL(adj_src_f64_kernel, dim3(tiles, ns), dim3(kAWG), 0, stream, q,
+                               (int)nq, gre, gim, ev, (int)p, eta, per, part);
+        profile_end(mk, stream);
+        hipError_t le = hipGetLastError();
+        if (le == hipSuccess) {
+            hipLaunchKernelGGL(adj_src_reduce_kernel, dim3((unsigned)((p + 255) / 256)), dim3(256),
+                               0, stream, part, ns, (int)p, grad_ev);
+            le = hipGetLastError();
+        }
+        const hipError_t fe = hipFreeAsync(part, stream);
+        if (le != hipSuccess) {
+            set_error(std::string("adjoint (eigenvalue sweep): ") + hipGetErrorString(le));
+            return 7;
+        }
+        SHK_HIP(fe);
+    }
+    if (grad_q || grad_eta) {
+        int tiles, ns, per;
+        plan(nq, p, &tiles, &ns, &per);
+        double *part = nullptr;
+        SHK_HIP(hipMallocAsync((void **)&part, sizeof(double) * (2 * (size_t)ns * nq + nq),
+                               stream));
+        double *eta_term = part + 2 * (size_t)ns * nq;
+        ProfileMark mk = profile_begin(stream);
+        if (f32c)
+            hipLaunchKernelGGL(adj_qry_f32c_kernel, dim3(tiles, ns), dim3(kAWG), 0, stream, q,
+                               (int)nq, ev, (int)p, eta, per, part);
+        else
+            hipLaunchKernelGGL(adj_qry_f64_kernel, dim3(tiles, ns), dim3(kAWG), 0, stream, q,
+                               (int)nq, ev, (int)p, eta, per, part);
+        profile_end(mk, stream);
+        hipError_t le = hipGetLastError();
+        if (le == hipSuccess) {
+            hipLaunchKernelGGL(adj_qry_reduce_kernel, dim3((unsigned)((nq + 255) / 256)),
+                               dim3(256), 0, stream, part, ns, (int)nq, eta, gre, gim, grad_q,
+                               eta_term);
+            le = hipGetLastError();
+        }
+        if (le == hipSuccess && grad_eta) {
+            hipLaunchKernelGGL(ordered_sum_kernel, dim3(1), dim3(kSumThreads), 0, stream,
+                               eta_term, (int)nq, 1.0, grad_eta);
+            le = hipGetLastError();
+        }
+        const hipError_t fe = hipFreeAsync(part, stream);
+        if (le != hipSuccess) {
+            set_error(std::string("adjoint (query sweep): ") + hipGetErrorString(le));
+            return 7;
+        }
+        SHK_HIP(fe);
+    }
+    return 0;
+}
+
+int ek_fit(const double *x, const double *g_re, const double *g_im, size_t m, double eta,
+           double c, const double *t, const double *w, size_t k, double *loss,
+           double *grad_g_re, double *grad_g_im, double *grad_w, double *grad_eta,
+           hipStream_t stream) {
+    if (m == 0 || k == 0) {
+        set_error("El Karoui fit: needs at least one grid point and one atom");
+        return 1;
+    }
+    if (m > (size_t)INT32_MAX / 2 || k > (size_t)INT32_MAX) {
+        set_error("El Karoui fit: sizes above 2^30 are not supported");
+        return 6;
+    }
+    double *scr = nullptr;
+    SHK_HIP(hipMallocAsync((void **)&scr, sizeof(double) * 6 * m, stream));
+    double *a_buf = scr, *r_buf = scr + 2 * m, *sq = scr + 4 * m, *et = scr + 5 * m;
+    const unsigned bm = (unsigned)((m + 255) / 256);
+    hipLaunchKernelGGL(ek_point_kernel, dim3(bm), dim3(256), 0, stream, x, g_re, g_im, (int)m,
+                       eta, c, t, w, (int)k, a_buf, r_buf, sq, et, grad_g_re, grad_g_im);
+    hipError_t le = hipGetLastError();
+    if (le == hipSuccess && grad_w) {
+        hipLaunchKernelGGL(ek_weight_kernel, dim3((unsigned)((k + 255) / 256)), dim3(256), 0,
+                           stream, x, (int)m, eta, t, (int)k, a_buf, r_buf, grad_w);
+        le = hipGetLastError();
+    }
+    if (le == hipSuccess && loss) {
+        hipLaunchKernelGGL(ordered_sum_kernel, dim3(1), dim3(kSumThreads), 0, stream, sq, (int)m,
+                           1.0 / (double)m, loss);
+        le = hipGetLastError();
+    }
+    if (le == hipSuccess && grad_eta) {
+        hipLaunchKernelGGL(ordered_sum_kernel, dim3(1), dim3(kSumThreads), 0, stream, et, (int)m,
+                           1.0, grad_eta);
+        le = hipGetLastError();
+    }
+    const hipError_t fe = hipFreeAsync(scr, stream);
+    if (le != hipSuccess) {
+        set_error(std::string("El Karoui fit kernels: ") + hipGetErrorString(le));
+        return 7;
+    }
+    SHK_HIP(fe);
+    return 0;
+}
+
+}  // namespace shk

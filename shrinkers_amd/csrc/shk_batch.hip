@@ -16,7 +16,7 @@
 // The counter is reset by a one-thread kernel inside the graph (no memset
 // node).  An autotuner times the (R, workers per CU) candidates on the
 // actual batch and keeps the fastest.
-#include "shk_f32c.hpp"
+#include "shk_pair.hpp"
 #include "shk_internal.hpp"
 #include "shrinkers_stieltjes.h"
 

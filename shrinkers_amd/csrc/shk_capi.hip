@@ -577,6 +577,70 @@ shk_status shk_at_points_dev(const double *d_query, size_t nq, const double *d_e
                                     d_out_re, d_out_im, (hipStream_t)stream);
 }
 
+shk_status shk_exact_at_points_dev(const double *d_query, size_t nq, const double *d_eigenvalues,
+                                   size_t p, double eta, int precision, double *d_out_re,
+                                   double *d_out_im, void *stream) {
+    if (nq == 0 || p == 0) return SHK_OK;
+    if (!d_query || !d_eigenvalues || !d_out_re || !d_out_im)
+        return fail(SHK_ERR_BAD_ARG, "NULL pointer");
+    SHK_CHECK(check_eta(eta));
+    if (precision != SHK_PREC_F64 && precision != SHK_PREC_F32C)
+        return fail(SHK_ERR_BAD_ARG, "precision must be SHK_PREC_F64 or SHK_PREC_F32C");
+    SHK_CHECK(ensure_device());
+    hipStream_t s = (hipStream_t)stream;
+    if (precision == SHK_PREC_F32C)
+        return (shk_status)exact_f32c(d_query, nq, d_eigenvalues, p, eta, 1.0, d_out_re, d_out_im,
+                                      0, s);
+    double *outs[2] = {d_out_re, d_out_im};
+    return (shk_status)exact_f64(d_query, nq, d_eigenvalues, p, eta, EXACT_PLAIN, 0.0, 1.0, outs,
+                                 s);
+}
+
+shk_status shk_exact_adjoint_dev(const double *d_query, size_t nq, const double *d_eigenvalues,
+                                 size_t p, double eta, const double *d_grad_re,
+                                 const double *d_grad_im, int precision,
+                                 double *d_grad_eigenvalues, double *d_grad_query,
+                                 double *d_grad_eta, void *stream) {
+    if (!d_grad_eigenvalues && !d_grad_query && !d_grad_eta) return SHK_OK;
+    if (nq == 0 || p == 0) {
+        // empty sums: every gradient is zero (a fill kernel, not a memset, so
+        // the call can be captured in a graph)
+        SHK_CHECK(ensure_device());
+        hipStream_t s = (hipStream_t)stream;
+        if (d_grad_eigenvalues) SHK_CHECK(zero_fill(d_grad_eigenvalues, p, s));
+        if (d_grad_query) SHK_CHECK(zero_fill(d_grad_query, nq, s));
+        if (d_grad_eta) SHK_CHECK(zero_fill(d_grad_eta, 1, s));
+        return SHK_OK;
+    }
+    if (!d_query || !d_eigenvalues || !d_grad_re || !d_grad_im)
+        return fail(SHK_ERR_BAD_ARG, "NULL pointer");
+    SHK_CHECK(check_eta(eta));
+    if (precision != SHK_PREC_F64 && precision != SHK_PREC_F32C)
+        return fail(SHK_ERR_BAD_ARG, "precision must be SHK_PREC_F64 or SHK_PREC_F32C");
+    SHK_CHECK(ensure_device());
+    return (shk_status)exact_adjoint(d_query, nq, d_eigenvalues, p, eta, d_grad_re, d_grad_im,
+                                     precision == SHK_PREC_F32C, d_grad_eigenvalues, d_grad_query,
+                                     d_grad_eta, (hipStream_t)stream);
+}
+
+shk_status shk_el_karoui_fit_dev(const double *d_x, const double *d_g_re, const double *d_g_im,
+                                 size_t m, double eta, double c, const double *d_atoms,
+                                 const double *d_weights, size_t k, double *d_loss,
+                                 double *d_grad_g_re, double *d_grad_g_im,
+                                 double *d_grad_weights, double *d_grad_eta, void *stream) {
+    if (m == 0) return fail(SHK_ERR_EMPTY, "the fit needs at least one grid point");
+    if (k == 0) return fail(SHK_ERR_EMPTY, "the fit needs at least one atom");
+    if (!d_x || !d_g_re || !d_g_im || !d_atoms || !d_weights)
+        return fail(SHK_ERR_BAD_ARG, "NULL pointer");
+    SHK_CHECK(check_eta(eta));
+    if (!std::isfinite(c) || !(c > 0.0))
+        return fail(SHK_ERR_BAD_ARG, "c must be a positive finite number, got " + std::to_string(c));
+    SHK_CHECK(ensure_device());
+    return (shk_status)ek_fit(d_x, d_g_re, d_g_im, m, eta, c, d_atoms, d_weights, k, d_loss,
+                              d_grad_g_re, d_grad_g_im, d_grad_weights, d_grad_eta,
+                              (hipStream_t)stream);
+}
+
 shk_status shk_chebcode_build(const double *eigenvalues, size_t p, double theta, int n,
                               size_t leaf_cap, int f32_far, shk_chebcode **out) {
     if (!out) return fail(SHK_ERR_BAD_ARG, "NULL handle pointer");

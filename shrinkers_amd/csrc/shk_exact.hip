@@ -32,7 +32,7 @@
 // v_rcp_f32 (1 ulp).  Partial sums run over kF32Run sources in FP32 and are
 // folded into FP64 accumulators; the cross-split reduction is FP64.
 #include "shk_internal.hpp"
-#include "shk_f32c.hpp"
+#include "shk_pair.hpp"
 
 #include <algorithm>
 #include <vector>
@@ -61,26 +61,6 @@ constexpr int kR = SHK_EXACT_R;            // queries per thread
 constexpr int kQT = kWG * kR;              // queries per tile
 constexpr int kPiece = SHK_EXACT_PIECE;    // sources staged in LDS per pass
 constexpr int kTargetWG = SHK_EXACT_TARGET_WG;  // workgroups to aim for
-
-// 1/x to ~1 ulp from the v_rcp_f64 seed r0.  With e = 1 - x*r0,
-// 1/x = r0 (1 + e + e^2 + ...): the default takes the cubic correction
-// r0 + r0*(e + e^2) (relative error e^3, three FMAs); the alternative takes
-// two Newton-Raphson steps r <- r + r*(1 - x*r) (error e^4, four FMAs).
-#ifndef SHK_F64_RCP_NEWTON2
-__device__ __forceinline__ double rcp_f64(double x) {
-    const double r = __builtin_amdgcn_rcp(x);
-    const double e = __builtin_fma(-x, r, 1.0);
-    return __builtin_fma(r, __builtin_fma(e, e, e), r);
-}
-#else
-__device__ __forceinline__ double rcp_f64(double x) {
-    double r = __builtin_amdgcn_rcp(x);
-    double e = __builtin_fma(-x, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    e = __builtin_fma(-x, r, 1.0);
-    return __builtin_fma(r, e, r);
-}
-#endif
 
 template <int KIND>
 struct NAcc {

@@ -71,6 +71,28 @@ int exact_f32c(const double *q, size_t nq, const double *src, size_t p, double e
 // float -> double widening on device (for the f32 entry points).
 int widen_f32(const float *in, double *out, size_t n, hipStream_t stream);
 
+// ---- reverse mode (shk_adjoint.hip) -----------------------------------------
+
+// Gradients of a real loss of the raw at-points sums, given the upstream
+// gradients gre/gim (w.r.t. Re S_m, Im S_m).  Outputs: grad_ev (p),
+// grad_q (nq), grad_eta (one device double); any may be NULL.  f32c != 0
+// selects the FP32-compensated pair arithmetic.
+int exact_adjoint(const double *q, size_t nq, const double *ev, size_t p, double eta,
+                  const double *gre, const double *gim, int f32c, double *grad_ev,
+                  double *grad_q, double *grad_eta, hipStream_t stream);
+
+// Writes n zeros with a kernel (graph-capturable); n = 0 is a no-op.
+int zero_fill(double *p, size_t n, hipStream_t stream);
+
+// El Karoui fit: loss = mean_m |g_m - sum_k w_k / (t_k a_m - z_m)|^2 with
+// a_m = 1 - c - c z_m g_m, z_m = x_m + i eta, and its gradients with respect
+// to (Re g, Im g), the weights and eta (through z only).  Device pointers;
+// every output may be NULL.
+int ek_fit(const double *x, const double *g_re, const double *g_im, size_t m, double eta,
+           double c, const double *t, const double *w, size_t k, double *loss,
+           double *grad_g_re, double *grad_g_im, double *grad_w, double *grad_eta,
+           hipStream_t stream);
+
 // ---- ChebCode (shk_chebcode.hip) --------------------------------------------
 
 struct ChebTree;  // device-resident tree

@@ -1,5 +1,5 @@
-// FP32-compensated pair arithmetic shared by the exact kernels
-// (shk_exact.hip) and the batched launch (shk_batch.hip).
+// Pair arithmetic shared by the exact, adjoint and batched kernels: the FP64
+// reciprocal and the FP32-compensated pair forms.
 //
 // Eigenvalues are re-expressed relative to a tile centre c and split into
 // f32 hi/lo parts; FORM selects how much of the residual enters the pair
@@ -10,6 +10,26 @@
 #include <hip/hip_runtime.h>
 
 namespace shk {
+
+// 1/x to ~1 ulp from the v_rcp_f64 seed r0.  With e = 1 - x*r0,
+// 1/x = r0 (1 + e + e^2 + ...): the default takes the cubic correction
+// r0 + r0*(e + e^2) (relative error e^3, three FMAs); the alternative takes
+// two Newton-Raphson steps r <- r + r*(1 - x*r) (error e^4, four FMAs).
+#ifndef SHK_F64_RCP_NEWTON2
+__device__ __forceinline__ double rcp_f64(double x) {
+    const double r = __builtin_amdgcn_rcp(x);
+    const double e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, __builtin_fma(e, e, e), r);
+}
+#else
+__device__ __forceinline__ double rcp_f64(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+#endif
 
 constexpr int kF32Run = 32;  // f32 terms per FP64 fold
 
