@@ -95,6 +95,31 @@ def cpu_baseline(ev: np.ndarray, eta: float, budget_s: float):
     for x in sample:
         oracle.autovec_sum(x, ev, eta)
     t1 = time.perf_counter() - t0
+
+    def median_time(fn, reps):
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts))
+
+    extra = {}
+    # sequential symmetric sweep (cacheblock.rs:841-954), one core: each unordered pair once
+    t_sym = median_time(lambda: oracle.symmetric_sweep(ev, eta), 1)
+    extra["symmetric_sweep_1core"] = {"ms_per_transform": t_sym * 1e3,
+                                      "pair_evals_per_s": p * p / t_sym}
+    # ChebCode FAST build + traversal (chebcode.rs:509-828), all cores
+    oracle.chebcode_preset(ev, eta, "chebcode_fast", threads)
+    t_cf = median_time(lambda: oracle.chebcode_preset(ev, eta, "chebcode_fast", threads), 7)
+    extra["chebcode_fast_all_cores"] = {"ms_per_transform": t_cf * 1e3}
+    # the reference's recorded benchmark spectrum (benches/support/mod.rs:65-82) at p = 50 000
+    hs = oracle.harness_spectrum(p)
+    he = 1.0 / np.sqrt(p)
+    oracle.tiled_parallel(hs, he, threads)
+    t_hs = median_time(lambda: oracle.tiled_parallel(hs, he, threads), 3)
+    extra["harness_spectrum_tiled_all_cores"] = {"ms_per_transform": t_hs * 1e3,
+                                                 "pair_evals_per_s": p * p / t_hs}
     return out, {
         "value": p * p / med,
         "unit": "pair-evals/s",
@@ -106,7 +131,36 @@ def cpu_baseline(ev: np.ndarray, eta: float, budget_s: float):
         "single_core": {"value": rows * p / t1, "unit": "pair-evals/s",
                         "sample": f"{rows} rows x {p} sources, 1 thread"},
         "cpu_model": _cpu_model(),
+        "extra": extra,
     }
+
+
+def box_peaks() -> dict:
+    """CU count and peak engine clock of the gfx950 agent from rocminfo, and
+    the vector peaks they imply (64 lanes per CU per clock, FMA = 2 FLOPs;
+    packed FP32 doubles the FP32 figure)."""
+    import subprocess
+    try:
+        txt = subprocess.run(["rocminfo"], capture_output=True, text=True, timeout=60).stdout
+    except (OSError, subprocess.SubprocessError):
+        return {}
+    cu = mhz = None
+    for block in txt.split("Agent ")[1:]:
+        if "gfx950" not in block:
+            continue
+        for line in block.splitlines():
+            s = line.strip()
+            if s.startswith("Compute Unit:"):
+                cu = int(s.split(":")[1])
+            elif s.startswith("Max Clock Freq. (MHz):"):
+                mhz = int(s.split(":")[1])
+        break
+    if not cu or not mhz:
+        return {}
+    fma_tflops = cu * 64 * 2 * mhz * 1e6 / 1e12
+    return {"compute_units": cu, "max_clock_mhz": mhz,
+            "fp32_vector_tflops": 2 * fma_tflops, "fp32_vector_nonpacked_tflops": fma_tflops,
+            "fp64_vector_tflops": fma_tflops}
 
 
 def chebcode_leg(shk, sync) -> dict:
@@ -128,7 +182,7 @@ def chebcode_leg(shk, sync) -> dict:
                                             ("chebcode", (0.5, 11, 32, False)),
                                             ("chebcode_n12", (0.5, 12, 32, False))):
             ts = []
-            for _ in range(3):
+            for _ in range(7):
                 sync()
                 t0 = time.perf_counter()
                 b = shk.ChebCodeBatch.build_dev(d, theta, n, leaf, f32)
@@ -258,6 +312,9 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU timing")
     ap.add_argument("--quick", action="store_true",
                     help="only the headline workload (no ChebCode / grid / reverse-mode legs)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
+                    help="first copy an MB-sized buffer on the device 5 times (a plain copy "
+                         "kernel of known size, to calibrate HBM byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -278,6 +335,15 @@ def main() -> int:
     out32 = (torch.empty_like(d_ev), torch.empty_like(d_ev))
     out64 = (torch.empty_like(d_ev), torch.empty_like(d_ev))
     sync = torch.cuda.synchronize
+    if args.copy_probe > 0:
+        n = args.copy_probe * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        sync()
+        log(f"copy probe: 5 x {args.copy_probe} MiB device copies")
+        del src, dst
 
     def step32():
         shk.all_points_dev(d_ev, eta, "blocked", precision="f32c", scaled=True, out=out32)
@@ -355,6 +421,7 @@ def main() -> int:
             "kernel": "exact_f32c_kernel",
             "kernel_avg_ms": k32,
             "flops_per_launch": FLOPS_PER_PAIR * pairs,
+            "peak_box": box_peaks(),
         },
         "cpu_baseline": cpu,
         "fp64_exact": {

@@ -27,6 +27,11 @@
  *
  * Thread safety: calls on distinct streams may run concurrently; scratch
  * memory is stream-ordered (hipMallocAsync) per call.
+ *
+ * Memory: scratch buffers and ChebCode trees come from the device's default
+ * stream-ordered memory pool.  On first use the library raises that pool's
+ * release threshold so freed blocks stay in the pool for reuse (the pool
+ * then holds as much as the largest calls needed until the process ends).
  */
 #ifndef SHRINKERS_STIELTJES_H
 #define SHRINKERS_STIELTJES_H

@@ -25,8 +25,8 @@
 //
 // The El Karoui fit (no reference counterpart; the reference inverts the MP
 // equation pointwise, src/deconvolution/mod.rs:116-259) is O(M K) FP64 work:
-// one thread per grid point forms a = 1 - c z g, the residual
-// r = g - sum_k w_k / (t_k a - z) and its gradients; one thread per atom
+// one wavefront per grid point forms a = 1 - c z g, the residual
+// r = g - sum_k w_k / (t_k a - z) and its gradients; one wavefront per atom
 // forms dL/dw_k; a one-workgroup tree sum gives the loss and the direct eta
 // term.  All sums run in a fixed order, so results are bitwise reproducible.
 #include "shk_internal.hpp"
@@ -41,7 +41,7 @@ constexpr int kAWG = 64;           // one wavefront per workgroup
 constexpr int kAR = 2;             // owned outputs per lane
 constexpr int kAT = kAWG * kAR;    // owned outputs per tile
 constexpr int kAPiece = 512;       // swept values staged in LDS per pass
-constexpr int kATargetWG = 32768;  // workgroups to aim for
+constexpr int kATargetWG = 16384;  // workgroups to aim for
 constexpr int kSumThreads = 256;
 
 void plan(size_t n_own, size_t n_sweep, int *tiles, int *n_split, int *per) {
@@ -437,27 +437,35 @@ __device__ __forceinline__ cplx cmul(cplx a, cplx b) {
     return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
 }
 __device__ __forceinline__ cplx crecip(cplx a) {
-    const double n = a.re * a.re + a.im * a.im;
-    return {a.re / n, -a.im / n};
+    const double inv = 1.0 / (a.re * a.re + a.im * a.im);
+    return {a.re * inv, -a.im * inv};
 }
 
-// Per grid point: a = 1 - c - c z g, h_k = 1/(t_k a - z),
-// r = g - sum w_k h_k, dr/dg = 1 - c z sum w_k t_k h_k^2,
-// dr/dz = -sum w_k h_k^2 (1 + c t_k g).
-__global__ __launch_bounds__(256) void ek_point_kernel(
+// Fixed-order sum over the 64 lanes of a wavefront (butterfly; every lane
+// ends with the total, and the order is the same on every run).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// One wavefront per grid point; lane l takes atoms l, l+64, ...:
+// a = 1 - c - c z g, h_k = 1/(t_k a - z), r = g - sum w_k h_k,
+// dr/dg = 1 - c z sum w_k t_k h_k^2, dr/dz = -sum w_k h_k^2 (1 + c t_k g).
+__global__ __launch_bounds__(64) void ek_point_kernel(
     const double *__restrict__ x, const double *__restrict__ g_re, const double *__restrict__ g_im,
     int m, double eta, double c, const double *__restrict__ t, const double *__restrict__ w, int k,
     double *__restrict__ a_out, double *__restrict__ r_out, double *__restrict__ sq_out,
     double *__restrict__ eta_out, double *__restrict__ grad_g_re,
     double *__restrict__ grad_g_im) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
     const cplx z = {x[i], eta};
     const cplx g = {g_re[i], g_im[i]};
     const cplx zg = cmul(z, g);
     const cplx a = {1.0 - c - c * zg.re, -c * zg.im};
     cplx s = {0.0, 0.0}, sd = {0.0, 0.0}, sz = {0.0, 0.0};
-    for (int q = 0; q < k; ++q) {
+    for (int q = lane; q < k; q += 64) {
         const double tq = t[q], wq = w[q];
         const cplx h = crecip({tq * a.re - z.re, tq * a.im - z.im});
         const cplx h2 = cmul(h, h);
@@ -469,6 +477,10 @@ __global__ __launch_bounds__(256) void ek_point_kernel(
         sz.re += wq * f.re;
         sz.im += wq * f.im;
     }
+    s = {wave_sum(s.re), wave_sum(s.im)};
+    sd = {wave_sum(sd.re), wave_sum(sd.im)};
+    sz = {wave_sum(sz.re), wave_sum(sz.im)};
+    if (lane != 0) return;
     const cplx r = {g.re - s.re, g.im - s.im};
     const cplx zsd = cmul(z, sd);
     const cplx jg = {1.0 - c * zsd.re, -c * zsd.im};
@@ -485,21 +497,23 @@ __global__ __launch_bounds__(256) void ek_point_kernel(
     r_out[2 * i + 1] = r.im;
 }
 
-// Per atom: dL/dw_k = -(2/M) sum_m Re(conj(r_m) h_mk).
-__global__ __launch_bounds__(256) void ek_weight_kernel(const double *__restrict__ x, int m,
-                                                        double eta, const double *__restrict__ t,
-                                                        int k, const double *__restrict__ a_in,
-                                                        const double *__restrict__ r_in,
-                                                        double *__restrict__ grad_w) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= k) return;
+// One wavefront per atom; lane l takes grid points l, l+64, ...:
+// dL/dw_k = -(2/M) sum_m Re(conj(r_m) h_mk).
+__global__ __launch_bounds__(64) void ek_weight_kernel(const double *__restrict__ x, int m,
+                                                       double eta, const double *__restrict__ t,
+                                                       const double *__restrict__ a_in,
+                                                       const double *__restrict__ r_in,
+                                                       double *__restrict__ grad_w) {
+    const int q = blockIdx.x;
+    const int lane = threadIdx.x;
     const double tq = t[q];
     double acc = 0.0;
-    for (int i = 0; i < m; ++i) {
+    for (int i = lane; i < m; i += 64) {
         const cplx h = crecip({tq * a_in[2 * i] - x[i], tq * a_in[2 * i + 1] - eta});
         acc += r_in[2 * i] * h.re + r_in[2 * i + 1] * h.im;
     }
-    grad_w[q] = -(2.0 / (double)m) * acc;
+    acc = wave_sum(acc);
+    if (lane == 0) grad_w[q] = -(2.0 / (double)m) * acc;
 }
 
 __global__ void zero_kernel(double *__restrict__ p, size_t n) {
@@ -595,20 +609,19 @@ int ek_fit(const double *x, const double *g_re, const double *g_im, size_t m, do
         set_error("El Karoui fit: needs at least one grid point and one atom");
         return 1;
     }
-    if (m > (size_t)INT32_MAX / 2 || k > (size_t)INT32_MAX) {
-        set_error("El Karoui fit: sizes above 2^30 are not supported");
+    if (m > ((size_t)1 << 24) || k > ((size_t)1 << 24)) {
+        set_error("El Karoui fit: more than 2^24 grid points or atoms is not supported");
         return 6;
     }
     double *scr = nullptr;
     SHK_HIP(hipMallocAsync((void **)&scr, sizeof(double) * 6 * m, stream));
     double *a_buf = scr, *r_buf = scr + 2 * m, *sq = scr + 4 * m, *et = scr + 5 * m;
-    const unsigned bm = (unsigned)((m + 255) / 256);
-    hipLaunchKernelGGL(ek_point_kernel, dim3(bm), dim3(256), 0, stream, x, g_re, g_im, (int)m,
-                       eta, c, t, w, (int)k, a_buf, r_buf, sq, et, grad_g_re, grad_g_im);
+    hipLaunchKernelGGL(ek_point_kernel, dim3((unsigned)m), dim3(64), 0, stream, x, g_re, g_im,
+                       (int)m, eta, c, t, w, (int)k, a_buf, r_buf, sq, et, grad_g_re, grad_g_im);
     hipError_t le = hipGetLastError();
     if (le == hipSuccess && grad_w) {
-        hipLaunchKernelGGL(ek_weight_kernel, dim3((unsigned)((k + 255) / 256)), dim3(256), 0,
-                           stream, x, (int)m, eta, t, (int)k, a_buf, r_buf, grad_w);
+        hipLaunchKernelGGL(ek_weight_kernel, dim3((unsigned)k), dim3(64), 0, stream, x, (int)m,
+                           eta, t, a_buf, r_buf, grad_w);
         le = hipGetLastError();
     }
     if (le == hipSuccess && loss) {

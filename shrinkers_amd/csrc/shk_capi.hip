@@ -223,6 +223,15 @@ shk_status ensure_device() {
             state = SHK_ERR_NO_DEVICE;
             msg = std::string("libshrinkers_hip is built for gfx950, device is ") +
                   prop.gcnArchName;
+            return;
+        }
+        // Scratch and trees come from the default stream-ordered pool; keep
+        // freed blocks in the pool for reuse instead of unmapping them at
+        // every synchronisation.
+        hipMemPool_t pool = nullptr;
+        if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess && pool) {
+            uint64_t keep = UINT64_MAX;
+            (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
         }
     });
     if (state != SHK_OK) set_error(msg);

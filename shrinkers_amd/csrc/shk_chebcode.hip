@@ -403,6 +403,19 @@ __device__ __forceinline__ void leaf_f64(const double *__restrict__ xs, int len,
                                          double eta2, double &lr, double &li) {
     double ar0 = 0.0, ar1 = 0.0, ai0 = 0.0, ai1 = 0.0;
     int k = 0;
+    // even sources to (ar0, ai0), odd to (ar1, ai1), eight per step
+    for (; k + 8 <= len; k += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; u += 2) {
+            const double d0 = x - xs[k + u], d1 = x - xs[k + u + 1];
+            const double i0 = rcp_f64(__builtin_fma(d0, d0, eta2));
+            const double i1 = rcp_f64(__builtin_fma(d1, d1, eta2));
+            ar0 = __builtin_fma(d0, i0, ar0);
+            ar1 = __builtin_fma(d1, i1, ar1);
+            ai0 += i0;
+            ai1 += i1;
+        }
+    }
     for (; k + 2 <= len; k += 2) {
         const double d0 = x - xs[k], d1 = x - xs[k + 1];
         const double i0 = rcp_f64(__builtin_fma(d0, d0, eta2));
@@ -422,31 +435,56 @@ __device__ __forceinline__ void leaf_f64(const double *__restrict__ xs, int len,
     }
 }
 
+// Traversal, one query per lane.  Every lane follows the reference's stack
+// DFS in pre-order (a pruned or finished node continues at its escape index,
+// a descended node at node + 1).  Two walks give bitwise the same per-lane
+// results:
+//
+// kWave = true: the wavefront walks together with one node pointer.  Lane L
+// is active at node u unless it accepted an ancestor of u, recorded as
+// skip_L = escape(ancestor) > u.  At an internal node the active lanes that
+// accept evaluate the panel and set skip; the wavefront descends if any
+// active lane rejects, and otherwise jumps to the escape index (every
+// inactive lane's skip is at least that far).  Each lane processes exactly
+// the nodes of its own traversal, in the same order; node records, panels and
+// leaf sources are read at wavefront-uniform addresses.  This suits dense
+// sorted queries, whose neighbours share their near fields.
+//
+// kWave = false: each lane walks on its own (sparse queries, such as a grid
+// spanning the spectrum, whose near fields differ).
+template <bool kWave>
 __global__ __launch_bounds__(256) void eval_f64_kernel(EvalView v, const double *__restrict__ query,
                                                        int nq, int queries_are_sources, double eta,
                                                        double scale, double *__restrict__ out_re,
                                                        double *__restrict__ out_im) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nq) return;
-    const double x = queries_are_sources ? v.sorted[i] : query[i];
+    const bool valid = i < nq;
+    if (!kWave && !valid) return;
+    const double x = valid ? (queries_are_sources ? v.sorted[i] : query[i]) : 0.0;
     const double eta2 = eta * eta;
     const int n = v.n;
     double re = 0.0, im = 0.0;
+    int skip = valid ? 0 : v.N;
     int node = 0;
     while (node < v.N) {
+        if constexpr (kWave) node = __builtin_amdgcn_readfirstlane(node);
         const int4 lk = v.link[node];  // right, esc, lo_idx, hi_idx
+        const bool active = !kWave || node >= skip;
         if (lk.x < 0) {
-            double lr, li;
-            leaf_f64(v.sorted + lk.z, lk.w - lk.z, x, eta2, lr, li);
-            re += lr;
-            im += eta * li;
+            if (active) {
+                double lr, li;
+                leaf_f64(v.sorted + lk.z, lk.w - lk.z, x, eta2, lr, li);
+                re += lr;
+                im += eta * li;
+            }
             node = lk.y;
             continue;
         }
         const double lo = v.lo[node], hi = v.hi[node];
         const double cl = fmax(fmax(lo - x, x - hi), 0.0);
         const double d_sq = __builtin_fma(cl, cl, eta2);
-        if (v.acc[node] < d_sq) {
+        const bool accept = v.acc[node] < d_sq;
+        if (active && accept) {
             const double *ts = v.nodes + (size_t)node * n;
             const double *ws = v.w + (size_t)node * n;
             double afr0 = 0.0, afr1 = 0.0, afi0 = 0.0, afi1 = 0.0;
@@ -469,23 +507,28 @@ __global__ __launch_bounds__(256) void eval_f64_kernel(EvalView v, const double 
             }
             re += fr;
             im += eta * fi;
-            node = lk.y;
-        } else {
-            node = node + 1;
+            skip = lk.y;
         }
+        if constexpr (kWave)
+            node = __any(active && !accept) ? node + 1 : lk.y;
+        else
+            node = accept ? lk.y : node + 1;
     }
+    if (!valid) return;
     const int o = (queries_are_sources && v.perm) ? v.perm[i] : i;
     out_re[o] = re * scale;
     out_im[o] = im * scale;
 }
 
+template <bool kWave>
 __global__ __launch_bounds__(256) void eval_fast_kernel(EvalView v, const double *__restrict__ query,
                                                         int nq, int queries_are_sources, double eta,
                                                         double scale, double *__restrict__ out_re,
                                                         double *__restrict__ out_im) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nq) return;
-    const double xd = queries_are_sources ? v.sorted[i] : query[i];
+    const bool valid = i < nq;
+    if (!kWave && !valid) return;
+    const double xd = valid ? (queries_are_sources ? v.sorted[i] : query[i]) : 0.0;
     const float x = (float)xd;
     const float x_lo = (float)(xd - (double)x);
     const float e32 = (float)eta;
@@ -494,30 +537,49 @@ __global__ __launch_bounds__(256) void eval_fast_kernel(EvalView v, const double
     const int n = v.n;
     float far_re[4] = {0.f, 0.f, 0.f, 0.f}, far_im[4] = {0.f, 0.f, 0.f, 0.f};
     double leaf_re = 0.0, leaf_inv = 0.0;
+    int skip = valid ? 0 : v.N;
     int node = 0;
     while (node < v.N) {
+        if constexpr (kWave) node = __builtin_amdgcn_readfirstlane(node);
         const int4 lk = v.link[node];
+        const bool active = !kWave || node >= skip;
         if (lk.x < 0) {
-            float ar0 = 0.f, ar1 = 0.f, ai0 = 0.f, ai1 = 0.f;
-            int k = lk.z;
-            for (; k + 2 <= lk.w; k += 2) {
-                const float d0 = (x - v.s_hi[k]) + (x_lo - v.s_lo[k]);
-                const float d1 = (x - v.s_hi[k + 1]) + (x_lo - v.s_lo[k + 1]);
-                const float r0 = __builtin_amdgcn_rcpf(__builtin_fmaf(d0, d0, eta2n));
-                const float r1 = __builtin_amdgcn_rcpf(__builtin_fmaf(d1, d1, eta2n));
-                ar0 = __builtin_fmaf(d0, r0, ar0);
-                ar1 = __builtin_fmaf(d1, r1, ar1);
-                ai0 += r0;
-                ai1 += r1;
+            if (active) {
+                float ar0 = 0.f, ar1 = 0.f, ai0 = 0.f, ai1 = 0.f;
+                int k = lk.z;
+                // even sources to (ar0, ai0), odd to (ar1, ai1), eight per step
+                for (; k + 8 <= lk.w; k += 8) {
+#pragma unroll
+                    for (int u = 0; u < 8; u += 2) {
+                        const float d0 = (x - v.s_hi[k + u]) + (x_lo - v.s_lo[k + u]);
+                        const float d1 = (x - v.s_hi[k + u + 1]) + (x_lo - v.s_lo[k + u + 1]);
+                        const float r0 = __builtin_amdgcn_rcpf(__builtin_fmaf(d0, d0, eta2n));
+                        const float r1 = __builtin_amdgcn_rcpf(__builtin_fmaf(d1, d1, eta2n));
+                        ar0 = __builtin_fmaf(d0, r0, ar0);
+                        ar1 = __builtin_fmaf(d1, r1, ar1);
+                        ai0 += r0;
+                        ai1 += r1;
+                    }
+                }
+                for (; k + 2 <= lk.w; k += 2) {
+                    const float d0 = (x - v.s_hi[k]) + (x_lo - v.s_lo[k]);
+                    const float d1 = (x - v.s_hi[k + 1]) + (x_lo - v.s_lo[k + 1]);
+                    const float r0 = __builtin_amdgcn_rcpf(__builtin_fmaf(d0, d0, eta2n));
+                    const float r1 = __builtin_amdgcn_rcpf(__builtin_fmaf(d1, d1, eta2n));
+                    ar0 = __builtin_fmaf(d0, r0, ar0);
+                    ar1 = __builtin_fmaf(d1, r1, ar1);
+                    ai0 += r0;
+                    ai1 += r1;
+                }
+                if (k < lk.w) {
+                    const float d0 = (x - v.s_hi[k]) + (x_lo - v.s_lo[k]);
+                    const float r0 = __builtin_amdgcn_rcpf(__builtin_fmaf(d0, d0, eta2n));
+                    ar0 = __builtin_fmaf(d0, r0, ar0);
+                    ai0 += r0;
+                }
+                leaf_re += (double)(ar0 + ar1);
+                leaf_inv += (double)(ai0 + ai1);
             }
-            if (k < lk.w) {
-                const float d0 = (x - v.s_hi[k]) + (x_lo - v.s_lo[k]);
-                const float r0 = __builtin_amdgcn_rcpf(__builtin_fmaf(d0, d0, eta2n));
-                ar0 = __builtin_fmaf(d0, r0, ar0);
-                ai0 += r0;
-            }
-            leaf_re += (double)(ar0 + ar1);
-            leaf_inv += (double)(ai0 + ai1);
             node = lk.y;
             continue;
         }
@@ -525,21 +587,38 @@ __global__ __launch_bounds__(256) void eval_fast_kernel(EvalView v, const double
         const float cl = fmaxf(fmaxf(r.x - x, x - r.y), 0.0f);
         const float d_sq = __builtin_fmaf(cl, cl, eta2);
         const float outside = (cl == 0.0f) ? d_sq : kOutsideTheta2 * d_sq;
-        if (r.z < outside) {
+        const bool accept = r.z < outside;
+        if (active && accept) {
             const float *ts = v.panel32 + (size_t)node * 2 * n;
             const float *ws = ts + n;
-            for (int j = 0; j < n; ++j) {
-                const int l = j & 3;
-                const float d = x - ts[j];
-                const float q = ws[j] * __builtin_amdgcn_rcpf(__builtin_fmaf(d, d, eta2));
-                far_re[l] = __builtin_fmaf(q, d, far_re[l]);
-                far_im[l] = far_im[l] + q;
+            // term j goes to accumulator j mod 4 (static indices)
+            int j = 0;
+            for (; j + 4 <= n; j += 4) {
+#pragma unroll
+                for (int l = 0; l < 4; ++l) {
+                    const float d = x - ts[j + l];
+                    const float q = ws[j + l] * __builtin_amdgcn_rcpf(__builtin_fmaf(d, d, eta2));
+                    far_re[l] = __builtin_fmaf(q, d, far_re[l]);
+                    far_im[l] = far_im[l] + q;
+                }
             }
-            node = lk.y;
-        } else {
-            node = node + 1;
+#pragma unroll
+            for (int l = 0; l < 3; ++l) {
+                if (j + l < n) {
+                    const float d = x - ts[j + l];
+                    const float q = ws[j + l] * __builtin_amdgcn_rcpf(__builtin_fmaf(d, d, eta2));
+                    far_re[l] = __builtin_fmaf(q, d, far_re[l]);
+                    far_im[l] = far_im[l] + q;
+                }
+            }
+            skip = lk.y;
         }
+        if constexpr (kWave)
+            node = __any(active && !accept) ? node + 1 : lk.y;
+        else
+            node = accept ? lk.y : node + 1;
     }
+    if (!valid) return;
     const float hr = (far_re[0] + far_re[1]) + (far_re[2] + far_re[3]);
     const float hi = (far_im[0] + far_im[1]) + (far_im[2] + far_im[3]);
     const double re = leaf_re + (double)hr;
@@ -631,8 +710,13 @@ inline unsigned blocks_for(size_t n, int bs) { return (unsigned)((n + bs - 1) / 
 
 void cheb_destroy(ChebTree *t) {
     if (!t) return;
-    if (t->arena) (void)hipFree(t->arena);
-    if (t->arena2) (void)hipFree(t->arena2);
+    if (t->arena || t->arena2) {
+        // Stream-ordered pool memory: wait until no stream can still read the
+        // tree (the synchronisation hipFree would do), then return it.
+        (void)hipDeviceSynchronize();
+        if (t->arena) (void)hipFreeAsync(t->arena, nullptr);
+        if (t->arena2) (void)hipFreeAsync(t->arena2, nullptr);
+    }
     if (t->ready) (void)hipEventDestroy(t->ready);
     delete t;
 }
@@ -731,7 +815,7 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
     // the arena, so the arena is allocated after a first pass with the
     // sources in scratch-independent storage: allocate the source part now.
     void *src_arena = nullptr;
-    SHK_HIP(hipMalloc(&src_arena, src_bytes));
+    SHK_HIP(hipMallocAsync(&src_arena, src_bytes, stream));
     ar.base = static_cast<char *>(src_arena);
     ar.off = 0;
     t->sorted = ar.take<double>(p);
@@ -787,7 +871,7 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
     (void)nr.take<int4>(N);
     (void)nr.take<float>((size_t)N * 2 * n);
     void *node_arena = nullptr;
-    SHK_HIP(hipMalloc(&node_arena, nr.off));
+    SHK_HIP(hipMallocAsync(&node_arena, nr.off, stream));
     t->arena2 = node_arena;
     nr.base = static_cast<char *>(node_arena);
     nr.off = 0;
@@ -894,13 +978,23 @@ int cheb_eval(const ChebTree *t, const double *d_query, size_t nq, double eta, d
     }
     SHK_HIP(hipStreamWaitEvent(stream, t->ready, 0));
     EvalView v = view_of(t);
+    // Dense queries (the tree's own sources, or at least as many points as
+    // sources) share near fields within a wavefront: walk together.
+    const bool wave = queries_are_sources || nq >= t->p;
+    const dim3 grid(blocks_for(nq, 256)), block(256);
     ProfileMark m = profile_begin(stream);
-    if (t->f32_far)
-        hipLaunchKernelGGL(eval_fast_kernel, dim3(blocks_for(nq, 256)), dim3(256), 0, stream, v,
-                           d_query, (int)nq, queries_are_sources, eta, scale, d_out_re, d_out_im);
+    if (t->f32_far && wave)
+        hipLaunchKernelGGL(eval_fast_kernel<true>, grid, block, 0, stream, v, d_query, (int)nq,
+                           queries_are_sources, eta, scale, d_out_re, d_out_im);
+    else if (t->f32_far)
+        hipLaunchKernelGGL(eval_fast_kernel<false>, grid, block, 0, stream, v, d_query, (int)nq,
+                           queries_are_sources, eta, scale, d_out_re, d_out_im);
+    else if (wave)
+        hipLaunchKernelGGL(eval_f64_kernel<true>, grid, block, 0, stream, v, d_query, (int)nq,
+                           queries_are_sources, eta, scale, d_out_re, d_out_im);
     else
-        hipLaunchKernelGGL(eval_f64_kernel, dim3(blocks_for(nq, 256)), dim3(256), 0, stream, v,
-                           d_query, (int)nq, queries_are_sources, eta, scale, d_out_re, d_out_im);
+        hipLaunchKernelGGL(eval_f64_kernel<false>, grid, block, 0, stream, v, d_query, (int)nq,
+                           queries_are_sources, eta, scale, d_out_re, d_out_im);
     profile_end(m, stream);
     SHK_HIP(hipGetLastError());
     return 0;

@@ -42,7 +42,9 @@ namespace {
 
 // Tile shape; the defaults are the measured choice on MI355X (one wavefront
 // per workgroup, 2 queries per lane: 128-query tiles, narrow enough that
-// the one-subtraction pair difference applies to most tiles) and can be
+// the one-subtraction pair difference applies to most tiles; about 16 384
+// workgroups: as fast as 32 768 at half the partial-sum traffic, while 8 192
+// and fewer leave the last round of workgroups unbalanced) and can be
 // overridden at compile time for tuning sweeps.
 #ifndef SHK_EXACT_WG
 #define SHK_EXACT_WG 64
@@ -54,7 +56,7 @@ namespace {
 #define SHK_EXACT_PIECE 512
 #endif
 #ifndef SHK_EXACT_TARGET_WG
-#define SHK_EXACT_TARGET_WG 32768
+#define SHK_EXACT_TARGET_WG 16384
 #endif
 constexpr int kWG = SHK_EXACT_WG;          // threads per workgroup
 constexpr int kR = SHK_EXACT_R;            // queries per thread
