@@ -42,6 +42,32 @@ def test_tree_identical_to_oracle(shk, orc, preset, p):
     assert np.max(np.abs(g["w"] - o["w"])) <= 1e-13 * wscale
 
 
+@pytest.mark.parametrize("preset", ["chebcode_fast", "chebcode"])
+@pytest.mark.parametrize("case", ["mp200k_c05", "mp1m_c01", "geometric_depth_cap", "duplicates"])
+def test_tree_topology_large_and_degenerate(shk, orc, preset, case):
+    """Topology, intervals and positions equal the oracle's tree on large MP
+    spectra, on a geometric spectrum whose midpoint splits peel one value per
+    level until the depth cap (48, chebcode.rs:95), and on heavy duplicates
+    (the index-midpoint fallback, chebcode.rs:373-381)."""
+    from shrinkers_amd import spectra as S
+    if case == "mp200k_c05":
+        ev = S.mp_inverse_cdf_spectrum(200_000, c=0.5)
+    elif case == "mp1m_c01":
+        ev = S.mp_inverse_cdf_spectrum(1_000_000, c=0.1)
+    elif case == "geometric_depth_cap":
+        ev = np.sort(1.3 ** (-np.arange(2000.0)))
+    else:
+        ev = np.sort(np.repeat(np.linspace(0.5, 2.0, 700), 37))
+    theta, n, leaf, _ = shk.CHEB_PRESETS[preset]
+    g = shk.ChebCodeBatch.build_preset(ev, preset).export(n)
+    o = orc.ChebTree(np.sort(ev), n, theta, leaf).export()
+    assert g["right"].size == o["right"].size
+    for k in ("lo_idx", "hi_idx", "right", "lo", "hi", "nodes"):
+        np.testing.assert_array_equal(g[k], o[k])
+    wscale = np.max(np.abs(o["w"]))
+    assert np.max(np.abs(g["w"] - o["w"])) <= 1e-13 * wscale
+
+
 @pytest.mark.parametrize("p", [1000, 2000, 5000, 10000])
 def test_f64_presets_match_oracle_and_record(shk, orc, p):
     from shrinkers_amd import spectra as S
