@@ -201,9 +201,14 @@ def chebcode_leg(shk, sync) -> dict:
             e1.record()
             sync()
             err = float(np.sqrt(np.sum((r.cpu().numpy() - er) ** 2 + (i.cpu().numpy() - ei) ** 2) / den))
+            ms_eval = e0.elapsed_time(e1) / 5
+            near, far = b.count_work(eta)
+            flops = 7.0 * near + 8.0 * far   # pair-eval 7 FLOPs, far-field panel term 8
             row[name] = {"ms_build_plus_eval": 1e3 * float(np.median(ts)),
-                         "ms_eval": e0.elapsed_time(e1) / 5, "rel_l2_vs_fp64_exact": err,
-                         "nodes": b.node_count}
+                         "ms_eval": ms_eval, "rel_l2_vs_fp64_exact": err,
+                         "nodes": b.node_count, "near_pairs": near, "far_terms": far,
+                         "eval_tflops": flops / (ms_eval * 1e-3) / 1e12,
+                         "near_field_tflops_lower_bound": 7.0 * near / (ms_eval * 1e-3) / 1e12}
             del b
         res[f"p={p},c={c}"] = row
     return res

@@ -220,6 +220,17 @@ shk_status shk_chebcode_build_dev(const double *d_eigenvalues, size_t p, double 
 shk_status shk_chebcode_evaluate_dev(const shk_chebcode *h, double eta, double scale,
                                      double *d_out_re, double *d_out_im, void *stream);
 
+/* Work of one traversal pass, with the evaluation kernels' node decisions:
+ * *near_pairs = sum over queries of the sizes of the leaves each visits
+ * (exact pair evaluations), *far_terms = n per accepted panel.  d_query =
+ * NULL (with nq = 0) counts the all-points pass at the tree's own sources.
+ * Counterpart of the near/far statistics the reference's traversal implies
+ * (chebcode.rs:607-828); used to express evaluation time as FLOP rates.
+ * Synchronises `stream`. */
+shk_status shk_chebcode_count_work(const shk_chebcode *h, const double *d_query, size_t nq,
+                                   double eta, int64_t *near_pairs, int64_t *far_terms,
+                                   void *stream);
+
 /* ---- batched spectra (no counterpart in the reference) ------------------- */
 
 /* Many independent spectra in one persistent launch captured in a hipGraph.

@@ -76,6 +76,7 @@ def lib() -> ctypes.CDLL:
                                    ctypes.c_int, _f64p, _f64p]
     L.orc_cheb_eval_points.argtypes = [ctypes.c_void_p, _f64p, _i64, _d, ctypes.c_int,
                                        _f64p, _f64p]
+    L.orc_cheb_count_work.argtypes = [ctypes.c_void_p, _f64p, _i64, _d, ctypes.c_int, _lp, _lp]
     L.orc_grid_leaf_cap.argtypes = [_i64, _i64, _i64, _i64]
     L.orc_grid_leaf_cap.restype = _i64
     L.orc_max_threads.restype = ctypes.c_int
@@ -286,6 +287,14 @@ class ChebTree:
         im = np.empty_like(q)
         lib().orc_cheb_eval_points(self._h, q, q.size, float(eta), int(bool(fast32)), re, im)
         return re, im
+
+    def count_work(self, q, eta: float, fast32: bool = False):
+        """(near-field pairs, accepted far panels) summed over the points q."""
+        q = _c(q)
+        nt, pt = ctypes.c_int64(), ctypes.c_int64()
+        lib().orc_cheb_count_work(self._h, q, q.size, float(eta), int(bool(fast32)),
+                                  ctypes.byref(nt), ctypes.byref(pt))
+        return nt.value, pt.value
 
     def eval_x2(self, x: float, e0: float, e1: float):
         v = [_d() for _ in range(4)]

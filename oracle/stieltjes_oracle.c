@@ -934,6 +934,25 @@ EXPORT void orc_cheb_eval_points(const orc_cheb_tree *t, const double *q, int64_
     }
 }
 
+/* Total traversal work over nq points: near-field pairs and accepted far
+ * panels of the f64 (chebcode.rs:607-711) or FAST (:722-828) traversal. */
+EXPORT void orc_cheb_count_work(const orc_cheb_tree *t, const double *q, int64_t nq, double eta,
+                                int fast32, int64_t *near_total, int64_t *panels_total) {
+    int64_t nt = 0, pt = 0;
+    for (int64_t i = 0; i < nq; i++) {
+        double r, im;
+        int64_t np = 0, fp = 0;
+        if (fast32)
+            orc_cheb_eval_fast32(t, q[i], eta, &r, &im, &np, &fp);
+        else
+            orc_cheb_eval_f64(t, q[i], eta, &r, &im, &np, &fp);
+        nt += np;
+        pt += fp;
+    }
+    *near_total = nt;
+    *panels_total = pt;
+}
+
 /* Grid leaf capacity L* = n*sqrt(2p/nq) clamped to [preset, 8192]:
  * src/stieltjes/mod.rs:361-370. */
 EXPORT int64_t orc_grid_leaf_cap(int64_t n, int64_t p, int64_t nq, int64_t preset_leaf) {

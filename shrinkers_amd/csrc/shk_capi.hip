@@ -789,6 +789,17 @@ shk_status shk_chebcode_evaluate_dev(const shk_chebcode *h, double eta, double s
                                  d_out_im, (hipStream_t)stream);
 }
 
+shk_status shk_chebcode_count_work(const shk_chebcode *h, const double *d_query, size_t nq,
+                                   double eta, int64_t *near_pairs, int64_t *far_terms,
+                                   void *stream) {
+    if (!h || !h->tree || !near_pairs || !far_terms) return fail(SHK_ERR_BAD_ARG, "NULL pointer");
+    if (d_query == nullptr && nq != 0)
+        return fail(SHK_ERR_BAD_ARG, "nq must be 0 when d_query is NULL (the tree's own sources)");
+    SHK_CHECK(check_eta(eta));
+    return (shk_status)cheb_count(h->tree, d_query, nq, eta, near_pairs, far_terms,
+                                  (hipStream_t)stream);
+}
+
 shk_status shk_chebcode_destroy(shk_chebcode *h) {
     if (!h) return SHK_OK;
     cheb_destroy(h->tree);

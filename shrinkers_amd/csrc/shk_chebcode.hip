@@ -435,6 +435,26 @@ __device__ __forceinline__ void leaf_f64(const double *__restrict__ xs, int len,
     }
 }
 
+// Far-field acceptance of an internal node for query x: the f64 test of
+// chebcode.rs:648-657 (accept_hw < cl^2 + eta^2, cl the clamped distance) and
+// the FAST f32 test with the OUTSIDE_THETA2 factor for panels that do not
+// contain the query (chebcode.rs:776-783).  Shared by the evaluation and the
+// work-counting kernels so both make the same decisions.
+__device__ __forceinline__ bool accept_f64(const EvalView &v, int node, double x, double eta2) {
+    const double lo = v.lo[node], hi = v.hi[node];
+    const double cl = fmax(fmax(lo - x, x - hi), 0.0);
+    const double d_sq = __builtin_fma(cl, cl, eta2);
+    return v.acc[node] < d_sq;
+}
+
+__device__ __forceinline__ bool accept_fast(const float4 r, float x, float eta2) {
+    // r = (lo, hi, accept_hw)
+    const float cl = fmaxf(fmaxf(r.x - x, x - r.y), 0.0f);
+    const float d_sq = __builtin_fmaf(cl, cl, eta2);
+    const float outside = (cl == 0.0f) ? d_sq : kOutsideTheta2 * d_sq;
+    return r.z < outside;
+}
+
 // Traversal, one query per lane.  Every lane follows the reference's stack
 // DFS in pre-order (a pruned or finished node continues at its escape index,
 // a descended node at node + 1).  Two walks give bitwise the same per-lane
@@ -480,10 +500,7 @@ __global__ __launch_bounds__(256) void eval_f64_kernel(EvalView v, const double 
             node = lk.y;
             continue;
         }
-        const double lo = v.lo[node], hi = v.hi[node];
-        const double cl = fmax(fmax(lo - x, x - hi), 0.0);
-        const double d_sq = __builtin_fma(cl, cl, eta2);
-        const bool accept = v.acc[node] < d_sq;
+        const bool accept = accept_f64(v, node, x, eta2);
         if (active && accept) {
             const double *ts = v.nodes + (size_t)node * n;
             const double *ws = v.w + (size_t)node * n;
@@ -583,11 +600,7 @@ __global__ __launch_bounds__(256) void eval_fast_kernel(EvalView v, const double
             node = lk.y;
             continue;
         }
-        const float4 r = v.rec32[node];  // lo, hi, acc
-        const float cl = fmaxf(fmaxf(r.x - x, x - r.y), 0.0f);
-        const float d_sq = __builtin_fmaf(cl, cl, eta2);
-        const float outside = (cl == 0.0f) ? d_sq : kOutsideTheta2 * d_sq;
-        const bool accept = r.z < outside;
+        const bool accept = accept_fast(v.rec32[node], x, eta2);
         if (active && accept) {
             const float *ts = v.panel32 + (size_t)node * 2 * n;
             const float *ws = ts + n;
@@ -626,6 +639,55 @@ __global__ __launch_bounds__(256) void eval_fast_kernel(EvalView v, const double
     const int o = (queries_are_sources && v.perm) ? v.perm[i] : i;
     out_re[o] = re * scale;
     out_im[o] = im * scale;
+}
+
+// Work of the traversal, without arithmetic: near-field pair-evaluations
+// (sum of the sizes of the leaves a query visits) and far-field panel terms
+// (n per accepted panel), with the evaluation kernels' decisions; one query
+// per lane, sums reduced per wavefront and added to counts[0..1].
+template <bool kFast>
+__global__ __launch_bounds__(256) void count_kernel(EvalView v, const double *__restrict__ query,
+                                                    int nq, int queries_are_sources, double eta,
+                                                    unsigned long long *__restrict__ counts) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long near = 0, far = 0;
+    if (i < nq) {
+        const double xd = queries_are_sources ? v.sorted[i] : query[i];
+        const float xf = (float)xd;
+        const float e32 = (float)eta;
+        const float eta2f = e32 * e32;
+        const double eta2 = eta * eta;
+        int node = 0;
+        while (node < v.N) {
+            const int4 lk = v.link[node];
+            if (lk.x < 0) {
+                near += (unsigned long long)(lk.w - lk.z);
+                node = lk.y;
+                continue;
+            }
+            const bool accept =
+                kFast ? accept_fast(v.rec32[node], xf, eta2f) : accept_f64(v, node, xd, eta2);
+            if (accept) {
+                far += (unsigned long long)v.n;
+                node = lk.y;
+            } else {
+                node = node + 1;
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        near += __shfl_xor(near, o, 64);
+        far += __shfl_xor(far, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&counts[0], near);
+        atomicAdd(&counts[1], far);
+    }
+}
+
+__global__ void clear_counts_kernel(unsigned long long *counts) {
+    if (threadIdx.x < 2) counts[threadIdx.x] = 0ull;
 }
 
 // Two-eta traversal (chebcode.rs:836-915); queries are the tree's sources.
@@ -997,6 +1059,43 @@ int cheb_eval(const ChebTree *t, const double *d_query, size_t nq, double eta, d
                            queries_are_sources, eta, scale, d_out_re, d_out_im);
     profile_end(m, stream);
     SHK_HIP(hipGetLastError());
+    return 0;
+}
+
+int cheb_count(const ChebTree *t, const double *d_query, size_t nq, double eta,
+               int64_t *near_pairs, int64_t *far_terms, hipStream_t stream) {
+    const int queries_are_sources = d_query == nullptr;
+    if (queries_are_sources) nq = t->p;
+    *near_pairs = *far_terms = 0;
+    if (nq == 0) return 0;
+    if (nq > (size_t)INT32_MAX) {
+        set_error("chebcode count: too many query points");
+        return 6;
+    }
+    SHK_HIP(hipStreamWaitEvent(stream, t->ready, 0));
+    unsigned long long *d_counts = nullptr;
+    SHK_HIP(hipMallocAsync((void **)&d_counts, 2 * sizeof(unsigned long long), stream));
+    EvalView v = view_of(t);
+    hipLaunchKernelGGL(clear_counts_kernel, dim3(1), dim3(64), 0, stream, d_counts);
+    if (t->f32_far)
+        hipLaunchKernelGGL(count_kernel<true>, dim3(blocks_for(nq, 256)), dim3(256), 0, stream, v,
+                           d_query, (int)nq, queries_are_sources, eta, d_counts);
+    else
+        hipLaunchKernelGGL(count_kernel<false>, dim3(blocks_for(nq, 256)), dim3(256), 0, stream, v,
+                           d_query, (int)nq, queries_are_sources, eta, d_counts);
+    hipError_t e = hipGetLastError();
+    unsigned long long h[2] = {0, 0};
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(h, d_counts, sizeof(h), hipMemcpyDeviceToHost, stream);
+    const hipError_t fe = hipFreeAsync(d_counts, stream);
+    if (e == hipSuccess) e = fe;
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) {
+        set_error(std::string("chebcode count: ") + hipGetErrorString(e));
+        return 7;
+    }
+    *near_pairs = (int64_t)h[0];
+    *far_terms = (int64_t)h[1];
     return 0;
 }
 

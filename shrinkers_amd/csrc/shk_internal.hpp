@@ -120,6 +120,10 @@ int cheb_export(const ChebTree *t, double *lo, double *hi, int64_t *lo_idx, int6
 int cheb_eval(const ChebTree *t, const double *d_query, size_t nq, double eta, double scale,
               int queries_are_sources, double *d_out_re, double *d_out_im,
               hipStream_t stream);
+// Traversal work at nq device query points (NULL: the tree's own sources):
+// near-field pair evaluations and far-field panel terms.  Synchronises.
+int cheb_count(const ChebTree *t, const double *d_query, size_t nq, double eta,
+               int64_t *near_pairs, int64_t *far_terms, hipStream_t stream);
 // Two-eta traversal for evaluate_many: acceptance at min(eta0, eta1).
 int cheb_eval_x2(const ChebTree *t, double eta0, double eta1, double *d_re0, double *d_im0,
                  double *d_re1, double *d_im1, hipStream_t stream);

@@ -333,6 +333,24 @@ class ChebCodeBatch:
                                                     _stream_handle(stream)))
         return out
 
+    def count_work(self, eta: float, query=None, stream=None):
+        """``(near_pairs, far_terms)`` of one traversal pass: the summed sizes
+        of the leaves each query visits and n per accepted panel, with the
+        evaluation's node decisions.  ``query`` is a CUDA float64 tensor of
+        points, or None for the tree's own eigenvalues.  Synchronises."""
+        import torch
+        near, far = ctypes.c_int64(), ctypes.c_int64()
+        if query is not None:
+            if not (query.is_cuda and query.dtype == torch.float64 and query.dim() == 1
+                    and query.is_contiguous()):
+                raise ValueError("query must be a contiguous 1-D float64 CUDA tensor")
+            qp, nq = query.data_ptr(), query.numel()
+        else:
+            qp, nq = None, 0
+        check(_lib.load().shk_chebcode_count_work(self._h, qp, nq, float(eta), ctypes.byref(near),
+                                                  ctypes.byref(far), _stream_handle(stream)))
+        return near.value, far.value
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:

@@ -200,3 +200,27 @@ def test_large_p_against_gpu_exact(shk, p, c):
         err = rel_l2(r.cpu().numpy(), i.cpu().numpy(), er, ei)
         print(f"{m} p={p} c={c}: rel-L2 vs GPU exact {err:.3e}", flush=True)
         assert err <= tol
+
+
+@pytest.mark.parametrize("preset", ["chebcode_fast", "chebcode", "chebcode_xtreme"])
+@pytest.mark.parametrize("p,eta_scale", [(5000, 1.0), (50000, 1.0), (50000, 0.05)])
+def test_work_counts_match_oracle(shk, orc, preset, p, eta_scale):
+    """The GPU traversal's work (near-field pairs, far-field terms) equals the
+    oracle traversal's, integer for integer, at the tree's own points and on a
+    grid: both make the same acceptance decisions (chebcode.rs:648-657,
+    776-783)."""
+    import torch
+    from shrinkers_amd import spectra as S
+    orc.set_outside_theta2(1.69)
+    ev = S.mp_inverse_cdf_spectrum(p, c=0.25)
+    eta = eta_scale / np.sqrt(p)
+    theta, n, leaf, fast = shk.CHEB_PRESETS[preset]
+    b = shk.ChebCodeBatch.build_dev(torch.from_numpy(ev).cuda(), theta, n, leaf, fast)
+    t = orc.ChebTree(np.sort(ev), n, theta, leaf)
+    near, far = b.count_work(eta)
+    o_near, o_panels = t.count_work(np.sort(ev), eta, fast)
+    assert (near, far) == (o_near, o_panels * n)
+    grid = np.linspace(ev[0] - 0.3, ev[-1] + 0.3, 333)
+    near, far = b.count_work(eta, query=torch.from_numpy(grid).cuda())
+    o_near, o_panels = t.count_work(grid, eta, fast)
+    assert (near, far) == (o_near, o_panels * n)
