@@ -253,6 +253,21 @@ def adjoint_leg(shk, sync, d_ev, eta) -> dict:
         grads[prec] = [r.detach().cpu().numpy() for r in res]
         out[f"fit_{prec}"] = {"ms_forward_backward": ms, "loss": float(grads[prec][0])}
 
+    # the same step as one library call (no autograd, no host round trips)
+    w = torch.softmax(torch.from_numpy(logits).cuda(), dim=0)
+    for prec in ("f32c", "f64"):
+        def fused():
+            ag.el_karoui_fit_step(t_ev, eta_f, w, t_atoms, 0.25, t_grid, prec)
+        for _ in range(3):
+            fused()
+        sync()
+        n = 50
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fused()
+        sync()
+        out[f"fit_{prec}"]["ms_fused_step"] = (time.perf_counter() - t0) / n * 1e3
+
     def rel(a, b):
         return float(np.linalg.norm(np.ravel(a - b)) / np.linalg.norm(np.ravel(b)))
     out["f32c_vs_f64_grad_rel"] = {"eigenvalues": rel(grads["f32c"][1], grads["f64"][1]),

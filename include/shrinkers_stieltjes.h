@@ -313,6 +313,20 @@ shk_status shk_el_karoui_fit_dev(const double *d_x, const double *d_g_re, const 
                                  double *d_grad_g_re, double *d_grad_g_im,
                                  double *d_grad_weights, double *d_grad_eta, void *stream);
 
+/* One El Karoui fit step on device, in one call: the raw exact sums at
+ * z_m = x_m + i eta (precision SHK_PREC_F64 or SHK_PREC_F32C), g = -conj(S)/p,
+ * the loss of shk_el_karoui_fit_dev, and its gradients with respect to the
+ * eigenvalues (p values), eta (one device double, through z and through S)
+ * and the weights (k values).  The same arithmetic as composing
+ * shk_exact_at_points_dev, shk_el_karoui_fit_dev and shk_exact_adjoint_dev,
+ * without the host round trips between them.  Gradient pointers may be NULL.
+ * No reference counterpart (see shk_el_karoui_fit_dev). */
+shk_status shk_el_karoui_fit_step_dev(const double *d_eigenvalues, size_t p, const double *d_x,
+                                      size_t m, double eta, double c, const double *d_atoms,
+                                      const double *d_weights, size_t k, int precision,
+                                      double *d_loss, double *d_grad_eigenvalues,
+                                      double *d_grad_eta, double *d_grad_weights, void *stream);
+
 /* ---- measurement ---------------------------------------------------------- */
 
 /* When enabled, the library brackets the dominant kernel of each exact or

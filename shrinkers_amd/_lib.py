@@ -78,6 +78,8 @@ SIGNATURES = {
     "shk_exact_adjoint_dev": (_i, [_vp, _sz, _vp, _sz, _d, _vp, _vp, _i, _vp, _vp, _vp, _vp]),
     "shk_el_karoui_fit_dev": (_i, [_vp, _vp, _vp, _sz, _d, _d, _vp, _vp, _sz, _vp, _vp, _vp,
                                    _vp, _vp, _vp]),
+    "shk_el_karoui_fit_step_dev": (_i, [_vp, _sz, _vp, _sz, _d, _d, _vp, _vp, _sz, _i, _vp,
+                                        _vp, _vp, _vp, _vp]),
     "shk_profile_enable": (_i, [_i]),
     "shk_profile_read": (_i, [_dp, _i64p]),
     "shk_profile_reset": (_i, []),

@@ -185,6 +185,36 @@ def el_karoui_fit_loss(eigenvalues, eta, logits, atoms, c: float, grid,
     return el_karoui_residual_loss(g_re, g_im, grid, eta, c, atoms, w)
 
 
+def el_karoui_fit_step(eigenvalues, eta: float, weights, atoms, c: float, grid,
+                       precision: str = "f32c", stream=None):
+    """One fit step in a single library call (no autograd, no host round
+    trips): returns ``(loss, grad_eigenvalues, grad_eta, grad_weights)`` as
+    CUDA float64 tensors (loss and grad_eta 0-d) for the loss of
+    ``el_karoui_residual_loss`` at g = sample transform of ``eigenvalues`` at
+    ``grid + i eta``.  ``eta`` is a Python float.  The weights' gradient
+    chains to logits as w * (g_w - <w, g_w>) for softmax weights."""
+    import torch
+    for t, nm in ((eigenvalues, "eigenvalues"), (weights, "weights"), (atoms, "atoms"),
+                  (grid, "grid")):
+        _need_cuda_f64(t, nm)
+    if weights.numel() != atoms.numel():
+        raise ValueError("one weight per atom")
+    ev = eigenvalues.detach().contiguous()
+    x = grid.detach().contiguous()
+    t_ = atoms.detach().contiguous()
+    w = weights.detach().contiguous()
+    dev = ev.device
+    loss = torch.empty((), dtype=torch.float64, device=dev)
+    g_ev = torch.empty_like(ev)
+    g_eta = torch.empty((), dtype=torch.float64, device=dev)
+    g_w = torch.empty_like(w)
+    check(_lib.load().shk_el_karoui_fit_step_dev(
+        ev.data_ptr(), ev.numel(), x.data_ptr(), x.numel(), _eta_value(eta), float(c),
+        t_.data_ptr(), w.data_ptr(), t_.numel(), _prec(precision), loss.data_ptr(),
+        g_ev.data_ptr(), g_eta.data_ptr(), g_w.data_ptr(), _stream_handle(stream)))
+    return loss, g_ev, g_eta, g_w
+
+
 def bulk_fit_problem(p: int = 10_000, c: float = 0.25, n_atoms: int = 256,
                      n_grid: int = 512, seed: int = 0):
     """The bulk-fit workload: MP(c, 1) eigenvalues by inverse-CDF sampling

@@ -411,10 +411,11 @@ __global__ __launch_bounds__(256) void adj_qry_reduce_kernel(
 }
 
 // One workgroup: out = scale * sum(v) in a fixed order (strided per-thread
-// sums, then a tree).
+// sums, then a tree); with accumulate != 0 the result is added to *out.
 __global__ __launch_bounds__(kSumThreads) void ordered_sum_kernel(const double *__restrict__ v,
                                                                   int n, double scale,
-                                                                  double *__restrict__ out) {
+                                                                  double *__restrict__ out,
+                                                                  int accumulate) {
     __shared__ double s[kSumThreads];
     const int tid = threadIdx.x;
     double a = 0.0;
@@ -425,7 +426,7 @@ __global__ __launch_bounds__(kSumThreads) void ordered_sum_kernel(const double *
         if (tid < o) s[tid] += s[tid + o];
         __syncthreads();
     }
-    if (tid == 0) *out = s[0] * scale;
+    if (tid == 0) *out = accumulate ? *out + s[0] * scale : s[0] * scale;
 }
 
 // ---- El Karoui fit ---------------------------------------------------------
@@ -449,7 +450,10 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// One wavefront per grid point; lane l takes atoms l, l+64, ...:
+// One wavefront per grid point; lane l takes atoms l, l+64, ...  The input is
+// (in_re * g_re, in_im * g_im) and the g-gradient is written scaled by
+// (out_re, out_im): 1s for g itself, or (-1/p, 1/p) both ways for the raw
+// sums S (g = -conj(S)/p).
 // a = 1 - c - c z g, h_k = 1/(t_k a - z), r = g - sum w_k h_k,
 // dr/dg = 1 - c z sum w_k t_k h_k^2, dr/dz = -sum w_k h_k^2 (1 + c t_k g).
 __global__ __launch_bounds__(64) void ek_point_kernel(
@@ -457,11 +461,11 @@ __global__ __launch_bounds__(64) void ek_point_kernel(
     int m, double eta, double c, const double *__restrict__ t, const double *__restrict__ w, int k,
     double *__restrict__ a_out, double *__restrict__ r_out, double *__restrict__ sq_out,
     double *__restrict__ eta_out, double *__restrict__ grad_g_re,
-    double *__restrict__ grad_g_im) {
+    double *__restrict__ grad_g_im, double in_re, double in_im, double out_re, double out_im) {
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
     const cplx z = {x[i], eta};
-    const cplx g = {g_re[i], g_im[i]};
+    const cplx g = {in_re * g_re[i], in_im * g_im[i]};
     const cplx zg = cmul(z, g);
     const cplx a = {1.0 - c - c * zg.re, -c * zg.im};
     cplx s = {0.0, 0.0}, sd = {0.0, 0.0}, sz = {0.0, 0.0};
@@ -486,8 +490,8 @@ __global__ __launch_bounds__(64) void ek_point_kernel(
     const cplx jg = {1.0 - c * zsd.re, -c * zsd.im};
     const double two_m = 2.0 / (double)m;
     // (2/M) r conj(jg)
-    if (grad_g_re) grad_g_re[i] = two_m * (r.re * jg.re + r.im * jg.im);
-    if (grad_g_im) grad_g_im[i] = two_m * (r.im * jg.re - r.re * jg.im);
+    if (grad_g_re) grad_g_re[i] = out_re * (two_m * (r.re * jg.re + r.im * jg.im));
+    if (grad_g_im) grad_g_im[i] = out_im * (two_m * (r.im * jg.re - r.re * jg.im));
     // (2/M) Re(conj(r) i (-sz)) = (2/M) Im(conj(r) sz)
     eta_out[i] = two_m * (r.re * sz.im - r.im * sz.re);
     sq_out[i] = r.re * r.re + r.im * r.im;
@@ -532,7 +536,7 @@ int zero_fill(double *p, size_t n, hipStream_t stream) {
 
 int exact_adjoint(const double *q, size_t nq, const double *ev, size_t p, double eta,
                   const double *gre, const double *gim, int f32c, double *grad_ev,
-                  double *grad_q, double *grad_eta, hipStream_t stream) {
+                  double *grad_q, double *grad_eta, hipStream_t stream, int accumulate_eta) {
     if (nq == 0 || p == 0) return 0;
     if (nq > (size_t)INT32_MAX || p > (size_t)INT32_MAX) {
         set_error("adjoint kernel: sizes above 2^31-1 are not supported");
@@ -588,7 +592,7 @@ int exact_adjoint(const double *q, size_t nq, const double *ev, size_t p, double
         }
         if (le == hipSuccess && grad_eta) {
             hipLaunchKernelGGL(ordered_sum_kernel, dim3(1), dim3(kSumThreads), 0, stream,
-                               eta_term, (int)nq, 1.0, grad_eta);
+                               eta_term, (int)nq, 1.0, grad_eta, accumulate_eta);
             le = hipGetLastError();
         }
         const hipError_t fe = hipFreeAsync(part, stream);
@@ -601,10 +605,12 @@ int exact_adjoint(const double *q, size_t nq, const double *ev, size_t p, double
     return 0;
 }
 
-int ek_fit(const double *x, const double *g_re, const double *g_im, size_t m, double eta,
-           double c, const double *t, const double *w, size_t k, double *loss,
-           double *grad_g_re, double *grad_g_im, double *grad_w, double *grad_eta,
-           hipStream_t stream) {
+namespace {
+
+int ek_fit_impl(const double *x, const double *g_re, const double *g_im, size_t m, double eta,
+                double c, const double *t, const double *w, size_t k, double *loss,
+                double *grad_g_re, double *grad_g_im, double *grad_w, double *grad_eta,
+                double in_re, double in_im, double out_re, double out_im, hipStream_t stream) {
     if (m == 0 || k == 0) {
         set_error("El Karoui fit: needs at least one grid point and one atom");
         return 1;
@@ -617,7 +623,8 @@ int ek_fit(const double *x, const double *g_re, const double *g_im, size_t m, do
     SHK_HIP(hipMallocAsync((void **)&scr, sizeof(double) * 6 * m, stream));
     double *a_buf = scr, *r_buf = scr + 2 * m, *sq = scr + 4 * m, *et = scr + 5 * m;
     hipLaunchKernelGGL(ek_point_kernel, dim3((unsigned)m), dim3(64), 0, stream, x, g_re, g_im,
-                       (int)m, eta, c, t, w, (int)k, a_buf, r_buf, sq, et, grad_g_re, grad_g_im);
+                       (int)m, eta, c, t, w, (int)k, a_buf, r_buf, sq, et, grad_g_re, grad_g_im,
+                       in_re, in_im, out_re, out_im);
     hipError_t le = hipGetLastError();
     if (le == hipSuccess && grad_w) {
         hipLaunchKernelGGL(ek_weight_kernel, dim3((unsigned)k), dim3(64), 0, stream, x, (int)m,
@@ -626,12 +633,12 @@ int ek_fit(const double *x, const double *g_re, const double *g_im, size_t m, do
     }
     if (le == hipSuccess && loss) {
         hipLaunchKernelGGL(ordered_sum_kernel, dim3(1), dim3(kSumThreads), 0, stream, sq, (int)m,
-                           1.0 / (double)m, loss);
+                           1.0 / (double)m, loss, 0);
         le = hipGetLastError();
     }
     if (le == hipSuccess && grad_eta) {
         hipLaunchKernelGGL(ordered_sum_kernel, dim3(1), dim3(kSumThreads), 0, stream, et, (int)m,
-                           1.0, grad_eta);
+                           1.0, grad_eta, 0);
         le = hipGetLastError();
     }
     const hipError_t fe = hipFreeAsync(scr, stream);
@@ -639,6 +646,48 @@ int ek_fit(const double *x, const double *g_re, const double *g_im, size_t m, do
         set_error(std::string("El Karoui fit kernels: ") + hipGetErrorString(le));
         return 7;
     }
+    SHK_HIP(fe);
+    return 0;
+}
+
+}  // namespace
+
+int ek_fit(const double *x, const double *g_re, const double *g_im, size_t m, double eta,
+           double c, const double *t, const double *w, size_t k, double *loss,
+           double *grad_g_re, double *grad_g_im, double *grad_w, double *grad_eta,
+           hipStream_t stream) {
+    return ek_fit_impl(x, g_re, g_im, m, eta, c, t, w, k, loss, grad_g_re, grad_g_im, grad_w,
+                       grad_eta, 1.0, 1.0, 1.0, 1.0, stream);
+}
+
+int ek_fit_step(const double *ev, size_t p, const double *x, size_t m, double eta, double c,
+                const double *t, const double *w, size_t k, int f32c, double *loss,
+                double *grad_ev, double *grad_eta, double *grad_w, hipStream_t stream) {
+    if (p == 0) {
+        set_error("El Karoui fit step: needs at least one eigenvalue");
+        return 1;
+    }
+    double *scr = nullptr;
+    SHK_HIP(hipMallocAsync((void **)&scr, sizeof(double) * 4 * m, stream));
+    double *s_re = scr, *s_im = scr + m, *g_re = scr + 2 * m, *g_im = scr + 3 * m;
+    const double inv_p = 1.0 / (double)p;
+    int st;
+    if (f32c) {
+        st = exact_f32c(x, m, ev, p, eta, 1.0, s_re, s_im, 0, stream);
+    } else {
+        double *outs[2] = {s_re, s_im};
+        st = exact_f64(x, m, ev, p, eta, EXACT_PLAIN, 0.0, 1.0, outs, stream);
+    }
+    // g = -conj(S)/p in, dL/dS = (-dL/dRe g, dL/dIm g)/p out; eta's direct term first
+    if (!st)
+        st = ek_fit_impl(x, s_re, s_im, m, eta, c, t, w, k, loss, g_re, g_im, grad_w, grad_eta,
+                         -inv_p, inv_p, -inv_p, inv_p, stream);
+    // then the part through S, added to it
+    if (!st && (grad_ev || grad_eta))
+        st = exact_adjoint(x, m, ev, p, eta, g_re, g_im, f32c, grad_ev, nullptr, grad_eta, stream,
+                           1);
+    const hipError_t fe = hipFreeAsync(scr, stream);
+    if (st) return st;
     SHK_HIP(fe);
     return 0;
 }

@@ -650,6 +650,28 @@ shk_status shk_el_karoui_fit_dev(const double *d_x, const double *d_g_re, const 
                               (hipStream_t)stream);
 }
 
+shk_status shk_el_karoui_fit_step_dev(const double *d_eigenvalues, size_t p, const double *d_x,
+                                      size_t m, double eta, double c, const double *d_atoms,
+                                      const double *d_weights, size_t k, int precision,
+                                      double *d_loss, double *d_grad_eigenvalues,
+                                      double *d_grad_eta, double *d_grad_weights, void *stream) {
+    if (p == 0) return fail(SHK_ERR_EMPTY, "the fit needs at least one eigenvalue");
+    if (m == 0) return fail(SHK_ERR_EMPTY, "the fit needs at least one grid point");
+    if (k == 0) return fail(SHK_ERR_EMPTY, "the fit needs at least one atom");
+    if (!d_eigenvalues || !d_x || !d_atoms || !d_weights) return fail(SHK_ERR_BAD_ARG, "NULL pointer");
+    if (p > (size_t)INT32_MAX || m > ((size_t)1 << 24) || k > ((size_t)1 << 24))
+        return fail(SHK_ERR_BAD_ARG, "size above the supported range");
+    SHK_CHECK(check_eta(eta));
+    if (!std::isfinite(c) || !(c > 0.0))
+        return fail(SHK_ERR_BAD_ARG, "c must be a positive finite number, got " + std::to_string(c));
+    if (precision != SHK_PREC_F64 && precision != SHK_PREC_F32C)
+        return fail(SHK_ERR_BAD_ARG, "precision must be SHK_PREC_F64 or SHK_PREC_F32C");
+    SHK_CHECK(ensure_device());
+    return (shk_status)ek_fit_step(d_eigenvalues, p, d_x, m, eta, c, d_atoms, d_weights, k,
+                                   precision == SHK_PREC_F32C, d_loss, d_grad_eigenvalues,
+                                   d_grad_eta, d_grad_weights, (hipStream_t)stream);
+}
+
 shk_status shk_chebcode_build(const double *eigenvalues, size_t p, double theta, int n,
                               size_t leaf_cap, int f32_far, shk_chebcode **out) {
     if (!out) return fail(SHK_ERR_BAD_ARG, "NULL handle pointer");

@@ -77,9 +77,10 @@ int widen_f32(const float *in, double *out, size_t n, hipStream_t stream);
 // gradients gre/gim (w.r.t. Re S_m, Im S_m).  Outputs: grad_ev (p),
 // grad_q (nq), grad_eta (one device double); any may be NULL.  f32c != 0
 // selects the FP32-compensated pair arithmetic.
+// accumulate_eta != 0 adds the eta gradient to *grad_eta instead of storing it.
 int exact_adjoint(const double *q, size_t nq, const double *ev, size_t p, double eta,
                   const double *gre, const double *gim, int f32c, double *grad_ev,
-                  double *grad_q, double *grad_eta, hipStream_t stream);
+                  double *grad_q, double *grad_eta, hipStream_t stream, int accumulate_eta = 0);
 
 // Writes n zeros with a kernel (graph-capturable); n = 0 is a no-op.
 int zero_fill(double *p, size_t n, hipStream_t stream);
@@ -92,6 +93,14 @@ int ek_fit(const double *x, const double *g_re, const double *g_im, size_t m, do
            double c, const double *t, const double *w, size_t k, double *loss,
            double *grad_g_re, double *grad_g_im, double *grad_w, double *grad_eta,
            hipStream_t stream);
+
+// One El Karoui fit step on device: the raw exact sums S at z_m = x_m + i eta
+// (FP64 or FP32-compensated), g = -conj(S)/p, the fit loss and the gradients
+// with respect to the eigenvalues (p), eta (one double: the direct term plus
+// the term through S) and the weights (k).  Gradient outputs may be NULL.
+int ek_fit_step(const double *ev, size_t p, const double *x, size_t m, double eta, double c,
+                const double *t, const double *w, size_t k, int f32c, double *loss,
+                double *grad_ev, double *grad_eta, double *grad_w, hipStream_t stream);
 
 // ---- ChebCode (shk_chebcode.hip) --------------------------------------------
 

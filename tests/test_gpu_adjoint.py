@@ -203,3 +203,31 @@ def test_bulk_fit_gradients_match_central_differences(shk):
         l0[k] -= h
         fd = (L(lg=l1) - L(lg=l0)) / (2 * h)
         assert abs(gl[k] - fd) <= 1e-6 * np.max(np.abs(gl)), (k, gl[k], fd)
+
+
+@pytest.mark.parametrize("precision,tol", [("f64", 1e-10), ("f32c", 5e-6)])
+def test_fused_fit_step_matches_reference_and_autograd(shk, orc, precision, tol):
+    """shk_el_karoui_fit_step_dev: the same loss and gradients as the FP64
+    reference gradient (to the path's tolerance) and as the autograd
+    composition of the same kernels (to rounding: 1e-12)."""
+    import torch
+    from shrinkers_amd import autograd as ag
+    ev, grid, atoms, eta, logits = _fit_inputs()
+    c = 0.25
+    w = np.exp(logits - logits.max())
+    w /= w.sum()
+    loss, gev, ge, gw = ag.el_karoui_fit_step(_t(ev), eta, _t(w), _t(atoms), c, _t(grid), precision)
+    torch.cuda.synchronize()
+    loss, gev, ge, gw = float(loss), gev.cpu().numpy(), float(ge), gw.cpu().numpy()
+    l_ref, gev_ref, ge_ref, gl_ref = _reference_fit_gradient(orc, ev, grid, atoms, eta, logits, c)
+    glog = w * (gw - np.dot(w, gw))
+    errs = (abs(loss - l_ref) / l_ref, _rel(gev, gev_ref), abs(ge - ge_ref) / abs(ge_ref),
+            _rel(glog, gl_ref))
+    print(f"fused step {precision}: loss {errs[0]:.2e} lam {errs[1]:.2e} eta {errs[2]:.2e} "
+          f"logits {errs[3]:.2e}", flush=True)
+    assert max(errs) <= tol
+    a_loss, a_gev, a_ge, a_gl = _gpu_fit(ev, grid, atoms, eta, logits, c, precision)
+    assert abs(loss - a_loss) <= 1e-12 * a_loss
+    assert _rel(gev, a_gev) <= 1e-12
+    assert abs(ge - a_ge) <= 1e-12 * abs(a_ge)
+    assert _rel(glog, a_gl) <= 1e-12
