@@ -209,6 +209,14 @@ def chebcode_leg(shk, sync) -> dict:
                          "nodes": b.node_count, "near_pairs": near, "far_terms": far,
                          "eval_tflops": flops / (ms_eval * 1e-3) / 1e12,
                          "near_field_tflops_lower_bound": 7.0 * near / (ms_eval * 1e-3) / 1e12}
+            if name == "chebcode":
+                # evaluate_many: four etas, two per traversal (host outputs)
+                etas = [eta * f for f in (0.5, 1.0, 2.0, 4.0)]
+                b.evaluate_many(etas)
+                t0 = time.perf_counter()
+                for _ in range(3):
+                    b.evaluate_many(etas)
+                row[name]["ms_evaluate_many_4eta_host"] = (time.perf_counter() - t0) / 3 * 1e3
             del b
         res[f"p={p},c={c}"] = row
     return res

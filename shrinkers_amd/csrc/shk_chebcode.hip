@@ -696,24 +696,32 @@ __global__ __launch_bounds__(256) void eval_x2_kernel(EvalView v, int nq, double
                                                       double *__restrict__ im0,
                                                       double *__restrict__ re1,
                                                       double *__restrict__ im1) {
+    // Queries are the tree's sorted sources (dense): the wavefront walks
+    // together as in eval_f64_kernel<true>; each lane's visits and arithmetic
+    // are those of its own traversal.
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nq) return;
-    const double x = v.sorted[i];
+    const bool valid = i < nq;
+    const double x = valid ? v.sorted[i] : 0.0;
     const double e[2] = {e0, e1}, e2[2] = {e0 * e0, e1 * e1};
     const double emin = e0 < e1 ? e0 : e1;
     double ar[2] = {0.0, 0.0}, ai[2] = {0.0, 0.0};
     const int n = v.n;
+    int skip = valid ? 0 : v.N;
     int node = 0;
     while (node < v.N) {
+        node = __builtin_amdgcn_readfirstlane(node);
         const int4 lk = v.link[node];
+        const bool active = node >= skip;
         if (lk.x < 0) {
-            for (int k = lk.z; k < lk.w; ++k) {
-                const double d = x - v.sorted[k];
+            if (active) {
+                for (int k = lk.z; k < lk.w; ++k) {
+                    const double d = x - v.sorted[k];
 #pragma unroll
-                for (int l = 0; l < 2; ++l) {
-                    const double inv = rcp_f64(__builtin_fma(d, d, e2[l]));
-                    ar[l] = __builtin_fma(d, inv, ar[l]);
-                    ai[l] = __builtin_fma(e[l], inv, ai[l]);
+                    for (int l = 0; l < 2; ++l) {
+                        const double inv = rcp_f64(__builtin_fma(d, d, e2[l]));
+                        ar[l] = __builtin_fma(d, inv, ar[l]);
+                        ai[l] = __builtin_fma(e[l], inv, ai[l]);
+                    }
                 }
             }
             node = lk.y;
@@ -723,7 +731,8 @@ __global__ __launch_bounds__(256) void eval_x2_kernel(EvalView v, int nq, double
         const double cl = x < lo ? lo - x : (x > hi ? x - hi : 0.0);
         const double d_sq = cl * cl + emin * emin;
         const double half_w = 0.5 * (hi - lo);
-        if (half_w * half_w < theta_sq * d_sq) {
+        const bool accept = half_w * half_w < theta_sq * d_sq;
+        if (active && accept) {
             const double *ts = v.nodes + (size_t)node * n;
             const double *ws = v.w + (size_t)node * n;
             for (int j = 0; j < n; ++j) {
@@ -735,11 +744,11 @@ __global__ __launch_bounds__(256) void eval_x2_kernel(EvalView v, int nq, double
                     ai[l] = __builtin_fma(wj * e[l], inv, ai[l]);
                 }
             }
-            node = lk.y;
-        } else {
-            node = node + 1;
+            skip = lk.y;
         }
+        node = __any(active && !accept) ? node + 1 : lk.y;
     }
+    if (!valid) return;
     const int o = v.perm ? v.perm[i] : i;
     re0[o] = ar[0];
     im0[o] = ai[0];

@@ -163,6 +163,37 @@ def test_evaluate_many_matches_single(shk, orc):
         assert abs(mr[1, k] - r1) <= 1e-12 * abs(i1) and abs(mi[1, k] - i1) <= 1e-12 * abs(i1)
 
 
+@pytest.mark.parametrize("p", [20_000, 200_000])
+def test_evaluate_many_large(shk, orc, p):
+    """Two etas per traversal at scale.  The pairwise traversal accepts a node
+    only when both etas' tests pass, so at acceptance boundaries it differs
+    from each single-eta traversal by one node's far-field error, on the
+    preset's own accuracy scale (chebcode.rs:1371-1399 bounds it by 5e-8 on
+    raw sums of a p = 512 spectrum).  Raw sums grow with p, so here the bound
+    is taken relative to their size: max |diff| <= 5e-8 max|S| and rel-L2
+    <= 5e-9.  Against the oracle's two-lane traversal (chebcode.rs:836-915)
+    the rows agree to rounding."""
+    from shrinkers_amd import spectra as S
+    ev = S.mp_inverse_cdf_spectrum(p, c=0.3)
+    etas = [0.5 / np.sqrt(p), 1.0 / np.sqrt(p), 3.0 / np.sqrt(p)]
+    b = shk.ChebCodeBatch.build_preset(ev, "chebcode")
+    mr, mi = b.evaluate_many(etas)
+    for k, eta in enumerate(etas):
+        sr, si = b.evaluate(eta)
+        scale = max(np.max(np.abs(sr)), np.max(np.abs(si)))
+        assert np.max(np.abs(mr[k] - sr)) <= 5e-8 * scale
+        assert np.max(np.abs(mi[k] - si)) <= 5e-8 * scale
+        assert rel_l2(mr[k], mi[k], sr, si) <= 5e-9
+    t = orc.ChebTree(np.sort(ev), 11, 0.5, 32)
+    mr, mi = b.evaluate_many(etas[:2])
+    srt = np.sort(ev)
+    rows = np.random.default_rng(0).choice(p, 64, replace=False)
+    for k in rows:   # the input is ascending: build order = sorted order
+        (r0, i0), (r1, i1) = t.eval_x2(srt[k], etas[0], etas[1])
+        assert abs(mr[0, k] - r0) <= 1e-12 * abs(i0) and abs(mi[0, k] - i0) <= 1e-12 * abs(i0)
+        assert abs(mr[1, k] - r1) <= 1e-12 * abs(i1) and abs(mi[1, k] - i1) <= 1e-12 * abs(i1)
+
+
 def test_grid_path_vs_exact(shk, orc):
     # mod.rs:1199-1254: grid ChebCode vs exact at-points; fast < 1e-3, balanced < 1e-6.
     p = 8000
