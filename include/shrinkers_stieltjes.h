@@ -166,6 +166,27 @@ shk_status shk_all_points_dev(const double *d_eigenvalues, size_t p, double eta,
                               double cutoff_ratio, int precision, int scaled,
                               double *d_out_re, double *d_out_im, void *stream);
 
+/* Many independent spectra in one call (no counterpart in the reference,
+ * which batches only across eta on one spectrum, chebcode.rs:1312).
+ * d_eigenvalues holds the spectra back to back; offsets (host, n_spectra+1
+ * entries, offsets[0] = 0) delimit them; etas (host) gives one eta each.
+ * Each spectrum's result is exactly shk_all_points_dev's for that spectrum
+ * and method/precision (scaled by 1/p_s when scaled != 0), written at the
+ * same offsets of d_out_re / d_out_im.  The library enqueues the spectra,
+ * largest first, on n_streams (1..8) streams of its own, each spectrum on
+ * the stream with the least work so far; the batch is ordered after earlier
+ * work on `stream`, and later work on `stream` waits for all of it.  If
+ * times_ms is not NULL (2*n_spectra floats) the call synchronises and
+ * writes each spectrum's start and end in ms from the batch start (device
+ * events), otherwise it returns without synchronising.  This is a host-
+ * orchestrated batch of the single-spectrum kernels; the persistent launch
+ * of shk_batch_* is switched off (see below). */
+shk_status shk_all_points_many_dev(const double *d_eigenvalues, const int64_t *offsets,
+                                   size_t n_spectra, const double *etas, int method,
+                                   int precision, int scaled, double *d_out_re,
+                                   double *d_out_im, int n_streams, float *times_ms,
+                                   void *stream);
+
 /* Raw sums at device query points (exact kernel or ChebCode per method). */
 shk_status shk_at_points_dev(const double *d_query, size_t nq, const double *d_eigenvalues,
                              size_t p, double eta, int method, double cutoff_ratio,

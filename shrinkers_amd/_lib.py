@@ -57,6 +57,8 @@ SIGNATURES = {
     "shk_compute_stieltjes_scaled_ray": (_i, [_dp, _sz, _d, _dp, _dp]),
     "shk_all_points_dev": (_i, [_vp, _sz, _d, _i, _d, _i, _i, _vp, _vp, _vp]),
     "shk_at_points_dev": (_i, [_vp, _sz, _vp, _sz, _d, _i, _d, _vp, _vp, _vp]),
+    "shk_all_points_many_dev": (_i, [_vp, _i64p, _sz, _dp, _i, _i, _i, _vp, _vp, _i,
+                                     ctypes.POINTER(ctypes.c_float), _vp]),
     "shk_chebcode_build": (_i, [_dp, _sz, _d, _i, _sz, _i, ctypes.POINTER(_vp)]),
     "shk_chebcode_build_preset": (_i, [_dp, _sz, _i, ctypes.POINTER(_vp)]),
     "shk_chebcode_info": (_i, [_vp, _szp, _szp]),

@@ -1,9 +1,12 @@
-"""Many spectra in one persistent GPU launch (hipGraph), FP32-compensated.
+"""Many independent spectra of uneven size.
 
-Each spectrum's result is its own all-points transform, as
-``compute_all_stieltjes(method="blocked", precision="f32c")`` would give it;
-the batch runs them all in one captured launch with a work queue, and keeps
-per-spectrum device timestamps for latency statistics.
+``SpectrumBatch`` / ``all_points_many_dev``: one library call enqueues every
+spectrum's all-points transform (the single-spectrum kernels, largest
+spectra first, spread over a few library streams) and can return per-spectrum
+device start/end times.  Each result equals the single-spectrum call.
+
+``BatchedStieltjes``: the persistent work-queue launch captured in a hipGraph.
+It is switched off in this build (the library refuses to create it).
 """
 from __future__ import annotations
 
@@ -14,7 +17,69 @@ import numpy as np
 
 from . import _lib
 from ._lib import check
-from .stieltjes import _stream_handle
+from .stieltjes import _PRECISIONS, _require_gpu_method, _stream_handle, parse_method
+
+
+def all_points_many_dev(ev, offsets, etas, method: str = "blocked", precision: str = "f32c",
+                        scaled: bool = True, n_streams: int = 4, out=None, times: bool = False,
+                        stream=None):
+    """All-points transforms of the spectra packed back to back in the CUDA
+    float64 tensor ``ev`` (``offsets``: n+1 int64 starts, ``etas``: n etas).
+    Returns ``(re, im)`` packed the same way, plus an (n, 2) array of
+    per-spectrum start/end milliseconds from the batch start when ``times``
+    (which synchronises)."""
+    import torch
+    if not (ev.is_cuda and ev.dtype == torch.float64 and ev.dim() == 1 and ev.is_contiguous()):
+        raise ValueError("ev must be a contiguous 1-D float64 CUDA tensor")
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    e = np.ascontiguousarray(etas, dtype=np.float64)
+    if off.size != e.size + 1 or (off.size and off[-1] != ev.numel()):
+        raise ValueError("offsets must have one more entry than etas and end at ev.numel()")
+    mid = parse_method(method)
+    _require_gpu_method(mid)
+    re, im = out if out is not None else (torch.empty_like(ev), torch.empty_like(ev))
+    t = np.zeros((e.size, 2), dtype=np.float32) if times else None
+    check(_lib.load().shk_all_points_many_dev(
+        ev.data_ptr(), off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), e.size, _lib.dptr(e),
+        mid, _PRECISIONS[precision], int(bool(scaled)), re.data_ptr(), im.data_ptr(),
+        int(n_streams),
+        t.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if times else None,
+        _stream_handle(stream)))
+    return (re, im, t.astype(np.float64)) if times else (re, im)
+
+
+class SpectrumBatch:
+    """Spectra (a list of 1-D arrays) copied to the GPU once, back to back,
+    with one eta each; ``run`` transforms them all in one library call."""
+
+    def __init__(self, spectra: Sequence, etas: Sequence[float]):
+        import torch
+        arrays = [np.ascontiguousarray(s, dtype=np.float64) for s in spectra]
+        if len(arrays) != len(etas):
+            raise ValueError("one eta per spectrum")
+        self.sizes = np.array([a.size for a in arrays], dtype=np.int64)
+        self.offsets = np.zeros(len(arrays) + 1, dtype=np.int64)
+        np.cumsum(self.sizes, out=self.offsets[1:])
+        self.etas = np.ascontiguousarray(etas, dtype=np.float64)
+        self.ev = torch.from_numpy(np.concatenate(arrays) if arrays else np.zeros(0)).cuda()
+        self.re = torch.empty_like(self.ev)
+        self.im = torch.empty_like(self.ev)
+
+    def run(self, method: str = "blocked", precision: str = "f32c", scaled: bool = True,
+            n_streams: int = 4, times: bool = False, stream=None):
+        """Enqueue every transform; returns the per-spectrum (start, end) ms
+        array when ``times``, else None."""
+        res = all_points_many_dev(self.ev, self.offsets, self.etas, method, precision, scaled,
+                                  n_streams, out=(self.re, self.im), times=times, stream=stream)
+        return res[2] if times else None
+
+    def results(self):
+        """Per-spectrum (re, im) numpy arrays of the last run (synchronises)."""
+        import torch
+        torch.cuda.synchronize()
+        re = self.re.cpu().numpy()
+        im = self.im.cpu().numpy()
+        return [(re[a:b], im[a:b]) for a, b in zip(self.offsets[:-1], self.offsets[1:])]
 
 
 class BatchedStieltjes:

@@ -335,6 +335,43 @@ int run_atpoints(const double *d_q, size_t nq, const double *d_ev, size_t p, dou
                      cutoff_ratio * eta, 1.0, outs, s);
 }
 
+// Streams and events the multi-spectrum entry point reuses: created on
+// first use on the calling thread (default priority), kept until the thread
+// ends.
+constexpr int kMaxManyStreams = 8;
+struct ManyPools {
+    std::vector<hipStream_t> streams;
+    std::vector<hipEvent_t> done;    // one per stream, no timing
+    std::vector<hipEvent_t> events;  // batch start + per-spectrum start/end
+    int reserve(int n_streams, size_t n_events) {
+        while ((int)streams.size() < n_streams) {
+            hipStream_t st = nullptr;
+            hipEvent_t ev = nullptr;
+            if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+                set_error("could not create a stream for the multi-spectrum launch");
+                return SHK_ERR_HIP;
+            }
+            streams.push_back(st);
+            done.push_back(ev);
+        }
+        while (events.size() < n_events) {
+            hipEvent_t ev = nullptr;
+            if (hipEventCreate(&ev) != hipSuccess) {
+                set_error("could not create a timing event");
+                return SHK_ERR_HIP;
+            }
+            events.push_back(ev);
+        }
+        return 0;
+    }
+};
+
+ManyPools &many_pools() {
+    thread_local ManyPools pools;
+    return pools;
+}
+
 #define SHK_CHECK(expr)                     \
     do {                                    \
         int s_ = (expr);                    \
@@ -570,6 +607,104 @@ shk_status shk_all_points_dev(const double *d_eigenvalues, size_t p, double eta,
     const double scale = scaled ? 1.0 / (double)p : 1.0;
     return (shk_status)run_allpoints(d_eigenvalues, p, eta, method, cutoff_ratio, precision,
                                      scale, d_out_re, d_out_im, (hipStream_t)stream);
+}
+
+shk_status shk_all_points_many_dev(const double *d_eigenvalues, const int64_t *offsets,
+                                   size_t n_spectra, const double *etas, int method,
+                                   int precision, int scaled, double *d_out_re,
+                                   double *d_out_im, int n_streams, float *times_ms,
+                                   void *stream) {
+    if (n_spectra == 0) return SHK_OK;
+    if (!d_eigenvalues || !offsets || !etas || !d_out_re || !d_out_im)
+        return fail(SHK_ERR_BAD_ARG, "NULL pointer");
+    if (offsets[0] != 0) return fail(SHK_ERR_BAD_ARG, "offsets[0] must be 0");
+    if (n_streams < 1 || n_streams > kMaxManyStreams)
+        return fail(SHK_ERR_BAD_ARG, "n_streams must be in [1, " +
+                                         std::to_string(kMaxManyStreams) + "]");
+    for (size_t k = 0; k < n_spectra; ++k) {
+        if (offsets[k + 1] < offsets[k]) return fail(SHK_ERR_BAD_ARG, "offsets must not decrease");
+        SHK_CHECK(check_eta(etas[k]));
+    }
+    if (precision != SHK_PREC_F64 && precision != SHK_PREC_F32C)
+        return fail(SHK_ERR_BAD_ARG, "precision must be SHK_PREC_F64 or SHK_PREC_F32C");
+    method = shk_resolve_auto_method(method, 0, (size_t)offsets[n_spectra]);
+    SHK_CHECK(check_method(method));
+    SHK_CHECK(ensure_device());
+    ManyPools &pool = many_pools();
+    SHK_CHECK(pool.reserve(n_streams, times_ms ? 2 * n_spectra + 1 : 1));
+    // Largest spectra first, each to the stream with the least p^2 so far.
+    std::vector<size_t> order(n_spectra);
+    for (size_t k = 0; k < n_spectra; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+        return offsets[a + 1] - offsets[a] > offsets[b + 1] - offsets[b];
+    });
+    // Assign streams first, then give each stream one partial-sum workspace of
+    // the largest size its exact-family spectra need (reused in stream order).
+    const bool exact = family_of(method) == FAM_EXACT;
+    const ExactKind kind = allpoints_kind(method, 0.0);
+    std::vector<double> load(n_streams, 0.0);
+    std::vector<int> stream_of(n_spectra, -1);
+    std::vector<size_t> ws_need(n_streams, 0);
+    for (size_t idx = 0; idx < n_spectra; ++idx) {
+        const size_t k = order[idx];
+        const size_t p = (size_t)(offsets[k + 1] - offsets[k]);
+        if (p == 0) continue;
+        const int j = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[j] += (double)p * (double)p;
+        stream_of[k] = j;
+        if (exact) ws_need[j] = std::max(ws_need[j], exact_workspace_doubles(p, p, kind));
+    }
+    hipStream_t caller = (hipStream_t)stream;
+    hipEvent_t t0 = pool.events[0];
+    SHK_HIP_S(hipEventRecord(t0, caller));
+    std::vector<double *> ws(n_streams, nullptr);
+    for (int j = 0; j < n_streams; ++j) {
+        SHK_HIP_S(hipStreamWaitEvent(pool.streams[j], t0, 0));
+        if (ws_need[j])
+            SHK_HIP_S(hipMallocAsync((void **)&ws[j], sizeof(double) * ws_need[j],
+                                     pool.streams[j]));
+    }
+    int st = 0;
+    for (size_t idx = 0; idx < n_spectra && !st; ++idx) {
+        const size_t k = order[idx];
+        const int j = stream_of[k];
+        if (j < 0) continue;
+        const size_t p = (size_t)(offsets[k + 1] - offsets[k]);
+        hipStream_t s = pool.streams[j];
+        if (times_ms) SHK_HIP_S(hipEventRecord(pool.events[1 + 2 * k], s));
+        const double scale = scaled ? 1.0 / (double)p : 1.0;
+        const double *ev = d_eigenvalues + offsets[k];
+        double *re = d_out_re + offsets[k], *im = d_out_im + offsets[k];
+        if (exact && precision == SHK_PREC_F32C && kind == EXACT_PLAIN) {
+            st = exact_f32c(ev, p, ev, p, etas[k], scale, re, im, 0, s, ws[j]);
+        } else if (exact) {
+            double *outs[2] = {re, im};
+            st = exact_f64(ev, p, ev, p, etas[k], kind, 0.0, scale, outs, s, ws[j]);
+        } else {
+            st = run_allpoints(ev, p, etas[k], method, 0.0, precision, scale, re, im, s);
+        }
+        if (!st && times_ms) SHK_HIP_S(hipEventRecord(pool.events[2 + 2 * k], s));
+    }
+    for (int j = 0; j < n_streams; ++j)
+        if (ws[j]) (void)hipFreeAsync(ws[j], pool.streams[j]);
+    if (st) return (shk_status)st;
+    for (int j = 0; j < n_streams; ++j) {
+        SHK_HIP_S(hipEventRecord(pool.done[j], pool.streams[j]));
+        SHK_HIP_S(hipStreamWaitEvent(caller, pool.done[j], 0));
+    }
+    if (times_ms) {
+        for (int j = 0; j < n_streams; ++j) SHK_HIP_S(hipEventSynchronize(pool.done[j]));
+        for (size_t k = 0; k < n_spectra; ++k) {
+            float a = 0.f, b = 0.f;
+            if (offsets[k + 1] > offsets[k]) {
+                SHK_HIP_S(hipEventElapsedTime(&a, t0, pool.events[1 + 2 * k]));
+                SHK_HIP_S(hipEventElapsedTime(&b, t0, pool.events[2 + 2 * k]));
+            }
+            times_ms[2 * k] = a;
+            times_ms[2 * k + 1] = b;
+        }
+    }
+    return SHK_OK;
 }
 
 shk_status shk_at_points_dev(const double *d_query, size_t nq, const double *d_eigenvalues,

@@ -290,9 +290,16 @@ int launch_reduce(const double *part, int n_split, size_t nq, double eta, double
 
 }  // namespace
 
+size_t exact_workspace_doubles(size_t nq, size_t p, ExactKind kind) {
+    if (nq == 0 || p == 0) return 0;
+    int qt, ns, per;
+    plan_splits(nq, p, &qt, &ns, &per);
+    return (size_t)((kind == EXACT_DERIV) ? 4 : 2) * (size_t)ns * nq;
+}
+
 int exact_f64(const double *q, size_t nq, const double *src, size_t p, double eta,
               ExactKind kind, double cut, double scale, double *const *out,
-              hipStream_t stream) {
+              hipStream_t stream, double *workspace) {
     if (nq == 0 || p == 0) return 0;
     if (nq > (size_t)INT32_MAX || p > (size_t)INT32_MAX) {
         set_error("exact kernel: sizes above 2^31-1 are not supported");
@@ -301,8 +308,9 @@ int exact_f64(const double *q, size_t nq, const double *src, size_t p, double et
     int qt, ns, per;
     plan_splits(nq, p, &qt, &ns, &per);
     const int na = (kind == EXACT_DERIV) ? 4 : 2;
-    double *part = nullptr;
-    SHK_HIP(hipMallocAsync((void **)&part, sizeof(double) * na * (size_t)ns * nq, stream));
+    double *part = workspace;
+    if (!part)
+        SHK_HIP(hipMallocAsync((void **)&part, sizeof(double) * na * (size_t)ns * nq, stream));
     int st = 0;
     switch (kind) {
         case EXACT_PLAIN:
@@ -329,7 +337,7 @@ int exact_f64(const double *q, size_t nq, const double *src, size_t p, double et
             st = 6;
             set_error("exact kernel: unknown kind");
     }
-    hipError_t fe = hipFreeAsync(part, stream);
+    hipError_t fe = workspace ? hipSuccess : hipFreeAsync(part, stream);
     if (st) return st;
     if (fe != hipSuccess) {
         set_error(std::string("hipFreeAsync: ") + hipGetErrorString(fe));
@@ -339,7 +347,8 @@ int exact_f64(const double *q, size_t nq, const double *src, size_t p, double et
 }
 
 int exact_f32c(const double *q, size_t nq, const double *src, size_t p, double eta,
-               double scale, void *out_re, void *out_im, int out_f32, hipStream_t stream) {
+               double scale, void *out_re, void *out_im, int out_f32, hipStream_t stream,
+               double *workspace) {
     if (nq == 0 || p == 0) return 0;
     if (nq > (size_t)INT32_MAX || p > (size_t)INT32_MAX) {
         set_error("exact kernel: sizes above 2^31-1 are not supported");
@@ -347,8 +356,9 @@ int exact_f32c(const double *q, size_t nq, const double *src, size_t p, double e
     }
     int qt, ns, per;
     plan_splits(nq, p, &qt, &ns, &per);
-    double *part = nullptr;
-    SHK_HIP(hipMallocAsync((void **)&part, sizeof(double) * 2 * (size_t)ns * nq, stream));
+    double *part = workspace;
+    if (!part)
+        SHK_HIP(hipMallocAsync((void **)&part, sizeof(double) * 2 * (size_t)ns * nq, stream));
     dim3 grid(qt, ns);
     ProfileMark m = profile_begin(stream);
     hipLaunchKernelGGL(exact_f32c_kernel, grid, dim3(kWG), 0, stream, q, (int)nq, src, (int)p,
@@ -369,7 +379,7 @@ int exact_f32c(const double *q, size_t nq, const double *src, size_t p, double e
                                                     (double *)out_re, (double *)out_im, nullptr,
                                                     nullptr, stream);
     }
-    hipError_t fe = hipFreeAsync(part, stream);
+    hipError_t fe = workspace ? hipSuccess : hipFreeAsync(part, stream);
     if (st) return st;
     if (fe != hipSuccess) {
         set_error(std::string("hipFreeAsync: ") + hipGetErrorString(fe));

@@ -59,14 +59,20 @@ enum ExactKind : int {
 //   cut      : absolute cutoff distance (ratio*eta) for CUTOFF/HYBRID
 //   scale    : multiplier applied to every output (1 or 1/p)
 //   out[0..n_out) : re, im (+ deriv re, deriv im for EXACT_DERIV)
+//   workspace: device scratch of exact_workspace_doubles(nq, p, kind)
+//              doubles, or NULL to allocate it stream-ordered per call
 int exact_f64(const double *q, size_t nq, const double *src, size_t p, double eta,
               ExactKind kind, double cut, double scale, double *const *out,
-              hipStream_t stream);
+              hipStream_t stream, double *workspace = nullptr);
 
 // FP32-compensated exact sums (no cutoff).  out_re/out_im are double, or
-// float when out_f32 != 0.
+// float when out_f32 != 0.  workspace as for exact_f64 (kind EXACT_PLAIN).
 int exact_f32c(const double *q, size_t nq, const double *src, size_t p, double eta,
-               double scale, void *out_re, void *out_im, int out_f32, hipStream_t stream);
+               double scale, void *out_re, void *out_im, int out_f32, hipStream_t stream,
+               double *workspace = nullptr);
+
+// Scratch doubles the exact kernels need for nq queries against p sources.
+size_t exact_workspace_doubles(size_t nq, size_t p, ExactKind kind);
 
 // float -> double widening on device (for the f32 entry points).
 int widen_f32(const float *in, double *out, size_t n, hipStream_t stream);

@@ -118,3 +118,29 @@ def test_batch_argument_checks(shk):
     assert lib.shk_batch_create(dummy, offp, 2, _lib.dptr(eta), 1, ctypes.byref(h)) == \
         _lib.SHK_ERR_UNSUPPORTED
     assert "switched off" in _lib.last_error() and not h.value
+
+
+def test_many_argument_checks(shk):
+    """shk_all_points_many_dev rejects bad arguments before any device work."""
+    from shrinkers_amd import _lib
+    lib = _lib.load()
+    eta = np.array([0.1, 0.1])
+    dummy = ctypes.c_void_p(16)  # never dereferenced: the checks fail first
+    i64 = lambda a: np.asarray(a, dtype=np.int64).ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    off = np.array([0, 3, 5], dtype=np.int64)
+    assert lib.shk_all_points_many_dev(dummy, i64([1, 3, 5]), 2, _lib.dptr(eta), 2, 1, 1, dummy,
+                                       dummy, 2, None, None) == _lib.SHK_ERR_BAD_ARG
+    assert lib.shk_all_points_many_dev(dummy, i64([0, 3, 2]), 2, _lib.dptr(eta), 2, 1, 1, dummy,
+                                       dummy, 2, None, None) == _lib.SHK_ERR_BAD_ARG
+    assert lib.shk_all_points_many_dev(dummy, off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                       2, _lib.dptr(eta), 2, 1, 1, dummy, dummy, 0, None,
+                                       None) == _lib.SHK_ERR_BAD_ARG
+    bad = np.array([0.1, -2.0])
+    assert lib.shk_all_points_many_dev(dummy, off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                       2, _lib.dptr(bad), 2, 1, 1, dummy, dummy, 2, None,
+                                       None) == _lib.SHK_ERR_BAD_ETA
+    assert lib.shk_all_points_many_dev(dummy, off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                       2, _lib.dptr(eta), 8, 1, 1, dummy, dummy, 2, None,
+                                       None) == _lib.SHK_ERR_UNSUPPORTED   # fft5
+    assert lib.shk_all_points_many_dev(None, None, 0, None, 2, 1, 1, None, None, 2, None,
+                                       None) == _lib.SHK_OK
