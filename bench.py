@@ -170,9 +170,11 @@ def box_peaks() -> dict:
 
 def many_leg(shk, sync) -> dict:
     """512 independent spectra of uneven p in one library call
-    (shk_all_points_many_dev: the single-spectrum FP32-compensated kernels,
-    largest spectra first, over 1 or 4 library streams); the persistent
-    hipGraph launch is switched off.  Per-spectrum times are device events."""
+    (shk_all_points_many_dev): the segmented launch (one grid over every
+    spectrum's query tiles, FP32-compensated and FP64) and the stream-pool
+    path (the single-spectrum kernels, largest spectra first, over 1 or 4
+    library streams); the persistent hipGraph launch is switched off.
+    Times are device events."""
     import torch
     from shrinkers_amd import spectra
     from shrinkers_amd.batch import SpectrumBatch
@@ -182,28 +184,32 @@ def many_leg(shk, sync) -> dict:
     pairs = float(np.sum(np.asarray([s.size for s in sp], dtype=np.float64) ** 2))
     out = {"workload": "512 spectra, p~U[500,20000], c~U[0.05,0.5], sigma2 log-U[0.5,2], 0-5 "
                        "spikes U[2,20]*sigma2 at forward-BBP images; eta=1/sqrt(p); "
-                       "FP32-compensated exact kernel per spectrum",
+                       "exact all-points sums per spectrum",
            "pair_evals_per_batch": pairs}
-    for ns in (1, 4):
+    for label, ns, prec in (("segmented", 0, "f32c"), ("segmented_f64", 0, "f64"),
+                            ("streams_1", 1, "f32c"), ("streams_4", 4, "f32c")):
         for _ in range(2):
-            b.run("blocked", "f32c", True, ns)
+            b.run("blocked", prec, True, ns)
         sync()
         reps = 5
         t0 = time.perf_counter()
         for _ in range(reps):
-            b.run("blocked", "f32c", True, ns)
+            b.run("blocked", prec, True, ns)
         sync()
         ms = (time.perf_counter() - t0) / reps * 1e3
-        t = b.run("blocked", "f32c", True, ns, times=True)
-        dur = t[:, 1] - t[:, 0]
-        out[f"streams_{ns}"] = {
-            "ms_per_batch": ms, "spectra_per_s": 512 / (ms * 1e-3),
-            "pair_evals_per_s": pairs / (ms * 1e-3),
-            "spectrum_ms_p50": float(np.percentile(dur, 50)),
-            "spectrum_ms_p99": float(np.percentile(dur, 99)),
-            "completion_ms_p50": float(np.percentile(t[:, 1], 50)),
-            "completion_ms_p99": float(np.percentile(t[:, 1], 99)),
-        }
+        t = b.run("blocked", prec, True, ns, times=True)
+        rec = {"precision": prec, "ms_per_batch": ms, "spectra_per_s": 512 / (ms * 1e-3),
+               "pair_evals_per_s": pairs / (ms * 1e-3)}
+        if ns == 0:
+            rec["launch_ms_events"] = float(t[:, 1].max())
+        else:
+            dur = t[:, 1] - t[:, 0]
+            rec.update({"spectrum_ms_p50": float(np.percentile(dur, 50)),
+                        "spectrum_ms_p99": float(np.percentile(dur, 99)),
+                        "completion_ms_p50": float(np.percentile(t[:, 1], 50)),
+                        "completion_ms_p99": float(np.percentile(t[:, 1], 99))})
+        out[label] = rec
+    b.run("blocked", "f32c", True, 0)
     res = b.results()
     worst = 0.0
     for k in np.random.default_rng(1).choice(len(sp), 8, replace=False):

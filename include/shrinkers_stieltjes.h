@@ -172,15 +172,19 @@ shk_status shk_all_points_dev(const double *d_eigenvalues, size_t p, double eta,
  * entries, offsets[0] = 0) delimit them; etas (host) gives one eta each.
  * Each spectrum's result is exactly shk_all_points_dev's for that spectrum
  * and method/precision (scaled by 1/p_s when scaled != 0), written at the
- * same offsets of d_out_re / d_out_im.  The library enqueues the spectra,
- * largest first, on n_streams (1..8) streams of its own, each spectrum on
- * the stream with the least work so far; the batch is ordered after earlier
- * work on `stream`, and later work on `stream` waits for all of it.  If
- * times_ms is not NULL (2*n_spectra floats) the call synchronises and
- * writes each spectrum's start and end in ms from the batch start (device
- * events), otherwise it returns without synchronising.  This is a host-
- * orchestrated batch of the single-spectrum kernels; the persistent launch
- * of shk_batch_* is switched off (see below). */
+ * same offsets of d_out_re / d_out_im.
+ * n_streams = 0 (recommended): for the exact family, ONE segmented launch on
+ * `stream` covers every spectrum (one workgroup per spectrum and 128-query
+ * tile, largest spectra first; no partial-sum scratch); for ChebCode, as
+ * n_streams = 4.  n_streams = 1..8: the library enqueues the single-spectrum
+ * kernels, largest spectrum first, on that many streams of its own, each
+ * spectrum on the stream with the least work so far.  Either way the batch
+ * is ordered after earlier work on `stream`, and later work on `stream`
+ * waits for all of it.  If times_ms is not NULL (2*n_spectra floats) the
+ * call synchronises and writes each spectrum's start and end in ms from the
+ * batch start (device events; on the segmented launch every spectrum gets
+ * 0 and the launch's end), otherwise it returns without synchronising.  The
+ * persistent launch of shk_batch_* is switched off (see below). */
 shk_status shk_all_points_many_dev(const double *d_eigenvalues, const int64_t *offsets,
                                    size_t n_spectra, const double *etas, int method,
                                    int precision, int scaled, double *d_out_re,

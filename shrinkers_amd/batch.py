@@ -1,9 +1,11 @@
 """Many independent spectra of uneven size.
 
-``SpectrumBatch`` / ``all_points_many_dev``: one library call enqueues every
-spectrum's all-points transform (the single-spectrum kernels, largest
-spectra first, spread over a few library streams) and can return per-spectrum
-device start/end times.  Each result equals the single-spectrum call.
+``SpectrumBatch`` / ``all_points_many_dev``: one library call transforms
+every spectrum -- for the exact family as one segmented kernel launch over
+all spectra (``n_streams=0``, the default), or as the single-spectrum kernels
+spread over a few library streams (``n_streams`` 1..8, which can also return
+per-spectrum device start/end times).  Each result equals the
+single-spectrum call bit for bit.
 
 ``BatchedStieltjes``: the persistent work-queue launch captured in a hipGraph.
 It is switched off in this build (the library refuses to create it).
@@ -21,7 +23,7 @@ from .stieltjes import _PRECISIONS, _require_gpu_method, _stream_handle, parse_m
 
 
 def all_points_many_dev(ev, offsets, etas, method: str = "blocked", precision: str = "f32c",
-                        scaled: bool = True, n_streams: int = 4, out=None, times: bool = False,
+                        scaled: bool = True, n_streams: int = 0, out=None, times: bool = False,
                         stream=None):
     """All-points transforms of the spectra packed back to back in the CUDA
     float64 tensor ``ev`` (``offsets``: n+1 int64 starts, ``etas``: n etas).
@@ -66,7 +68,7 @@ class SpectrumBatch:
         self.im = torch.empty_like(self.ev)
 
     def run(self, method: str = "blocked", precision: str = "f32c", scaled: bool = True,
-            n_streams: int = 4, times: bool = False, stream=None):
+            n_streams: int = 0, times: bool = False, stream=None):
         """Enqueue every transform; returns the per-spectrum (start, end) ms
         array when ``times``, else None."""
         res = all_points_many_dev(self.ev, self.offsets, self.etas, method, precision, scaled,

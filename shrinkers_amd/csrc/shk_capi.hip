@@ -339,6 +339,7 @@ int run_atpoints(const double *d_q, size_t nq, const double *d_ev, size_t p, dou
 // first use on the calling thread (default priority), kept until the thread
 // ends.
 constexpr int kMaxManyStreams = 8;
+constexpr int kDefaultManyStreams = 4;  // n_streams = 0 on the stream-pool path
 struct ManyPools {
     std::vector<hipStream_t> streams;
     std::vector<hipEvent_t> done;    // one per stream, no timing
@@ -363,6 +364,34 @@ struct ManyPools {
             }
             events.push_back(ev);
         }
+        return 0;
+    }
+    // Pinned host copy of the segmented launch's spectrum table.  `staged`
+    // marks the end of the last copy out of it; it is waited on before the
+    // table is rewritten, so back-to-back calls never overwrite a pending copy.
+    SegSpectrum *staging = nullptr;
+    size_t staging_cap = 0;
+    hipEvent_t staged = nullptr;
+    int reserve_table(size_t n) {
+        if (!staged && hipEventCreateWithFlags(&staged, hipEventDisableTiming) != hipSuccess) {
+            set_error("could not create an event for the segmented launch");
+            return SHK_ERR_HIP;
+        }
+        if (hipEventSynchronize(staged) != hipSuccess) {
+            set_error("waiting for the previous spectrum-table copy failed");
+            return SHK_ERR_HIP;
+        }
+        if (staging_cap >= n) return 0;
+        if (staging) (void)hipHostFree(staging);
+        staging = nullptr;
+        staging_cap = 0;
+        const size_t cap = std::max<size_t>(n, 1024);
+        if (hipHostMalloc((void **)&staging, cap * sizeof(SegSpectrum), hipHostMallocDefault) !=
+            hipSuccess) {
+            set_error("could not allocate the pinned spectrum table");
+            return SHK_ERR_HIP;
+        }
+        staging_cap = cap;
         return 0;
     }
 };
@@ -618,8 +647,8 @@ shk_status shk_all_points_many_dev(const double *d_eigenvalues, const int64_t *o
     if (!d_eigenvalues || !offsets || !etas || !d_out_re || !d_out_im)
         return fail(SHK_ERR_BAD_ARG, "NULL pointer");
     if (offsets[0] != 0) return fail(SHK_ERR_BAD_ARG, "offsets[0] must be 0");
-    if (n_streams < 1 || n_streams > kMaxManyStreams)
-        return fail(SHK_ERR_BAD_ARG, "n_streams must be in [1, " +
+    if (n_streams < 0 || n_streams > kMaxManyStreams)
+        return fail(SHK_ERR_BAD_ARG, "n_streams must be in [0, " +
                                          std::to_string(kMaxManyStreams) + "]");
     for (size_t k = 0; k < n_spectra; ++k) {
         if (offsets[k + 1] < offsets[k]) return fail(SHK_ERR_BAD_ARG, "offsets must not decrease");
@@ -630,18 +659,54 @@ shk_status shk_all_points_many_dev(const double *d_eigenvalues, const int64_t *o
     method = shk_resolve_auto_method(method, 0, (size_t)offsets[n_spectra]);
     SHK_CHECK(check_method(method));
     SHK_CHECK(ensure_device());
+    const bool exact = family_of(method) == FAM_EXACT;
+    const ExactKind kind = allpoints_kind(method, 0.0);
     ManyPools &pool = many_pools();
-    SHK_CHECK(pool.reserve(n_streams, times_ms ? 2 * n_spectra + 1 : 1));
-    // Largest spectra first, each to the stream with the least p^2 so far.
+    // Largest spectra first (in the segmented grid and on the streams alike).
     std::vector<size_t> order(n_spectra);
     for (size_t k = 0; k < n_spectra; ++k) order[k] = k;
     std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
         return offsets[a + 1] - offsets[a] > offsets[b + 1] - offsets[b];
     });
-    // Assign streams first, then give each stream one partial-sum workspace of
-    // the largest size its exact-family spectra need (reused in stream order).
-    const bool exact = family_of(method) == FAM_EXACT;
-    const ExactKind kind = allpoints_kind(method, 0.0);
+    hipStream_t caller = (hipStream_t)stream;
+    if (n_streams == 0 && exact && kind == EXACT_PLAIN) {
+        // One segmented launch on the caller's stream: no library streams,
+        // no partial sums.
+        SHK_CHECK(pool.reserve(0, times_ms ? 2 : 0));
+        SHK_CHECK(pool.reserve_table(n_spectra));
+        int n_table = 0, n_tiles = 0;
+        SHK_CHECK(exact_many_plan(offsets, order.data(), n_spectra, etas, scaled, pool.staging,
+                                  &n_table, &n_tiles));
+        float ms = 0.f;
+        if (n_table > 0) {
+            SegSpectrum *d_table = nullptr;
+            SHK_HIP_S(hipMallocAsync((void **)&d_table, sizeof(SegSpectrum) * n_table, caller));
+            SHK_HIP_S(hipMemcpyAsync(d_table, pool.staging, sizeof(SegSpectrum) * n_table,
+                                     hipMemcpyHostToDevice, caller));
+            SHK_HIP_S(hipEventRecord(pool.staged, caller));
+            if (times_ms) SHK_HIP_S(hipEventRecord(pool.events[0], caller));
+            const int st = exact_many(d_eigenvalues, d_table, n_table, n_tiles,
+                                      precision == SHK_PREC_F32C, d_out_re, d_out_im, caller);
+            if (!st && times_ms) SHK_HIP_S(hipEventRecord(pool.events[1], caller));
+            (void)hipFreeAsync(d_table, caller);
+            if (st) return (shk_status)st;
+            if (times_ms) {
+                SHK_HIP_S(hipEventSynchronize(pool.events[1]));
+                SHK_HIP_S(hipEventElapsedTime(&ms, pool.events[0], pool.events[1]));
+            }
+        }
+        if (times_ms)
+            for (size_t k = 0; k < n_spectra; ++k) {
+                times_ms[2 * k] = 0.f;
+                times_ms[2 * k + 1] = (offsets[k + 1] > offsets[k]) ? ms : 0.f;
+            }
+        return SHK_OK;
+    }
+    if (n_streams == 0) n_streams = kDefaultManyStreams;
+    SHK_CHECK(pool.reserve(n_streams, times_ms ? 2 * n_spectra + 1 : 1));
+    // Each spectrum to the stream with the least p^2 so far; then each stream
+    // gets one partial-sum workspace of the largest size its exact-family
+    // spectra need (reused in stream order).
     std::vector<double> load(n_streams, 0.0);
     std::vector<int> stream_of(n_spectra, -1);
     std::vector<size_t> ws_need(n_streams, 0);
@@ -654,7 +719,6 @@ shk_status shk_all_points_many_dev(const double *d_eigenvalues, const int64_t *o
         stream_of[k] = j;
         if (exact) ws_need[j] = std::max(ws_need[j], exact_workspace_doubles(p, p, kind));
     }
-    hipStream_t caller = (hipStream_t)stream;
     hipEvent_t t0 = pool.events[0];
     SHK_HIP_S(hipEventRecord(t0, caller));
     std::vector<double *> ws(n_streams, nullptr);

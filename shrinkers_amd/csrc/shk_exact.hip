@@ -139,24 +139,20 @@ __global__ __launch_bounds__(kWG) void exact_f64_kernel(
 
 // ---- FP32-compensated pair kernel ------------------------------------------
 
-__global__ __launch_bounds__(kWG) void exact_f32c_kernel(const double *__restrict__ q, int nq,
-                                                         const double *__restrict__ src, int p,
-                                                         double eta, int src_per_split,
-                                                         double *__restrict__ part) {
-    __shared__ float s_hi[kPiece];
-    __shared__ float s_lo[kPiece];
-    __shared__ double s_red[2][kWG / 64];
+// Loads the kR queries each lane owns in query tile `tile` of q[0, nq), finds
+// the tile centre c (midpoint of its query range, FP64) and the pair form,
+// and splits x - c into FP32 hi/lo parts.  Shared by the single-spectrum and
+// the segmented kernels so that both see the same tile arithmetic.
+__device__ __forceinline__ int f32c_tile_setup(const double *__restrict__ q, int nq, int tile,
+                                               double eta, double (*s_red)[kWG / 64],
+                                               double *c_out, float (&qh)[kR],
+                                               float (&ql)[kR]) {
     const int tid = threadIdx.x;
-    const int split = blockIdx.y;
-    const int n_split = gridDim.y;
-    const int j_begin = split * src_per_split;
-    const int j_end = min(j_begin + src_per_split, p);
-
     double x[kR];
     double mn = __builtin_inf(), mx = -__builtin_inf();
 #pragma unroll
     for (int k = 0; k < kR; ++k) {
-        const int i = blockIdx.x * kQT + k * kWG + tid;
+        const int i = tile * kQT + k * kWG + tid;
         x[k] = (i < nq) ? q[i] : 0.0;
         if (i < nq) {
             mn = fmin(mn, x[k]);
@@ -178,36 +174,66 @@ __global__ __launch_bounds__(kWG) void exact_f32c_kernel(const double *__restric
         mx = fmax(mx, s_red[1][w]);
     }
     const double c = 0.5 * (mn + mx);
-    const double half_w = 0.5 * (mx - mn);
-    const int form = f32c_form(half_w, eta);
-
-    float qh[kR], ql[kR];
-    double re[kR], iv[kR];
 #pragma unroll
     for (int k = 0; k < kR; ++k) {
         const double t = x[k] - c;
         qh[k] = (float)t;
         ql[k] = (float)(t - (double)qh[k]);
-        re[k] = iv[k] = 0.0;
     }
+    *c_out = c;
+    return f32c_form(0.5 * (mx - mn), eta);
+}
+
+// Stages cnt sources as FP32 hi/lo parts of (src - c) in LDS.
+__device__ __forceinline__ void f32c_stage(const double *__restrict__ src, int cnt, double c,
+                                           float *__restrict__ s_hi, float *__restrict__ s_lo) {
+    for (int j = threadIdx.x; j < cnt; j += kWG) {
+        const double t = src[j] - c;
+        const float h = (float)t;
+        s_hi[j] = h;
+        s_lo[j] = (float)(t - (double)h);
+    }
+}
+
+__device__ __forceinline__ void f32c_sweep_any(int form, const float *s_hi, const float *s_lo,
+                                               int cnt, const float (&qh)[kR],
+                                               const float (&ql)[kR], float eta2f,
+                                               double (&re)[kR], double (&iv)[kR]) {
+    if (form == 0)
+        f32c_sweep<kR, 0>(s_hi, s_lo, cnt, qh, ql, eta2f, re, iv);
+    else if (form == 1)
+        f32c_sweep<kR, 1>(s_hi, s_lo, cnt, qh, ql, eta2f, re, iv);
+    else
+        f32c_sweep<kR, 2>(s_hi, s_lo, cnt, qh, ql, eta2f, re, iv);
+}
+
+__global__ __launch_bounds__(kWG) void exact_f32c_kernel(const double *__restrict__ q, int nq,
+                                                         const double *__restrict__ src, int p,
+                                                         double eta, int src_per_split,
+                                                         double *__restrict__ part) {
+    __shared__ float s_hi[kPiece];
+    __shared__ float s_lo[kPiece];
+    __shared__ double s_red[2][kWG / 64];
+    const int tid = threadIdx.x;
+    const int split = blockIdx.y;
+    const int n_split = gridDim.y;
+    const int j_begin = split * src_per_split;
+    const int j_end = min(j_begin + src_per_split, p);
+
+    double c;
+    float qh[kR], ql[kR];
+    const int form = f32c_tile_setup(q, nq, blockIdx.x, eta, s_red, &c, qh, ql);
+    double re[kR], iv[kR];
+#pragma unroll
+    for (int k = 0; k < kR; ++k) re[k] = iv[k] = 0.0;
     const float eta2f = (float)(eta * eta);
 
     for (int base = j_begin; base < j_end; base += kPiece) {
         const int cnt = min(kPiece, j_end - base);
         __syncthreads();
-        for (int j = tid; j < cnt; j += kWG) {
-            const double t = src[base + j] - c;
-            const float h = (float)t;
-            s_hi[j] = h;
-            s_lo[j] = (float)(t - (double)h);
-        }
+        f32c_stage(src + base, cnt, c, s_hi, s_lo);
         __syncthreads();
-        if (form == 0)
-            f32c_sweep<kR, 0>(s_hi, s_lo, cnt, qh, ql, eta2f, re, iv);
-        else if (form == 1)
-            f32c_sweep<kR, 1>(s_hi, s_lo, cnt, qh, ql, eta2f, re, iv);
-        else
-            f32c_sweep<kR, 2>(s_hi, s_lo, cnt, qh, ql, eta2f, re, iv);
+        f32c_sweep_any(form, s_hi, s_lo, cnt, qh, ql, eta2f, re, iv);
     }
 
 #pragma unroll
@@ -250,6 +276,144 @@ __global__ __launch_bounds__(256) void reduce_kernel(const double *__restrict__ 
 __global__ void widen_kernel(const float *__restrict__ in, double *__restrict__ out, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = (double)in[i];
+}
+
+// ---- segmented launch: the all-points sums of many spectra at once ----------
+//
+// One workgroup per (spectrum, query tile); the tiles of all spectra form one
+// grid, enumerated spectrum by spectrum in the order of the table (largest
+// spectra first, so the long tiles start first and the short ones fill in).
+// A workgroup sweeps its spectrum's whole source range with the per-tile
+// arithmetic of the single-spectrum kernels, and keeps a chunk accumulator
+// that it adds to the running total every `per` sources -- `per` being the
+// split length that the single-spectrum plan picks for that spectrum.  Runs
+// of kF32Run sources start at multiples of 32 from each split start in both
+// (per and kPiece are multiples of 32), so the FP64 additions happen in the
+// same order as split partials followed by the in-order cross-split
+// reduction: every output equals the single-spectrum call bit for bit, with
+// no partial-sum scratch and no second kernel.
+
+__device__ __forceinline__ int seg_find(const SegSpectrum *__restrict__ specs, int n, int tile) {
+    int lo = 0, hi = n - 1;  // the last entry whose first tile is <= tile
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (specs[mid].tile0 <= tile) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Sources [base, base + cnt) are in LDS; sweeps them in sub-ranges cut at the
+// chunk ends, folding the chunk accumulator into the total at each end.
+template <typename Sweep>
+__device__ __forceinline__ void seg_sweep_piece(int base, int cnt, int per, int p,
+                                                int &chunk_end, double (&re)[kR],
+                                                double (&iv)[kR], double (&tr)[kR],
+                                                double (&ti)[kR], Sweep sweep) {
+    int j = 0;
+    while (j < cnt) {
+        const int stop = min(cnt, chunk_end - base);
+        sweep(j, stop - j);
+        j = stop;
+        if (base + j == chunk_end) {
+#pragma unroll
+            for (int k = 0; k < kR; ++k) {
+                tr[k] += re[k];
+                ti[k] += iv[k];
+                re[k] = iv[k] = 0.0;
+            }
+            chunk_end = (p - chunk_end > per) ? chunk_end + per : p;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kWG) void many_f32c_kernel(const double *__restrict__ ev,
+                                                        const SegSpectrum *__restrict__ specs,
+                                                        int n_specs, double *__restrict__ out_re,
+                                                        double *__restrict__ out_im) {
+    __shared__ float s_hi[kPiece];
+    __shared__ float s_lo[kPiece];
+    __shared__ double s_red[2][kWG / 64];
+    const SegSpectrum sp = specs[seg_find(specs, n_specs, (int)blockIdx.x)];
+    const double *__restrict__ src = ev + sp.off;
+    const int p = sp.p;
+    const int tile = (int)blockIdx.x - sp.tile0;
+
+    double c;
+    float qh[kR], ql[kR];
+    const int form = f32c_tile_setup(src, p, tile, sp.eta, s_red, &c, qh, ql);
+    double re[kR], iv[kR], tr[kR], ti[kR];
+#pragma unroll
+    for (int k = 0; k < kR; ++k) re[k] = iv[k] = tr[k] = ti[k] = 0.0;
+    const float eta2f = (float)(sp.eta * sp.eta);
+
+    int chunk_end = min(sp.per, p);
+    for (int base = 0; base < p; base += kPiece) {
+        const int cnt = min(kPiece, p - base);
+        __syncthreads();
+        f32c_stage(src + base, cnt, c, s_hi, s_lo);
+        __syncthreads();
+        seg_sweep_piece(base, cnt, sp.per, p, chunk_end, re, iv, tr, ti, [&](int j0, int n) {
+            f32c_sweep_any(form, s_hi + j0, s_lo + j0, n, qh, ql, eta2f, re, iv);
+        });
+    }
+#pragma unroll
+    for (int k = 0; k < kR; ++k) {
+        const int i = tile * kQT + k * kWG + (int)threadIdx.x;
+        if (i < p) {
+            out_re[sp.off + i] = tr[k] * sp.scale;
+            out_im[sp.off + i] = (sp.eta * ti[k]) * sp.scale;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kWG) void many_f64_kernel(const double *__restrict__ ev,
+                                                       const SegSpectrum *__restrict__ specs,
+                                                       int n_specs, double *__restrict__ out_re,
+                                                       double *__restrict__ out_im) {
+    __shared__ double s_src[kPiece];
+    const SegSpectrum sp = specs[seg_find(specs, n_specs, (int)blockIdx.x)];
+    const double *__restrict__ src = ev + sp.off;
+    const int p = sp.p;
+    const int tile = (int)blockIdx.x - sp.tile0;
+    const int tid = threadIdx.x;
+
+    double x[kR], e2[kR], re[kR], iv[kR], tr[kR], ti[kR];
+#pragma unroll
+    for (int k = 0; k < kR; ++k) {
+        const int i = tile * kQT + k * kWG + tid;
+        x[k] = (i < p) ? src[i] : 0.0;
+        e2[k] = sp.eta * sp.eta;
+        re[k] = iv[k] = tr[k] = ti[k] = 0.0;
+    }
+    int chunk_end = min(sp.per, p);
+    for (int base = 0; base < p; base += kPiece) {
+        const int cnt = min(kPiece, p - base);
+        __syncthreads();
+        for (int j = tid; j < cnt; j += kWG) s_src[j] = src[base + j];
+        __syncthreads();
+        seg_sweep_piece(base, cnt, sp.per, p, chunk_end, re, iv, tr, ti, [&](int j0, int n) {
+#pragma unroll 2
+            for (int j = j0; j < j0 + n; ++j) {
+                const double s = s_src[j];
+#pragma unroll
+                for (int k = 0; k < kR; ++k) {
+                    const double d = x[k] - s;
+                    const double r = rcp_f64(__builtin_fma(d, d, e2[k]));
+                    re[k] = __builtin_fma(d, r, re[k]);
+                    iv[k] += r;
+                }
+            }
+        });
+    }
+#pragma unroll
+    for (int k = 0; k < kR; ++k) {
+        const int i = tile * kQT + k * kWG + tid;
+        if (i < p) {
+            out_re[sp.off + i] = tr[k] * sp.scale;
+            out_im[sp.off + i] = (sp.eta * ti[k]) * sp.scale;
+        }
+    }
 }
 
 // Source splits so that about kTargetWG workgroups are in flight.
@@ -385,6 +549,54 @@ int exact_f32c(const double *q, size_t nq, const double *src, size_t p, double e
         set_error(std::string("hipFreeAsync: ") + hipGetErrorString(fe));
         return 7;
     }
+    return 0;
+}
+
+int exact_many_plan(const int64_t *offsets, const size_t *order, size_t n, const double *etas,
+                    int scaled, SegSpectrum *table, int *n_table, int *n_tiles) {
+    int64_t tiles = 0;
+    int m = 0;
+    for (size_t idx = 0; idx < n; ++idx) {
+        const size_t k = order[idx];
+        const int64_t p = offsets[k + 1] - offsets[k];
+        if (p <= 0) continue;
+        if (p > (int64_t)INT32_MAX) {
+            set_error("exact kernel: sizes above 2^31-1 are not supported");
+            return 6;
+        }
+        int qt, ns, per;
+        plan_splits((size_t)p, (size_t)p, &qt, &ns, &per);
+        SegSpectrum &e = table[m++];
+        e.off = offsets[k];
+        e.p = (int32_t)p;
+        e.per = per;
+        e.tile0 = (int32_t)tiles;
+        e.pad = 0;
+        e.eta = etas[k];
+        e.scale = scaled ? 1.0 / (double)p : 1.0;
+        tiles += qt;
+        if (tiles > (int64_t)INT32_MAX) {
+            set_error("segmented launch: more than 2^31-1 query tiles");
+            return 6;
+        }
+    }
+    *n_table = m;
+    *n_tiles = (int)tiles;
+    return 0;
+}
+
+int exact_many(const double *ev, const SegSpectrum *d_table, int n_table, int n_tiles,
+               int f32c, double *out_re, double *out_im, hipStream_t stream) {
+    if (n_table == 0 || n_tiles == 0) return 0;
+    ProfileMark m = profile_begin(stream);
+    if (f32c)
+        hipLaunchKernelGGL(many_f32c_kernel, dim3(n_tiles), dim3(kWG), 0, stream, ev, d_table,
+                           n_table, out_re, out_im);
+    else
+        hipLaunchKernelGGL(many_f64_kernel, dim3(n_tiles), dim3(kWG), 0, stream, ev, d_table,
+                           n_table, out_re, out_im);
+    profile_end(m, stream);
+    SHK_HIP(hipGetLastError());
     return 0;
 }
 

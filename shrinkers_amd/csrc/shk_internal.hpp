@@ -74,6 +74,27 @@ int exact_f32c(const double *q, size_t nq, const double *src, size_t p, double e
 // Scratch doubles the exact kernels need for nq queries against p sources.
 size_t exact_workspace_doubles(size_t nq, size_t p, ExactKind kind);
 
+// One spectrum of a segmented all-points launch (device table entry).
+struct SegSpectrum {
+    int64_t off;    // index of its first eigenvalue (and first output)
+    int32_t p;      // eigenvalues
+    int32_t per;    // chunk length: the single-spectrum split length for p
+    int32_t tile0;  // its first query tile in the launch
+    int32_t pad;
+    double eta;
+    double scale;   // 1/p or 1
+};
+
+// Fills table[0, *n_table) for the non-empty spectra in `order` (offsets[k],
+// offsets[k+1] bound spectrum k) and counts their query tiles.
+int exact_many_plan(const int64_t *offsets, const size_t *order, size_t n, const double *etas,
+                    int scaled, SegSpectrum *table, int *n_table, int *n_tiles);
+// All-points sums (kind EXACT_PLAIN) of every spectrum in the device table in
+// one launch, written at each spectrum's offset in out_re/out_im; bit-equal to
+// exact_f32c / exact_f64 on each spectrum alone.
+int exact_many(const double *ev, const SegSpectrum *d_table, int n_table, int n_tiles,
+               int f32c, double *out_re, double *out_im, hipStream_t stream);
+
 // float -> double widening on device (for the f32 entry points).
 int widen_f32(const float *in, double *out, size_t n, hipStream_t stream);
 

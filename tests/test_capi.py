@@ -132,9 +132,10 @@ def test_many_argument_checks(shk):
                                        dummy, 2, None, None) == _lib.SHK_ERR_BAD_ARG
     assert lib.shk_all_points_many_dev(dummy, i64([0, 3, 2]), 2, _lib.dptr(eta), 2, 1, 1, dummy,
                                        dummy, 2, None, None) == _lib.SHK_ERR_BAD_ARG
-    assert lib.shk_all_points_many_dev(dummy, off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
-                                       2, _lib.dptr(eta), 2, 1, 1, dummy, dummy, 0, None,
-                                       None) == _lib.SHK_ERR_BAD_ARG
+    for ns in (-1, 9):
+        assert lib.shk_all_points_many_dev(dummy, off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                           2, _lib.dptr(eta), 2, 1, 1, dummy, dummy, ns, None,
+                                           None) == _lib.SHK_ERR_BAD_ARG
     bad = np.array([0.1, -2.0])
     assert lib.shk_all_points_many_dev(dummy, off.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
                                        2, _lib.dptr(bad), 2, 1, 1, dummy, dummy, 2, None,

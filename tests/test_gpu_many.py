@@ -1,6 +1,8 @@
 """Many spectra in one library call (shk_all_points_many_dev): every
-spectrum's result equals its own single-spectrum transform bit for bit
-(the same kernels run on each), and the FP64 oracle to the path's bar."""
+spectrum's result equals its own single-spectrum transform bit for bit --
+on the stream-pool path (the same kernels run on each) and on the segmented
+launch (one grid over all spectra that reproduces the single-spectrum
+summation order) -- and the FP64 oracle to the path's bar."""
 import numpy as np
 import pytest
 
@@ -13,19 +15,20 @@ def small_batch(seed=0):
     rng = np.random.default_rng(seed)
     from shrinkers_amd import spectra as S
     sizes = [1, 2, 63, 64, 65, 127, 129, 500, 1000, 2047, 3000] + list(rng.integers(3, 3000, 13))
+    sizes.insert(4, 0)   # an empty spectrum in the middle
     spectra, etas = [], []
     for k, p in enumerate(sizes):
         c = float(rng.uniform(0.05, 0.5))
         ev = S.mp_inverse_cdf_spectrum(int(p), c=c, seed=k, spikes=()) if p >= 8 \
             else np.sort(rng.uniform(0.3, 3.0, int(p)))
         spectra.append(ev)
-        etas.append(float(rng.uniform(0.1, 1.0)) / np.sqrt(p))
+        etas.append(float(rng.uniform(0.1, 1.0)) / np.sqrt(max(p, 1)))
     return spectra, etas
 
 
 @pytest.mark.parametrize("precision", ["f32c", "f64"])
 @pytest.mark.parametrize("scaled", [True, False])
-@pytest.mark.parametrize("n_streams", [1, 4])
+@pytest.mark.parametrize("n_streams", [0, 1, 4])
 def test_many_equals_single_calls(shk, orc, precision, scaled, n_streams):
     import torch
     from shrinkers_amd.batch import SpectrumBatch
@@ -36,6 +39,9 @@ def test_many_equals_single_calls(shk, orc, precision, scaled, n_streams):
     assert t.shape == (len(spectra), 2) and np.all(t[:, 1] >= t[:, 0]) and np.all(t >= 0)
     tol = 1e-6 if precision == "f32c" else 1e-13
     for k, (ev, eta) in enumerate(zip(spectra, etas)):
+        if ev.size == 0:
+            assert out[k][0].size == 0 and tuple(t[k]) == (0.0, 0.0)
+            continue
         d = torch.from_numpy(ev).cuda()
         sr, si = shk.all_points_dev(d, eta, "blocked", precision=precision, scaled=scaled)
         torch.cuda.synchronize()
@@ -77,9 +83,35 @@ def test_many_rejects_bad_arguments(shk):
     with pytest.raises(ShkError):
         all_points_many_dev(ev, [0, 4, 10], [0.1, -1.0])        # bad eta
     with pytest.raises(ShkError):
-        all_points_many_dev(ev, [0, 4, 10], [0.1, 0.1], n_streams=0)
+        all_points_many_dev(ev, [0, 4, 10], [0.1, 0.1], n_streams=9)
+    with pytest.raises(ShkError):
+        all_points_many_dev(ev, [0, 4, 10], [0.1, 0.1], n_streams=-1)
     with pytest.raises(ShkError):
         all_points_many_dev(ev, [0, 6, 4, 10], [0.1, 0.1, 0.1])  # decreasing offsets
+
+
+@pytest.mark.parametrize("precision", ["f32c", "f64"])
+def test_segmented_large_spectra(shk, precision):
+    """Spectra whose single-spectrum split length exceeds the LDS piece
+    (p = 40 000: 768 sources per split; p = 100 003: 4 768) or divides it
+    unevenly (p = 20 000: 192), behind a tiny one: the segmented launch
+    still equals the single-spectrum call bit for bit."""
+    import torch
+    from shrinkers_amd import spectra as S
+    from shrinkers_amd.batch import SpectrumBatch
+    sizes = [20000, 3, 100003, 40000]
+    spectra = [S.mp_inverse_cdf_spectrum(p, c=0.3, seed=p, spikes=(4.0,)) if p > 8
+               else np.array([0.5, 1.0, 1.5]) for p in sizes]
+    etas = [1.0 / np.sqrt(p) for p in sizes]
+    b = SpectrumBatch(spectra, etas)
+    b.run("blocked", precision, True, 0)
+    out = b.results()
+    for k, (ev, eta) in enumerate(zip(spectra, etas)):
+        sr, si = shk.all_points_dev(torch.from_numpy(ev).cuda(), eta, "blocked",
+                                    precision=precision, scaled=True)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(out[k][0], sr.cpu().numpy(), err_msg=f"p={ev.size}")
+        np.testing.assert_array_equal(out[k][1], si.cpu().numpy(), err_msg=f"p={ev.size}")
 
 
 def test_config4_workload_sample(shk):
@@ -92,7 +124,12 @@ def test_config4_workload_sample(shk):
     etas = [1.0 / np.sqrt(s.size) for s in spectra]
     b = SpectrumBatch(spectra, etas)
     b.run("blocked", "f32c", True, 4)
+    out4 = b.results()
+    b.run("blocked", "f32c", True, 0)
     out = b.results()
+    for k in range(len(spectra)):   # segmented launch == stream-pool path, all 512
+        np.testing.assert_array_equal(out[k][0], out4[k][0])
+        np.testing.assert_array_equal(out[k][1], out4[k][1])
     worst = 0.0
     for k in np.random.default_rng(1).choice(len(spectra), 24, replace=False):
         d = torch.from_numpy(spectra[k]).cuda()
