@@ -182,9 +182,11 @@ shk_status shk_all_points_dev(const double *d_eigenvalues, size_t p, double eta,
  * is ordered after earlier work on `stream`, and later work on `stream`
  * waits for all of it.  If times_ms is not NULL (2*n_spectra floats) the
  * call synchronises and writes each spectrum's start and end in ms from the
- * batch start (device events; on the segmented launch every spectrum gets
- * 0 and the launch's end), otherwise it returns without synchronising.  The
- * persistent launch of shk_batch_* is switched off (see below). */
+ * batch start (stream pool: device events; segmented launch: device wall
+ * clock at its first workgroup's start and last workgroup's end, from the
+ * launch's earliest workgroup start), otherwise it returns without
+ * synchronising.  The persistent launch of shk_batch_* is switched off (see
+ * below). */
 shk_status shk_all_points_many_dev(const double *d_eigenvalues, const int64_t *offsets,
                                    size_t n_spectra, const double *etas, int method,
                                    int precision, int scaled, double *d_out_re,

@@ -672,34 +672,64 @@ shk_status shk_all_points_many_dev(const double *d_eigenvalues, const int64_t *o
     if (n_streams == 0 && exact && kind == EXACT_PLAIN) {
         // One segmented launch on the caller's stream: no library streams,
         // no partial sums.
-        SHK_CHECK(pool.reserve(0, times_ms ? 2 : 0));
         SHK_CHECK(pool.reserve_table(n_spectra));
         int n_table = 0, n_tiles = 0;
         SHK_CHECK(exact_many_plan(offsets, order.data(), n_spectra, etas, scaled, pool.staging,
                                   &n_table, &n_tiles));
-        float ms = 0.f;
+        std::vector<long long> stamps;
         if (n_table > 0) {
             SegSpectrum *d_table = nullptr;
+            long long *d_stamps = nullptr;
             SHK_HIP_S(hipMallocAsync((void **)&d_table, sizeof(SegSpectrum) * n_table, caller));
+            if (times_ms)
+                SHK_HIP_S(hipMallocAsync((void **)&d_stamps, sizeof(long long) * 2 * n_tiles,
+                                         caller));
             SHK_HIP_S(hipMemcpyAsync(d_table, pool.staging, sizeof(SegSpectrum) * n_table,
                                      hipMemcpyHostToDevice, caller));
             SHK_HIP_S(hipEventRecord(pool.staged, caller));
-            if (times_ms) SHK_HIP_S(hipEventRecord(pool.events[0], caller));
             const int st = exact_many(d_eigenvalues, d_table, n_table, n_tiles,
-                                      precision == SHK_PREC_F32C, d_out_re, d_out_im, caller);
-            if (!st && times_ms) SHK_HIP_S(hipEventRecord(pool.events[1], caller));
+                                      precision == SHK_PREC_F32C, d_out_re, d_out_im, d_stamps,
+                                      caller);
             (void)hipFreeAsync(d_table, caller);
+            if (!st && times_ms) {
+                stamps.resize(2 * (size_t)n_tiles);
+                hipError_t ce = hipMemcpyAsync(stamps.data(), d_stamps,
+                                               sizeof(long long) * stamps.size(),
+                                               hipMemcpyDeviceToHost, caller);
+                if (ce == hipSuccess) ce = hipStreamSynchronize(caller);
+                (void)hipFreeAsync(d_stamps, caller);
+                SHK_HIP_S(ce);
+            } else if (d_stamps) {
+                (void)hipFreeAsync(d_stamps, caller);
+            }
             if (st) return (shk_status)st;
-            if (times_ms) {
-                SHK_HIP_S(hipEventSynchronize(pool.events[1]));
-                SHK_HIP_S(hipEventElapsedTime(&ms, pool.events[0], pool.events[1]));
+        }
+        if (times_ms) {
+            // Each spectrum: its first workgroup's start and last workgroup's
+            // end, in ms from the earliest workgroup start of the launch.
+            int dev = 0, khz = 0;
+            SHK_HIP_S(hipGetDevice(&dev));
+            SHK_HIP_S(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+            if (khz <= 0) return fail(SHK_ERR_HIP, "device reports no wall-clock rate");
+            long long t0 = 0;
+            if (!stamps.empty()) t0 = *std::min_element(stamps.begin(), stamps.end());
+            for (size_t k = 0; k < n_spectra; ++k) times_ms[2 * k] = times_ms[2 * k + 1] = 0.f;
+            int e = 0;
+            for (size_t idx = 0; idx < n_spectra && e < n_table; ++idx) {
+                const size_t k = order[idx];
+                if (offsets[k + 1] <= offsets[k]) continue;
+                const int a = pool.staging[e].tile0;
+                const int b = (e + 1 < n_table) ? pool.staging[e + 1].tile0 : n_tiles;
+                long long lo = stamps[2 * (size_t)a], hi = stamps[2 * (size_t)a + 1];
+                for (int t = a; t < b; ++t) {
+                    lo = std::min(lo, stamps[2 * (size_t)t]);
+                    hi = std::max(hi, stamps[2 * (size_t)t + 1]);
+                }
+                times_ms[2 * k] = (float)((double)(lo - t0) / khz);
+                times_ms[2 * k + 1] = (float)((double)(hi - t0) / khz);
+                ++e;
             }
         }
-        if (times_ms)
-            for (size_t k = 0; k < n_spectra; ++k) {
-                times_ms[2 * k] = 0.f;
-                times_ms[2 * k + 1] = (offsets[k + 1] > offsets[k]) ? ms : 0.f;
-            }
         return SHK_OK;
     }
     if (n_streams == 0) n_streams = kDefaultManyStreams;

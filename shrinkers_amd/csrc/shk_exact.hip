@@ -248,19 +248,40 @@ __global__ __launch_bounds__(kWG) void exact_f32c_kernel(const double *__restric
 
 // ---- deterministic cross-split reduction ------------------------------------
 
+// One lane per query, one wavefront per workgroup (nq/64 workgroups spread
+// over every CU).  Loads run kRedAhead splits ahead of the additions, which
+// stay in split order.
+constexpr int kRedWG = 64;
+constexpr int kRedAhead = 8;
+
 template <int KIND, typename OutT>
-__global__ __launch_bounds__(256) void reduce_kernel(const double *__restrict__ part, int n_split,
-                                                     int nq, double eta, double scale,
-                                                     const double *__restrict__ q,
-                                                     OutT *__restrict__ o0, OutT *__restrict__ o1,
-                                                     OutT *__restrict__ o2, OutT *__restrict__ o3) {
+__global__ __launch_bounds__(kRedWG) void reduce_kernel(const double *__restrict__ part,
+                                                        int n_split, int nq, double eta,
+                                                        double scale,
+                                                        const double *__restrict__ q,
+                                                        OutT *__restrict__ o0,
+                                                        OutT *__restrict__ o1,
+                                                        OutT *__restrict__ o2,
+                                                        OutT *__restrict__ o3) {
     constexpr int NA = NAcc<KIND>::value;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nq) return;
     double a[NA];
 #pragma unroll
     for (int t = 0; t < NA; ++t) a[t] = 0.0;
-    for (int s = 0; s < n_split; ++s) {
+    int s = 0;
+    for (; s + kRedAhead <= n_split; s += kRedAhead) {
+        double v[kRedAhead][NA];
+#pragma unroll
+        for (int u = 0; u < kRedAhead; ++u)
+#pragma unroll
+            for (int t = 0; t < NA; ++t) v[u][t] = part[((size_t)t * n_split + s + u) * nq + i];
+#pragma unroll
+        for (int u = 0; u < kRedAhead; ++u)
+#pragma unroll
+            for (int t = 0; t < NA; ++t) a[t] += v[u][t];
+    }
+    for (; s < n_split; ++s) {
 #pragma unroll
         for (int t = 0; t < NA; ++t) a[t] += part[((size_t)t * n_split + s) * nq + i];
     }
@@ -303,6 +324,13 @@ __device__ __forceinline__ int seg_find(const SegSpectrum *__restrict__ specs, i
     return lo;
 }
 
+// Device wall clock (constant rate) at a workgroup's start (which = 0) or
+// end (which = 1), written by lane 0 to stamps[2 * blockIdx.x + which].
+__device__ __forceinline__ void seg_stamp(long long *stamps, int which) {
+    if (which) __syncthreads();  // the end stamp follows every lane's stores
+    if (threadIdx.x == 0) stamps[2 * (size_t)blockIdx.x + which] = (long long)wall_clock64();
+}
+
 // Sources [base, base + cnt) are in LDS; sweeps them in sub-ranges cut at the
 // chunk ends, folding the chunk accumulator into the total at each end.
 template <typename Sweep>
@@ -327,13 +355,16 @@ __device__ __forceinline__ void seg_sweep_piece(int base, int cnt, int per, int 
     }
 }
 
+template <bool STAMP>
 __global__ __launch_bounds__(kWG) void many_f32c_kernel(const double *__restrict__ ev,
                                                         const SegSpectrum *__restrict__ specs,
                                                         int n_specs, double *__restrict__ out_re,
-                                                        double *__restrict__ out_im) {
+                                                        double *__restrict__ out_im,
+                                                        long long *__restrict__ stamps) {
     __shared__ float s_hi[kPiece];
     __shared__ float s_lo[kPiece];
     __shared__ double s_red[2][kWG / 64];
+    if (STAMP) seg_stamp(stamps, 0);
     const SegSpectrum sp = specs[seg_find(specs, n_specs, (int)blockIdx.x)];
     const double *__restrict__ src = ev + sp.off;
     const int p = sp.p;
@@ -365,13 +396,17 @@ __global__ __launch_bounds__(kWG) void many_f32c_kernel(const double *__restrict
             out_im[sp.off + i] = (sp.eta * ti[k]) * sp.scale;
         }
     }
+    if (STAMP) seg_stamp(stamps, 1);
 }
 
+template <bool STAMP>
 __global__ __launch_bounds__(kWG) void many_f64_kernel(const double *__restrict__ ev,
                                                        const SegSpectrum *__restrict__ specs,
                                                        int n_specs, double *__restrict__ out_re,
-                                                       double *__restrict__ out_im) {
+                                                       double *__restrict__ out_im,
+                                                       long long *__restrict__ stamps) {
     __shared__ double s_src[kPiece];
+    if (STAMP) seg_stamp(stamps, 0);
     const SegSpectrum sp = specs[seg_find(specs, n_specs, (int)blockIdx.x)];
     const double *__restrict__ src = ev + sp.off;
     const int p = sp.p;
@@ -414,6 +449,7 @@ __global__ __launch_bounds__(kWG) void many_f64_kernel(const double *__restrict_
             out_im[sp.off + i] = (sp.eta * ti[k]) * sp.scale;
         }
     }
+    if (STAMP) seg_stamp(stamps, 1);
 }
 
 // Source splits so that about kTargetWG workgroups are in flight.
@@ -445,8 +481,8 @@ int launch_f64(const double *q, size_t nq, const double *src, size_t p, double e
 template <int KIND, typename OutT>
 int launch_reduce(const double *part, int n_split, size_t nq, double eta, double scale,
                   const double *q, OutT *o0, OutT *o1, OutT *o2, OutT *o3, hipStream_t stream) {
-    const int blocks = (int)((nq + 255) / 256);
-    hipLaunchKernelGGL((reduce_kernel<KIND, OutT>), dim3(blocks), dim3(256), 0, stream, part,
+    const int blocks = (int)((nq + kRedWG - 1) / kRedWG);
+    hipLaunchKernelGGL((reduce_kernel<KIND, OutT>), dim3(blocks), dim3(kRedWG), 0, stream, part,
                        n_split, (int)nq, eta, scale, q, o0, o1, o2, o3);
     SHK_HIP(hipGetLastError());
     return 0;
@@ -586,15 +622,23 @@ int exact_many_plan(const int64_t *offsets, const size_t *order, size_t n, const
 }
 
 int exact_many(const double *ev, const SegSpectrum *d_table, int n_table, int n_tiles,
-               int f32c, double *out_re, double *out_im, hipStream_t stream) {
+               int f32c, double *out_re, double *out_im, long long *d_stamps,
+               hipStream_t stream) {
     if (n_table == 0 || n_tiles == 0) return 0;
     ProfileMark m = profile_begin(stream);
-    if (f32c)
-        hipLaunchKernelGGL(many_f32c_kernel, dim3(n_tiles), dim3(kWG), 0, stream, ev, d_table,
-                           n_table, out_re, out_im);
+    const dim3 grid(n_tiles), block(kWG);
+    if (f32c && d_stamps)
+        hipLaunchKernelGGL(many_f32c_kernel<true>, grid, block, 0, stream, ev, d_table, n_table,
+                           out_re, out_im, d_stamps);
+    else if (f32c)
+        hipLaunchKernelGGL(many_f32c_kernel<false>, grid, block, 0, stream, ev, d_table, n_table,
+                           out_re, out_im, nullptr);
+    else if (d_stamps)
+        hipLaunchKernelGGL(many_f64_kernel<true>, grid, block, 0, stream, ev, d_table, n_table,
+                           out_re, out_im, d_stamps);
     else
-        hipLaunchKernelGGL(many_f64_kernel, dim3(n_tiles), dim3(kWG), 0, stream, ev, d_table,
-                           n_table, out_re, out_im);
+        hipLaunchKernelGGL(many_f64_kernel<false>, grid, block, 0, stream, ev, d_table, n_table,
+                           out_re, out_im, nullptr);
     profile_end(m, stream);
     SHK_HIP(hipGetLastError());
     return 0;

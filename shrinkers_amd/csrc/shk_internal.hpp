@@ -91,9 +91,11 @@ int exact_many_plan(const int64_t *offsets, const size_t *order, size_t n, const
                     int scaled, SegSpectrum *table, int *n_table, int *n_tiles);
 // All-points sums (kind EXACT_PLAIN) of every spectrum in the device table in
 // one launch, written at each spectrum's offset in out_re/out_im; bit-equal to
-// exact_f32c / exact_f64 on each spectrum alone.
+// exact_f32c / exact_f64 on each spectrum alone.  d_stamps, if not NULL,
+// receives 2 * n_tiles device wall-clock ticks: each workgroup's start and end.
 int exact_many(const double *ev, const SegSpectrum *d_table, int n_table, int n_tiles,
-               int f32c, double *out_re, double *out_im, hipStream_t stream);
+               int f32c, double *out_re, double *out_im, long long *d_stamps,
+               hipStream_t stream);
 
 // float -> double widening on device (for the f32 entry points).
 int widen_f32(const float *in, double *out, size_t n, hipStream_t stream);
