@@ -26,12 +26,18 @@
  * buffers alive until the stream has passed the work.
  *
  * Thread safety: calls on distinct streams may run concurrently; scratch
- * memory is stream-ordered (hipMallocAsync) per call.
+ * memory is stream-ordered per call (pool allocations, or the calling
+ * thread's cached buffers, whose reuse on another stream waits for the
+ * previous use).
  *
- * Memory: scratch buffers and ChebCode trees come from the device's default
- * stream-ordered memory pool.  On first use the library raises that pool's
- * release threshold so freed blocks stay in the pool for reuse (the pool
- * then holds as much as the largest calls needed until the process ends).
+ * Memory: scratch buffers and ChebCode trees built through the handle API
+ * come from the device's default stream-ordered memory pool.  On first use
+ * the library raises that pool's release threshold so freed blocks stay in
+ * the pool for reuse (the pool then holds as much as the largest calls
+ * needed until the process ends).  The ChebCode build scratch, and the
+ * trees of one-shot ChebCode transforms (all-points and at-points calls),
+ * live in buffers cached per calling thread and device, which grow to the
+ * largest size used; shk_release_cached_memory() frees them.
  */
 #ifndef SHRINKERS_STIELTJES_H
 #define SHRINKERS_STIELTJES_H
@@ -106,6 +112,12 @@ const char *shk_last_error(void);
 
 /* Library version string. */
 const char *shk_version(void);
+
+/* Frees the buffers the library caches for the calling thread on the
+ * current device (ChebCode build scratch, one-shot ChebCode trees), after
+ * their last uses have finished.  No counterpart in the reference; for
+ * long-running callers that want the memory back. */
+shk_status shk_release_cached_memory(void);
 
 /* Resolve a Python method name or alias (src/python.rs:321-350) to its id.
  * Returns SHK_ERR_BAD_METHOD for an unknown name. */

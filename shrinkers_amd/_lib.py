@@ -45,6 +45,7 @@ _ip = ctypes.POINTER(ctypes.c_int)
 
 SIGNATURES = {
     "shk_last_error": (ctypes.c_char_p, []),
+    "shk_release_cached_memory": (_i, []),
     "shk_version": (ctypes.c_char_p, []),
     "shk_method_from_name": (_i, [ctypes.c_char_p, _ip]),
     "shk_method_name": (ctypes.c_char_p, [_i]),

@@ -229,13 +229,27 @@ __global__ __launch_bounds__(kAWG) void adj_src_f64_kernel(
     }
 }
 
-__global__ __launch_bounds__(256) void adj_src_reduce_kernel(const double *__restrict__ part,
-                                                             int n_split, int p,
-                                                             double *__restrict__ out) {
+// Split reductions: one lane per output, one wavefront per workgroup (so
+// they spread over every CU), loads kRedAhead splits ahead of the additions,
+// which stay in split order.
+constexpr int kRedWG = 64;
+constexpr int kRedAhead = 8;
+
+__global__ __launch_bounds__(kRedWG) void adj_src_reduce_kernel(const double *__restrict__ part,
+                                                                int n_split, int p,
+                                                                double *__restrict__ out) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= p) return;
     double a = 0.0;
-    for (int s = 0; s < n_split; ++s) a += part[(size_t)s * p + j];
+    int s = 0;
+    for (; s + kRedAhead <= n_split; s += kRedAhead) {
+        double v[kRedAhead];
+#pragma unroll
+        for (int u = 0; u < kRedAhead; ++u) v[u] = part[(size_t)(s + u) * p + j];
+#pragma unroll
+        for (int u = 0; u < kRedAhead; ++u) a += v[u];
+    }
+    for (; s < n_split; ++s) a += part[(size_t)s * p + j];
     out[j] = a;
 }
 
@@ -393,14 +407,28 @@ __global__ __launch_bounds__(kAWG) void adj_qry_f64_kernel(const double *__restr
 }
 
 // Split reduction of (P, Q), then grad_x and the per-query eta term.
-__global__ __launch_bounds__(256) void adj_qry_reduce_kernel(
+__global__ __launch_bounds__(kRedWG) void adj_qry_reduce_kernel(
     const double *__restrict__ part, int n_split, int nq, double eta,
     const double *__restrict__ gre, const double *__restrict__ gim,
     double *__restrict__ grad_q, double *__restrict__ eta_term) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nq) return;
     double P = 0.0, Q = 0.0;
-    for (int s = 0; s < n_split; ++s) {
+    int s = 0;
+    for (; s + kRedAhead <= n_split; s += kRedAhead) {
+        double vp[kRedAhead], vq[kRedAhead];
+#pragma unroll
+        for (int u = 0; u < kRedAhead; ++u) {
+            vp[u] = part[((size_t)0 * n_split + s + u) * nq + i];
+            vq[u] = part[((size_t)1 * n_split + s + u) * nq + i];
+        }
+#pragma unroll
+        for (int u = 0; u < kRedAhead; ++u) {
+            P += vp[u];
+            Q += vq[u];
+        }
+    }
+    for (; s < n_split; ++s) {
         P += part[((size_t)0 * n_split + s) * nq + i];
         Q += part[((size_t)1 * n_split + s) * nq + i];
     }
@@ -557,8 +585,8 @@ int exact_adjoint(const double *q, size_t nq, const double *ev, size_t p, double
         profile_end(mk, stream);
         hipError_t le = hipGetLastError();
         if (le == hipSuccess) {
-            hipLaunchKernelGGL(adj_src_reduce_kernel, dim3((unsigned)((p + 255) / 256)), dim3(256),
-                               0, stream, part, ns, (int)p, grad_ev);
+            hipLaunchKernelGGL(adj_src_reduce_kernel, dim3((unsigned)((p + kRedWG - 1) / kRedWG)),
+                               dim3(kRedWG), 0, stream, part, ns, (int)p, grad_ev);
             le = hipGetLastError();
         }
         const hipError_t fe = hipFreeAsync(part, stream);
@@ -585,8 +613,8 @@ int exact_adjoint(const double *q, size_t nq, const double *ev, size_t p, double
         profile_end(mk, stream);
         hipError_t le = hipGetLastError();
         if (le == hipSuccess) {
-            hipLaunchKernelGGL(adj_qry_reduce_kernel, dim3((unsigned)((nq + 255) / 256)),
-                               dim3(256), 0, stream, part, ns, (int)nq, eta, gre, gim, grad_q,
+            hipLaunchKernelGGL(adj_qry_reduce_kernel, dim3((unsigned)((nq + kRedWG - 1) / kRedWG)),
+                               dim3(kRedWG), 0, stream, part, ns, (int)nq, eta, gre, gim, grad_q,
                                eta_term);
             le = hipGetLastError();
         }

@@ -6,6 +6,7 @@
 #include "shrinkers_stieltjes.h"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstring>
@@ -292,16 +293,11 @@ int run_allpoints(const double *d_ev, size_t p, double eta, int method, double c
                   int precision, double scale, double *d_re, double *d_im, hipStream_t s) {
     const int fam = family_of(method);
     if (fam == FAM_CHEB) {
+        // transient tree: its arrays are the thread's cached buffers, so the
+        // evaluation stays asynchronous and destruction needs no sync
         ChebTree *t = nullptr;
-        SHK_TRY(cheb_build(d_ev, p, preset_of(method), &t, s));
-        int st = cheb_eval(t, nullptr, p, eta, scale, 1, d_re, d_im, s);
-        if (!st) {
-            hipError_t e = hipStreamSynchronize(s);
-            if (e != hipSuccess) {
-                set_error(std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
-                st = SHK_ERR_HIP;
-            }
-        }
+        SHK_TRY(cheb_build(d_ev, p, preset_of(method), &t, s, 1));
+        const int st = cheb_eval(t, nullptr, p, eta, scale, 1, d_re, d_im, s);
         cheb_destroy(t);
         return st;
     }
@@ -318,15 +314,8 @@ int run_atpoints(const double *d_q, size_t nq, const double *d_ev, size_t p, dou
         ChebParams prm = preset_of(method);
         prm.leaf_cap = grid_leaf_cap(prm.n, (int64_t)p, (int64_t)nq, prm.leaf_cap);
         ChebTree *t = nullptr;
-        SHK_TRY(cheb_build(d_ev, p, prm, &t, s));
-        int st = cheb_eval(t, d_q, nq, eta, 1.0, 0, d_re, d_im, s);
-        if (!st) {
-            hipError_t e = hipStreamSynchronize(s);
-            if (e != hipSuccess) {
-                set_error(std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
-                st = SHK_ERR_HIP;
-            }
-        }
+        SHK_TRY(cheb_build(d_ev, p, prm, &t, s, 1));
+        const int st = cheb_eval(t, d_q, nq, eta, 1.0, 0, d_re, d_im, s);
         cheb_destroy(t);
         return st;
     }
@@ -334,6 +323,91 @@ int run_atpoints(const double *d_q, size_t nq, const double *d_ev, size_t p, dou
     return exact_f64(d_q, nq, d_ev, p, eta, atpoints_kind(method, cutoff_ratio),
                      cutoff_ratio * eta, 1.0, outs, s);
 }
+
+}  // namespace
+
+// ---- per-thread device buffer cache ----------------------------------------
+
+namespace {
+
+struct CacheEntry {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipEvent_t last = nullptr;     // recorded at the end of the last use
+    hipStream_t last_stream = nullptr;
+    bool used = false;
+};
+
+struct ThreadCache {
+    // per device id; the buffers live until cache_free_all or process end
+    // (no destructor: the HIP runtime may already be gone at thread exit)
+    std::vector<std::array<CacheEntry, kCacheSlots>> dev;
+    CacheEntry *entry(int slot) {
+        int d = 0;
+        if (hipGetDevice(&d) != hipSuccess || d < 0) return nullptr;
+        if ((size_t)d >= dev.size()) dev.resize((size_t)d + 1);
+        return &dev[(size_t)d][(size_t)slot];
+    }
+};
+
+ThreadCache &thread_cache() {
+    thread_local ThreadCache tc;
+    return tc;
+}
+
+}  // namespace
+
+namespace shk {
+
+int cache_acquire(int slot, size_t bytes, hipStream_t s, void **out) {
+    *out = nullptr;
+    CacheEntry *c = (slot >= 0 && slot < kCacheSlots) ? thread_cache().entry(slot) : nullptr;
+    if (!c) {
+        set_error("buffer cache: no current device");
+        return SHK_ERR_HIP;
+    }
+    if (!c->last) SHK_HIP(hipEventCreateWithFlags(&c->last, hipEventDisableTiming));
+    if (c->used && c->last_stream != s) SHK_HIP(hipStreamWaitEvent(s, c->last, 0));
+    if (c->cap < bytes) {
+        if (c->p) {
+            SHK_HIP(hipEventSynchronize(c->last));
+            SHK_HIP(hipFree(c->p));
+            c->p = nullptr;
+            c->cap = 0;
+        }
+        const size_t cap = bytes + bytes / 4 + 4096;  // headroom for slowly growing sizes
+        SHK_HIP(hipMalloc(&c->p, cap));
+        c->cap = cap;
+    }
+    *out = c->p;
+    return 0;
+}
+
+int cache_release(int slot, hipStream_t s) {
+    CacheEntry *c = (slot >= 0 && slot < kCacheSlots) ? thread_cache().entry(slot) : nullptr;
+    if (!c || !c->last) return 0;
+    SHK_HIP(hipEventRecord(c->last, s));
+    c->last_stream = s;
+    c->used = true;
+    return 0;
+}
+
+int cache_free_all() {
+    for (int slot = 0; slot < kCacheSlots; ++slot) {
+        CacheEntry *c = thread_cache().entry(slot);
+        if (!c || !c->p) continue;
+        if (c->used) SHK_HIP(hipEventSynchronize(c->last));
+        SHK_HIP(hipFree(c->p));
+        c->p = nullptr;
+        c->cap = 0;
+        c->used = false;
+    }
+    return 0;
+}
+
+}  // namespace shk
+
+namespace {
 
 // Streams and events the multi-spectrum entry point reuses: created on
 // first use on the calling thread (default priority), kept until the thread
@@ -427,6 +501,8 @@ extern "C" {
 const char *shk_last_error(void) { return last_error(); }
 
 const char *shk_version(void) { return "shrinkers_amd 0.1.0 (gfx950)"; }
+
+shk_status shk_release_cached_memory(void) { return (shk_status)cache_free_all(); }
 
 shk_status shk_method_from_name(const char *name, int *method_out) {
     if (!name || !method_out) return fail(SHK_ERR_BAD_ARG, "NULL argument");

@@ -8,14 +8,15 @@
 //   1. sort (rocPRIM radix sort of (value, index) pairs) unless the input is
 //      already ascending; the permutation scatters results back to input
 //      order (ChebCodeBatch::build_mode_parallel, :1186-1217);
-//   2. topology: one workgroup walks the tree breadth-first, one thread per
-//      frontier node, with a workgroup scan placing the children so the BFS
-//      order is deterministic; subtree sizes bottom-up and pre-order numbers
-//      top-down follow in the same kernel;
+//   2. topology: one workgroup walks the top of the tree breadth-first until
+//      a level holds 512 nodes, then one workgroup per node of that level
+//      builds its subtree; a partition search narrowed by a value-bucket
+//      table finds each split; subtree sizes and pre-order numbers follow,
+//      with per-depth node lists for the merge;
 //   3. finalize: pre-order node records, Chebyshev positions (:269-279);
 //   4. leaf fill: one thread per leaf, barycentric rows in source order
 //      (:286-337, :401-460);
-//   5. merge: parents from their children's weights, deepest level first
+//   5. merge: parents from their children's weights, deepest depth first
 //      (:474-506);
 //   6. f32 mirrors for the FAST traversal (:551-570) and f32 hi/lo splits of
 //      the sorted sources for the FAST near field.
@@ -67,6 +68,10 @@ struct ChebTree {
     float *panel32 = nullptr;               // N*2n: nodes then weights
     void *arena = nullptr;   // sources (sorted, perm, hi/lo)
     void *arena2 = nullptr;  // node records and panels
+    // transient trees: arena / arena2 are the thread's cached buffers
+    // (CACHE_CHEB_SRC / CACHE_CHEB_NODES), released at destruction on `stream`
+    bool cached_src = false, cached_nodes = false;
+    hipStream_t stream = nullptr;
     hipEvent_t ready = nullptr;  // recorded when the build's last kernel is enqueued
 };
 
@@ -96,117 +101,419 @@ __global__ void iota_kernel(int32_t *__restrict__ v, int p) {
     if (i < p) v[i] = i;
 }
 
-// Workgroup exclusive scan of one int per thread (1024 threads).
-__device__ int block_exclusive_scan(int v, int *s, int *total) {
-    const int tid = threadIdx.x;
-    s[tid] = v;
-    __syncthreads();
-    for (int off = 1; off < kTopoThreads; off <<= 1) {
-        int t = (tid >= off) ? s[tid - off] : 0;
-        __syncthreads();
-        s[tid] += t;
-        __syncthreads();
+// Workgroup exclusive scan of one int per thread (THREADS threads):
+// wavefront scans by shuffles, then the wavefront totals through LDS.
+template <int THREADS>
+__device__ int block_exclusive_scan(int v, int *s_wave, int *total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
     }
-    int incl = s[tid];
-    *total = s[kTopoThreads - 1];
+    if (lane == 63) s_wave[wid] = x;
     __syncthreads();
-    return incl - v;
+    int before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < THREADS / 64; ++w) {
+        const int t = s_wave[w];
+        before += (w < wid) ? t : 0;
+        all += t;
+    }
+    __syncthreads();  // s_wave is reused by the next call
+    *total = all;
+    return before + x - v;
 }
 
-// Breadth-first topology, then subtree sizes and pre-order numbers.
-// b_* arrays are indexed by BFS id and hold `cap` entries.  info[0] = node
-// count, info[1] = number of levels, info[2] = overflow flag.
-__global__ __launch_bounds__(kTopoThreads) void topo_kernel(
-    const double *__restrict__ sorted, int p, int leaf_cap, int cap, int *__restrict__ b_lo,
-    int *__restrict__ b_hi, int *__restrict__ b_left, int *__restrict__ b_size,
-    int *__restrict__ b_pre, int *__restrict__ info) {
-    __shared__ int s_scan[kTopoThreads];
-    __shared__ int s_level[kMaxDepth + 3];
+// Bucket hint for the partition search.  bucket(v) = floor((v - x0) * R /
+// (x_last - x0)) clamped to [0, R-1] is monotone in v (rounded subtraction,
+// multiplication by a non-negative constant and truncation all are), so for
+// k = bucket(mid) every element in a lower bucket is < mid and every element
+// in a higher bucket is > mid: upper_bound(mid) lies in [bstart[k],
+// bstart[k+1]], where bstart[k] is the first index whose bucket is >= k.
+struct Hint {
+    double x0, inv;
+    int R;
+};
+
+__device__ __forceinline__ Hint hint_of(const double *__restrict__ sorted, int p, int R) {
+    const double x0 = sorted[0];
+    const double range = sorted[p - 1] - x0;
+    double inv = (range > 0.0) ? (double)R / range : 0.0;
+    if (!(inv <= 1.0e300)) inv = 0.0;  // a range too small to scale
+    return {x0, inv, R};
+}
+
+__device__ __forceinline__ int hint_bucket(double v, const Hint &h) {
+    if (h.inv == 0.0) return 0;  // also a range that overflowed
+    const double f = (v - h.x0) * h.inv;
+    if (!(f < (double)h.R)) return h.R - 1;
+    return (f > 0.0) ? (int)f : 0;
+}
+
+__global__ void hint_clear_kernel(int *__restrict__ bstart, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) bstart[i] = -1;
+}
+
+// bstart[0, R]: element i fills the buckets (bucket(x[i-1]), bucket(x[i])]
+// with i, the last element also (bucket(x[p-1]), R] with p.  Runs longer
+// than kHintGap buckets (gaps in the spectrum) stay -1 and the search falls
+// back to the node's own range there.
+constexpr int kHintGap = 64;
+
+__global__ void hint_scatter_kernel(const double *__restrict__ sorted, int p, int R,
+                                    int *__restrict__ bstart) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p) return;
+    const Hint h = hint_of(sorted, p, R);
+    const int bi = hint_bucket(sorted[i], h);
+    const int bp = (i == 0) ? -1 : hint_bucket(sorted[i - 1], h);
+    if (bi - bp <= kHintGap)
+        for (int k = bp + 1; k <= bi; ++k) bstart[k] = i;
+    if (i == p - 1 && R - bi <= kHintGap)
+        for (int k = bi + 1; k <= R; ++k) bstart[k] = p;
+}
+
+// ---- topology ---------------------------------------------------------------
+//
+// The tree is built in node "slots"; the output (pre-order records) does not
+// depend on how slots are numbered, only on the split rule, so any schedule
+// gives the same tree.  Four kernels:
+//   A  one workgroup walks breadth-first from the root until a level holds
+//      kFrontier nodes or more (or the tree ends); that level is the
+//      frontier.  Top slots are BFS ids [0, T).
+//   B  one workgroup per frontier node builds its subtree breadth-first into
+//      slots [T + 2a, T + 2b) (a subtree over sources [a, b) has at most
+//      2(b - a) - 1 nodes, and the frontier ranges are disjoint), then the
+//      subtree sizes bottom-up.
+//   C  one workgroup: sizes of the top levels, pre-order numbers of the top
+//      levels and the frontier roots, and the per-depth node lists (the top
+//      levels in BFS order, then for each deeper depth the subtrees' levels
+//      in frontier order).
+//   D  one workgroup per frontier node: pre-order numbers inside its subtree
+//      and its nodes' places in the depth lists.
+// Every split is the reference's (midpoint value split with index-midpoint
+// fallback, leaf if count <= leaf_cap or depth >= 48, chebcode.rs:347-388).
+
+constexpr int kTopoK = 4;        // nodes per thread per pass (interleaved searches)
+constexpr int kSubThreads = 256; // workgroup of the subtree kernels
+constexpr int kFrontier = 512;   // top walk stops at a level this large
+constexpr int kMaxFrontier = 2 * kFrontier;  // the frontier level is smaller
+constexpr int kLv = kMaxDepth + 2;           // levels per subtree (root level 0)
+
+// info[] layout (ints), copied to the host after the topology.
+constexpr int kInfoN = 0, kInfoLevels = 1, kInfoOverflow = 2, kInfoUnsorted = 3;
+constexpr int kInfoT = 4, kInfoF = 5, kInfoD = 6, kInfoFront = 7;
+constexpr int kInfoDepthOff = 8;                          // kLv + 1 entries
+constexpr int kInfoDepthCnt = kInfoDepthOff + kLv + 1;    // kLv + 1 entries
+constexpr int kInfoTopLevel = kInfoDepthCnt + kLv + 1;    // kMaxDepth + 3 entries
+constexpr int kInfoInts = kInfoTopLevel + kMaxDepth + 3;
+
+struct TopoCtx {
+    const double *sorted;
+    int p, leaf_cap;
+    const int *bstart;
+    int R;   // hint buckets (bstart holds R + 1 entries)
+    Hint h;  // set on the device at kernel entry (hint_of)
+    int *b_lo, *b_hi, *b_left, *b_size, *b_pre;
+};
+
+// Splits the nodes of slots [lb, le) at depth `depth`: nodes with more than
+// leaf_cap sources (and depth < kMaxDepth) get two children, placed in node
+// order at slots alloc, alloc + 1, ...; leaves get b_left = -1.  Returns
+// the number of child slots used (uniform), or sets *overflow when they
+// would pass `limit`.  Loads are unconditional at clamped, in-bounds indices
+// and masked afterwards, so the kTopoK nodes of a thread issue them together.
+template <int THREADS>
+__device__ int expand_level(const TopoCtx &c, int lb, int le, int depth, int alloc, int limit,
+                            int *s_wave, int *overflow) {
     const int tid = threadIdx.x;
-    if (tid == 0) {
-        b_lo[0] = 0;
-        b_hi[0] = p;
-        s_level[0] = 0;
-        s_level[1] = 1;
-    }
-    __syncthreads();
-    int lvl_begin = 0, lvl_end = 1, total = 1, depth = 0, overflow = 0;
-    while (lvl_begin < lvl_end && !overflow) {
-        for (int base = lvl_begin; base < lvl_end; base += kTopoThreads) {
-            const int v = base + tid;
-            int a = 0, b = 0, split = 0, nchild = 0;
-            if (v < lvl_end) {
-                a = b_lo[v];
-                b = b_hi[v];
-                if (b - a > leaf_cap && depth < kMaxDepth) {
-                    const double lo = sorted[a], hi = sorted[b - 1];
-                    const double mid = lo + 0.5 * (hi - lo);
-                    int l = a, r = b;  // partition_point(x <= mid)
-                    while (l < r) {
-                        const int m = l + (r - l) / 2;
-                        if (sorted[m] <= mid) l = m + 1;
-                        else r = m;
-                    }
-                    split = (l == a || l == b) ? (a + b) / 2 : l;
-                    nchild = 2;
-                }
+    int used = 0;
+    for (int base = lb; base < le; base += THREADS * kTopoK) {
+        const int v0 = base + tid * kTopoK;
+        int a[kTopoK], b[kTopoK], l[kTopoK], r[kTopoK];
+        bool need[kTopoK];
+        double mid[kTopoK];
+#pragma unroll
+        for (int k = 0; k < kTopoK; ++k) {
+            const int v = min(v0 + k, le - 1);
+            a[k] = c.b_lo[v];
+            b[k] = c.b_hi[v];
+        }
+#pragma unroll
+        for (int k = 0; k < kTopoK; ++k)
+            need[k] = (v0 + k < le) && (b[k] - a[k] > c.leaf_cap) && (depth < kMaxDepth);
+        double xa[kTopoK], xb[kTopoK];
+#pragma unroll
+        for (int k = 0; k < kTopoK; ++k) {
+            xa[k] = c.sorted[need[k] ? a[k] : 0];
+            xb[k] = c.sorted[need[k] ? b[k] - 1 : 0];
+        }
+#pragma unroll
+        for (int k = 0; k < kTopoK; ++k) mid[k] = xa[k] + 0.5 * (xb[k] - xa[k]);
+        int hl[kTopoK], hr[kTopoK];
+#pragma unroll
+        for (int k = 0; k < kTopoK; ++k) {
+            const int kb = hint_bucket(mid[k], c.h);  // in [0, R-1]
+            hl[k] = c.bstart[kb];
+            hr[k] = c.bstart[kb + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < kTopoK; ++k) {
+            const bool ok = hl[k] >= 0 && hr[k] >= 0;
+            // upper_bound(mid) in [hl, hr], or partition_point in [a, b)
+            l[k] = need[k] ? (ok ? hl[k] : a[k]) : 0;
+            r[k] = need[k] ? (ok ? hr[k] : b[k]) : 0;
+        }
+        // partition_point(x <= mid), the kTopoK searches in lockstep
+        bool busy = false;
+#pragma unroll
+        for (int k = 0; k < kTopoK; ++k) busy |= l[k] < r[k];
+        while (busy) {
+            int m[kTopoK];
+            double xm[kTopoK];
+#pragma unroll
+            for (int k = 0; k < kTopoK; ++k) {
+                m[k] = l[k] + (r[k] - l[k]) / 2;
+                xm[k] = c.sorted[min(m[k], c.p - 1)];
             }
-            int added;
-            const int off = block_exclusive_scan(nchild, s_scan, &added);
-            if (total + added > cap) {
-                overflow = 1;
-                break;
+            busy = false;
+#pragma unroll
+            for (int k = 0; k < kTopoK; ++k) {
+                const bool act = l[k] < r[k];
+                const bool le_mid = xm[k] <= mid[k];
+                l[k] = (act && le_mid) ? m[k] + 1 : l[k];
+                r[k] = (act && !le_mid) ? m[k] : r[k];
+                busy |= l[k] < r[k];
             }
-            if (v < lvl_end) {
-                if (nchild) {
-                    const int c = total + off;
-                    b_left[v] = c;
-                    b_lo[c] = a;
-                    b_hi[c] = split;
-                    b_lo[c + 1] = split;
-                    b_hi[c + 1] = b;
+        }
+        int split[kTopoK], mine = 0;
+#pragma unroll
+        for (int k = 0; k < kTopoK; ++k) {
+            split[k] = 0;
+            if (need[k]) {
+                const int u = min(max(l[k], a[k]), b[k]);
+                split[k] = (u == a[k] || u == b[k]) ? (a[k] + b[k]) / 2 : u;
+                mine += 2;
+            }
+        }
+        int added;
+        const int off = block_exclusive_scan<THREADS>(mine, s_wave, &added);
+        if (alloc + used + added > limit) {
+            *overflow = 1;
+            return used;
+        }
+        int ch = alloc + used + off;
+#pragma unroll
+        for (int k = 0; k < kTopoK; ++k) {
+            const int v = v0 + k;
+            if (v < le) {
+                if (need[k]) {
+                    c.b_left[v] = ch;
+                    c.b_lo[ch] = a[k];
+                    c.b_hi[ch] = split[k];
+                    c.b_lo[ch + 1] = split[k];
+                    c.b_hi[ch + 1] = b[k];
+                    ch += 2;
                 } else {
-                    b_left[v] = -1;
+                    c.b_left[v] = -1;
                 }
             }
-            total += added;
         }
-        __syncthreads();
-        lvl_begin = lvl_end;
-        lvl_end = total;
-        ++depth;
-        if (tid == 0) s_level[depth + 1] = total;
-        __syncthreads();
+        used += added;
     }
-    const int n_levels = depth;  // levels 0..n_levels-1
-    // subtree sizes, deepest level first
-    for (int d = n_levels - 1; d >= 0 && !overflow; --d) {
-        for (int v = s_level[d] + tid; v < s_level[d + 1]; v += kTopoThreads) {
-            const int l = b_left[v];
-            b_size[v] = (l < 0) ? 1 : 1 + b_size[l] + b_size[l + 1];
+    return used;
+}
+
+// b_size of the nodes in slots [lb, le) from their children's sizes.
+template <int THREADS>
+__device__ void sizes_level(const TopoCtx &c, int lb, int le) {
+    for (int v0 = lb + (int)threadIdx.x * kTopoK; v0 < le; v0 += THREADS * kTopoK) {
+        int lf[kTopoK], s0[kTopoK], s1[kTopoK];
+#pragma unroll
+        for (int k = 0; k < kTopoK; ++k) {
+            lf[k] = c.b_left[min(v0 + k, le - 1)];
+            if (v0 + k >= le) lf[k] = -1;
         }
-        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kTopoK; ++k) {
+            s0[k] = c.b_size[lf[k] >= 0 ? lf[k] : 0];
+            s1[k] = c.b_size[lf[k] >= 0 ? lf[k] + 1 : 0];
+        }
+#pragma unroll
+        for (int k = 0; k < kTopoK; ++k)
+            if (v0 + k < le) c.b_size[v0 + k] = (lf[k] < 0) ? 1 : 1 + s0[k] + s1[k];
     }
-    // pre-order numbers, root first
-    if (tid == 0) b_pre[0] = 0;
-    __syncthreads();
-    for (int d = 0; d < n_levels && !overflow; ++d) {
-        for (int v = s_level[d] + tid; v < s_level[d + 1]; v += kTopoThreads) {
-            const int l = b_left[v];
-            if (l >= 0) {
-                b_pre[l] = b_pre[v] + 1;
-                b_pre[l + 1] = b_pre[v] + 1 + b_size[l];
+}
+
+// Pre-order numbers of the children of the nodes in slots [lb, le).
+template <int THREADS>
+__device__ void preorder_level(const TopoCtx &c, int lb, int le) {
+    for (int v0 = lb + (int)threadIdx.x * kTopoK; v0 < le; v0 += THREADS * kTopoK) {
+        int lf[kTopoK], pv[kTopoK], sl[kTopoK];
+#pragma unroll
+        for (int k = 0; k < kTopoK; ++k) {
+            const int v = min(v0 + k, le - 1);
+            lf[k] = c.b_left[v];
+            pv[k] = c.b_pre[v];
+            if (v0 + k >= le) lf[k] = -1;
+        }
+#pragma unroll
+        for (int k = 0; k < kTopoK; ++k) sl[k] = c.b_size[lf[k] >= 0 ? lf[k] : 0];
+#pragma unroll
+        for (int k = 0; k < kTopoK; ++k) {
+            if (lf[k] >= 0) {
+                c.b_pre[lf[k]] = pv[k] + 1;
+                c.b_pre[lf[k] + 1] = pv[k] + 1 + sl[k];
             }
         }
+    }
+}
+
+// A: breadth-first from the root while levels are smaller than kFrontier.
+__global__ __launch_bounds__(kTopoThreads) void topo_top_kernel(TopoCtx c, int cap_top,
+                                                                int *__restrict__ info) {
+    __shared__ int s_wave[kTopoThreads / 64];
+    const int tid = threadIdx.x;
+    c.h = hint_of(c.sorted, c.p, c.R);
+    if (tid == 0) {
+        c.b_lo[0] = 0;
+        c.b_hi[0] = c.p;
+        info[kInfoTopLevel + 0] = 0;
+        info[kInfoTopLevel + 1] = 1;
+    }
+    __syncthreads();
+    int lb = 0, le = 1, depth = 0, overflow = 0;
+    while (lb < le && le - lb < kFrontier && !overflow) {
+        const int used = expand_level<kTopoThreads>(c, lb, le, depth, le, cap_top, s_wave,
+                                                    &overflow);
         __syncthreads();
+        lb = le;
+        le += used;
+        ++depth;
+        if (tid == 0) info[kInfoTopLevel + depth + 1] = le;
     }
     if (tid == 0) {
-        info[0] = total;
-        info[1] = n_levels;
-        info[2] = overflow;
+        info[kInfoOverflow] = overflow;
+        info[kInfoT] = le;
+        info[kInfoF] = overflow ? 0 : le - lb;  // 0 when the tree ended within the top walk
+        info[kInfoD] = depth;
+        info[kInfoFront] = lb;
     }
-    // level offsets for the merge pass
-    for (int d = tid; d <= n_levels; d += kTopoThreads) info[4 + d] = s_level[d];
+}
+
+// B: the subtree of frontier node blockIdx.x, then its sizes.
+__global__ __launch_bounds__(kSubThreads) void topo_sub_kernel(TopoCtx c,
+                                                               const int *__restrict__ info,
+                                                               int *__restrict__ lv_start,
+                                                               int *__restrict__ lv_cnt) {
+    __shared__ int s_wave[kSubThreads / 64];
+    __shared__ int s_lb[kLv], s_le[kLv];
+    const int f = blockIdx.x;
+    if (f >= info[kInfoF]) return;
+    c.h = hint_of(c.sorted, c.p, c.R);
+    const int T = info[kInfoT], D = info[kInfoD];
+    const int v = info[kInfoFront] + f;
+    const int a = c.b_lo[v], b = c.b_hi[v];
+    int lb = v, le = v + 1, alloc = T + 2 * a, lev = 0, overflow = 0;
+    const int limit = T + 2 * b;
+    if (threadIdx.x == 0) {
+        s_lb[0] = lb;
+        s_le[0] = le;
+    }
+    while (true) {
+        const int used = expand_level<kSubThreads>(c, lb, le, D + lev, alloc, limit, s_wave,
+                                                   &overflow);
+        __syncthreads();
+        if (used == 0 || overflow) break;
+        lb = alloc;
+        le = alloc + used;
+        alloc += used;
+        ++lev;
+        if (threadIdx.x == 0) {
+            s_lb[lev] = lb;
+            s_le[lev] = le;
+        }
+    }
+    __syncthreads();
+    const int n_lv = lev + 1;
+    for (int l = n_lv - 1; l >= 0; --l) {
+        sizes_level<kSubThreads>(c, s_lb[l], s_le[l]);
+        __syncthreads();
+    }
+    for (int l = threadIdx.x; l < kLv; l += kSubThreads) {
+        lv_start[f * kLv + l] = (l < n_lv) ? s_lb[l] : 0;
+        lv_cnt[f * kLv + l] = (l < n_lv) ? s_le[l] - s_lb[l] : 0;
+    }
+}
+
+// C: top sizes and pre-order numbers, then the per-depth node lists.
+__global__ __launch_bounds__(kTopoThreads) void topo_mid_kernel(
+    TopoCtx c, int *__restrict__ info, const int *__restrict__ lv_cnt, int *__restrict__ lv_off,
+    int *__restrict__ list) {
+    __shared__ int s_wave[kTopoThreads / 64];
+    const int tid = threadIdx.x;
+    const int T = info[kInfoT], F = info[kInfoF], D = info[kInfoD];
+    const int *lvl = info + kInfoTopLevel;  // top level d = BFS slots [lvl[d], lvl[d+1])
+    // Expanded top levels are 0..D-1; with a frontier (F > 0) level D holds
+    // the subtree roots, whose sizes come from B.
+    for (int d = D - 1; d >= 0; --d) {
+        sizes_level<kTopoThreads>(c, lvl[d], lvl[d + 1]);
+        __syncthreads();
+    }
+    if (tid == 0) c.b_pre[0] = 0;
+    __syncthreads();
+    for (int d = 0; d < D; ++d) {
+        preorder_level<kTopoThreads>(c, lvl[d], lvl[d + 1]);
+        __syncthreads();
+    }
+    for (int i = tid; i < T; i += kTopoThreads) list[i] = i;
+    // depth lists: top depths are BFS ranges; deeper ones are filled by D
+    const int top_levels = (F > 0) ? D + 1 : D;
+    for (int d = tid; d < top_levels; d += kTopoThreads) {
+        info[kInfoDepthOff + d] = lvl[d];
+        info[kInfoDepthCnt + d] = lvl[d + 1] - lvl[d];
+    }
+    int n = T, levels = top_levels;
+    for (int l = 1; F > 0 && l < kLv && D + l <= kMaxDepth; ++l) {
+        const int mine = (tid < F) ? lv_cnt[tid * kLv + l] : 0;
+        int tot;
+        const int off = block_exclusive_scan<kTopoThreads>(mine, s_wave, &tot);
+        if (tot == 0) break;
+        if (tid < F) lv_off[tid * kLv + l] = n + off;
+        if (tid == 0) {
+            info[kInfoDepthOff + D + l] = n;
+            info[kInfoDepthCnt + D + l] = tot;
+        }
+        n += tot;
+        levels = D + l + 1;
+    }
+    if (tid == 0) {
+        info[kInfoN] = n;
+        info[kInfoLevels] = levels;
+    }
+}
+
+// D: pre-order numbers inside the subtree and its entries of the depth lists.
+__global__ __launch_bounds__(kSubThreads) void topo_fin_kernel(TopoCtx c,
+                                                               const int *__restrict__ info,
+                                                               const int *__restrict__ lv_start,
+                                                               const int *__restrict__ lv_cnt,
+                                                               const int *__restrict__ lv_off,
+                                                               int *__restrict__ list) {
+    const int f = blockIdx.x;
+    if (f >= info[kInfoF]) return;
+    for (int l = 0; l + 1 < kLv && lv_cnt[f * kLv + l + 1] > 0; ++l) {
+        const int lb = lv_start[f * kLv + l];
+        preorder_level<kSubThreads>(c, lb, lb + lv_cnt[f * kLv + l]);
+        __syncthreads();
+    }
+    for (int l = 1; l < kLv && lv_cnt[f * kLv + l] > 0; ++l) {
+        const int lb = lv_start[f * kLv + l], n = lv_cnt[f * kLv + l], o = lv_off[f * kLv + l];
+        for (int i = threadIdx.x; i < n; i += kSubThreads) list[o + i] = lb + i;
+    }
 }
 
 struct TreeArrays {
@@ -214,13 +521,15 @@ struct TreeArrays {
     int32_t *lo_idx, *hi_idx, *right, *esc;
 };
 
+// One thread per node (node slots list[0, N)).
 __global__ void finalize_kernel(const double *__restrict__ sorted, int N, int n, double theta_sq,
-                                const int *__restrict__ b_lo, const int *__restrict__ b_hi,
-                                const int *__restrict__ b_left, const int *__restrict__ b_size,
-                                const int *__restrict__ b_pre, const double *__restrict__ sm1,
-                                TreeArrays t) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= N) return;
+                                const int *__restrict__ list, const int *__restrict__ b_lo,
+                                const int *__restrict__ b_hi, const int *__restrict__ b_left,
+                                const int *__restrict__ b_size, const int *__restrict__ b_pre,
+                                const double *__restrict__ sm1, TreeArrays t) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int v = list[i];
     const int k = b_pre[v];
     const int a = b_lo[v], b = b_hi[v];
     const double lo = sorted[a], hi = sorted[b - 1];
@@ -337,16 +646,17 @@ __global__ __launch_bounds__(kBaryLanes) void leaf_fill_kernel(const double *__r
                     t.w + (size_t)k * n, n, sq, sm, shit);
 }
 
-// Parents of one tree level (BFS ids [v0, v0 + gridDim.x)), one wavefront
+// Parents of one tree depth (node slots level[0, gridDim.x)), one wavefront
 // each.  The 2n rows are the children's Chebyshev nodes with their weights
 // as masses, left child first (merge_all_weights, chebcode.rs:474-506).
-__global__ __launch_bounds__(kBaryLanes) void merge_level_kernel(int n, int v0,
+__global__ __launch_bounds__(kBaryLanes) void merge_level_kernel(int n,
+                                                                 const int *__restrict__ level,
                                                                  const int *__restrict__ b_left,
                                                                  const int *__restrict__ b_pre,
                                                                  const double *__restrict__ lam,
                                                                  TreeArrays t) {
     extern __shared__ double lds[];
-    const int v = v0 + blockIdx.x;
+    const int v = level[blockIdx.x];
     if (b_left[v] < 0) return;
     const int k = b_pre[v];
     double *sq = lds;
@@ -781,12 +1091,17 @@ inline unsigned blocks_for(size_t n, int bs) { return (unsigned)((n + bs - 1) / 
 
 void cheb_destroy(ChebTree *t) {
     if (!t) return;
-    if (t->arena || t->arena2) {
+    // Cached arrays of a transient tree: mark their last use (the work its
+    // thread enqueued on the tree's stream); the next use waits for it.
+    if (t->cached_src) (void)cache_release(CACHE_CHEB_SRC, t->stream);
+    if (t->cached_nodes) (void)cache_release(CACHE_CHEB_NODES, t->stream);
+    const bool own_src = t->arena && !t->cached_src, own_nodes = t->arena2 && !t->cached_nodes;
+    if (own_src || own_nodes) {
         // Stream-ordered pool memory: wait until no stream can still read the
         // tree (the synchronisation hipFree would do), then return it.
         (void)hipDeviceSynchronize();
-        if (t->arena) (void)hipFreeAsync(t->arena, nullptr);
-        if (t->arena2) (void)hipFreeAsync(t->arena2, nullptr);
+        if (own_src) (void)hipFreeAsync(t->arena, nullptr);
+        if (own_nodes) (void)hipFreeAsync(t->arena2, nullptr);
     }
     if (t->ready) (void)hipEventDestroy(t->ready);
     delete t;
@@ -807,20 +1122,20 @@ struct Carver {
     }
 };
 
-// Stream-ordered scratch freed when it goes out of scope.
-struct Scratch {
-    void *p = nullptr;
+// Marks the end of a cache slot's use when it goes out of scope.
+struct CacheUse {
+    int slot;
     hipStream_t s;
-    explicit Scratch(hipStream_t st) : s(st) {}
-    ~Scratch() {
-        if (p) (void)hipFreeAsync(p, s);
+    bool on = false;
+    ~CacheUse() {
+        if (on) (void)cache_release(slot, s);
     }
 };
 
 }  // namespace
 
 int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **out,
-               hipStream_t stream) {
+               hipStream_t stream, int transient) {
     *out = nullptr;
     if (p == 0 || p > (size_t)INT32_MAX / 2) {
         set_error("chebcode build: p must be in [1, 2^30)");
@@ -844,33 +1159,45 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
     const int ip = (int)p;
     const int n = prm.n;
     const int leaf_cap = (int)std::min<int64_t>(prm.leaf_cap, INT32_MAX);
-    const size_t cap_a = 2 * p - 1;
-    const size_t cap_b = 2 * (size_t)(kMaxDepth + 1) * ((p + leaf_cap) / (leaf_cap + 1)) + 1;
-    const int cap = (int)std::min(cap_a, cap_b);
 
-    // Scratch: info words, BFS arrays, node/weight tables, sort temporaries.
-    const int n_info = 4 + kMaxDepth + 3;
-    Scratch scratch(stream);
+    // Node slots: the top walk's BFS ids (at most kFrontier per level before
+    // the frontier, which holds fewer than kMaxFrontier) and 2p for the
+    // subtrees.
+    const int cap_top = (int)std::min<size_t>(2 * p - 1, (size_t)(kMaxDepth + 3) * kFrontier);
+    const size_t slots = (size_t)cap_top + 2 * p;
+
+    // Scratch: info words, slot arrays, depth lists, subtree levels, node/
+    // weight tables, sort temporaries.
+    const int n_info = kInfoInts;
     Carver sc;
     (void)sc.take<int>(n_info);
-    (void)sc.take<int>(5 * (size_t)cap);
+    (void)sc.take<int>(6 * slots);
+    (void)sc.take<int>(3 * (size_t)kMaxFrontier * kLv);
     (void)sc.take<double>(2 * (size_t)n);
     (void)sc.take<int32_t>(p);
+    const int R = ip;  // hint buckets: about one element each on average
+    (void)sc.take<int>((size_t)R + 1);
     size_t sort_bytes = 0;
     SHK_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, d_ev, (double *)nullptr,
                                       (const int32_t *)nullptr, (int32_t *)nullptr, p, 0, 64,
                                       stream));
     (void)sc.take<char>(sort_bytes);
-    SHK_HIP(hipMallocAsync(&scratch.p, sc.off, stream));
-    sc.base = static_cast<char *>(scratch.p);
+    void *scratch = nullptr;
+    if (int st = cache_acquire(CACHE_CHEB_SCRATCH, sc.off, stream, &scratch)) return st;
+    CacheUse scratch_use{CACHE_CHEB_SCRATCH, stream, true};
+    sc.base = static_cast<char *>(scratch);
     sc.off = 0;
     int *d_info = sc.take<int>(n_info);
-    int *b_buf = sc.take<int>(5 * (size_t)cap);
+    int *b_buf = sc.take<int>(6 * slots);
+    int *lv_buf = sc.take<int>(3 * (size_t)kMaxFrontier * kLv);
     double *d_tab = sc.take<double>(2 * (size_t)n);
     int32_t *d_iota = sc.take<int32_t>(p);
+    int *d_hint = sc.take<int>((size_t)R + 1);
     void *d_sort_tmp = sc.take<char>(sort_bytes);
-    int *b_lo = b_buf, *b_hi = b_buf + cap, *b_left = b_buf + 2 * (size_t)cap,
-        *b_size = b_buf + 3 * (size_t)cap, *b_pre = b_buf + 4 * (size_t)cap;
+    int *b_lo = b_buf, *b_hi = b_buf + slots, *b_left = b_buf + 2 * slots,
+        *b_size = b_buf + 3 * slots, *b_pre = b_buf + 4 * slots, *d_list = b_buf + 5 * slots;
+    int *lv_start = lv_buf, *lv_cnt = lv_buf + (size_t)kMaxFrontier * kLv,
+        *lv_off = lv_buf + 2 * (size_t)kMaxFrontier * kLv;
 
     // Arena part 1 (sizes known before the topology): sources.
     Carver ar;
@@ -886,7 +1213,13 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
     // the arena, so the arena is allocated after a first pass with the
     // sources in scratch-independent storage: allocate the source part now.
     void *src_arena = nullptr;
-    SHK_HIP(hipMallocAsync(&src_arena, src_bytes, stream));
+    t->stream = stream;
+    if (transient) {
+        if (int st = cache_acquire(CACHE_CHEB_SRC, src_bytes, stream, &src_arena)) return st;
+        t->cached_src = true;
+    } else {
+        SHK_HIP(hipMallocAsync(&src_arena, src_bytes, stream));
+    }
     ar.base = static_cast<char *>(src_arena);
     ar.off = 0;
     t->sorted = ar.take<double>(p);
@@ -895,40 +1228,62 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
     t->s_lo = ar.take<float>(p);
     t->arena = src_arena;  // owned from here on (freed by cheb_destroy)
 
+    // 1-2. Speculate that the input is ascending (eigenvalues usually are):
+    // copy it, check it and build the topology on the copy, with one host
+    // synchronisation for the check and the node count together.  An input
+    // that is not ascending is then sorted (+ permutation) and the topology
+    // rebuilt; on unsorted values the first topology still stays in bounds
+    // (every split lies strictly inside its node), it is only wasted work.
+    auto topology = [&]() -> int {
+        hipLaunchKernelGGL(hint_clear_kernel, dim3(blocks_for((size_t)R + 1, 256)), dim3(256), 0,
+                           stream, d_hint, R + 1);
+        SHK_HIP(hipGetLastError());
+        hipLaunchKernelGGL(hint_scatter_kernel, dim3(blocks_for(p, 256)), dim3(256), 0, stream,
+                           t->sorted, ip, R, d_hint);
+        SHK_HIP(hipGetLastError());
+        const TopoCtx c{t->sorted, ip, leaf_cap, d_hint, R, Hint{}, b_lo, b_hi, b_left, b_size,
+                        b_pre};
+        hipLaunchKernelGGL(topo_top_kernel, dim3(1), dim3(kTopoThreads), 0, stream, c, cap_top,
+                           d_info);
+        SHK_HIP(hipGetLastError());
+        hipLaunchKernelGGL(topo_sub_kernel, dim3(kMaxFrontier), dim3(kSubThreads), 0, stream, c,
+                           d_info, lv_start, lv_cnt);
+        SHK_HIP(hipGetLastError());
+        hipLaunchKernelGGL(topo_mid_kernel, dim3(1), dim3(kTopoThreads), 0, stream, c, d_info,
+                           lv_cnt, lv_off, d_list);
+        SHK_HIP(hipGetLastError());
+        hipLaunchKernelGGL(topo_fin_kernel, dim3(kMaxFrontier), dim3(kSubThreads), 0, stream, c,
+                           d_info, lv_start, lv_cnt, lv_off, d_list);
+        SHK_HIP(hipGetLastError());
+        SHK_HIP(hipMemcpyAsync(info.data(), d_info, sizeof(int) * n_info,
+                               hipMemcpyDeviceToHost, stream));
+        SHK_HIP(hipStreamSynchronize(stream));
+        return 0;
+    };
     int zero4[4] = {0, 0, 0, 0};
     SHK_HIP(hipMemcpyAsync(d_info, zero4, sizeof(zero4), hipMemcpyHostToDevice, stream));
+    SHK_HIP(hipMemcpyAsync(t->sorted, d_ev, sizeof(double) * p, hipMemcpyDeviceToDevice, stream));
     if (p > 1) {
         hipLaunchKernelGGL(check_sorted_kernel, dim3(blocks_for(p, 256)), dim3(256), 0, stream,
                            d_ev, ip, d_info + 3);
         SHK_HIP(hipGetLastError());
     }
-    SHK_HIP(hipMemcpyAsync(info.data(), d_info, sizeof(int) * 4, hipMemcpyDeviceToHost, stream));
-    SHK_HIP(hipStreamSynchronize(stream));
-    if (info[3] == 0) {
-        SHK_HIP(hipMemcpyAsync(t->sorted, d_ev, sizeof(double) * p, hipMemcpyDeviceToDevice,
-                               stream));
-    } else {
+    if (int st = topology()) return st;
+    if (info[kInfoUnsorted] != 0) {
         t->perm = perm_buf;
         hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(p, 256)), dim3(256), 0, stream, d_iota, ip);
         SHK_HIP(hipGetLastError());
         SHK_HIP(rocprim::radix_sort_pairs(d_sort_tmp, sort_bytes, d_ev, t->sorted, d_iota, t->perm,
                                           p, 0, 64, stream));
+        if (int st = topology()) return st;
     }
-
-    // 2. topology
-    hipLaunchKernelGGL(topo_kernel, dim3(1), dim3(kTopoThreads), 0, stream, t->sorted, ip, leaf_cap,
-                       cap, b_lo, b_hi, b_left, b_size, b_pre, d_info);
-    SHK_HIP(hipGetLastError());
-    SHK_HIP(hipMemcpyAsync(info.data(), d_info, sizeof(int) * n_info, hipMemcpyDeviceToHost,
-                           stream));
-    SHK_HIP(hipStreamSynchronize(stream));
-    if (info[2]) {
+    if (info[kInfoOverflow]) {
         set_error("chebcode build: node capacity exceeded");
         return 6;
     }
-    const int N = info[0];
+    const int N = info[kInfoN];
     t->N = (size_t)N;
-    t->n_levels = info[1];
+    t->n_levels = info[kInfoLevels];
 
     // Arena part 2: node records and panels.
     Carver nr;
@@ -942,7 +1297,12 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
     (void)nr.take<int4>(N);
     (void)nr.take<float>((size_t)N * 2 * n);
     void *node_arena = nullptr;
-    SHK_HIP(hipMallocAsync(&node_arena, nr.off, stream));
+    if (transient) {
+        if (int st = cache_acquire(CACHE_CHEB_NODES, nr.off, stream, &node_arena)) return st;
+        t->cached_nodes = true;
+    } else {
+        SHK_HIP(hipMallocAsync(&node_arena, nr.off, stream));
+    }
     t->arena2 = node_arena;
     nr.base = static_cast<char *>(node_arena);
     nr.off = 0;
@@ -972,19 +1332,19 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
                            stream));
     TreeArrays ta{t->lo, t->hi, t->acc, t->nodes, t->w, t->lo_idx, t->hi_idx, t->right, t->esc};
     hipLaunchKernelGGL(finalize_kernel, dim3(blocks_for(N, 256)), dim3(256), 0, stream, t->sorted, N,
-                       n, t->theta_sq, b_lo, b_hi, b_left, b_size, b_pre, d_tab, ta);
+                       n, t->theta_sq, d_list, b_lo, b_hi, b_left, b_size, b_pre, d_tab, ta);
     SHK_HIP(hipGetLastError());
     // 4. leaves: one wavefront per node
     const size_t lds = bary_lds_bytes(n);
     hipLaunchKernelGGL(leaf_fill_kernel, dim3(N), dim3(kBaryLanes), lds, stream, t->sorted, n,
                        d_tab + n, ta);
     SHK_HIP(hipGetLastError());
-    // 5. parents, deepest level first: one launch per level, one wavefront per node
+    // 5. parents, deepest depth first: one launch per depth, one wavefront per node
     for (int d = t->n_levels - 2; d >= 0; --d) {
-        const int v0 = info[4 + d], v1 = info[4 + d + 1];
-        if (v1 <= v0) continue;
-        hipLaunchKernelGGL(merge_level_kernel, dim3(v1 - v0), dim3(kBaryLanes), lds, stream, n, v0,
-                           b_left, b_pre, d_tab + n, ta);
+        const int off = info[kInfoDepthOff + d], cnt = info[kInfoDepthCnt + d];
+        if (cnt <= 0) continue;
+        hipLaunchKernelGGL(merge_level_kernel, dim3(cnt), dim3(kBaryLanes), lds, stream, n,
+                           d_list + off, b_left, b_pre, d_tab + n, ta);
         SHK_HIP(hipGetLastError());
     }
     // 6. mirrors

@@ -15,6 +15,26 @@ namespace shk {
 void set_error(const std::string &msg);
 const char *last_error();
 
+// ---- per-thread device buffer cache ---------------------------------------
+// One buffer per slot, per calling thread and device, grown on demand (and
+// never shrunk until cache_free_all).  Each slot records an event at its
+// last use; a use on another stream first waits for it, so reuse never
+// races with work still in flight.  Growing waits for the last use and
+// frees the old buffer.
+enum CacheSlot : int {
+    CACHE_CHEB_SCRATCH = 0,  // tree build scratch
+    CACHE_CHEB_SRC = 1,      // transient tree: sorted sources
+    CACHE_CHEB_NODES = 2,    // transient tree: node records and panels
+    kCacheSlots = 3
+};
+// Returns the slot's buffer (>= bytes) in *out, ordered on stream s.
+int cache_acquire(int slot, size_t bytes, hipStream_t s, void **out);
+// Marks the end of the slot's use by the work enqueued on s so far.
+int cache_release(int slot, hipStream_t s);
+// Frees the calling thread's cached buffers on the current device (waits
+// for their last uses).
+int cache_free_all();
+
 // Evaluates a HIP call; on failure records the message and returns
 // SHK_ERR_HIP (7) from the enclosing function.
 #define SHK_HIP(expr)                                                           \
@@ -142,9 +162,12 @@ struct ChebParams {
     int f32_far;
 };
 
-// Builds the tree on the GPU from device eigenvalues (any order).
+// Builds the tree on the GPU from device eigenvalues (any order).  A
+// transient tree (one evaluated and destroyed by the same thread before its
+// next build) keeps its arrays in that thread's buffer cache instead of
+// allocating them; its destruction then needs no synchronisation.
 int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **out,
-               hipStream_t stream);
+               hipStream_t stream, int transient = 0);
 void cheb_destroy(ChebTree *t);
 size_t cheb_node_count(const ChebTree *t);
 size_t cheb_p(const ChebTree *t);

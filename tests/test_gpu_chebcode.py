@@ -255,3 +255,38 @@ def test_work_counts_match_oracle(shk, orc, preset, p, eta_scale):
     near, far = b.count_work(eta, query=torch.from_numpy(grid).cuda())
     o_near, o_panels = t.count_work(grid, eta, fast)
     assert (near, far) == (o_near, o_panels * n)
+
+
+def test_cached_buffers_across_streams_and_release(shk):
+    """One-shot ChebCode transforms keep the build scratch and the tree in
+    buffers cached per thread: calls on two streams with no synchronisation
+    between them, sizes that grow and shrink the cache, and a release of the
+    cache in between all give the same bits as the first call."""
+    import torch
+    from shrinkers_amd import _lib, spectra
+    specs = [spectra.mp_inverse_cdf_spectrum(p, c=c) for p, c in ((30_000, 0.3), (120_000, 0.2),
+                                                                 (5_000, 0.5))]
+    dev = [torch.from_numpy(e).cuda() for e in specs]
+    etas = [1.0 / np.sqrt(e.size) for e in specs]
+    ref = []
+    for d, eta in zip(dev, etas):
+        re, im = shk.all_points_dev(d, eta, "chebcode")
+        torch.cuda.synchronize()
+        ref.append((re.cpu().numpy(), im.cpu().numpy()))
+    grid = torch.linspace(0.2, 3.0, 200, dtype=torch.float64, device="cuda")
+    g_ref = [t.cpu().numpy() for t in shk.at_points_dev(grid, dev[1], 0.01, "chebcode_fast")]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    outs = []
+    for rep in range(3):
+        for k, (d, eta) in enumerate(zip(dev, etas)):
+            with torch.cuda.stream(streams[(k + rep) % 2]):
+                outs.append((k, shk.all_points_dev(d, eta, "chebcode")))
+                outs.append((-1, shk.at_points_dev(grid, dev[1], 0.01, "chebcode_fast")))
+        if rep == 1:
+            torch.cuda.synchronize()
+            assert _lib.load().shk_release_cached_memory() == _lib.SHK_OK
+    torch.cuda.synchronize()
+    for k, (re, im) in outs:
+        want = ref[k] if k >= 0 else g_ref
+        np.testing.assert_array_equal(re.cpu().numpy(), want[0])
+        np.testing.assert_array_equal(im.cpu().numpy(), want[1])
