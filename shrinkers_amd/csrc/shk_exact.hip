@@ -97,13 +97,14 @@ __global__ __launch_bounds__(kWG) void exact_f64_kernel(
         __syncthreads();
         for (int j = tid; j < cnt; j += kWG) s_src[j] = src[base + j];
         __syncthreads();
-#pragma unroll 2
-        for (int j = 0; j < cnt; ++j) {
-            const double s = s_src[j];
+        // Two sources per step (one combined skip-branch for their paired
+        // reciprocals); each pair's reciprocal depends on its own
+        // denominators only, so the grouping does not change any result.
+        auto accumulate = [&](const double (&dd)[kR], const double (&rr)[kR]) {
 #pragma unroll
             for (int k = 0; k < kR; ++k) {
-                const double d = x[k] - s;
-                const double r = rcp_f64(__builtin_fma(d, d, e2[k]));
+                const double d = dd[k];
+                const double r = rr[k];
                 if constexpr (KIND == EXACT_CUTOFF) {
                     const double rm = (__builtin_fabs(d) <= cut) ? r : 0.0;
                     acc[0][k] = __builtin_fma(d, rm, acc[0][k]);
@@ -123,6 +124,40 @@ __global__ __launch_bounds__(kWG) void exact_f64_kernel(
                     acc[1][k] += r;
                 }
             }
+        };
+        int j = 0;
+        for (; j + 2 <= cnt; j += 2) {
+            const double s0 = s_src[j], s1 = s_src[j + 1];
+            double dd[2 * kR], tt[2 * kR], rr[2 * kR];
+#pragma unroll
+            for (int k = 0; k < kR; ++k) {
+                dd[k] = x[k] - s0;
+                dd[kR + k] = x[k] - s1;
+                tt[k] = __builtin_fma(dd[k], dd[k], e2[k]);
+                tt[kR + k] = __builtin_fma(dd[kR + k], dd[kR + k], e2[k]);
+            }
+            rcp_pairs_f64(tt, rr);
+            double d0[kR], r0[kR], d1[kR], r1[kR];
+#pragma unroll
+            for (int k = 0; k < kR; ++k) {
+                d0[k] = dd[k];
+                r0[k] = rr[k];
+                d1[k] = dd[kR + k];
+                r1[k] = rr[kR + k];
+            }
+            accumulate(d0, r0);
+            accumulate(d1, r1);
+        }
+        if (j < cnt) {
+            const double s0 = s_src[j];
+            double dd[kR], tt[kR], rr[kR];
+#pragma unroll
+            for (int k = 0; k < kR; ++k) {
+                dd[k] = x[k] - s0;
+                tt[k] = __builtin_fma(dd[k], dd[k], e2[k]);
+            }
+            rcp_pairs_f64(tt, rr);
+            accumulate(dd, rr);
         }
     }
 
@@ -428,15 +463,43 @@ __global__ __launch_bounds__(kWG) void many_f64_kernel(const double *__restrict_
         for (int j = tid; j < cnt; j += kWG) s_src[j] = src[base + j];
         __syncthreads();
         seg_sweep_piece(base, cnt, sp.per, p, chunk_end, re, iv, tr, ti, [&](int j0, int n) {
-#pragma unroll 2
-            for (int j = j0; j < j0 + n; ++j) {
-                const double s = s_src[j];
+            // two sources per step, as in exact_f64_kernel (same bits)
+            int j = j0;
+            for (; j + 2 <= j0 + n; j += 2) {
+                const double s0 = s_src[j], s1 = s_src[j + 1];
+                double dd[2 * kR], tt[2 * kR], rr[2 * kR];
 #pragma unroll
                 for (int k = 0; k < kR; ++k) {
-                    const double d = x[k] - s;
-                    const double r = rcp_f64(__builtin_fma(d, d, e2[k]));
-                    re[k] = __builtin_fma(d, r, re[k]);
-                    iv[k] += r;
+                    dd[k] = x[k] - s0;
+                    dd[kR + k] = x[k] - s1;
+                    tt[k] = __builtin_fma(dd[k], dd[k], e2[k]);
+                    tt[kR + k] = __builtin_fma(dd[kR + k], dd[kR + k], e2[k]);
+                }
+                rcp_pairs_f64(tt, rr);
+#pragma unroll
+                for (int k = 0; k < kR; ++k) {
+                    re[k] = __builtin_fma(dd[k], rr[k], re[k]);
+                    iv[k] += rr[k];
+                }
+#pragma unroll
+                for (int k = 0; k < kR; ++k) {
+                    re[k] = __builtin_fma(dd[kR + k], rr[kR + k], re[k]);
+                    iv[k] += rr[kR + k];
+                }
+            }
+            if (j < j0 + n) {
+                const double s0 = s_src[j];
+                double dd[kR], tt[kR], rr[kR];
+#pragma unroll
+                for (int k = 0; k < kR; ++k) {
+                    dd[k] = x[k] - s0;
+                    tt[k] = __builtin_fma(dd[k], dd[k], e2[k]);
+                }
+                rcp_pairs_f64(tt, rr);
+#pragma unroll
+                for (int k = 0; k < kR; ++k) {
+                    re[k] = __builtin_fma(dd[k], rr[k], re[k]);
+                    iv[k] += rr[k];
                 }
             }
         });

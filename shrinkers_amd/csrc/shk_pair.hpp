@@ -31,6 +31,39 @@ __device__ __forceinline__ double rcp_f64(double x) {
 }
 #endif
 
+// 1/t[k] for R (even) positive denominators, two per v_rcp_f64: with
+// P = t0 t1 and Q = 1/P (rcp_f64), 1/t0 = t1 Q and 1/t1 = t0 Q.  That holds
+// while P is a positive normal number; a lane whose product is not (t0 t1
+// overflowing or underflowing, i.e. |x - lam| near 1e154 or eta near 1e-154)
+// takes rcp_f64 of each denominator instead.  Every lane's result depends
+// on its own two denominators only.  Relative error about two ulp (one
+// rounding each for P, Q and the product) against one for rcp_f64; per pair
+// half a quarter-rate v_rcp_f64 and two FMAs' worth of full-rate work
+// instead of one v_rcp_f64 and three.
+template <int R>
+__device__ __forceinline__ void rcp_pairs_f64(const double (&t)[R], double (&r)[R]) {
+    static_assert(R % 2 == 0, "denominators are paired");
+    bool ok[R / 2], all = true;
+#pragma unroll
+    for (int k = 0; k < R; k += 2) {
+        const double p = t[k] * t[k + 1];
+        const double q = rcp_f64(p);
+        r[k] = t[k + 1] * q;
+        r[k + 1] = t[k] * q;
+        ok[k / 2] = __builtin_amdgcn_class(p, 0x100);  // a positive normal number
+        all = all && ok[k / 2];
+    }
+    if (!all) {  // one branch for all R/2 products, skipped unless a lane needs it
+#pragma unroll
+        for (int k = 0; k < R; k += 2) {
+            if (!ok[k / 2]) {
+                r[k] = rcp_f64(t[k]);
+                r[k + 1] = rcp_f64(t[k + 1]);
+            }
+        }
+    }
+}
+
 constexpr int kF32Run = 32;  // f32 terms per FP64 fold
 
 template <int FORM>
