@@ -210,16 +210,48 @@ __global__ __launch_bounds__(kAWG) void adj_src_f64_kernel(
             s_c[u] = -e2 * a;
         }
         __syncthreads();
-#pragma unroll 2
-        for (int u = 0; u < cnt; ++u) {
-            const double x = s_x[u], A = s_a[u], B = s_b[u], C = s_c[u];
+        // two query points per step; the kAR reciprocals of each come in
+        // pairs from one v_rcp_f64 (rcp_pairs_f64)
+        auto step = [&](int u, const double (&dd)[kAR], const double (&rr)[kAR]) {
+            const double A = s_a[u], B = s_b[u], C = s_c[u];
 #pragma unroll
             for (int k = 0; k < kAR; ++k) {
-                const double dd = lam[k] - x;
-                const double r = rcp_f64(__builtin_fma(dd, dd, e2));
-                const double poly = __builtin_fma(__builtin_fma(A, dd, B), dd, C);
-                acc[k] = __builtin_fma(r, r * poly, acc[k]);
+                const double poly = __builtin_fma(__builtin_fma(A, dd[k], B), dd[k], C);
+                acc[k] = __builtin_fma(rr[k], rr[k] * poly, acc[k]);
             }
+        };
+        int u = 0;
+        for (; u + 2 <= cnt; u += 2) {
+            const double x0 = s_x[u], x1 = s_x[u + 1];
+            double dd[2 * kAR], tt[2 * kAR], rr[2 * kAR];
+#pragma unroll
+            for (int k = 0; k < kAR; ++k) {
+                dd[k] = lam[k] - x0;
+                dd[kAR + k] = lam[k] - x1;
+                tt[k] = __builtin_fma(dd[k], dd[k], e2);
+                tt[kAR + k] = __builtin_fma(dd[kAR + k], dd[kAR + k], e2);
+            }
+            rcp_pairs_f64(tt, rr);
+            double d0[kAR], r0[kAR], d1[kAR], r1[kAR];
+#pragma unroll
+            for (int k = 0; k < kAR; ++k) {
+                d0[k] = dd[k];
+                r0[k] = rr[k];
+                d1[k] = dd[kAR + k];
+                r1[k] = rr[kAR + k];
+            }
+            step(u, d0, r0);
+            step(u + 1, d1, r1);
+        }
+        if (u < cnt) {
+            double dd[kAR], tt[kAR], rr[kAR];
+#pragma unroll
+            for (int k = 0; k < kAR; ++k) {
+                dd[k] = lam[k] - s_x[u];
+                tt[k] = __builtin_fma(dd[k], dd[k], e2);
+            }
+            rcp_pairs_f64(tt, rr);
+            step(u, dd, rr);
         }
     }
 #pragma unroll
@@ -383,17 +415,49 @@ __global__ __launch_bounds__(kAWG) void adj_qry_f64_kernel(const double *__restr
         __syncthreads();
         for (int u = lane; u < cnt; u += kAWG) s_src[u] = ev[base + u];
         __syncthreads();
-#pragma unroll 2
-        for (int u = 0; u < cnt; ++u) {
-            const double s = s_src[u];
+        // two sources per step; the kAR reciprocals of each come in pairs
+        // from one v_rcp_f64 (rcp_pairs_f64)
+        auto step = [&](const double (&dd)[kAR], const double (&rr)[kAR]) {
 #pragma unroll
             for (int k = 0; k < kAR; ++k) {
-                const double d = x[k] - s;
-                const double r = rcp_f64(__builtin_fma(d, d, e2));
+                const double d = dd[k], r = rr[k];
                 const double e = __builtin_fma(-d, d, e2);
                 P[k] = __builtin_fma(r, r * e, P[k]);
                 Q[k] = __builtin_fma(r, r * d, Q[k]);
             }
+        };
+        int u = 0;
+        for (; u + 2 <= cnt; u += 2) {
+            const double s0 = s_src[u], s1 = s_src[u + 1];
+            double dd[2 * kAR], tt[2 * kAR], rr[2 * kAR];
+#pragma unroll
+            for (int k = 0; k < kAR; ++k) {
+                dd[k] = x[k] - s0;
+                dd[kAR + k] = x[k] - s1;
+                tt[k] = __builtin_fma(dd[k], dd[k], e2);
+                tt[kAR + k] = __builtin_fma(dd[kAR + k], dd[kAR + k], e2);
+            }
+            rcp_pairs_f64(tt, rr);
+            double d0[kAR], r0[kAR], d1[kAR], r1[kAR];
+#pragma unroll
+            for (int k = 0; k < kAR; ++k) {
+                d0[k] = dd[k];
+                r0[k] = rr[k];
+                d1[k] = dd[kAR + k];
+                r1[k] = rr[kAR + k];
+            }
+            step(d0, r0);
+            step(d1, r1);
+        }
+        if (u < cnt) {
+            double dd[kAR], tt[kAR], rr[kAR];
+#pragma unroll
+            for (int k = 0; k < kAR; ++k) {
+                dd[k] = x[k] - s_src[u];
+                tt[k] = __builtin_fma(dd[k], dd[k], e2);
+            }
+            rcp_pairs_f64(tt, rr);
+            step(dd, rr);
         }
     }
 #pragma unroll

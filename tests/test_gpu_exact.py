@@ -218,3 +218,33 @@ def test_errors_are_raised(shk):
         shk.compute_all_stieltjes(np.array([1.0, np.nan]), 0.1, "blocked")
     with pytest.raises(shk.ShkError):
         shk.compute_all_stieltjes(np.array([1.0, 2.0]), -0.1, "blocked")
+
+
+def test_f64_paired_reciprocal_fallback(shk, orc):
+    """The FP64 kernels take two reciprocals from one v_rcp_f64 of the
+    product of two pair denominators and fall back to one reciprocal each
+    where that product is not a normal number.  Scaling eigenvalues and eta
+    by 2^300 makes every product overflow (all pairs take the fallback) and
+    changes the sums by exactly 2^-300; a bulk near 1 with outliers near
+    2^400 mixes both paths; a tiny eta with duplicated eigenvalues makes the
+    products of coinciding pairs underflow."""
+    from shrinkers_amd import spectra as S
+    ev = S.mp_inverse_cdf_spectrum(3000, c=0.3)
+    eta = 1.0 / np.sqrt(ev.size)
+    r, i = shk.compute_all_stieltjes(ev, eta, "blocked")
+    big = np.ldexp(1.0, 300)
+    rs, is_ = shk.compute_all_stieltjes(ev * big, eta * big, "blocked")
+    assert_close(rs * big, is_ * big, r, i, 1e-13, "all pairs on the fallback")
+
+    rng = np.random.default_rng(3)
+    mixed = np.sort(np.concatenate([ev[:2000], np.ldexp(rng.uniform(1, 2, 40), 400)]))
+    eta_m = 0.01
+    r, i = shk.compute_all_stieltjes(mixed, eta_m, "blocked")
+    rr, ri = orc.tiled_parallel(mixed, eta_m)
+    assert_close(r, i, rr / mixed.size, ri / mixed.size, 1e-13, "paired + fallback mix")
+
+    dup = np.sort(np.concatenate([ev[:500], ev[:500], ev[200:300]]))
+    eta_t = np.ldexp(1.0, -300)
+    r, i = shk.compute_all_stieltjes(dup, eta_t, "blocked")
+    rr, ri = orc.tiled_parallel(dup, eta_t)
+    assert_close(r, i, rr / dup.size, ri / dup.size, 1e-13, "underflowing products")
