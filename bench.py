@@ -173,8 +173,10 @@ def many_leg(shk, sync) -> dict:
     (shk_all_points_many_dev): the segmented launch (one grid over every
     spectrum's query tiles, FP32-compensated and FP64) and the stream-pool
     path (the single-spectrum kernels, largest spectra first, over 1 or 4
-    library streams); the persistent hipGraph launch is switched off.
-    Times are device events."""
+    library streams); then the same spectra as a prepared batch
+    (shk_batch_*: device table, launch captured in a hipGraph and replayed,
+    tile height tuned).  Per-spectrum times are device events (stream pool)
+    or per-workgroup device wall-clock stamps (segmented launch)."""
     import torch
     from shrinkers_amd import spectra
     from shrinkers_amd.batch import SpectrumBatch
@@ -208,6 +210,32 @@ def many_leg(shk, sync) -> dict:
                     "completion_ms_p99": float(np.percentile(t[:, 1], 99)),
                     "last_completion_ms": float(t[:, 1].max())})
         out[label] = rec
+    # the same spectra as a prepared batch: table on the device, the launch
+    # captured once in a hipGraph, tile height tuned on this batch
+    from shrinkers_amd.batch import BatchedStieltjes
+    gb = BatchedStieltjes(sp, etas)
+    r_best, ms_tune = gb.tune()
+    for _ in range(2):
+        gb.run()
+    sync()
+    reps = 10
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        gb.run()
+    sync()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    s0, s1 = gb.spectrum_times_us()
+    dur = (s1 - s0) * 1e-3
+    out["hipgraph_replay"] = {
+        "precision": "f32c", "tuned_queries_per_lane": r_best, "tune_best_ms": ms_tune,
+        "ms_per_batch": ms, "spectra_per_s": 512 / (ms * 1e-3),
+        "pair_evals_per_s": pairs / (ms * 1e-3),
+        "spectrum_ms_p50": float(np.percentile(dur, 50)),
+        "spectrum_ms_p99": float(np.percentile(dur, 99)),
+        "completion_ms_p50": float(np.percentile(s1 * 1e-3, 50)),
+        "completion_ms_p99": float(np.percentile(s1 * 1e-3, 99)),
+        "last_completion_ms": float(s1.max() * 1e-3)}
+    del gb
     b.run("blocked", "f32c", True, 0)
     res = b.results()
     worst = 0.0
@@ -529,12 +557,10 @@ def main() -> int:
     if cpu:
         line["cpu_baseline"]["gpu_over_cpu"] = value / cpu["value"]
     if not args.quick:
-        # Many spectra through the single-spectrum kernels in one call; the
-        # persistent launch (shk_batch_create) is switched off in the library.
+        # 512 spectra: one call (segmented launch / stream pool) and the
+        # prepared batch replayed from a hipGraph
         log("512 spectra in one call")
         line["batched"] = many_leg(shk, sync)
-        line["batched"]["persistent_hipgraph_launch"] = (
-            "switched off: its first GPU run did not finish, cause not found")
         log("ChebCode at p = 2e5 / 1e6")
         line["chebcode_large"] = chebcode_leg(shk, sync)
         log("grid (at-points) leg")

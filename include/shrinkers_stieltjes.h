@@ -197,8 +197,9 @@ shk_status shk_all_points_dev(const double *d_eigenvalues, size_t p, double eta,
  * batch start (stream pool: device events; segmented launch: device wall
  * clock at its first workgroup's start and last workgroup's end, from the
  * launch's earliest workgroup start), otherwise it returns without
- * synchronising.  The persistent launch of shk_batch_* is switched off (see
- * below). */
+ * synchronising.  For a fixed set of spectra run repeatedly, shk_batch_*
+ * (below) keeps the table on the device and replays the launch from a
+ * hipGraph. */
 shk_status shk_all_points_many_dev(const double *d_eigenvalues, const int64_t *offsets,
                                    size_t n_spectra, const double *etas, int method,
                                    int precision, int scaled, double *d_out_re,
@@ -272,34 +273,43 @@ shk_status shk_chebcode_count_work(const shk_chebcode *h, const double *d_query,
 
 /* ---- batched spectra (no counterpart in the reference) ------------------- */
 
-/* Many independent spectra in one persistent launch captured in a hipGraph.
+/* Many independent spectra as one prepared launch replayed from a hipGraph.
  * The reference batches only across eta on one spectrum
  * (ChebCodeBatch::evaluate_many, chebcode.rs:1312); each spectrum's result
  * here is its own all-points transform (compute_all_stieltjes with the exact
  * family), in FP32-compensated arithmetic, scaled by 1/p_s when scaled != 0.
  *
  * d_eigenvalues: all spectra concatenated on the device (kept alive by the
- * caller); offsets: host array of n_spectra+1 starts (offsets[0] = 0);
- * etas: host array of one eta per spectrum.  Outputs share the layout of
- * d_eigenvalues.
+ * caller for the batch's life); offsets: host array of n_spectra+1 starts
+ * (offsets[0] = 0, every spectrum non-empty); etas: host array of one eta per
+ * spectrum.  Outputs share the layout of d_eigenvalues.
  *
- * SWITCHED OFF: unless the library is built with -DSHK_ENABLE_BATCH,
- * shk_batch_create checks its arguments and then returns SHK_ERR_UNSUPPORTED
- * (the first GPU run of this path did not finish and no cause has been
- * shown); the other batch calls then only see a NULL handle. */
+ * The launch is the segmented kernel of shk_all_points_many_dev (one
+ * workgroup per spectrum and query tile, largest spectra first, no
+ * partial-sum scratch); the batch keeps its spectrum table on the device and
+ * captures the launch in a hipGraph on first use (again when the outputs or
+ * the tile height change).  Every run stamps each workgroup's device wall
+ * clock for shk_batch_spectrum_times.  With the default tile height r = 2
+ * every result equals shk_all_points_dev's (FP32-compensated) bit for bit;
+ * r = 1 or 4 (64- or 256-query tiles) agree within the FP32-compensated
+ * bound.  A batch is used from one thread at a time; its runs share the
+ * stamp buffer, so a run's times are read before the next run. */
 typedef struct shk_batch shk_batch;
 
 shk_status shk_batch_create(const double *d_eigenvalues, const int64_t *offsets, size_t n_spectra,
                             const double *etas, int scaled, shk_batch **out);
-/* Enqueue the graph on `stream` (re-captured when the outputs change). */
+/* Enqueue the graph on `stream`; returns without synchronising. */
 shk_status shk_batch_run(shk_batch *b, double *d_out_re, double *d_out_im, void *stream);
-/* Time every (queries per lane, workgroups per CU) candidate on this batch
- * and keep the fastest; the chosen values and time are returned. */
+/* Time each tile height r in {1, 2, 4} on this batch (three replays each,
+ * synchronising `stream`) and keep the fastest; the chosen r and its time
+ * are returned. */
 shk_status shk_batch_tune(shk_batch *b, double *d_out_re, double *d_out_im, void *stream,
-                          int *best_r, int *best_workers_per_cu, double *best_ms);
-shk_status shk_batch_set_config(shk_batch *b, int r, int workers_per_cu);
-/* Per-spectrum start/end of the last completed run in microseconds from the
- * run's first item (device wall clock); the caller synchronises first. */
+                          int *best_r, double *best_ms);
+/* Set the tile height r (1, 2 or 4 queries per lane). */
+shk_status shk_batch_set_config(shk_batch *b, int r);
+/* Per-spectrum start/end of the last run in microseconds from that run's
+ * first workgroup start (device wall clock); the caller synchronises first.
+ * Fails before the first run. */
 shk_status shk_batch_spectrum_times(const shk_batch *b, double *start_us, double *end_us);
 shk_status shk_batch_destroy(shk_batch *b);
 

@@ -73,8 +73,8 @@ SIGNATURES = {
     "shk_chebcode_count_work": (_i, [_vp, _vp, _sz, _d, _i64p, _i64p, _vp]),
     "shk_batch_create": (_i, [_vp, _i64p, _sz, _dp, _i, ctypes.POINTER(_vp)]),
     "shk_batch_run": (_i, [_vp, _vp, _vp, _vp]),
-    "shk_batch_tune": (_i, [_vp, _vp, _vp, _vp, _ip, _ip, _dp]),
-    "shk_batch_set_config": (_i, [_vp, _i, _i]),
+    "shk_batch_tune": (_i, [_vp, _vp, _vp, _vp, _ip, _dp]),
+    "shk_batch_set_config": (_i, [_vp, _i]),
     "shk_batch_spectrum_times": (_i, [_vp, _dp, _dp]),
     "shk_batch_destroy": (_i, [_vp]),
     "shk_exact_at_points_dev": (_i, [_vp, _sz, _vp, _sz, _d, _i, _vp, _vp, _vp]),

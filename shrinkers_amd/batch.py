@@ -7,8 +7,9 @@ spread over a few library streams (``n_streams`` 1..8, which can also return
 per-spectrum device start/end times).  Each result equals the
 single-spectrum call bit for bit.
 
-``BatchedStieltjes``: the persistent work-queue launch captured in a hipGraph.
-It is switched off in this build (the library refuses to create it).
+``BatchedStieltjes``: a fixed set of spectra kept on the device with its
+spectrum table; the segmented launch is captured in a hipGraph and replayed
+by each ``run``, with per-spectrum device times and a tile-height tuner.
 """
 from __future__ import annotations
 
@@ -85,8 +86,9 @@ class SpectrumBatch:
 
 
 class BatchedStieltjes:
-    """All-points transforms of ``spectra`` (a list of 1-D arrays) with one
-    eta per spectrum.  Eigenvalues are copied to the GPU once."""
+    """All-points transforms of ``spectra`` (a list of non-empty 1-D arrays)
+    with one eta per spectrum, FP32-compensated.  Eigenvalues are copied to
+    the GPU once; each ``run`` replays the captured launch."""
 
     def __init__(self, spectra: Sequence, etas: Sequence[float], scaled: bool = True):
         import torch
@@ -116,17 +118,18 @@ class BatchedStieltjes:
             self._h = None
 
     def tune(self, stream=None):
-        """Time the launch candidates on this batch and keep the fastest;
-        returns ``(queries_per_lane, workgroups_per_cu, ms)``."""
-        r, w, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        """Time the tile heights (1, 2, 4 queries per lane) on this batch and
+        keep the fastest; returns ``(queries_per_lane, ms)``."""
+        r, ms = ctypes.c_int(), ctypes.c_double()
         check(_lib.load().shk_batch_tune(self._h, self.re.data_ptr(), self.im.data_ptr(),
-                                         _stream_handle(stream), ctypes.byref(r), ctypes.byref(w),
+                                         _stream_handle(stream), ctypes.byref(r),
                                          ctypes.byref(ms)))
-        return r.value, w.value, ms.value
+        return r.value, ms.value
 
-    def set_config(self, queries_per_lane: int, workgroups_per_cu: int) -> None:
-        check(_lib.load().shk_batch_set_config(self._h, int(queries_per_lane),
-                                               int(workgroups_per_cu)))
+    def set_config(self, queries_per_lane: int) -> None:
+        """Tile height: 1, 2 (default: results equal the single-spectrum
+        kernel's bit for bit) or 4 queries per lane."""
+        check(_lib.load().shk_batch_set_config(self._h, int(queries_per_lane)))
 
     def run(self, stream=None):
         """Enqueue the batch; returns the (re, im) CUDA tensors (concatenated)."""

@@ -1,22 +1,23 @@
-// Many spectra of uneven size in one persistent launch, captured in a hipGraph.
+// Many spectra of uneven size as one prepared launch replayed from a hipGraph.
 //
 // There is no batched path in the reference (its batching is across eta on
-// one spectrum, chebcode.rs:1312); each spectrum's result here is exactly the
-// single-spectrum all-points transform (compute_all_stieltjes with the
-// exact family, src/stieltjes/mod.rs:709-723) in FP32-compensated arithmetic.
+// one spectrum, chebcode.rs:1312); each spectrum's result here is its own
+// all-points transform (compute_all_stieltjes with the exact family,
+// src/stieltjes/mod.rs:709-723) in FP32-compensated arithmetic.
 //
-// Work items are (spectrum, 64*R-query tile) pairs; an item sweeps all of its
-// spectrum's sources, so it owns its outputs and needs no cross-workgroup
-// reduction.  Items are ordered by descending spectrum size with each
-// spectrum's tiles contiguous, and one-wavefront workgroups pull them from an
-// atomic counter until the list is exhausted (every wavefront reaches the
-// exit).  Each item records device wall-clock start/end stamps; a spectrum's
-// latency is its last item's end minus its first item's start.
+// The launch is the segmented all-points kernel of shk_exact.hip: one
+// one-wavefront workgroup per (spectrum, query tile), the tiles of all
+// spectra in one grid, largest spectra first.  A batch keeps its spectrum
+// table on the device and captures the launch (a single kernel node) in a
+// hipGraph on first use; later runs replay the graph.  Every workgroup stamps
+// the device wall clock at its start and end, from which the per-spectrum
+// start and end times are derived.
 //
-// The counter is reset by a one-thread kernel inside the graph (no memset
-// node).  An autotuner times the (R, workers per CU) candidates on the
-// actual batch and keeps the fastest.
-#include "shk_pair.hpp"
+// Tuning: the tile height (r = 1, 2 or 4 queries per lane, tiles of 64, 128
+// or 256 queries) is timed on the actual batch and the fastest kept.  With
+// r = 2 the tiles are the single-spectrum kernel's, so every result equals
+// that kernel's bit for bit; r = 1 and 4 use other tiles (their own centres
+// and pair forms) and agree within the FP32-compensated bound.
 #include "shk_internal.hpp"
 #include "shrinkers_stieltjes.h"
 
@@ -26,139 +27,36 @@
 #include <string>
 #include <vector>
 
-namespace shk {
+using namespace shk;
+
 namespace {
 
-constexpr int kBLanes = 64;    // one wavefront per workgroup
-constexpr int kBPiece = 512;   // sources staged in LDS per pass
+constexpr int kRs[3] = {1, 2, 4};
 
-struct BatchSpec {
-    long long off;
-    int p;
-    int pad;
-    double eta;
-    double scale;
-};
-
-__global__ void batch_reset_kernel(int *counter) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *counter = 0;
-}
-
-__device__ __forceinline__ double bwave_min(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ double bwave_max(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
-template <int R>
-__global__ __launch_bounds__(kBLanes) void batch_f32c_kernel(
-    const double *__restrict__ ev, const BatchSpec *__restrict__ specs,
-    const int2 *__restrict__ items, int n_items, int *__restrict__ counter,
-    double *__restrict__ out_re, double *__restrict__ out_im, long long *__restrict__ t_start,
-    long long *__restrict__ t_end) {
-    __shared__ float s_hi[kBPiece];
-    __shared__ float s_lo[kBPiece];
-    __shared__ int s_item;
-    const int lane = threadIdx.x;
-    for (;;) {
-        if (lane == 0) s_item = atomicAdd(counter, 1);
-        __syncthreads();
-        const int item = s_item;
-        __syncthreads();
-        if (item >= n_items) break;
-        const long long t0 = wall_clock64();
-        const int2 it = items[item];
-        const BatchSpec sp = specs[it.x];
-        const double *src = ev + sp.off;
-        const int p = sp.p;
-        const int base_q = it.y * (kBLanes * R);
-
-        double x[R];
-        double mn = __builtin_inf(), mx = -__builtin_inf();
-#pragma unroll
-        for (int k = 0; k < R; ++k) {
-            const int i = base_q + k * kBLanes + lane;
-            x[k] = (i < p) ? src[i] : 0.0;
-            if (i < p) {
-                mn = fmin(mn, x[k]);
-                mx = fmax(mx, x[k]);
-            }
-        }
-        mn = bwave_min(mn);
-        mx = bwave_max(mx);
-        const double c = 0.5 * (mn + mx);
-        const int form = f32c_form(0.5 * (mx - mn), sp.eta);
-        float qh[R], ql[R];
-        double re[R], iv[R];
-#pragma unroll
-        for (int k = 0; k < R; ++k) {
-            const double t = x[k] - c;
-            qh[k] = (float)t;
-            ql[k] = (float)(t - (double)qh[k]);
-            re[k] = iv[k] = 0.0;
-        }
-        const float eta2f = (float)(sp.eta * sp.eta);
-        for (int base = 0; base < p; base += kBPiece) {
-            const int cnt = min(kBPiece, p - base);
-            __syncthreads();
-            for (int j = lane; j < cnt; j += kBLanes) {
-                const double t = src[base + j] - c;
-                const float h = (float)t;
-                s_hi[j] = h;
-                s_lo[j] = (float)(t - (double)h);
-            }
-            __syncthreads();
-            if (form == 0)
-                f32c_sweep<R, 0>(s_hi, s_lo, cnt, qh, ql, eta2f, re, iv);
-            else if (form == 1)
-                f32c_sweep<R, 1>(s_hi, s_lo, cnt, qh, ql, eta2f, re, iv);
-            else
-                f32c_sweep<R, 2>(s_hi, s_lo, cnt, qh, ql, eta2f, re, iv);
-        }
-#pragma unroll
-        for (int k = 0; k < R; ++k) {
-            const int i = base_q + k * kBLanes + lane;
-            if (i < p) {
-                out_re[sp.off + i] = re[k] * sp.scale;
-                out_im[sp.off + i] = (sp.eta * iv[k]) * sp.scale;
-            }
-        }
-        if (lane == 0) {
-            t_start[item] = t0;
-            t_end[item] = wall_clock64();
-        }
-    }
-}
+int r_index(int r) { return r == 1 ? 0 : (r == 2 ? 1 : 2); }
 
 }  // namespace
-}  // namespace shk
-
-using namespace shk;
 
 struct shk_batch {
     const double *d_ev = nullptr;
-    std::vector<long long> off;
+    size_t n = 0;
+    std::vector<int64_t> off;
     std::vector<double> eta;
     int scaled = 1;
-    size_t n = 0;
-    // device state
-    BatchSpec *d_specs = nullptr;
-    int2 *d_items = nullptr;
-    int *d_counter = nullptr;
-    long long *d_t0 = nullptr, *d_t1 = nullptr;
-    int n_items = 0, items_cap = 0;
-    std::vector<int> item_spec;
-    int R = 2, workers_per_cu = 16, n_cu = 0;
+    std::vector<size_t> order;   // spectra by descending size (the grid order)
+    int r = 2;
+    // per tile height: device spectrum table and its tile count
+    SegSpectrum *d_table[3] = {nullptr, nullptr, nullptr};
+    int n_table = 0;
+    int n_tiles[3] = {0, 0, 0};
+    std::vector<int> tile0[3];   // host copy of each table's first tiles
+    long long *d_stamps = nullptr;  // 2 per tile of the largest tile count
     // captured graph
     hipGraphExec_t exec = nullptr;
     hipStream_t cap = nullptr;
     double *g_re = nullptr, *g_im = nullptr;
-    int g_R = 0, g_workers = 0;
+    int g_r = 0;
+    bool ran = false;
 };
 
 namespace {
@@ -177,41 +75,22 @@ shk_status bfail(shk_status s, const std::string &m) {
         }                                                                       \
     } while (0)
 
-// Item list for tile width 64*R: spectra by descending size, tiles in order.
-shk_status build_items(shk_batch *b, int R) {
-    // A launch of the previous item list may still be reading it.
-    SHK_BHIP(hipDeviceSynchronize());
-    std::vector<int> order(b->n);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
-        return (b->off[a + 1] - b->off[a]) > (b->off[c + 1] - b->off[c]);
-    });
-    std::vector<int2> items;
-    b->item_spec.clear();
-    const long long qt = (long long)kBLanes * R;
-    for (int s : order) {
-        const long long p = b->off[s + 1] - b->off[s];
-        for (long long t = 0; t * qt < p; ++t) {
-            items.push_back(make_int2(s, (int)t));
-            b->item_spec.push_back(s);
-        }
-    }
-    b->n_items = (int)items.size();
-    if (b->n_items > b->items_cap) {
-        if (b->d_items) (void)hipFree(b->d_items);
-        if (b->d_t0) (void)hipFree(b->d_t0);
-        if (b->d_t1) (void)hipFree(b->d_t1);
-        b->d_items = nullptr;
-        b->d_t0 = b->d_t1 = nullptr;
-        SHK_BHIP(hipMalloc(&b->d_items, sizeof(int2) * std::max(1, b->n_items)));
-        SHK_BHIP(hipMalloc(&b->d_t0, sizeof(long long) * std::max(1, b->n_items)));
-        SHK_BHIP(hipMalloc(&b->d_t1, sizeof(long long) * std::max(1, b->n_items)));
-        b->items_cap = b->n_items;
-    }
-    if (b->n_items)
-        SHK_BHIP(hipMemcpy(b->d_items, items.data(), sizeof(int2) * b->n_items,
-                           hipMemcpyHostToDevice));
-    b->R = R;
+// Device table for tile height r (built once per r).
+shk_status prepare(shk_batch *b, int r) {
+    const int ri = r_index(r);
+    if (b->d_table[ri]) return SHK_OK;
+    std::vector<SegSpectrum> table(b->n);
+    int n_table = 0, n_tiles = 0;
+    const int st = exact_many_plan(b->off.data(), b->order.data(), b->n, b->eta.data(), b->scaled,
+                                   table.data(), &n_table, &n_tiles, r);
+    if (st) return (shk_status)st;
+    SHK_BHIP(hipMalloc(&b->d_table[ri], sizeof(SegSpectrum) * std::max(1, n_table)));
+    SHK_BHIP(hipMemcpy(b->d_table[ri], table.data(), sizeof(SegSpectrum) * n_table,
+                       hipMemcpyHostToDevice));
+    b->n_table = n_table;
+    b->n_tiles[ri] = n_tiles;
+    b->tile0[ri].resize(n_table);
+    for (int e = 0; e < n_table; ++e) b->tile0[ri][e] = table[e].tile0;
     return SHK_OK;
 }
 
@@ -221,28 +100,16 @@ shk_status capture(shk_batch *b, double *d_re, double *d_im) {
         (void)hipGraphExecDestroy(b->exec);
         b->exec = nullptr;
     }
-    const int workers = std::max(1, b->n_cu * b->workers_per_cu);
+    const int ri = r_index(b->r);
     hipGraph_t g = nullptr;
     SHK_BHIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
-    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(1), 0, b->cap, b->d_counter);
-    if (b->R == 1)
-        hipLaunchKernelGGL(batch_f32c_kernel<1>, dim3(workers), dim3(kBLanes), 0, b->cap, b->d_ev,
-                           b->d_specs, b->d_items, b->n_items, b->d_counter, d_re, d_im, b->d_t0,
-                           b->d_t1);
-    else if (b->R == 2)
-        hipLaunchKernelGGL(batch_f32c_kernel<2>, dim3(workers), dim3(kBLanes), 0, b->cap, b->d_ev,
-                           b->d_specs, b->d_items, b->n_items, b->d_counter, d_re, d_im, b->d_t0,
-                           b->d_t1);
-    else
-        hipLaunchKernelGGL(batch_f32c_kernel<4>, dim3(workers), dim3(kBLanes), 0, b->cap, b->d_ev,
-                           b->d_specs, b->d_items, b->n_items, b->d_counter, d_re, d_im, b->d_t0,
-                           b->d_t1);
-    const hipError_t le = hipGetLastError();
+    const int st = exact_many(b->d_ev, b->d_table[ri], b->n_table, b->n_tiles[ri], 1, d_re, d_im,
+                              b->d_stamps, b->cap, b->r);
     const hipError_t ce = hipStreamEndCapture(b->cap, &g);
-    if (le != hipSuccess || ce != hipSuccess) {
+    if (st || ce != hipSuccess) {
         if (g) (void)hipGraphDestroy(g);
-        return bfail(SHK_ERR_HIP, std::string("batch graph capture: ") +
-                                      hipGetErrorString(le != hipSuccess ? le : ce));
+        if (st) return (shk_status)st;
+        return bfail(SHK_ERR_HIP, std::string("batch graph capture: ") + hipGetErrorString(ce));
     }
     const hipError_t ie = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
@@ -250,20 +117,19 @@ shk_status capture(shk_batch *b, double *d_re, double *d_im) {
         return bfail(SHK_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
     b->g_re = d_re;
     b->g_im = d_im;
-    b->g_R = b->R;
-    b->g_workers = b->workers_per_cu;
+    b->g_r = b->r;
     return SHK_OK;
 }
 
 shk_status launch(shk_batch *b, double *d_re, double *d_im, hipStream_t s) {
-    if (!b->exec || b->g_re != d_re || b->g_im != d_im || b->g_R != b->R ||
-        b->g_workers != b->workers_per_cu) {
+    if (!b->exec || b->g_re != d_re || b->g_im != d_im || b->g_r != b->r) {
         shk_status st = capture(b, d_re, d_im);
         if (st != SHK_OK) return st;
     }
     ProfileMark m = profile_begin(s);
     SHK_BHIP(hipGraphLaunch(b->exec, s));
     profile_end(m, s);
+    b->ran = true;
     return SHK_OK;
 }
 
@@ -286,40 +152,29 @@ shk_status shk_batch_create(const double *d_eigenvalues, const int64_t *offsets,
             return bfail(SHK_ERR_BAD_ETA, "eta must be a positive finite number, got " +
                                               std::to_string(etas[s]));
     }
-#ifndef SHK_ENABLE_BATCH
-    // Switched off: the first GPU run of this path (tuning pass on a 24-spectrum
-    // batch) did not finish and no cause has been shown yet.  Every call is
-    // refused after the argument checks, before any device work.
-    return bfail(SHK_ERR_UNSUPPORTED,
-                 "the batched launch is switched off in this build: its first GPU run did not "
-                 "finish and the cause has not been found; use the single-spectrum entry points");
-#endif
-    int dev = 0;
-    SHK_BHIP(hipGetDevice(&dev));
+    if (const int dv = device_ready()) return (shk_status)dv;
     shk_batch *b = new shk_batch();
     b->d_ev = d_eigenvalues;
     b->n = n_spectra;
     b->off.assign(offsets, offsets + n_spectra + 1);
     b->eta.assign(etas, etas + n_spectra);
     b->scaled = scaled ? 1 : 0;
-    hipError_t e = hipDeviceGetAttribute(&b->n_cu, hipDeviceAttributeMultiprocessorCount, dev);
-    std::vector<BatchSpec> specs(n_spectra);
-    for (size_t s = 0; s < n_spectra; ++s) {
-        const long long p = offsets[s + 1] - offsets[s];
-        specs[s] = BatchSpec{offsets[s], (int)p, 0, etas[s], scaled ? 1.0 / (double)p : 1.0};
-    }
-    if (e == hipSuccess) e = hipMalloc(&b->d_specs, sizeof(BatchSpec) * n_spectra);
-    if (e == hipSuccess)
-        e = hipMemcpy(b->d_specs, specs.data(), sizeof(BatchSpec) * n_spectra,
-                      hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMalloc(&b->d_counter, sizeof(int));
+    b->order.resize(n_spectra);
+    std::iota(b->order.begin(), b->order.end(), (size_t)0);
+    std::stable_sort(b->order.begin(), b->order.end(), [&](size_t a, size_t c) {
+        return (b->off[a + 1] - b->off[a]) > (b->off[c + 1] - b->off[c]);
+    });
+    // stamps for the largest tile count (r = 1)
+    int64_t max_tiles = 0;
+    for (size_t s = 0; s < n_spectra; ++s) max_tiles += (b->off[s + 1] - b->off[s] + 63) / 64;
+    hipError_t e = hipMalloc(&b->d_stamps, sizeof(long long) * 2 * (size_t)std::max<int64_t>(1, max_tiles));
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
     if (e != hipSuccess) {
         set_error(std::string("shk_batch_create: ") + hipGetErrorString(e));
         shk_batch_destroy(b);
         return SHK_ERR_HIP;
     }
-    shk_status st = build_items(b, 2);
+    const shk_status st = prepare(b, 2);
     if (st != SHK_OK) {
         shk_batch_destroy(b);
         return st;
@@ -334,39 +189,33 @@ shk_status shk_batch_run(shk_batch *b, double *d_out_re, double *d_out_im, void 
 }
 
 shk_status shk_batch_tune(shk_batch *b, double *d_out_re, double *d_out_im, void *stream,
-                          int *best_r, int *best_workers_per_cu, double *best_ms) {
+                          int *best_r, double *best_ms) {
     if (!b || !d_out_re || !d_out_im) return bfail(SHK_ERR_BAD_ARG, "NULL pointer");
     hipStream_t s = (hipStream_t)stream;
-    const int Rs[3] = {1, 2, 4};
-    const int Ws[3] = {8, 16, 32};
     double best = 1e300;
-    int bR = 2, bW = 16;
-    hipEvent_t e0, e1;
+    int bR = 2;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     SHK_BHIP(hipEventCreate(&e0));
     SHK_BHIP(hipEventCreate(&e1));
     shk_status st = SHK_OK;
-    for (int R : Rs) {
-        if ((st = build_items(b, R)) != SHK_OK) break;
-        for (int W : Ws) {
-            b->workers_per_cu = W;
-            if ((st = launch(b, d_out_re, d_out_im, s)) != SHK_OK) break;  // warm-up / capture
-            float ms = 0.f, tot = 1e30f;
-            for (int rep = 0; rep < 3 && st == SHK_OK; ++rep) {
-                if (hipEventRecord(e0, s) != hipSuccess) st = SHK_ERR_HIP;
-                if (st == SHK_OK) st = launch(b, d_out_re, d_out_im, s);
-                if (st == SHK_OK && hipEventRecord(e1, s) != hipSuccess) st = SHK_ERR_HIP;
-                if (st == SHK_OK && hipEventSynchronize(e1) != hipSuccess) st = SHK_ERR_HIP;
-                if (st == SHK_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) st = SHK_ERR_HIP;
-                tot = std::min(tot, ms);
-            }
-            if (st != SHK_OK) break;
-            if (tot < best) {
-                best = tot;
-                bR = R;
-                bW = W;
-            }
+    for (int R : kRs) {
+        if ((st = prepare(b, R)) != SHK_OK) break;
+        b->r = R;
+        if ((st = launch(b, d_out_re, d_out_im, s)) != SHK_OK) break;  // capture + warm-up
+        float ms = 0.f, tot = 1e30f;
+        for (int rep = 0; rep < 3 && st == SHK_OK; ++rep) {
+            if (hipEventRecord(e0, s) != hipSuccess) st = SHK_ERR_HIP;
+            if (st == SHK_OK) st = launch(b, d_out_re, d_out_im, s);
+            if (st == SHK_OK && hipEventRecord(e1, s) != hipSuccess) st = SHK_ERR_HIP;
+            if (st == SHK_OK && hipEventSynchronize(e1) != hipSuccess) st = SHK_ERR_HIP;
+            if (st == SHK_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) st = SHK_ERR_HIP;
+            tot = std::min(tot, ms);
         }
         if (st != SHK_OK) break;
+        if (tot < best) {
+            best = tot;
+            bR = R;
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -374,58 +223,59 @@ shk_status shk_batch_tune(shk_batch *b, double *d_out_re, double *d_out_im, void
         if (std::string(last_error()).empty()) set_error("batch tuning failed");
         return st;
     }
-    if ((st = build_items(b, bR)) != SHK_OK) return st;
-    b->workers_per_cu = bW;
+    b->r = bR;
     if (best_r) *best_r = bR;
-    if (best_workers_per_cu) *best_workers_per_cu = bW;
     if (best_ms) *best_ms = best;
     return SHK_OK;
 }
 
-shk_status shk_batch_set_config(shk_batch *b, int r, int workers_per_cu) {
+shk_status shk_batch_set_config(shk_batch *b, int r) {
     if (!b) return bfail(SHK_ERR_BAD_ARG, "NULL handle");
     if (r != 1 && r != 2 && r != 4) return bfail(SHK_ERR_BAD_ARG, "r must be 1, 2 or 4");
-    if (workers_per_cu < 1 || workers_per_cu > 64)
-        return bfail(SHK_ERR_BAD_ARG, "workers_per_cu must be in [1, 64]");
-    if (r != b->R) {
-        shk_status st = build_items(b, r);
-        if (st != SHK_OK) return st;
-    }
-    b->workers_per_cu = workers_per_cu;
+    const shk_status st = prepare(b, r);
+    if (st != SHK_OK) return st;
+    b->r = r;
     return SHK_OK;
 }
 
 shk_status shk_batch_spectrum_times(const shk_batch *b, double *start_us, double *end_us) {
     if (!b || !start_us || !end_us) return bfail(SHK_ERR_BAD_ARG, "NULL pointer");
-    std::vector<long long> t0(b->n_items), t1(b->n_items);
-    SHK_BHIP(hipMemcpy(t0.data(), b->d_t0, sizeof(long long) * b->n_items, hipMemcpyDeviceToHost));
-    SHK_BHIP(hipMemcpy(t1.data(), b->d_t1, sizeof(long long) * b->n_items, hipMemcpyDeviceToHost));
+    if (!b->ran || !b->exec) return bfail(SHK_ERR_BAD_ARG, "the batch has not run yet");
+    const int ri = r_index(b->g_r);
+    const int nt = b->n_tiles[ri];
+    std::vector<long long> t(2 * (size_t)std::max(1, nt));
+    SHK_BHIP(hipMemcpy(t.data(), b->d_stamps, sizeof(long long) * 2 * (size_t)nt,
+                       hipMemcpyDeviceToHost));
     int dev = 0, khz = 0;
     SHK_BHIP(hipGetDevice(&dev));
     SHK_BHIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
     if (khz <= 0) return bfail(SHK_ERR_HIP, "device wall-clock rate unavailable");
     const double us_per_tick = 1e3 / (double)khz;
-    long long origin = t0.empty() ? 0 : *std::min_element(t0.begin(), t0.end());
-    std::vector<long long> s0(b->n, -1), s1(b->n, -1);
-    for (int k = 0; k < b->n_items; ++k) {
-        const int s = b->item_spec[k];
-        if (s0[s] < 0 || t0[k] < s0[s]) s0[s] = t0[k];
-        if (s1[s] < 0 || t1[k] > s1[s]) s1[s] = t1[k];
-    }
-    for (size_t s = 0; s < b->n; ++s) {
-        start_us[s] = (double)(s0[s] - origin) * us_per_tick;
-        end_us[s] = (double)(s1[s] - origin) * us_per_tick;
+    long long origin = t[0];
+    for (int k = 0; k < nt; ++k) origin = std::min(origin, t[2 * (size_t)k]);
+    const std::vector<int> &t0 = b->tile0[ri];
+    for (int e = 0; e < b->n_table; ++e) {
+        const int a = t0[e], z = (e + 1 < b->n_table) ? t0[e + 1] : nt;
+        long long lo = t[2 * (size_t)a], hi = t[2 * (size_t)a + 1];
+        for (int k = a; k < z; ++k) {
+            lo = std::min(lo, t[2 * (size_t)k]);
+            hi = std::max(hi, t[2 * (size_t)k + 1]);
+        }
+        const size_t s = b->order[e];  // every spectrum is non-empty: entry e is order[e]
+        start_us[s] = (double)(lo - origin) * us_per_tick;
+        end_us[s] = (double)(hi - origin) * us_per_tick;
     }
     return SHK_OK;
 }
 
 shk_status shk_batch_destroy(shk_batch *b) {
     if (!b) return SHK_OK;
+    if (b->exec || b->ran) (void)hipDeviceSynchronize();  // a replay may still read the table
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->cap) (void)hipStreamDestroy(b->cap);
-    void *ptrs[] = {b->d_specs, b->d_items, b->d_counter, b->d_t0, b->d_t1};
-    for (void *p : ptrs)
-        if (p) (void)hipFree(p);
+    for (SegSpectrum *t : b->d_table)
+        if (t) (void)hipFree(t);
+    if (b->d_stamps) (void)hipFree(b->d_stamps);
     delete b;
     return SHK_OK;
 }

@@ -33,6 +33,10 @@ const char *last_error() { return g_err.c_str(); }
 ProfileMark profile_begin(hipStream_t stream) {
     ProfileMark m;
     if (!g_prof_on.load(std::memory_order_relaxed)) return m;
+    // no marks inside a graph capture (the graph's launch is marked instead)
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+        return m;
     if (hipEventCreate(&m.a) != hipSuccess || hipEventCreate(&m.b) != hipSuccess) {
         if (m.a) (void)hipEventDestroy(m.a);
         return ProfileMark{};
@@ -325,6 +329,10 @@ int run_atpoints(const double *d_q, size_t nq, const double *d_ev, size_t p, dou
 }
 
 }  // namespace
+
+namespace shk {
+int device_ready() { return ensure_device(); }
+}  // namespace shk
 
 // ---- per-thread device buffer cache ----------------------------------------
 

@@ -174,20 +174,22 @@ __global__ __launch_bounds__(kWG) void exact_f64_kernel(
 
 // ---- FP32-compensated pair kernel ------------------------------------------
 
-// Loads the kR queries each lane owns in query tile `tile` of q[0, nq), finds
-// the tile centre c (midpoint of its query range, FP64) and the pair form,
-// and splits x - c into FP32 hi/lo parts.  Shared by the single-spectrum and
-// the segmented kernels so that both see the same tile arithmetic.
+// Loads the R queries each lane owns in query tile `tile` (kWG*R queries)
+// of q[0, nq), finds the tile centre c (midpoint of its query range, FP64)
+// and the pair form, and splits x - c into FP32 hi/lo parts.  Shared by the
+// single-spectrum and the segmented kernels so that both see the same tile
+// arithmetic.
+template <int R>
 __device__ __forceinline__ int f32c_tile_setup(const double *__restrict__ q, int nq, int tile,
                                                double eta, double (*s_red)[kWG / 64],
-                                               double *c_out, float (&qh)[kR],
-                                               float (&ql)[kR]) {
+                                               double *c_out, float (&qh)[R],
+                                               float (&ql)[R]) {
     const int tid = threadIdx.x;
-    double x[kR];
+    double x[R];
     double mn = __builtin_inf(), mx = -__builtin_inf();
 #pragma unroll
-    for (int k = 0; k < kR; ++k) {
-        const int i = tile * kQT + k * kWG + tid;
+    for (int k = 0; k < R; ++k) {
+        const int i = tile * (kWG * R) + k * kWG + tid;
         x[k] = (i < nq) ? q[i] : 0.0;
         if (i < nq) {
             mn = fmin(mn, x[k]);
@@ -210,7 +212,7 @@ __device__ __forceinline__ int f32c_tile_setup(const double *__restrict__ q, int
     }
     const double c = 0.5 * (mn + mx);
 #pragma unroll
-    for (int k = 0; k < kR; ++k) {
+    for (int k = 0; k < R; ++k) {
         const double t = x[k] - c;
         qh[k] = (float)t;
         ql[k] = (float)(t - (double)qh[k]);
@@ -230,16 +232,17 @@ __device__ __forceinline__ void f32c_stage(const double *__restrict__ src, int c
     }
 }
 
+template <int R>
 __device__ __forceinline__ void f32c_sweep_any(int form, const float *s_hi, const float *s_lo,
-                                               int cnt, const float (&qh)[kR],
-                                               const float (&ql)[kR], float eta2f,
-                                               double (&re)[kR], double (&iv)[kR]) {
+                                               int cnt, const float (&qh)[R],
+                                               const float (&ql)[R], float eta2f,
+                                               double (&re)[R], double (&iv)[R]) {
     if (form == 0)
-        f32c_sweep<kR, 0>(s_hi, s_lo, cnt, qh, ql, eta2f, re, iv);
+        f32c_sweep<R, 0>(s_hi, s_lo, cnt, qh, ql, eta2f, re, iv);
     else if (form == 1)
-        f32c_sweep<kR, 1>(s_hi, s_lo, cnt, qh, ql, eta2f, re, iv);
+        f32c_sweep<R, 1>(s_hi, s_lo, cnt, qh, ql, eta2f, re, iv);
     else
-        f32c_sweep<kR, 2>(s_hi, s_lo, cnt, qh, ql, eta2f, re, iv);
+        f32c_sweep<R, 2>(s_hi, s_lo, cnt, qh, ql, eta2f, re, iv);
 }
 
 __global__ __launch_bounds__(kWG) void exact_f32c_kernel(const double *__restrict__ q, int nq,
@@ -257,7 +260,7 @@ __global__ __launch_bounds__(kWG) void exact_f32c_kernel(const double *__restric
 
     double c;
     float qh[kR], ql[kR];
-    const int form = f32c_tile_setup(q, nq, blockIdx.x, eta, s_red, &c, qh, ql);
+    const int form = f32c_tile_setup<kR>(q, nq, blockIdx.x, eta, s_red, &c, qh, ql);
     double re[kR], iv[kR];
 #pragma unroll
     for (int k = 0; k < kR; ++k) re[k] = iv[k] = 0.0;
@@ -268,7 +271,7 @@ __global__ __launch_bounds__(kWG) void exact_f32c_kernel(const double *__restric
         __syncthreads();
         f32c_stage(src + base, cnt, c, s_hi, s_lo);
         __syncthreads();
-        f32c_sweep_any(form, s_hi, s_lo, cnt, qh, ql, eta2f, re, iv);
+        f32c_sweep_any<kR>(form, s_hi, s_lo, cnt, qh, ql, eta2f, re, iv);
     }
 
 #pragma unroll
@@ -368,11 +371,11 @@ __device__ __forceinline__ void seg_stamp(long long *stamps, int which) {
 
 // Sources [base, base + cnt) are in LDS; sweeps them in sub-ranges cut at the
 // chunk ends, folding the chunk accumulator into the total at each end.
-template <typename Sweep>
+template <int R, typename Sweep>
 __device__ __forceinline__ void seg_sweep_piece(int base, int cnt, int per, int p,
-                                                int &chunk_end, double (&re)[kR],
-                                                double (&iv)[kR], double (&tr)[kR],
-                                                double (&ti)[kR], Sweep sweep) {
+                                                int &chunk_end, double (&re)[R],
+                                                double (&iv)[R], double (&tr)[R],
+                                                double (&ti)[R], Sweep sweep) {
     int j = 0;
     while (j < cnt) {
         const int stop = min(cnt, chunk_end - base);
@@ -380,7 +383,7 @@ __device__ __forceinline__ void seg_sweep_piece(int base, int cnt, int per, int 
         j = stop;
         if (base + j == chunk_end) {
 #pragma unroll
-            for (int k = 0; k < kR; ++k) {
+            for (int k = 0; k < R; ++k) {
                 tr[k] += re[k];
                 ti[k] += iv[k];
                 re[k] = iv[k] = 0.0;
@@ -390,7 +393,9 @@ __device__ __forceinline__ void seg_sweep_piece(int base, int cnt, int per, int 
     }
 }
 
-template <bool STAMP>
+// R queries per lane (tiles of kWG*R); R = kR gives the single-spectrum
+// tiles and hence the same bits as exact_f32c.
+template <int R, bool STAMP>
 __global__ __launch_bounds__(kWG) void many_f32c_kernel(const double *__restrict__ ev,
                                                         const SegSpectrum *__restrict__ specs,
                                                         int n_specs, double *__restrict__ out_re,
@@ -406,11 +411,11 @@ __global__ __launch_bounds__(kWG) void many_f32c_kernel(const double *__restrict
     const int tile = (int)blockIdx.x - sp.tile0;
 
     double c;
-    float qh[kR], ql[kR];
-    const int form = f32c_tile_setup(src, p, tile, sp.eta, s_red, &c, qh, ql);
-    double re[kR], iv[kR], tr[kR], ti[kR];
+    float qh[R], ql[R];
+    const int form = f32c_tile_setup<R>(src, p, tile, sp.eta, s_red, &c, qh, ql);
+    double re[R], iv[R], tr[R], ti[R];
 #pragma unroll
-    for (int k = 0; k < kR; ++k) re[k] = iv[k] = tr[k] = ti[k] = 0.0;
+    for (int k = 0; k < R; ++k) re[k] = iv[k] = tr[k] = ti[k] = 0.0;
     const float eta2f = (float)(sp.eta * sp.eta);
 
     int chunk_end = min(sp.per, p);
@@ -419,13 +424,13 @@ __global__ __launch_bounds__(kWG) void many_f32c_kernel(const double *__restrict
         __syncthreads();
         f32c_stage(src + base, cnt, c, s_hi, s_lo);
         __syncthreads();
-        seg_sweep_piece(base, cnt, sp.per, p, chunk_end, re, iv, tr, ti, [&](int j0, int n) {
-            f32c_sweep_any(form, s_hi + j0, s_lo + j0, n, qh, ql, eta2f, re, iv);
+        seg_sweep_piece<R>(base, cnt, sp.per, p, chunk_end, re, iv, tr, ti, [&](int j0, int n) {
+            f32c_sweep_any<R>(form, s_hi + j0, s_lo + j0, n, qh, ql, eta2f, re, iv);
         });
     }
 #pragma unroll
-    for (int k = 0; k < kR; ++k) {
-        const int i = tile * kQT + k * kWG + (int)threadIdx.x;
+    for (int k = 0; k < R; ++k) {
+        const int i = tile * (kWG * R) + k * kWG + (int)threadIdx.x;
         if (i < p) {
             out_re[sp.off + i] = tr[k] * sp.scale;
             out_im[sp.off + i] = (sp.eta * ti[k]) * sp.scale;
@@ -462,7 +467,7 @@ __global__ __launch_bounds__(kWG) void many_f64_kernel(const double *__restrict_
         __syncthreads();
         for (int j = tid; j < cnt; j += kWG) s_src[j] = src[base + j];
         __syncthreads();
-        seg_sweep_piece(base, cnt, sp.per, p, chunk_end, re, iv, tr, ti, [&](int j0, int n) {
+        seg_sweep_piece<kR>(base, cnt, sp.per, p, chunk_end, re, iv, tr, ti, [&](int j0, int n) {
             // two sources per step, as in exact_f64_kernel (same bits)
             int j = j0;
             for (; j + 2 <= j0 + n; j += 2) {
@@ -652,7 +657,12 @@ int exact_f32c(const double *q, size_t nq, const double *src, size_t p, double e
 }
 
 int exact_many_plan(const int64_t *offsets, const size_t *order, size_t n, const double *etas,
-                    int scaled, SegSpectrum *table, int *n_table, int *n_tiles) {
+                    int scaled, SegSpectrum *table, int *n_table, int *n_tiles, int r) {
+    if (r != 1 && r != 2 && r != 4) {
+        set_error("segmented launch: queries per lane must be 1, 2 or 4");
+        return 6;
+    }
+    const int64_t tile_q = (int64_t)kWG * r;
     int64_t tiles = 0;
     int m = 0;
     for (size_t idx = 0; idx < n; ++idx) {
@@ -673,7 +683,7 @@ int exact_many_plan(const int64_t *offsets, const size_t *order, size_t n, const
         e.pad = 0;
         e.eta = etas[k];
         e.scale = scaled ? 1.0 / (double)p : 1.0;
-        tiles += qt;
+        tiles += (p + tile_q - 1) / tile_q;
         if (tiles > (int64_t)INT32_MAX) {
             set_error("segmented launch: more than 2^31-1 query tiles");
             return 6;
@@ -684,18 +694,42 @@ int exact_many_plan(const int64_t *offsets, const size_t *order, size_t n, const
     return 0;
 }
 
+namespace {
+
+template <int R>
+void launch_many_f32c(const dim3 &grid, const double *ev, const SegSpectrum *d_table,
+                      int n_table, double *out_re, double *out_im, long long *d_stamps,
+                      hipStream_t stream) {
+    if (d_stamps)
+        hipLaunchKernelGGL((many_f32c_kernel<R, true>), grid, dim3(kWG), 0, stream, ev, d_table,
+                           n_table, out_re, out_im, d_stamps);
+    else
+        hipLaunchKernelGGL((many_f32c_kernel<R, false>), grid, dim3(kWG), 0, stream, ev, d_table,
+                           n_table, out_re, out_im, nullptr);
+}
+
+}  // namespace
+
 int exact_many(const double *ev, const SegSpectrum *d_table, int n_table, int n_tiles,
                int f32c, double *out_re, double *out_im, long long *d_stamps,
-               hipStream_t stream) {
+               hipStream_t stream, int r) {
     if (n_table == 0 || n_tiles == 0) return 0;
+    if (r != 1 && r != 2 && r != 4) {
+        set_error("segmented launch: queries per lane must be 1, 2 or 4");
+        return 6;
+    }
+    if (!f32c && r != kR) {
+        set_error("segmented launch: the FP64 kernel takes 2 queries per lane");
+        return 6;
+    }
     ProfileMark m = profile_begin(stream);
     const dim3 grid(n_tiles), block(kWG);
-    if (f32c && d_stamps)
-        hipLaunchKernelGGL(many_f32c_kernel<true>, grid, block, 0, stream, ev, d_table, n_table,
-                           out_re, out_im, d_stamps);
+    if (f32c && r == 1)
+        launch_many_f32c<1>(grid, ev, d_table, n_table, out_re, out_im, d_stamps, stream);
+    else if (f32c && r == 2)
+        launch_many_f32c<2>(grid, ev, d_table, n_table, out_re, out_im, d_stamps, stream);
     else if (f32c)
-        hipLaunchKernelGGL(many_f32c_kernel<false>, grid, block, 0, stream, ev, d_table, n_table,
-                           out_re, out_im, nullptr);
+        launch_many_f32c<4>(grid, ev, d_table, n_table, out_re, out_im, d_stamps, stream);
     else if (d_stamps)
         hipLaunchKernelGGL(many_f64_kernel<true>, grid, block, 0, stream, ev, d_table, n_table,
                            out_re, out_im, d_stamps);

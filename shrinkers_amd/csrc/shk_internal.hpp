@@ -15,6 +15,9 @@ namespace shk {
 void set_error(const std::string &msg);
 const char *last_error();
 
+// Checks once that a gfx950 device is visible (SHK_ERR_NO_DEVICE otherwise).
+int device_ready();
+
 // ---- per-thread device buffer cache ---------------------------------------
 // One buffer per slot, per calling thread and device, grown on demand (and
 // never shrunk until cache_free_all).  Each slot records an event at its
@@ -106,16 +109,20 @@ struct SegSpectrum {
 };
 
 // Fills table[0, *n_table) for the non-empty spectra in `order` (offsets[k],
-// offsets[k+1] bound spectrum k) and counts their query tiles.
+// offsets[k+1] bound spectrum k) and counts their query tiles of 64*r
+// queries (r = 1, 2 or 4 queries per lane).
 int exact_many_plan(const int64_t *offsets, const size_t *order, size_t n, const double *etas,
-                    int scaled, SegSpectrum *table, int *n_table, int *n_tiles);
+                    int scaled, SegSpectrum *table, int *n_table, int *n_tiles, int r = 2);
 // All-points sums (kind EXACT_PLAIN) of every spectrum in the device table in
-// one launch, written at each spectrum's offset in out_re/out_im; bit-equal to
-// exact_f32c / exact_f64 on each spectrum alone.  d_stamps, if not NULL,
-// receives 2 * n_tiles device wall-clock ticks: each workgroup's start and end.
+// one launch, written at each spectrum's offset in out_re/out_im.  With r = 2
+// (the single-spectrum tiles) each result is bit-equal to exact_f32c /
+// exact_f64 on that spectrum alone; FP32-compensated r = 1 or 4 use other
+// tiles (their own centres and pair forms).  FP64 takes r = 2 only.
+// d_stamps, if not NULL, receives 2 * n_tiles device wall-clock ticks: each
+// workgroup's start and end.  The table must match r.
 int exact_many(const double *ev, const SegSpectrum *d_table, int n_table, int n_tiles,
                int f32c, double *out_re, double *out_im, long long *d_stamps,
-               hipStream_t stream);
+               hipStream_t stream, int r = 2);
 
 // float -> double widening on device (for the f32 entry points).
 int widen_f32(const float *in, double *out, size_t n, hipStream_t stream);

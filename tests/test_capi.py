@@ -113,11 +113,9 @@ def test_batch_argument_checks(shk):
                                 _lib.dptr(eta), 1, ctypes.byref(h)) == _lib.SHK_ERR_EMPTY
     neg = np.array([0.1, -1.0])
     assert lib.shk_batch_create(dummy, offp, 2, _lib.dptr(neg), 1, ctypes.byref(h)) == _lib.SHK_ERR_BAD_ETA
-    assert lib.shk_batch_set_config(None, 2, 16) == _lib.SHK_ERR_BAD_ARG
-    # valid arguments: the batched launch is switched off and says so
-    assert lib.shk_batch_create(dummy, offp, 2, _lib.dptr(eta), 1, ctypes.byref(h)) == \
-        _lib.SHK_ERR_UNSUPPORTED
-    assert "switched off" in _lib.last_error() and not h.value
+    assert lib.shk_batch_set_config(None, 2) == _lib.SHK_ERR_BAD_ARG
+    assert lib.shk_batch_run(None, dummy, dummy, None) == _lib.SHK_ERR_BAD_ARG
+    assert lib.shk_batch_destroy(None) == _lib.SHK_OK
 
 
 def test_many_argument_checks(shk):
