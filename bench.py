@@ -31,11 +31,16 @@ METRIC = "Stieltjes pair-evals/sec at p=50k: FP32-comp (≤1e-6) vs FP64 exact, 
 FLOPS_PER_PAIR = 7          # sub, fma(2), rcp, fma(2), add
 PEAK_FP32_TFLOPS = 157.3    # MI355X FP32 vector (published)
 PEAK_FP64_TFLOPS = 78.6     # MI355X FP64 vector (published)
-# HBM bytes per exact_f32c_kernel launch at p = 50 000 with the library's default tiles,
+# The all-points FP32-compensated entry runs the symmetric kernel (each unordered pair
+# once) for SYM_MIN_P <= p <= SYM_MAX_P and the split-tile kernel otherwise (the
+# library's rule, shk_internal.hpp).
+SYM_MIN_P, SYM_MAX_P = 24576, 262144
+# HBM bytes per launch of the kernel the headline runs at p = 50 000 (sym_f32c_kernel),
 # from one rocprofv3 pass per counter (FETCH_SIZE, WRITE_SIZE) scaled by the readings for a
-# 256 MiB device copy (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000): 1607.5 KiB fetched (the
-# eigenvalues once per XCD L2) and 32812.5 KiB written (the FP64 split partial sums).
-TRAFFIC_F32C_P50000 = (1607.5 * 2.000 + 32812.5 * 1.000) * 1024
+# 256 MiB device copy (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000): 1681.0 KiB fetched (the
+# eigenvalues, once per XCD L2) and 76562.5 KiB written (the FP64 per-tile partials,
+# 2 x 98 tiles x 50 000 x 8 B).
+TRAFFIC_P50000 = (1681.0 * 2.000 + 76562.5 * 1.000) * 1024
 
 
 def log(msg: str) -> None:
@@ -532,9 +537,9 @@ def main() -> int:
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": TRAFFIC_F32C_P50000 if p == 50_000 else None,
+            "traffic": TRAFFIC_P50000 if p == 50_000 else None,
             "traffic_unit": "bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
-            "kernel": "exact_f32c_kernel",
+            "kernel": "sym_f32c_kernel" if SYM_MIN_P <= p <= SYM_MAX_P else "exact_f32c_kernel",
             "kernel_avg_ms": k32,
             "flops_per_launch": FLOPS_PER_PAIR * pairs,
             "peak_box": box_peaks(),

@@ -558,6 +558,12 @@ int launch_reduce(const double *part, int n_split, size_t nq, double eta, double
 
 }  // namespace
 
+int exact_reduce_plain(const double *part, int n_split, size_t nq, double eta, double scale,
+                       const double *q, double *out_re, double *out_im, hipStream_t stream) {
+    return launch_reduce<EXACT_PLAIN, double>(part, n_split, nq, eta, scale, q, out_re, out_im,
+                                              nullptr, nullptr, stream);
+}
+
 size_t exact_workspace_doubles(size_t nq, size_t p, ExactKind kind) {
     if (nq == 0 || p == 0) return 0;
     int qt, ns, per;

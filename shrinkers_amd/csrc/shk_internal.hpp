@@ -96,6 +96,24 @@ int exact_f32c(const double *q, size_t nq, const double *src, size_t p, double e
 
 // Scratch doubles the exact kernels need for nq queries against p sources.
 size_t exact_workspace_doubles(size_t nq, size_t p, ExactKind kind);
+// In-order sum of n_split FP64 partials laid out [re/im][split][query], times
+// scale, with eta applied to the imaginary part (kind EXACT_PLAIN).
+int exact_reduce_plain(const double *part, int n_split, size_t nq, double eta, double scale,
+                       const double *q, double *out_re, double *out_im, hipStream_t stream);
+
+// Symmetric FP32-compensated all-points kernel (each unordered pair once,
+// credited to both ends); partials of 16 * nt * p bytes, nt = ceil(p/512).
+// Used by the all-points entry for kSymMinP <= p <= kSymMaxP.
+// Below about 2.4e4 the split-tile kernel is faster (measured on MI355X:
+// equal at p = 2e4, split ahead by 20 % at 1.6e4, symmetric ahead by 9 % at
+// 3e4 and 23 % at 5e4).
+#ifndef SHK_SYM_MIN_P
+#define SHK_SYM_MIN_P 24576
+#endif
+constexpr size_t kSymMinP = SHK_SYM_MIN_P;
+constexpr size_t kSymMaxP = 262144;
+int exact_sym_f32c(const double *ev, size_t p, double eta, double scale, double *out_re,
+                   double *out_im, hipStream_t stream);
 
 // One spectrum of a segmented all-points launch (device table entry).
 struct SegSpectrum {

@@ -248,3 +248,49 @@ def test_f64_paired_reciprocal_fallback(shk, orc):
     r, i = shk.compute_all_stieltjes(dup, eta_t, "blocked")
     rr, ri = orc.tiled_parallel(dup, eta_t)
     assert_close(r, i, rr / dup.size, ri / dup.size, 1e-13, "underflowing products")
+
+
+@pytest.mark.parametrize("p", [24576, 24577, 25087, 50000, 65536 + 511])
+def test_symmetric_f32c_kernel(shk, orc, p):
+    """The all-points FP32-compensated entry takes the symmetric kernel for
+    24 576 <= p <= 262 144 (each unordered pair once, credited to both
+    ends; tiles of 512).  Against the FP64 oracle within 1e-6 normwise, and
+    per element within 1e-6 of max|S|; against the split-tile kernel (the
+    spectrum as its own query points) within 1e-7.  Sizes: whole tiles, one
+    element over, a partial last tile, the headline size, a partial tile
+    past 65 536."""
+    import torch
+    from shrinkers_amd import _lib
+    from shrinkers_amd import spectra as S
+    ev = S.mp_inverse_cdf_spectrum(p, c=0.25)
+    eta = 1.0 / np.sqrt(p)
+    d = torch.from_numpy(ev).cuda()
+    re, im = shk.all_points_dev(d, eta, "blocked", precision="f32c", scaled=False)
+    sr, si = torch.empty_like(d), torch.empty_like(d)
+    assert _lib.load().shk_exact_at_points_dev(d.data_ptr(), p, d.data_ptr(), p, eta, 1,
+                                               sr.data_ptr(), si.data_ptr(), None) == _lib.SHK_OK
+    torch.cuda.synchronize()
+    re, im, sr, si = (t.cpu().numpy() for t in (re, im, sr, si))
+    rr, ri = orc.tiled_parallel(ev, eta)
+    assert_close(re, im, rr, ri, 1e-6, f"symmetric f32c p={p}")
+    assert rel_l2(re, im, sr, si) <= 1e-7
+
+
+def test_symmetric_f32c_hard_inputs(shk, orc):
+    """Unsorted input (wide per-lane query spans: the compensated forms),
+    heavy duplicates with a small eta, and a log-spaced spectrum, all above
+    the symmetric kernel's threshold."""
+    from shrinkers_amd import spectra as S
+    rng = np.random.default_rng(5)
+    cases = {
+        "unsorted": (orc.shuffled(S.mp_inverse_cdf_spectrum(30000), 3), 1.0 / np.sqrt(30000)),
+        "duplicates": (np.sort(np.repeat(rng.uniform(0.5, 2.0, 3200), 8)), 1e-4),
+        "log": (S.log_spectrum(30000), 1.0 / np.sqrt(30000)),
+    }
+    for name, (ev, eta) in cases.items():
+        r, i = shk.compute_all_stieltjes(ev, eta, "blocked", precision="f32c")
+        rr, ri = orc.tiled_parallel(ev, eta)
+        n = ev.size
+        err = rel_l2(r, i, rr / n, ri / n)
+        print(f"symmetric f32c {name}: rel-L2 {err:.3e}", flush=True)
+        assert err <= 1e-6, name

@@ -95,20 +95,28 @@ def test_segmented_large_spectra(shk, precision):
     """Spectra whose single-spectrum split length exceeds the LDS piece
     (p = 40 000: 768 sources per split; p = 100 003: 4 768) or divides it
     unevenly (p = 20 000: 192), behind a tiny one: the segmented launch
-    still equals the single-spectrum call bit for bit."""
+    equals the split-tile single-spectrum kernel bit for bit (raw sums
+    through shk_exact_at_points_dev with the spectrum as its own query
+    points; the all-points entry takes the symmetric kernel at these
+    sizes)."""
     import torch
+    from shrinkers_amd import _lib
     from shrinkers_amd import spectra as S
     from shrinkers_amd.batch import SpectrumBatch
+    from shrinkers_amd.stieltjes import _PRECISIONS
     sizes = [20000, 3, 100003, 40000]
     spectra = [S.mp_inverse_cdf_spectrum(p, c=0.3, seed=p, spikes=(4.0,)) if p > 8
                else np.array([0.5, 1.0, 1.5]) for p in sizes]
     etas = [1.0 / np.sqrt(p) for p in sizes]
     b = SpectrumBatch(spectra, etas)
-    b.run("blocked", precision, True, 0)
+    b.run("blocked", precision, False, 0)
     out = b.results()
     for k, (ev, eta) in enumerate(zip(spectra, etas)):
-        sr, si = shk.all_points_dev(torch.from_numpy(ev).cuda(), eta, "blocked",
-                                    precision=precision, scaled=True)
+        d = torch.from_numpy(ev).cuda()
+        sr, si = torch.empty_like(d), torch.empty_like(d)
+        assert _lib.load().shk_exact_at_points_dev(
+            d.data_ptr(), d.numel(), d.data_ptr(), d.numel(), eta, _PRECISIONS[precision],
+            sr.data_ptr(), si.data_ptr(), None) == _lib.SHK_OK
         torch.cuda.synchronize()
         np.testing.assert_array_equal(out[k][0], sr.cpu().numpy(), err_msg=f"p={ev.size}")
         np.testing.assert_array_equal(out[k][1], si.cpu().numpy(), err_msg=f"p={ev.size}")
