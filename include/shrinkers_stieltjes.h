@@ -182,9 +182,13 @@ shk_status shk_all_points_dev(const double *d_eigenvalues, size_t p, double eta,
  * which batches only across eta on one spectrum, chebcode.rs:1312).
  * d_eigenvalues holds the spectra back to back; offsets (host, n_spectra+1
  * entries, offsets[0] = 0) delimit them; etas (host) gives one eta each.
- * Each spectrum's result is exactly shk_all_points_dev's for that spectrum
- * and method/precision (scaled by 1/p_s when scaled != 0), written at the
- * same offsets of d_out_re / d_out_im.
+ * Each spectrum's result is its all-points transform for the method and
+ * precision (scaled by 1/p_s when scaled != 0), written at the same offsets
+ * of d_out_re / d_out_im.  For the exact family it is bit for bit the
+ * split-tile single-spectrum kernel's: shk_all_points_dev's, except that
+ * shk_all_points_dev takes the symmetric kernel for FP32-compensated
+ * spectra of 24 576 to 262 144 eigenvalues (same as
+ * shk_exact_at_points_dev with the spectrum as its own query points, raw).
  * n_streams = 0 (recommended): for the exact family, ONE segmented launch on
  * `stream` covers every spectrum (one workgroup per spectrum and 128-query
  * tile, largest spectra first; no partial-sum scratch); for ChebCode, as
@@ -290,7 +294,8 @@ shk_status shk_chebcode_count_work(const shk_chebcode *h, const double *d_query,
  * captures the launch in a hipGraph on first use (again when the outputs or
  * the tile height change).  Every run stamps each workgroup's device wall
  * clock for shk_batch_spectrum_times.  With the default tile height r = 2
- * every result equals shk_all_points_dev's (FP32-compensated) bit for bit;
+ * every result equals the split-tile single-spectrum kernel's bit for bit
+ * (shk_all_points_dev's outside its symmetric range, see above);
  * r = 1 or 4 (64- or 256-query tiles) agree within the FP32-compensated
  * bound.  A batch is used from one thread at a time; its runs share the
  * stamp buffer, so a run's times are read before the next run. */
