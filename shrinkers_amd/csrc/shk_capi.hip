@@ -308,9 +308,11 @@ int run_allpoints(const double *d_ev, size_t p, double eta, int method, double c
     const ExactKind kind = allpoints_kind(method, cutoff_ratio);
     if (precision == SHK_PREC_F32C && kind == EXACT_PLAIN) {
         if (p >= kSymMinP && p <= kSymMaxP)
-            return exact_sym_f32c(d_ev, p, eta, scale, d_re, d_im, s);
+            return exact_sym(d_ev, p, eta, scale, 0, d_re, d_im, s);
         return exact_f32c(d_ev, p, d_ev, p, eta, scale, d_re, d_im, 0, s);
     }
+    if (kind == EXACT_PLAIN && p >= kSym64MinP && p <= kSymMaxP)
+        return exact_sym(d_ev, p, eta, scale, 1, d_re, d_im, s);
     double *outs[2] = {d_re, d_im};
     return exact_f64(d_ev, p, d_ev, p, eta, kind, cutoff_ratio * eta, scale, outs, s);
 }

@@ -101,9 +101,10 @@ size_t exact_workspace_doubles(size_t nq, size_t p, ExactKind kind);
 int exact_reduce_plain(const double *part, int n_split, size_t nq, double eta, double scale,
                        const double *q, double *out_re, double *out_im, hipStream_t stream);
 
-// Symmetric FP32-compensated all-points kernel (each unordered pair once,
-// credited to both ends); partials of 16 * nt * p bytes, nt = ceil(p/512).
-// Used by the all-points entry for kSymMinP <= p <= kSymMaxP.
+// Symmetric all-points kernels (each unordered pair once, credited to both
+// ends); partials of 16 * nt * p bytes, nt = ceil(p/512).  The FP32-
+// compensated one is used by the all-points entry for kSymMinP <= p <=
+// kSymMaxP.
 // Below about 2.4e4 the split-tile kernel is faster (measured on MI355X:
 // equal at p = 2e4, split ahead by 20 % at 1.6e4, symmetric ahead by 9 % at
 // 3e4 and 23 % at 5e4).
@@ -112,8 +113,16 @@ int exact_reduce_plain(const double *part, int n_split, size_t nq, double eta, d
 #endif
 constexpr size_t kSymMinP = SHK_SYM_MIN_P;
 constexpr size_t kSymMaxP = 262144;
-int exact_sym_f32c(const double *ev, size_t p, double eta, double scale, double *out_re,
-                   double *out_im, hipStream_t stream);
+// f64 != 0: the FP64 kernel (pair reciprocals as in rcp_pairs_f64), used for
+// kSym64MinP <= p <= kSymMaxP.
+int exact_sym(const double *ev, size_t p, double eta, double scale, int f64, double *out_re,
+              double *out_im, hipStream_t stream);
+// FP64 crossover (measured on MI355X): split ahead by 19 % at 1.64e4,
+// symmetric ahead by 9 % at 2e4, 52 % at 5e4 and 57 % at 1e5.
+#ifndef SHK_SYM64_MIN_P
+#define SHK_SYM64_MIN_P 18432
+#endif
+constexpr size_t kSym64MinP = SHK_SYM64_MIN_P;
 
 // One spectrum of a segmented all-points launch (device table entry).
 struct SegSpectrum {

@@ -130,6 +130,139 @@ __device__ __forceinline__ void sym_sweep(const double *__restrict__ s_src, int 
     }
 }
 
+// i-side finish: waves 1..3 hand their partial sums to wave 0 in turn (fixed
+// order, through `buf`, 2 * kST doubles), and wave 0 writes part[.][b][i].
+__device__ __forceinline__ void sym_finish(double (&re)[kSR], double (&iv)[kSR],
+                                           double *__restrict__ buf, int w, int lane,
+                                           double *__restrict__ part, int nt, int p, int b,
+                                           int q0) {
+    for (int r = 1; r < kSW; ++r) {
+        __syncthreads();
+        if (w == r) {
+#pragma unroll
+            for (int k = 0; k < kSR; ++k) {
+                buf[lane * kSR + k] = re[k];
+                buf[kST + lane * kSR + k] = iv[k];
+            }
+        }
+        __syncthreads();
+        if (w == 0) {
+#pragma unroll
+            for (int k = 0; k < kSR; ++k) {
+                re[k] += buf[lane * kSR + k];
+                iv[k] += buf[kST + lane * kSR + k];
+            }
+        }
+    }
+    if (w == 0) {
+        double *i_re = part + (size_t)b * p;          // part[0][b][i]
+        double *i_im = part + ((size_t)nt + b) * p;   // part[1][b][i]
+#pragma unroll
+        for (int k = 0; k < kSR; ++k) {
+            const int i = q0 + k;
+            if (i < p) {
+                i_re[i] = re[k];
+                i_im[i] = iv[k];
+            }
+        }
+    }
+}
+
+// FP64 variant: the j-side lane sums travel through LDS in FP64, kSB64
+// sources per batch (the same LDS bytes as the FP32 batches).
+constexpr int kSB64 = kSB / 2;
+
+template <bool JSIDE>
+__device__ __forceinline__ void sym_sweep_f64(const double *__restrict__ s_src, int n_src,
+                                              const double (&x)[kSR], double e2,
+                                              double (&re)[kSR], double (&iv)[kSR],
+                                              double *__restrict__ s_jb, double *__restrict__ j_re,
+                                              double *__restrict__ j_im, int lane) {
+    for (int j0 = 0; j0 < kSQ; j0 += kSB64) {  // same trip count in every wave
+        for (int u = 0; u < kSB64; ++u) {
+            const int j = j0 + u;
+            if (j < n_src) {  // wave-uniform
+                const double sv = s_src[j];
+                double d[kSR], t[kSR], r[kSR];
+#pragma unroll
+                for (int k = 0; k < kSR; ++k) {
+                    d[k] = x[k] - sv;
+                    t[k] = __builtin_fma(d[k], d[k], e2);
+                }
+                rcp_pairs_f64(t, r);
+                double bre = 0.0, bim = 0.0;
+#pragma unroll
+                for (int k = 0; k < kSR; ++k) {
+                    re[k] = __builtin_fma(d[k], r[k], re[k]);
+                    iv[k] += r[k];
+                    if (JSIDE) {
+                        bre = __builtin_fma(d[k], r[k], bre);
+                        bim += r[k];
+                    }
+                }
+                if (JSIDE) {
+                    s_jb[(2 * u + 0) * kJRow + lane] = bre;
+                    s_jb[(2 * u + 1) * kJRow + lane] = bim;
+                }
+            }
+        }
+        if (JSIDE) {
+            sym_handover();
+            if (lane < 2 * kSB64) {
+                const int u = lane >> 1, comp = lane & 1;
+                if (j0 + u < n_src) {
+                    const double *row = s_jb + (2 * u + comp) * kJRow;
+                    double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
+#pragma unroll
+                    for (int l = 0; l < 16; ++l) {
+                        c0 += row[l];
+                        c1 += row[16 + l];
+                        c2 += row[32 + l];
+                        c3 += row[48 + l];
+                    }
+                    const double tot = (c0 + c1) + (c2 + c3);
+                    if (comp) j_im[j0 + u] = tot;
+                    else j_re[j0 + u] = -tot;
+                }
+            }
+            sym_handover();  // the next batch overwrites s_jb
+        }
+    }
+}
+
+__global__ __launch_bounds__(kSL * kSW) void sym_f64_kernel(const double *__restrict__ ev, int p,
+                                                            double eta, int nt,
+                                                            double *__restrict__ part) {
+    const int b = blockIdx.x, a = blockIdx.y;
+    if (b < a) return;  // upper triangle of tile pairs only (uniform per workgroup)
+    __shared__ double s_src[kSW][kSQ];
+    __shared__ double s_jbuf[kSW * 2 * kSB64 * kJRow];  // j-side lane sums / i-side combine
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int q0 = a * kST + lane * kSR;
+    double x[kSR], re[kSR], iv[kSR];
+#pragma unroll
+    for (int k = 0; k < kSR; ++k) {
+        const int i = q0 + k;
+        x[k] = (i < p) ? ev[i] : 0.0;  // only the last tile (a = b) is partial
+        re[k] = iv[k] = 0.0;
+    }
+    const int js = b * kST + w * kSQ;
+    const int n_src = max(0, min(kSQ, p - js));
+    for (int u = lane; u < n_src; u += kSL) s_src[w][u] = ev[js + u];
+    __syncthreads();
+    double *s_jb = s_jbuf + w * (2 * kSB64 * kJRow);
+    double *j_re = part + (size_t)a * p + js;         // part[0][a][j]
+    double *j_im = part + ((size_t)nt + a) * p + js;  // part[1][a][j]
+    const double e2 = eta * eta;
+    if (a < b)
+        sym_sweep_f64<true>(s_src[w], n_src, x, e2, re, iv, s_jb, j_re, j_im, lane);
+    else
+        sym_sweep_f64<false>(s_src[w], n_src, x, e2, re, iv, s_jb, j_re, j_im, lane);
+    sym_finish(re, iv, s_jbuf, w, lane, part, nt, p, b, q0);
+}
+
+static_assert(kSW * 2 * kSB64 * kJRow >= 2 * kST, "FP64 combine buffer fits in s_jbuf");
+
 __global__ __launch_bounds__(kSL * kSW) void sym_f32c_kernel(const double *__restrict__ ev, int p,
                                                              double eta, int nt,
                                                              double *__restrict__ part) {
@@ -195,38 +328,7 @@ __global__ __launch_bounds__(kSL * kSW) void sym_f32c_kernel(const double *__res
             sym_sweep<2, false>(s_src[w], n_src, c, qh, ql, eta2f, re, iv, s_jb, j_re, j_im, lane);
     }
 
-    // i-side: waves 1..3 hand their partial sums to wave 0 in turn (fixed order)
-    double *buf = s_jbuf;  // 2 * kST doubles fit: kSW*2*kSB*kJRow/2 >= 2*kST
-    for (int r = 1; r < kSW; ++r) {
-        __syncthreads();
-        if (w == r) {
-#pragma unroll
-            for (int k = 0; k < kSR; ++k) {
-                buf[lane * kSR + k] = re[k];
-                buf[kST + lane * kSR + k] = iv[k];
-            }
-        }
-        __syncthreads();
-        if (w == 0) {
-#pragma unroll
-            for (int k = 0; k < kSR; ++k) {
-                re[k] += buf[lane * kSR + k];
-                iv[k] += buf[kST + lane * kSR + k];
-            }
-        }
-    }
-    if (w == 0) {
-        double *i_re = part + (size_t)b * p;          // part[0][b][i]
-        double *i_im = part + ((size_t)nt + b) * p;   // part[1][b][i]
-#pragma unroll
-        for (int k = 0; k < kSR; ++k) {
-            const int i = q0 + k;
-            if (i < p) {
-                i_re[i] = re[k];
-                i_im[i] = iv[k];
-            }
-        }
-    }
+    sym_finish(re, iv, s_jbuf, w, lane, part, nt, p, b, q0);
 }
 
 static_assert(kSW * 2 * kSB * kJRow / 2 >= 2 * kST, "i-side combine buffer fits in s_jbuf");
@@ -234,8 +336,8 @@ static_assert(kSQ % kSB == 0, "batches tile the wave's sources");
 
 }  // namespace
 
-int exact_sym_f32c(const double *ev, size_t p, double eta, double scale, double *out_re,
-                   double *out_im, hipStream_t stream) {
+int exact_sym(const double *ev, size_t p, double eta, double scale, int f64, double *out_re,
+              double *out_im, hipStream_t stream) {
     if (p == 0) return 0;
     if (p > (size_t)kSymMaxP) {
         set_error("symmetric kernel: p above its maximum");
@@ -245,13 +347,17 @@ int exact_sym_f32c(const double *ev, size_t p, double eta, double scale, double 
     double *part = nullptr;
     SHK_HIP(hipMallocAsync((void **)&part, sizeof(double) * 2 * (size_t)nt * p, stream));
     ProfileMark m = profile_begin(stream);
-    hipLaunchKernelGGL(sym_f32c_kernel, dim3(nt, nt), dim3(kSL * kSW), 0, stream, ev, (int)p, eta,
-                       nt, part);
+    if (f64)
+        hipLaunchKernelGGL(sym_f64_kernel, dim3(nt, nt), dim3(kSL * kSW), 0, stream, ev, (int)p,
+                           eta, nt, part);
+    else
+        hipLaunchKernelGGL(sym_f32c_kernel, dim3(nt, nt), dim3(kSL * kSW), 0, stream, ev, (int)p,
+                           eta, nt, part);
     profile_end(m, stream);
     int st = 0;
     const hipError_t le = hipGetLastError();
     if (le != hipSuccess) {
-        set_error(std::string("sym_f32c_kernel launch: ") + hipGetErrorString(le));
+        set_error(std::string("symmetric kernel launch: ") + hipGetErrorString(le));
         st = 7;
     }
     if (!st) st = exact_reduce_plain(part, nt, p, eta, scale, ev, out_re, out_im, stream);

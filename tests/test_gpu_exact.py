@@ -294,3 +294,39 @@ def test_symmetric_f32c_hard_inputs(shk, orc):
         err = rel_l2(r, i, rr / n, ri / n)
         print(f"symmetric f32c {name}: rel-L2 {err:.3e}", flush=True)
         assert err <= 1e-6, name
+
+
+@pytest.mark.parametrize("p", [24576, 24577, 50000])
+def test_symmetric_f64_kernel(shk, orc, p):
+    """The FP64 all-points entry takes the symmetric FP64 kernel for
+    24 576 <= p <= 262 144: within 1e-13 of the FP64 oracle (normwise and per
+    element against max|S|), and within 1e-13 of the split-tile FP64 kernel
+    (the spectrum as its own query points)."""
+    import torch
+    from shrinkers_amd import _lib
+    from shrinkers_amd import spectra as S
+    ev = S.mp_inverse_cdf_spectrum(p, c=0.25)
+    eta = 1.0 / np.sqrt(p)
+    d = torch.from_numpy(ev).cuda()
+    re, im = shk.all_points_dev(d, eta, "blocked", precision="f64", scaled=False)
+    sr, si = torch.empty_like(d), torch.empty_like(d)
+    assert _lib.load().shk_exact_at_points_dev(d.data_ptr(), p, d.data_ptr(), p, eta, 0,
+                                               sr.data_ptr(), si.data_ptr(), None) == _lib.SHK_OK
+    torch.cuda.synchronize()
+    re, im, sr, si = (t.cpu().numpy() for t in (re, im, sr, si))
+    rr, ri = orc.tiled_parallel(ev, eta)
+    assert_close(re, im, rr, ri, 1e-13, f"symmetric f64 p={p}")
+    assert_close(re, im, sr, si, 1e-13, f"symmetric vs split f64 p={p}")
+
+
+def test_symmetric_f64_paired_fallback(shk):
+    """The symmetric FP64 kernel's paired reciprocals fall back where the
+    denominators' product is not normal: eigenvalues and eta scaled by 2^300
+    (every pair on the fallback) change the sums by exactly 2^-300."""
+    from shrinkers_amd import spectra as S
+    ev = S.mp_inverse_cdf_spectrum(30000, c=0.3)
+    eta = 1.0 / np.sqrt(ev.size)
+    r, i = shk.compute_all_stieltjes(ev, eta, "blocked")
+    big = np.ldexp(1.0, 300)
+    rs, is_ = shk.compute_all_stieltjes(ev * big, eta * big, "blocked")
+    assert_close(rs * big, is_ * big, r, i, 1e-13, "symmetric f64, all pairs on the fallback")
