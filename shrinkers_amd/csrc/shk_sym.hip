@@ -16,8 +16,9 @@
 //   a < b: every pair (i in a, j in b) adds d r and r to query i (i-side, as
 //   the full-square kernel does) and -d r and r to query j (j-side).  A lane
 //   sums its kSR pairs of a source in FP32; the 64 lane sums of each source
-//   go through LDS and are added by one lane per (source, component) in four
-//   FP32 chains of 16 joined in FP64, once per batch of kSB sources.
+//   go through LDS and are added, per (source, component), in four FP32
+//   chains of 16 (two lanes, two chains each) joined in FP64, once per batch
+//   of kSB sources.
 //   a = b: the full square of tile a, i-side only (the diagonal term
 //   included, d = 0 exactly).
 //
@@ -108,19 +109,24 @@ __device__ __forceinline__ void sym_sweep(const double *__restrict__ s_src, int 
         }
         if (JSIDE) {
             sym_handover();
-            if (lane < 2 * kSB) {
-                const int u = lane >> 1, comp = lane & 1;
-                if (j0 + u < n_src) {
-                    const float *row = s_jb + (2 * u + comp) * kJRow;
-                    float c0 = 0.0f, c1 = 0.0f, c2 = 0.0f, c3 = 0.0f;
+            {
+                // two lanes per (source, component) row: lane h = 0 adds
+                // row[0, 32) and h = 1 row[32, 64), each as two FP32 chains
+                // of 16 joined in FP64; the row total is (h0 + h1) in that
+                // order on both lanes, ((c0 + c1) + (c2 + c3)) overall
+                const int row_id = lane >> 1, h = lane & 1;
+                const int u = row_id >> 1, comp = row_id & 1;
+                const float *row = s_jb + (2 * u + comp) * kJRow + 32 * h;
+                float ca = 0.0f, cb = 0.0f;
 #pragma unroll
-                    for (int l = 0; l < 16; ++l) {
-                        c0 += row[l];
-                        c1 += row[16 + l];
-                        c2 += row[32 + l];
-                        c3 += row[48 + l];
-                    }
-                    const double tot = ((double)c0 + (double)c1) + ((double)c2 + (double)c3);
+                for (int l = 0; l < 16; ++l) {
+                    ca += row[l];
+                    cb += row[16 + l];
+                }
+                const double mine = (double)ca + (double)cb;
+                const double other = __shfl_xor(mine, 1, 64);
+                const double tot = h ? other + mine : mine + other;
+                if (h == 0 && j0 + u < n_src) {
                     if (comp) j_im[j0 + u] = tot;
                     else j_re[j0 + u] = -tot;
                 }
@@ -208,19 +214,20 @@ __device__ __forceinline__ void sym_sweep_f64(const double *__restrict__ s_src, 
         }
         if (JSIDE) {
             sym_handover();
-            if (lane < 2 * kSB64) {
-                const int u = lane >> 1, comp = lane & 1;
-                if (j0 + u < n_src) {
-                    const double *row = s_jb + (2 * u + comp) * kJRow;
-                    double c0 = 0.0, c1 = 0.0, c2 = 0.0, c3 = 0.0;
+            {
+                // four lanes per (source, component) row, one chain of 16
+                // each; joined as (c0 + c1) + (c2 + c3) on every lane
+                const int row_id = lane >> 2, q = lane & 3;
+                const int u = row_id >> 1, comp = row_id & 1;
+                const double *row = s_jb + (2 * u + comp) * kJRow + 16 * q;
+                double cq = 0.0;
 #pragma unroll
-                    for (int l = 0; l < 16; ++l) {
-                        c0 += row[l];
-                        c1 += row[16 + l];
-                        c2 += row[32 + l];
-                        c3 += row[48 + l];
-                    }
-                    const double tot = (c0 + c1) + (c2 + c3);
+                for (int l = 0; l < 16; ++l) cq += row[l];
+                const double o1 = __shfl_xor(cq, 1, 64);
+                const double pair = (q & 1) ? o1 + cq : cq + o1;   // c0+c1 or c2+c3
+                const double o2 = __shfl_xor(pair, 2, 64);
+                const double tot = (q & 2) ? o2 + pair : pair + o2;
+                if (q == 0 && j0 + u < n_src) {
                     if (comp) j_im[j0 + u] = tot;
                     else j_re[j0 + u] = -tot;
                 }
