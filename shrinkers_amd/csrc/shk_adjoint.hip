@@ -638,7 +638,8 @@ int exact_adjoint(const double *q, size_t nq, const double *ev, size_t p, double
         int tiles, ns, per;
         plan(p, nq, &tiles, &ns, &per);
         double *part = nullptr;
-        SHK_HIP(hipMallocAsync((void **)&part, sizeof(double) * (size_t)ns * p, stream));
+        CachedBuf cache(CACHE_ADJ_PART, stream);
+        if (int e = cache.get((size_t)ns * p, &part)) return e;
         ProfileMark mk = profile_begin(stream);
         if (f32c)
             hipLaunchKernelGGL(adj_src_f32c_kernel, dim3(tiles, ns), dim3(kAWG), 0, stream, q,
@@ -653,19 +654,17 @@ int exact_adjoint(const double *q, size_t nq, const double *ev, size_t p, double
                                dim3(kRedWG), 0, stream, part, ns, (int)p, grad_ev);
             le = hipGetLastError();
         }
-        const hipError_t fe = hipFreeAsync(part, stream);
         if (le != hipSuccess) {
             set_error(std::string("adjoint (eigenvalue sweep): ") + hipGetErrorString(le));
             return 7;
         }
-        SHK_HIP(fe);
     }
     if (grad_q || grad_eta) {
         int tiles, ns, per;
         plan(nq, p, &tiles, &ns, &per);
         double *part = nullptr;
-        SHK_HIP(hipMallocAsync((void **)&part, sizeof(double) * (2 * (size_t)ns * nq + nq),
-                               stream));
+        CachedBuf cache(CACHE_ADJ_PART, stream);
+        if (int e = cache.get(2 * (size_t)ns * nq + nq, &part)) return e;
         double *eta_term = part + 2 * (size_t)ns * nq;
         ProfileMark mk = profile_begin(stream);
         if (f32c)
@@ -687,12 +686,10 @@ int exact_adjoint(const double *q, size_t nq, const double *ev, size_t p, double
                                eta_term, (int)nq, 1.0, grad_eta, accumulate_eta);
             le = hipGetLastError();
         }
-        const hipError_t fe = hipFreeAsync(part, stream);
         if (le != hipSuccess) {
             set_error(std::string("adjoint (query sweep): ") + hipGetErrorString(le));
             return 7;
         }
-        SHK_HIP(fe);
     }
     return 0;
 }
@@ -712,7 +709,8 @@ int ek_fit_impl(const double *x, const double *g_re, const double *g_im, size_t 
         return 6;
     }
     double *scr = nullptr;
-    SHK_HIP(hipMallocAsync((void **)&scr, sizeof(double) * 6 * m, stream));
+    CachedBuf cache(CACHE_EK_SCR, stream);
+    if (int e = cache.get(6 * m, &scr)) return e;
     double *a_buf = scr, *r_buf = scr + 2 * m, *sq = scr + 4 * m, *et = scr + 5 * m;
     hipLaunchKernelGGL(ek_point_kernel, dim3((unsigned)m), dim3(64), 0, stream, x, g_re, g_im,
                        (int)m, eta, c, t, w, (int)k, a_buf, r_buf, sq, et, grad_g_re, grad_g_im,
@@ -733,12 +731,10 @@ int ek_fit_impl(const double *x, const double *g_re, const double *g_im, size_t 
                            1.0, grad_eta, 0);
         le = hipGetLastError();
     }
-    const hipError_t fe = hipFreeAsync(scr, stream);
     if (le != hipSuccess) {
         set_error(std::string("El Karoui fit kernels: ") + hipGetErrorString(le));
         return 7;
     }
-    SHK_HIP(fe);
     return 0;
 }
 
@@ -760,7 +756,8 @@ int ek_fit_step(const double *ev, size_t p, const double *x, size_t m, double et
         return 1;
     }
     double *scr = nullptr;
-    SHK_HIP(hipMallocAsync((void **)&scr, sizeof(double) * 4 * m, stream));
+    CachedBuf cache(CACHE_EK_STEP, stream);
+    if (int e = cache.get(4 * m, &scr)) return e;
     double *s_re = scr, *s_im = scr + m, *g_re = scr + 2 * m, *g_im = scr + 3 * m;
     const double inv_p = 1.0 / (double)p;
     int st;
@@ -778,10 +775,7 @@ int ek_fit_step(const double *ev, size_t p, const double *x, size_t m, double et
     if (!st && (grad_ev || grad_eta))
         st = exact_adjoint(x, m, ev, p, eta, g_re, g_im, f32c, grad_ev, nullptr, grad_eta, stream,
                            1);
-    const hipError_t fe = hipFreeAsync(scr, stream);
-    if (st) return st;
-    SHK_HIP(fe);
-    return 0;
+    return st;
 }
 
 }  // namespace shk

@@ -583,8 +583,9 @@ int exact_f64(const double *q, size_t nq, const double *src, size_t p, double et
     plan_splits(nq, p, &qt, &ns, &per);
     const int na = (kind == EXACT_DERIV) ? 4 : 2;
     double *part = workspace;
+    CachedBuf cache(CACHE_EXACT_PART, stream);
     if (!part)
-        SHK_HIP(hipMallocAsync((void **)&part, sizeof(double) * na * (size_t)ns * nq, stream));
+        if (int e = cache.get(na * (size_t)ns * nq, &part)) return e;
     int st = 0;
     switch (kind) {
         case EXACT_PLAIN:
@@ -611,13 +612,7 @@ int exact_f64(const double *q, size_t nq, const double *src, size_t p, double et
             st = 6;
             set_error("exact kernel: unknown kind");
     }
-    hipError_t fe = workspace ? hipSuccess : hipFreeAsync(part, stream);
-    if (st) return st;
-    if (fe != hipSuccess) {
-        set_error(std::string("hipFreeAsync: ") + hipGetErrorString(fe));
-        return 7;
-    }
-    return 0;
+    return st;
 }
 
 int exact_f32c(const double *q, size_t nq, const double *src, size_t p, double eta,
@@ -631,8 +626,9 @@ int exact_f32c(const double *q, size_t nq, const double *src, size_t p, double e
     int qt, ns, per;
     plan_splits(nq, p, &qt, &ns, &per);
     double *part = workspace;
+    CachedBuf cache(CACHE_EXACT_PART, stream);
     if (!part)
-        SHK_HIP(hipMallocAsync((void **)&part, sizeof(double) * 2 * (size_t)ns * nq, stream));
+        if (int e = cache.get(2 * (size_t)ns * nq, &part)) return e;
     dim3 grid(qt, ns);
     ProfileMark m = profile_begin(stream);
     hipLaunchKernelGGL(exact_f32c_kernel, grid, dim3(kWG), 0, stream, q, (int)nq, src, (int)p,
@@ -653,13 +649,7 @@ int exact_f32c(const double *q, size_t nq, const double *src, size_t p, double e
                                                     (double *)out_re, (double *)out_im, nullptr,
                                                     nullptr, stream);
     }
-    hipError_t fe = workspace ? hipSuccess : hipFreeAsync(part, stream);
-    if (st) return st;
-    if (fe != hipSuccess) {
-        set_error(std::string("hipFreeAsync: ") + hipGetErrorString(fe));
-        return 7;
-    }
-    return 0;
+    return st;
 }
 
 int exact_many_plan(const int64_t *offsets, const size_t *order, size_t n, const double *etas,

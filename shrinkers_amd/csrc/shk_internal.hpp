@@ -28,7 +28,11 @@ enum CacheSlot : int {
     CACHE_CHEB_SCRATCH = 0,  // tree build scratch
     CACHE_CHEB_SRC = 1,      // transient tree: sorted sources
     CACHE_CHEB_NODES = 2,    // transient tree: node records and panels
-    kCacheSlots = 3
+    CACHE_EXACT_PART = 3,    // exact / symmetric kernels: FP64 partial sums
+    CACHE_ADJ_PART = 4,      // adjoint sweeps: FP64 partial sums
+    CACHE_EK_SCR = 5,        // El Karoui fit kernels: per-point scratch
+    CACHE_EK_STEP = 6,       // fused fit step: sample transform and its gradient
+    kCacheSlots = 7
 };
 // Returns the slot's buffer (>= bytes) in *out, ordered on stream s.
 int cache_acquire(int slot, size_t bytes, hipStream_t s, void **out);
@@ -37,6 +41,26 @@ int cache_release(int slot, hipStream_t s);
 // Frees the calling thread's cached buffers on the current device (waits
 // for their last uses).
 int cache_free_all();
+
+// A cache slot held for the rest of the enclosing scope: get() acquires it on
+// stream s, and leaving the scope marks the end of its use there.
+struct CachedBuf {
+    int slot;
+    hipStream_t s;
+    void *p = nullptr;
+    CachedBuf(int sl, hipStream_t st) : slot(sl), s(st) {}
+    CachedBuf(const CachedBuf &) = delete;
+    CachedBuf &operator=(const CachedBuf &) = delete;
+    template <class T>
+    int get(size_t count, T **out) {
+        const int st = cache_acquire(slot, sizeof(T) * count, s, &p);
+        *out = static_cast<T *>(p);
+        return st;
+    }
+    ~CachedBuf() {
+        if (p) (void)cache_release(slot, s);
+    }
+};
 
 // Evaluates a HIP call; on failure records the message and returns
 // SHK_ERR_HIP (7) from the enclosing function.

@@ -352,7 +352,8 @@ int exact_sym(const double *ev, size_t p, double eta, double scale, int f64, dou
     }
     const int nt = (int)((p + kST - 1) / kST);
     double *part = nullptr;
-    SHK_HIP(hipMallocAsync((void **)&part, sizeof(double) * 2 * (size_t)nt * p, stream));
+    CachedBuf cache(CACHE_EXACT_PART, stream);
+    if (int e = cache.get(2 * (size_t)nt * p, &part)) return e;
     ProfileMark m = profile_begin(stream);
     if (f64)
         hipLaunchKernelGGL(sym_f64_kernel, dim3(nt, nt), dim3(kSL * kSW), 0, stream, ev, (int)p,
@@ -368,13 +369,7 @@ int exact_sym(const double *ev, size_t p, double eta, double scale, int f64, dou
         st = 7;
     }
     if (!st) st = exact_reduce_plain(part, nt, p, eta, scale, ev, out_re, out_im, stream);
-    const hipError_t fe = hipFreeAsync(part, stream);
-    if (st) return st;
-    if (fe != hipSuccess) {
-        set_error(std::string("hipFreeAsync: ") + hipGetErrorString(fe));
-        return 7;
-    }
-    return 0;
+    return st;
 }
 
 }  // namespace shk
