@@ -13,13 +13,16 @@
 //      builds its subtree; a partition search narrowed by a value-bucket
 //      table finds each split; subtree sizes and pre-order numbers follow,
 //      with per-depth node lists for the merge;
-//   3. finalize: pre-order node records, Chebyshev positions (:269-279);
-//   4. leaf fill: one thread per leaf, barycentric rows in source order
-//      (:286-337, :401-460);
-//   5. merge: parents from their children's weights, deepest depth first
-//      (:474-506);
-//   6. f32 mirrors for the FAST traversal (:551-570) and f32 hi/lo splits of
-//      the sorted sources for the FAST near field.
+//   3. node records, one wavefront per node: pre-order arrays, Chebyshev
+//      positions (:269-279), the f32 mirrors for the FAST traversal
+//      (:551-570), and for a leaf its weights, barycentric rows in source
+//      order (:286-337, :401-460);
+//   4. merge: parents from their children's weights, deepest depth first
+//      (:474-506): one workgroup per frontier subtree merges that subtree's
+//      depths, then the top levels (one launch per large level, one
+//      workgroup for the small ones);
+//   5. the f32 hi/lo splits of the sorted sources for the FAST near field
+//      are written with the partition hint.
 //
 // Evaluation is one thread per query, queries in ascending order so that the
 // 64 lanes of a wavefront walk nearly the same path.  The reference's DFS
@@ -163,12 +166,19 @@ __global__ void hint_clear_kernel(int *__restrict__ bstart, int n) {
 // back to the node's own range there.
 constexpr int kHintGap = 64;
 
+// The same pass writes the f32 hi/lo split of each source for the FAST near
+// field (hi = f32(x), lo = f32(x - hi)).
 __global__ void hint_scatter_kernel(const double *__restrict__ sorted, int p, int R,
-                                    int *__restrict__ bstart) {
+                                    int *__restrict__ bstart, float *__restrict__ s_hi,
+                                    float *__restrict__ s_lo) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p) return;
+    const double x = sorted[i];
+    const float xh = (float)x;
+    s_hi[i] = xh;
+    s_lo[i] = (float)(x - (double)xh);
     const Hint h = hint_of(sorted, p, R);
-    const int bi = hint_bucket(sorted[i], h);
+    const int bi = hint_bucket(x, h);
     const int bp = (i == 0) ? -1 : hint_bucket(sorted[i - 1], h);
     if (bi - bp <= kHintGap)
         for (int k = bp + 1; k <= bi; ++k) bstart[k] = i;
@@ -521,36 +531,17 @@ struct TreeArrays {
     int32_t *lo_idx, *hi_idx, *right, *esc;
 };
 
-// One thread per node (node slots list[0, N)).
-__global__ void finalize_kernel(const double *__restrict__ sorted, int N, int n, double theta_sq,
-                                const int *__restrict__ list, const int *__restrict__ b_lo,
-                                const int *__restrict__ b_hi, const int *__restrict__ b_left,
-                                const int *__restrict__ b_size, const int *__restrict__ b_pre,
-                                const double *__restrict__ sm1, TreeArrays t) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    const int v = list[i];
-    const int k = b_pre[v];
-    const int a = b_lo[v], b = b_hi[v];
-    const double lo = sorted[a], hi = sorted[b - 1];
-    t.lo[k] = lo;
-    t.hi[k] = hi;
-    const double hw = 0.5 * (hi - lo);
-    t.acc[k] = (hw * hw) / theta_sq;
-    t.lo_idx[k] = a;
-    t.hi_idx[k] = b;
-    const int l = b_left[v];
-    t.right[k] = (l < 0) ? -1 : b_pre[l + 1];
-    t.esc[k] = k + b_size[v];
-    const double c = 0.5 * (lo + hi), hl = 0.5 * (hi - lo);
-    for (int j = 0; j < n; ++j) {
-        t.nodes[(size_t)k * n + j] = c + hl * sm1[j];
-        t.w[(size_t)k * n + j] = 0.0;
-    }
+// Makes a wavefront's LDS writes visible to its own lanes' reads: the build
+// kernels below give each wavefront its own LDS region.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Barycentric accumulation of `nrows` rows (x_r, mass_r) into one panel's
-// weights w[0..n) (chebcode.rs:309-337), one wavefront per panel.
+// weights w[0..n) (chebcode.rs:309-337), one wavefront per panel; w32
+// receives the f32 copy of the weights (the FAST panel).
 //
 // Lanes first evaluate 64 rows at a time: the quotients lam_j/(x_r - t_j)
 // go to LDS together with m_r = mass_r * (1/s_r), or the node index hit by
@@ -558,16 +549,34 @@ __global__ void finalize_kernel(const double *__restrict__ sorted, int N, int n,
 // w_j = fma(m_r, q_rj, w_j), which is the reference's per-row update in the
 // reference's order, so the weights equal a sequential build bit for bit.
 // Rows with zero mass are skipped (merge_all_weights, chebcode.rs:496-499).
-// `row(r, x, mass)` supplies row r; w must hold zeros on entry.
+// `row(r, x, mass)` supplies row r; `t` (the panel's Chebyshev positions)
+// may be in LDS or global memory.
+//
+// NT > 0 fixes n = NT (the presets' orders): positions and weights sit in
+// registers and every quotient of a row is formed before the first exact
+// hit is picked, so the loads and divisions of a row issue together.  A
+// row without a hit gets the same quotients and the same sum s in the same
+// order as the sequential loop; a row with a hit uses only its mass and
+// the hit's index, as there.
 constexpr int kBaryLanes = 64;
 constexpr int kWaccPerLane = (kMaxN + kBaryLanes - 1) / kBaryLanes;
 
-template <class RowFn>
+template <int NT, class RowFn>
 __device__ void bary_accumulate(RowFn row, int nrows, const double *__restrict__ t,
-                                const double *__restrict__ lam, double *__restrict__ w, int n,
-                                double *__restrict__ sq, double *__restrict__ sm,
-                                int *__restrict__ shit) {
-    const int lane = threadIdx.x;
+                                const double *__restrict__ lam, double *__restrict__ w,
+                                float *__restrict__ w32, int n_rt, double *__restrict__ sq,
+                                double *__restrict__ sm, int *__restrict__ shit) {
+    const int n = (NT > 0) ? NT : n_rt;
+    const int lane = threadIdx.x & (kBaryLanes - 1);
+    constexpr int kReg = (NT > 0) ? NT : 1;
+    double tr[kReg], lr[kReg];
+    if (NT > 0) {
+#pragma unroll
+        for (int j = 0; j < kReg; ++j) {
+            tr[j] = t[j];
+            lr[j] = lam[j];
+        }
+    }
     double wacc[kWaccPerLane];
 #pragma unroll
     for (int u = 0; u < kWaccPerLane; ++u) wacc[u] = 0.0;
@@ -581,22 +590,33 @@ __device__ void bary_accumulate(RowFn row, int nrows, const double *__restrict__
             if (mass != 0.0) {
                 double s = 0.0;
                 hit = -1;
-                for (int j = 0; j < n; ++j) {
-                    const double d = x - t[j];
-                    if (d == 0.0) {
-                        hit = j;
-                        break;
+                if (NT > 0) {
+#pragma unroll
+                    for (int j = 0; j < kReg; ++j) {
+                        const double d = x - tr[j];
+                        const double q = lr[j] / d;
+                        sq[lane * kReg + j] = q;
+                        s += q;
+                        hit = (hit < 0 && d == 0.0) ? j : hit;
                     }
-                    const double q = lam[j] / d;
-                    sq[lane * n + j] = q;
-                    s += q;
+                } else {
+                    for (int j = 0; j < n; ++j) {
+                        const double d = x - t[j];
+                        if (d == 0.0) {
+                            hit = j;
+                            break;
+                        }
+                        const double q = lam[j] / d;
+                        sq[lane * n + j] = q;
+                        s += q;
+                    }
                 }
                 m = (hit >= 0) ? mass : mass * (1.0 / s);
             }
         }
         sm[lane] = m;
         shit[lane] = hit;
-        __syncthreads();
+        wave_sync();
         const int rows = min(kBaryLanes, nrows - base);
 #pragma unroll
         for (int u = 0; u < kWaccPerLane; ++u) {
@@ -611,58 +631,113 @@ __device__ void bary_accumulate(RowFn row, int nrows, const double *__restrict__
                 wacc[u] = acc;
             }
         }
-        __syncthreads();
+        wave_sync();  // the next rows overwrite sq, sm, shit
     }
 #pragma unroll
     for (int u = 0; u < kWaccPerLane; ++u) {
         const int j = lane + u * kBaryLanes;
-        if (j < n) w[j] = wacc[u];
+        if (j < n) {
+            w[j] = wacc[u];
+            w32[j] = (float)wacc[u];
+        }
     }
 }
 
-// LDS layout of the two build kernels below: sq[64*n] doubles, sm[64]
-// doubles, shit[64] ints.
-inline size_t bary_lds_bytes(int n) {
-    return sizeof(double) * ((size_t)kBaryLanes * n + kBaryLanes) + sizeof(int) * kBaryLanes;
+// LDS of one wavefront's accumulation: sq[64*n] doubles, sm[64] doubles,
+// shit[64] ints (counted as 32 doubles).
+__host__ __device__ inline int bary_lds_doubles(int n) {
+    return kBaryLanes * n + kBaryLanes + kBaryLanes / 2;
+}
+inline size_t bary_lds_bytes(int n) { return sizeof(double) * (size_t)bary_lds_doubles(n); }
+
+// bary_accumulate with the register form for the presets' orders.
+template <class RowFn>
+__device__ __forceinline__ void bary_any(RowFn row, int nrows, const double *__restrict__ t,
+                                         const double *__restrict__ lam, double *__restrict__ w,
+                                         float *__restrict__ w32, int n, double *__restrict__ sq,
+                                         double *__restrict__ sm, int *__restrict__ shit) {
+    switch (n) {
+    case 8: bary_accumulate<8>(row, nrows, t, lam, w, w32, n, sq, sm, shit); break;
+    case 11: bary_accumulate<11>(row, nrows, t, lam, w, w32, n, sq, sm, shit); break;
+    case 12: bary_accumulate<12>(row, nrows, t, lam, w, w32, n, sq, sm, shit); break;
+    default: bary_accumulate<0>(row, nrows, t, lam, w, w32, n, sq, sm, shit); break;
+    }
 }
 
-// One wavefront per node; internal nodes exit at once.
-__global__ __launch_bounds__(kBaryLanes) void leaf_fill_kernel(const double *__restrict__ sorted,
-                                                               int n,
-                                                               const double *__restrict__ lam,
-                                                               TreeArrays t) {
+struct BaryLds {
+    double *sq, *sm;
+    int *shit;
+};
+__device__ __forceinline__ BaryLds bary_lds(double *base, int n) {
+    BaryLds b;
+    b.sq = base;
+    b.sm = base + kBaryLanes * n;
+    b.shit = reinterpret_cast<int *>(b.sm + kBaryLanes);
+    return b;
+}
+
+// Node records, one wavefront per node (node slots list[0, N)): pre-order
+// arrays, Chebyshev positions (chebcode.rs:269-279), the slot's merge record
+// (pre-order number, right child) and the f32 mirrors (chebcode.rs:551-570);
+// a leaf also gets its weights from its sources (chebcode.rs:401-460).
+// Internal nodes get theirs from the merge kernels.
+__global__ __launch_bounds__(kBaryLanes) void node_kernel(
+    const double *__restrict__ sorted, int n, double theta_sq, const int *__restrict__ list,
+    const int *__restrict__ b_lo, const int *__restrict__ b_hi, const int *__restrict__ b_left,
+    const int *__restrict__ b_size, const int *__restrict__ b_pre, const double *__restrict__ tab,
+    TreeArrays t, int2 *__restrict__ mrec, float4 *__restrict__ rec32, int4 *__restrict__ link,
+    float *__restrict__ panel32) {
     extern __shared__ double lds[];
-    const int k = blockIdx.x;
-    if (t.right[k] >= 0) return;
-    double *sq = lds;
-    double *sm = lds + kBaryLanes * n;
-    int *shit = reinterpret_cast<int *>(sm + kBaryLanes);
-    const double *xs = sorted + t.lo_idx[k];
+    const int lane = threadIdx.x;
+    const int v = list[blockIdx.x];
+    const int k = b_pre[v], a = b_lo[v], b = b_hi[v], l = b_left[v], size = b_size[v];
+    const int right = (l < 0) ? -1 : b_pre[l + 1];
+    const double lo = sorted[a], hi = sorted[b - 1];
+    const double hw = 0.5 * (hi - lo);
+    const double acc = (hw * hw) / theta_sq;
+    if (lane == 0) {
+        t.lo[k] = lo;
+        t.hi[k] = hi;
+        t.acc[k] = acc;
+        t.lo_idx[k] = a;
+        t.hi_idx[k] = b;
+        t.right[k] = right;
+        t.esc[k] = k + size;
+        mrec[v] = make_int2(k, right);
+        rec32[k] = make_float4((float)lo, (float)hi, (float)acc, 0.0f);
+        link[k] = make_int4(right, k + size, a, b);
+    }
+    const BaryLds bl = bary_lds(lds, n);
+    double *pos = lds + bary_lds_doubles(n);  // n doubles
+    const double c = 0.5 * (lo + hi), hl = 0.5 * (hi - lo);
+    for (int j = lane; j < n; j += kBaryLanes) {
+        const double x = c + hl * tab[j];
+        t.nodes[(size_t)k * n + j] = x;
+        panel32[(size_t)k * 2 * n + j] = (float)x;
+        pos[j] = x;
+    }
+    if (right >= 0) return;
+    wave_sync();
+    const double *xs = sorted + a;
     auto row = [xs](int r, double &x, double &mass) {
         x = xs[r];
         mass = 1.0;
     };
-    bary_accumulate(row, t.hi_idx[k] - t.lo_idx[k], t.nodes + (size_t)k * n, lam,
-                    t.w + (size_t)k * n, n, sq, sm, shit);
+    bary_any(row, b - a, pos, tab + n, t.w + (size_t)k * n,
+                    panel32 + (size_t)k * 2 * n + n, n, bl.sq, bl.sm, bl.shit);
 }
 
-// Parents of one tree depth (node slots level[0, gridDim.x)), one wavefront
-// each.  The 2n rows are the children's Chebyshev nodes with their weights
-// as masses, left child first (merge_all_weights, chebcode.rs:474-506).
-__global__ __launch_bounds__(kBaryLanes) void merge_level_kernel(int n,
-                                                                 const int *__restrict__ level,
-                                                                 const int *__restrict__ b_left,
-                                                                 const int *__restrict__ b_pre,
-                                                                 const double *__restrict__ lam,
-                                                                 TreeArrays t) {
-    extern __shared__ double lds[];
-    const int v = level[blockIdx.x];
-    if (b_left[v] < 0) return;
-    const int k = b_pre[v];
-    double *sq = lds;
-    double *sm = lds + kBaryLanes * n;
-    int *shit = reinterpret_cast<int *>(sm + kBaryLanes);
-    const size_t lb = (size_t)(k + 1) * n, rb = (size_t)t.right[k] * n;
+// Weights of the node in slot v from its children's: the 2n rows are the
+// children's Chebyshev nodes with their weights as masses, left child first
+// (merge_all_weights, chebcode.rs:474-506).  One wavefront; leaves return.
+__device__ __forceinline__ void merge_node(int v, int n, const int2 *__restrict__ mrec,
+                                           const double *__restrict__ lam, const TreeArrays &t,
+                                           float *__restrict__ panel32, double *lds_wave) {
+    const int2 m = mrec[v];
+    if (m.y < 0) return;
+    const int k = m.x;
+    const BaryLds bl = bary_lds(lds_wave, n);
+    const size_t lb = (size_t)(k + 1) * n, rb = (size_t)m.y * n;
     const double *nodes = t.nodes;
     const double *w = t.w;
     auto row = [=](int r, double &x, double &mass) {
@@ -670,29 +745,152 @@ __global__ __launch_bounds__(kBaryLanes) void merge_level_kernel(int n,
         x = nodes[o];
         mass = w[o];
     };
-    bary_accumulate(row, 2 * n, t.nodes + (size_t)k * n, lam, t.w + (size_t)k * n, n, sq, sm,
-                    shit);
+    bary_any(row, 2 * n, t.nodes + (size_t)k * n, lam, t.w + (size_t)k * n,
+                    panel32 + (size_t)k * 2 * n + n, n, bl.sq, bl.sm, bl.shit);
 }
 
-__global__ void mirror_kernel(int N, int n, TreeArrays t, float4 *__restrict__ rec32,
-                              int4 *__restrict__ link, float *__restrict__ panel32) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= N) return;
-    rec32[k] = make_float4((float)t.lo[k], (float)t.hi[k], (float)t.acc[k], 0.0f);
-    link[k] = make_int4(t.right[k], t.esc[k], t.lo_idx[k], t.hi_idx[k]);
-    for (int j = 0; j < n; ++j) {
-        panel32[(size_t)k * 2 * n + j] = (float)t.nodes[(size_t)k * n + j];
-        panel32[(size_t)k * 2 * n + n + j] = (float)t.w[(size_t)k * n + j];
+// Merges of the nodes in slots lb + [base, base + G) that are below lb + cnt,
+// G = 64 / 2NT nodes per wavefront: lane group g (2NT lanes) forms node g's
+// rows, then lane g*NT + j adds w_j of node g over that node's rows in row
+// order.  Per node this is the arithmetic of the one-node accumulation in
+// the same order, so the weights are the same bit for bit.
+template <int NT>
+__device__ __forceinline__ void merge_group(int lb, int base, int cnt,
+                                            const int2 *__restrict__ mrec,
+                                            const double *__restrict__ lam, const TreeArrays &t,
+                                            float *__restrict__ panel32, double *lds_wave) {
+    constexpr int R = 2 * NT, G = kBaryLanes / R;
+    const int lane = threadIdx.x & (kBaryLanes - 1);
+    const int g = lane / R, r = lane - g * R;
+    const BaryLds bl = bary_lds(lds_wave, NT);
+    int2 m = make_int2(0, -1);
+    if (g < G && base + g < cnt) m = mrec[lb + base + g];
+    int hit = -2;  // as in bary_accumulate
+    double mm = 0.0;
+    if (m.y >= 0) {
+        const double *tp = t.nodes + (size_t)m.x * NT;
+        double tr[NT], lr[NT];
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+            tr[j] = tp[j];
+            lr[j] = lam[j];
+        }
+        const size_t o = (r < NT) ? (size_t)(m.x + 1) * NT + r : (size_t)m.y * NT + (r - NT);
+        const double x = t.nodes[o], mass = t.w[o];
+        if (mass != 0.0) {
+            double sum = 0.0;
+            hit = -1;
+#pragma unroll
+            for (int j = 0; j < NT; ++j) {
+                const double d = x - tr[j];
+                const double q = lr[j] / d;
+                bl.sq[lane * NT + j] = q;
+                sum += q;
+                hit = (hit < 0 && d == 0.0) ? j : hit;
+            }
+            mm = (hit >= 0) ? mass : mass * (1.0 / sum);
+        }
+    }
+    bl.sm[lane] = mm;
+    bl.shit[lane] = hit;
+    wave_sync();
+    const int ga = lane / NT, j = lane - ga * NT;
+    const int src = min(ga, G - 1) * R;  // first lane of group ga
+    const int kx = __shfl(m.x, src, kBaryLanes), ky = __shfl(m.y, src, kBaryLanes);
+    if (ga < G && ky >= 0) {
+        const int r0 = ga * R;
+        double acc = 0.0;
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const int h = bl.shit[r0 + u];
+            if (h == -1) acc = __builtin_fma(bl.sm[r0 + u], bl.sq[(r0 + u) * NT + j], acc);
+            else if (h == j) acc += bl.sm[r0 + u];
+        }
+        t.w[(size_t)kx * NT + j] = acc;
+        panel32[(size_t)kx * 2 * NT + NT + j] = (float)acc;
+    }
+    wave_sync();  // the next group overwrites sq, sm, shit
+}
+
+// Nodes merged at once by one wavefront for order n (1 outside the presets).
+__host__ __device__ inline int merge_group_size(int n) {
+    return (n == 8 || n == 11 || n == 12) ? kBaryLanes / (2 * n) : 1;
+}
+
+// The nodes in slots [lb, lb + cnt), wavefront wv of `waves` taking groups
+// wv, wv + waves, ...
+template <int NT>
+__device__ __forceinline__ void merge_level_t(int lb, int cnt, int wv, int waves,
+                                              const int2 *__restrict__ mrec,
+                                              const double *__restrict__ lam, const TreeArrays &t,
+                                              float *__restrict__ panel32, double *lds_wave) {
+    constexpr int G = kBaryLanes / (2 * NT);
+    for (int c = wv * G; c < cnt; c += waves * G)
+        merge_group<NT>(lb, c, cnt, mrec, lam, t, panel32, lds_wave);
+}
+
+__device__ __forceinline__ void merge_level(int n, int lb, int cnt, int wv, int waves,
+                                            const int2 *__restrict__ mrec,
+                                            const double *__restrict__ lam, const TreeArrays &t,
+                                            float *__restrict__ panel32, double *lds_wave) {
+    switch (n) {
+    case 8: merge_level_t<8>(lb, cnt, wv, waves, mrec, lam, t, panel32, lds_wave); break;
+    case 11: merge_level_t<11>(lb, cnt, wv, waves, mrec, lam, t, panel32, lds_wave); break;
+    case 12: merge_level_t<12>(lb, cnt, wv, waves, mrec, lam, t, panel32, lds_wave); break;
+    default:
+        for (int i = wv; i < cnt; i += waves) merge_node(lb + i, n, mrec, lam, t, panel32, lds_wave);
+        break;
     }
 }
 
-__global__ void split_kernel(const double *__restrict__ v, int p, float *__restrict__ hi,
-                             float *__restrict__ lo) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= p) return;
-    const float h = (float)v[i];
-    hi[i] = h;
-    lo[i] = (float)(v[i] - (double)h);
+constexpr int kMergeMaxThreads = 1024;
+constexpr int kMergeSubWaves = 4;   // wavefronts per subtree workgroup
+constexpr int kMergeTopRounds = 2;  // a top level of at most this many groups per wavefront
+                                    // merges in the one-workgroup kernel
+
+// One top level (slots [lb, lb + cnt)), one wavefront per group of nodes.
+__global__ __launch_bounds__(kBaryLanes) void merge_range_kernel(int n, int lb, int cnt,
+                                                                 const int2 *__restrict__ mrec,
+                                                                 const double *__restrict__ lam,
+                                                                 TreeArrays t,
+                                                                 float *__restrict__ panel32) {
+    extern __shared__ double lds[];
+    merge_level(n, lb, cnt, blockIdx.x, gridDim.x, mrec, lam, t, panel32, lds);
+}
+
+// The subtree of frontier node blockIdx.x, its depths bottom-up (level l
+// holds slots lv_start[l] + [0, lv_cnt[l])); the workgroup barrier orders
+// the levels.
+__global__ __launch_bounds__(kMergeMaxThreads) void merge_sub_kernel(
+    int n, const int *__restrict__ lv_start, const int *__restrict__ lv_cnt,
+    const int2 *__restrict__ mrec, const double *__restrict__ lam, TreeArrays t,
+    float *__restrict__ panel32) {
+    extern __shared__ double lds[];
+    const int f = blockIdx.x;
+    const int waves = blockDim.x / kBaryLanes, wv = threadIdx.x / kBaryLanes;
+    double *mine = lds + (size_t)wv * bary_lds_doubles(n);
+    int n_lv = 1;
+    while (n_lv < kLv && lv_cnt[f * kLv + n_lv] > 0) ++n_lv;
+    for (int l = n_lv - 2; l >= 0; --l) {
+        merge_level(n, lv_start[f * kLv + l], lv_cnt[f * kLv + l], wv, waves, mrec, lam, t,
+                    panel32, mine);
+        __syncthreads();
+    }
+}
+
+// Top levels [0, levels) bottom-up in one workgroup (level d holds the BFS
+// slots [lvl[d], lvl[d+1]) recorded by the top walk).
+__global__ __launch_bounds__(kMergeMaxThreads) void merge_top_kernel(
+    int n, int levels, const int *__restrict__ info, const int2 *__restrict__ mrec,
+    const double *__restrict__ lam, TreeArrays t, float *__restrict__ panel32) {
+    extern __shared__ double lds[];
+    const int waves = blockDim.x / kBaryLanes, wv = threadIdx.x / kBaryLanes;
+    double *mine = lds + (size_t)wv * bary_lds_doubles(n);
+    const int *lvl = info + kInfoTopLevel;
+    for (int d = levels - 1; d >= 0; --d) {
+        merge_level(n, lvl[d], lvl[d + 1] - lvl[d], wv, waves, mrec, lam, t, panel32, mine);
+        __syncthreads();
+    }
 }
 
 // ---- evaluation --------------------------------------------------------------
@@ -1172,6 +1370,7 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
     Carver sc;
     (void)sc.take<int>(n_info);
     (void)sc.take<int>(6 * slots);
+    (void)sc.take<int2>(slots);
     (void)sc.take<int>(3 * (size_t)kMaxFrontier * kLv);
     (void)sc.take<double>(2 * (size_t)n);
     (void)sc.take<int32_t>(p);
@@ -1189,6 +1388,7 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
     sc.off = 0;
     int *d_info = sc.take<int>(n_info);
     int *b_buf = sc.take<int>(6 * slots);
+    int2 *d_mrec = sc.take<int2>(slots);
     int *lv_buf = sc.take<int>(3 * (size_t)kMaxFrontier * kLv);
     double *d_tab = sc.take<double>(2 * (size_t)n);
     int32_t *d_iota = sc.take<int32_t>(p);
@@ -1239,7 +1439,7 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
                            stream, d_hint, R + 1);
         SHK_HIP(hipGetLastError());
         hipLaunchKernelGGL(hint_scatter_kernel, dim3(blocks_for(p, 256)), dim3(256), 0, stream,
-                           t->sorted, ip, R, d_hint);
+                           t->sorted, ip, R, d_hint, t->s_hi, t->s_lo);
         SHK_HIP(hipGetLastError());
         const TopoCtx c{t->sorted, ip, leaf_cap, d_hint, R, Hint{}, b_lo, b_hi, b_left, b_size,
                         b_pre};
@@ -1260,6 +1460,16 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
         SHK_HIP(hipStreamSynchronize(stream));
         return 0;
     };
+    // Chebyshev nodes of the second kind s_j = cos(j*pi/(n-1)) and
+    // barycentric weights (-1)^j halved at both ends (chebcode.rs:242-261).
+    std::vector<double> tab(2 * n);
+    for (int j = 0; j < n; ++j) {
+        tab[j] = std::cos((double)j * kPi / (double)(n - 1));
+        const double sign = (j % 2 == 0) ? 1.0 : -1.0;
+        tab[n + j] = (j == 0 || j == n - 1) ? 0.5 * sign : sign;
+    }
+    SHK_HIP(hipMemcpyAsync(d_tab, tab.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice,
+                           stream));
     int zero4[4] = {0, 0, 0, 0};
     SHK_HIP(hipMemcpyAsync(d_info, zero4, sizeof(zero4), hipMemcpyHostToDevice, stream));
     SHK_HIP(hipMemcpyAsync(t->sorted, d_ev, sizeof(double) * p, hipMemcpyDeviceToDevice, stream));
@@ -1319,41 +1529,43 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
     t->link = nr.take<int4>(N);
     t->panel32 = nr.take<float>((size_t)N * 2 * n);
 
-    // 3. pre-order arrays and positions.  Chebyshev nodes of the second kind
-    // s_j = cos(j*pi/(n-1)) and barycentric weights (-1)^j halved at both
-    // ends (chebcode.rs:242-261).
-    std::vector<double> tab(2 * n);
-    for (int j = 0; j < n; ++j) {
-        tab[j] = std::cos((double)j * kPi / (double)(n - 1));
-        const double sign = (j % 2 == 0) ? 1.0 : -1.0;
-        tab[n + j] = (j == 0 || j == n - 1) ? 0.5 * sign : sign;
-    }
-    SHK_HIP(hipMemcpyAsync(d_tab, tab.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice,
-                           stream));
+    // 3. node records and mirrors, leaf weights: one wavefront per node
     TreeArrays ta{t->lo, t->hi, t->acc, t->nodes, t->w, t->lo_idx, t->hi_idx, t->right, t->esc};
-    hipLaunchKernelGGL(finalize_kernel, dim3(blocks_for(N, 256)), dim3(256), 0, stream, t->sorted, N,
-                       n, t->theta_sq, d_list, b_lo, b_hi, b_left, b_size, b_pre, d_tab, ta);
-    SHK_HIP(hipGetLastError());
-    // 4. leaves: one wavefront per node
     const size_t lds = bary_lds_bytes(n);
-    hipLaunchKernelGGL(leaf_fill_kernel, dim3(N), dim3(kBaryLanes), lds, stream, t->sorted, n,
-                       d_tab + n, ta);
+    hipLaunchKernelGGL(node_kernel, dim3(N), dim3(kBaryLanes), lds + sizeof(double) * n, stream,
+                       t->sorted, n, t->theta_sq, d_list, b_lo, b_hi, b_left, b_size, b_pre, d_tab,
+                       ta, d_mrec, t->rec32, t->link, t->panel32);
     SHK_HIP(hipGetLastError());
-    // 5. parents, deepest depth first: one launch per depth, one wavefront per node
-    for (int d = t->n_levels - 2; d >= 0; --d) {
-        const int off = info[kInfoDepthOff + d], cnt = info[kInfoDepthCnt + d];
-        if (cnt <= 0) continue;
-        hipLaunchKernelGGL(merge_level_kernel, dim3(cnt), dim3(kBaryLanes), lds, stream, n,
-                           d_list + off, b_left, b_pre, d_tab + n, ta);
+    // 4. parents, deepest first.  Workgroups of several wavefronts keep
+    // their LDS within 64 KiB.
+    const int lds_waves = (int)std::max<size_t>(1, 65536 / lds);
+    const int F = info[kInfoF], D = info[kInfoD];
+    if (F > 0) {
+        const int waves = std::min(lds_waves, kMergeSubWaves);
+        hipLaunchKernelGGL(merge_sub_kernel, dim3(F), dim3(waves * kBaryLanes), waves * lds, stream,
+                           n, lv_start, lv_cnt, d_mrec, d_tab + n, ta, t->panel32);
         SHK_HIP(hipGetLastError());
     }
-    // 6. mirrors
-    hipLaunchKernelGGL(mirror_kernel, dim3(blocks_for(N, 256)), dim3(256), 0, stream, N, n, ta,
-                       t->rec32, t->link, t->panel32);
-    SHK_HIP(hipGetLastError());
-    hipLaunchKernelGGL(split_kernel, dim3(blocks_for(p, 256)), dim3(256), 0, stream, t->sorted, ip,
-                       t->s_hi, t->s_lo);
-    SHK_HIP(hipGetLastError());
+    // Top levels with internal nodes: [0, D) below a frontier, else [0, D-1)
+    // (the last level of a tree that ended in the top walk holds leaves).
+    const int top = (F > 0) ? D : std::max(D - 1, 0);
+    const int *lvl = info.data() + kInfoTopLevel;
+    const int top_waves = std::min(lds_waves, kMergeMaxThreads / kBaryLanes);
+    const int group = merge_group_size(n);
+    auto groups = [&](int d) { return (lvl[d + 1] - lvl[d] + group - 1) / group; };
+    int small = 0;  // leading levels merged together in one workgroup
+    while (small < top && groups(small) <= kMergeTopRounds * top_waves) ++small;
+    for (int d = top - 1; d >= small; --d) {
+        hipLaunchKernelGGL(merge_range_kernel, dim3(groups(d)), dim3(kBaryLanes), lds, stream, n,
+                           lvl[d], lvl[d + 1] - lvl[d], d_mrec, d_tab + n, ta, t->panel32);
+        SHK_HIP(hipGetLastError());
+    }
+    if (small > 0) {
+        hipLaunchKernelGGL(merge_top_kernel, dim3(1), dim3(top_waves * kBaryLanes),
+                           top_waves * lds, stream, n, small, d_info, d_mrec, d_tab + n, ta,
+                           t->panel32);
+        SHK_HIP(hipGetLastError());
+    }
     // Evaluations on any stream wait for this event (cheb_eval*).
     SHK_HIP(hipEventCreateWithFlags(&t->ready, hipEventDisableTiming));
     SHK_HIP(hipEventRecord(t->ready, stream));
