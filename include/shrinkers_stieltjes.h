@@ -337,7 +337,7 @@ shk_status shk_exact_at_points_dev(const double *d_query, size_t nq, const doubl
  * dL/dRe S_m, dL/dIm S_m of a real loss L, writes dL/dlam_j (p values),
  * dL/dx_m (nq values) and dL/deta (one device double); each output pointer
  * may be NULL to skip it.  With d = x_m - lam_j, D = d^2 + eta^2:
- *   dL/dlam_j = sum_m (G_re (d^2 - eta^2) - 2 G_im eta d) / D^2
+ *   dL/dlam_j = sum_m (G_re (d^2 - eta^2) + 2 G_im eta d) / D^2
  *   dS_m/dx   = sum_j ((eta^2 - d^2) - 2 i eta d) / D^2   (autovec.rs:38-59)
  *   dL/dx_m   = G_re Re dS/dx + G_im Im dS/dx
  *   dL/deta   = sum_m G_re Im dS/dx - G_im Re dS/dx       (dS/deta = -i dS/dx)
@@ -348,6 +348,22 @@ shk_status shk_exact_adjoint_dev(const double *d_query, size_t nq, const double 
                                  const double *d_grad_im, int precision,
                                  double *d_grad_eigenvalues, double *d_grad_query,
                                  double *d_grad_eta, void *stream);
+
+/* Adjoint of the all-points transform (raw sums S_i = sum_j
+ * 1/((lam_i - lam_j) - i eta), diagonal included).  Given d_grad_re[i],
+ * d_grad_im[i] = dL/dRe S_i, dL/dIm S_i, writes the total dL/dlam_i (the
+ * eigenvalue as query and as source: the sum of shk_exact_adjoint_dev's
+ * eigenvalue and query gradients at queries = eigenvalues) and dL/deta (one
+ * device double; NULL skips it).  A pair (i, j) adds T to dL/dlam_i and -T to
+ * dL/dlam_j, so for 8192 <= p <= 262144 each unordered pair is evaluated
+ * once (symmetric kernels; the FP32-compensated one for eta >= 1e-9); other
+ * sizes add the two sweeps of shk_exact_adjoint_dev.  Deterministic.  The
+ * reference differentiates with respect to the query only (autovec.rs:38-59)
+ * and has no counterpart. */
+shk_status shk_exact_adjoint_all_points_dev(const double *d_eigenvalues, size_t p, double eta,
+                                            const double *d_grad_re, const double *d_grad_im,
+                                            int precision, double *d_grad_eigenvalues,
+                                            double *d_grad_eta, void *stream);
 
 /* El Karoui bulk fit, FP64.  The residual of the Marchenko-Pastur equation
  * as written in src/deconvolution/mod.rs:6 for a discrete population

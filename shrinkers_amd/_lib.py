@@ -79,6 +79,7 @@ SIGNATURES = {
     "shk_batch_destroy": (_i, [_vp]),
     "shk_exact_at_points_dev": (_i, [_vp, _sz, _vp, _sz, _d, _i, _vp, _vp, _vp]),
     "shk_exact_adjoint_dev": (_i, [_vp, _sz, _vp, _sz, _d, _vp, _vp, _i, _vp, _vp, _vp, _vp]),
+    "shk_exact_adjoint_all_points_dev": (_i, [_vp, _sz, _d, _vp, _vp, _i, _vp, _vp, _vp]),
     "shk_el_karoui_fit_dev": (_i, [_vp, _vp, _vp, _sz, _d, _d, _vp, _vp, _sz, _vp, _vp, _vp,
                                    _vp, _vp, _vp]),
     "shk_el_karoui_fit_step_dev": (_i, [_vp, _sz, _vp, _sz, _d, _d, _vp, _vp, _sz, _i, _vp,

@@ -21,7 +21,9 @@ import numpy as np
 
 from . import _lib
 from ._lib import check
-from .stieltjes import _PRECISIONS, _stream_handle
+from .stieltjes import _PRECISIONS, _stream_handle, parse_method
+
+_BLOCKED = parse_method("blocked")  # the exact all-points family
 
 
 def _need_cuda_f64(t, name):
@@ -88,6 +90,47 @@ def _function_classes():
                 ge = ge.reshape(ctx.eta_shape)
             return gq, gev, ge, None
 
+    class StieltjesAllPoints(torch.autograd.Function):
+        """Raw all-points sums S_i = sum_j 1/((lam_i - lam_j) - i eta) ->
+        (Re S, Im S); the backward gives the total eigenvalue gradient (query
+        and source roles) in one call."""
+
+        @staticmethod
+        def forward(ctx, eigenvalues, eta, precision):
+            ev = eigenvalues.detach().contiguous()
+            e = _eta_value(eta)
+            prec = _prec(precision)
+            re = torch.empty_like(ev)
+            im = torch.empty_like(ev)
+            check(_lib.load().shk_all_points_dev(ev.data_ptr(), ev.numel(), e, _BLOCKED, 0.0,
+                                                 prec, 0, re.data_ptr(), im.data_ptr(),
+                                                 _stream_handle(None)))
+            ctx.save_for_backward(ev)
+            ctx.eta = e
+            ctx.prec = prec
+            ctx.eta_tensor = torch.is_tensor(eta)
+            ctx.eta_shape = eta.shape if torch.is_tensor(eta) else None
+            return re, im
+
+        @staticmethod
+        def backward(ctx, g_re, g_im):
+            (ev,) = ctx.saved_tensors
+            need_ev, need_eta = ctx.needs_input_grad[:2]
+            need_eta = need_eta and ctx.eta_tensor
+            if not (need_ev or need_eta):
+                return None, None, None
+            g_re = torch.zeros_like(ev) if g_re is None else g_re.contiguous()
+            g_im = torch.zeros_like(ev) if g_im is None else g_im.contiguous()
+            gev = torch.empty_like(ev) if need_ev else None
+            ge = torch.empty((), dtype=torch.float64, device=ev.device) if need_eta else None
+            ptr = lambda t: t.data_ptr() if t is not None else None
+            check(_lib.load().shk_exact_adjoint_all_points_dev(
+                ev.data_ptr(), ev.numel(), ctx.eta, g_re.data_ptr(), g_im.data_ptr(), ctx.prec,
+                ptr(gev), ptr(ge), _stream_handle(None)))
+            if ge is not None:
+                ge = ge.reshape(ctx.eta_shape)
+            return gev, ge, None
+
     class ElKaroui(torch.autograd.Function):
         """Mean squared residual of the MP equation (FP64 kernel)."""
 
@@ -122,7 +165,7 @@ def _function_classes():
             return (gl * ggr if ng[0] else None, gl * ggi if ng[1] else None, None, out_ge,
                     None, None, gl * gw if ng[6] else None)
 
-    return StieltjesAtPoints, ElKaroui
+    return StieltjesAtPoints, ElKaroui, StieltjesAllPoints
 
 
 _CLASSES = None
@@ -146,8 +189,11 @@ def stieltjes_at_points(queries, eigenvalues, eta, precision: str = "f32c"):
 
 def stieltjes_all_points(eigenvalues, eta, precision: str = "f32c", scaled: bool = True):
     """All-points transform (queries = eigenvalues, diagonal included),
-    differentiable; scaled by 1/p when ``scaled``."""
-    re, im = stieltjes_at_points(eigenvalues, eigenvalues, eta, precision)
+    differentiable with respect to the eigenvalues and ``eta``; scaled by 1/p
+    when ``scaled``.  The backward is one call of the all-points adjoint
+    (each unordered pair once where the size allows)."""
+    _need_cuda_f64(eigenvalues, "eigenvalues")
+    re, im = _classes()[2].apply(eigenvalues, eta, precision)
     if scaled:
         p = eigenvalues.numel()
         return re / p, im / p

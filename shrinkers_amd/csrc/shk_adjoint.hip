@@ -7,7 +7,7 @@
 // (G_re[m], G_im[m]) = (dL/dRe S_m, dL/dIm S_m).  With d = x_m - lam_j and
 // D = d^2 + eta^2:
 //
-//   grad_lam[j] = sum_m (G_re (d^2 - eta^2) - 2 G_im eta d) / D^2        (A)
+//   grad_lam[j] = sum_m (G_re (d^2 - eta^2) + 2 G_im eta d) / D^2        (A)
 //   dS_m/dx     = sum_j ((eta^2 - d^2) - i 2 eta d) / D^2                 (B)
 //   grad_x[m]   = G_re Re dS/dx + G_im Im dS/dx
 //   grad_eta    = sum_m G_re Im dS/dx - G_im Re dS/dx     (dS/deta = -i dS/dx)
@@ -617,7 +617,32 @@ __global__ void zero_kernel(double *__restrict__ p, size_t n) {
     if (i < n) p[i] = 0.0;
 }
 
+__global__ void add_kernel(double *__restrict__ out, const double *__restrict__ in, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += in[i];
+}
+
 }  // namespace
+
+int add_into(double *out, const double *in, size_t n, hipStream_t stream) {
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(add_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, out,
+                       in, n);
+    SHK_HIP(hipGetLastError());
+    return 0;
+}
+
+int adjoint_ordered_sum(const double *v, size_t n, double *out, int accumulate,
+                        hipStream_t stream) {
+    if (n > (size_t)INT32_MAX) {
+        set_error("ordered sum: n above 2^31-1");
+        return 6;
+    }
+    hipLaunchKernelGGL(ordered_sum_kernel, dim3(1), dim3(kSumThreads), 0, stream, v, (int)n, 1.0,
+                       out, accumulate);
+    SHK_HIP(hipGetLastError());
+    return 0;
+}
 
 int zero_fill(double *p, size_t n, hipStream_t stream) {
     if (n == 0) return 0;

@@ -950,6 +950,40 @@ shk_status shk_exact_adjoint_dev(const double *d_query, size_t nq, const double 
                                      d_grad_eta, (hipStream_t)stream);
 }
 
+shk_status shk_exact_adjoint_all_points_dev(const double *d_eigenvalues, size_t p, double eta,
+                                            const double *d_grad_re, const double *d_grad_im,
+                                            int precision, double *d_grad_eigenvalues,
+                                            double *d_grad_eta, void *stream) {
+    if (!d_grad_eigenvalues && !d_grad_eta) return SHK_OK;
+    if (p == 0) {
+        SHK_CHECK(ensure_device());
+        if (d_grad_eta) SHK_CHECK(zero_fill(d_grad_eta, 1, (hipStream_t)stream));
+        return SHK_OK;
+    }
+    if (!d_eigenvalues || !d_grad_re || !d_grad_im) return fail(SHK_ERR_BAD_ARG, "NULL pointer");
+    SHK_CHECK(check_eta(eta));
+    if (precision != SHK_PREC_F64 && precision != SHK_PREC_F32C)
+        return fail(SHK_ERR_BAD_ARG, "precision must be SHK_PREC_F64 or SHK_PREC_F32C");
+    SHK_CHECK(ensure_device());
+    hipStream_t s = (hipStream_t)stream;
+    const bool f64 = precision == SHK_PREC_F64;
+    const bool sym = d_grad_eigenvalues && p >= (f64 ? kSymAdj64MinP : kSymAdjMinP) &&
+                     p <= kSymMaxP && (f64 || eta >= kSymAdjMinEta);
+    if (sym)
+        return (shk_status)exact_adjoint_sym(d_eigenvalues, p, eta, d_grad_re, d_grad_im, f64,
+                                             d_grad_eigenvalues, d_grad_eta, s);
+    if (!d_grad_eigenvalues)  // the eta gradient alone: the query sweep
+        return (shk_status)exact_adjoint(d_eigenvalues, p, d_eigenvalues, p, eta, d_grad_re,
+                                         d_grad_im, !f64, nullptr, nullptr, d_grad_eta, s);
+    // two sweeps: the eigenvalue gradient, plus the query gradient
+    double *tmp = nullptr;
+    CachedBuf cache(CACHE_ADJ_TMP, s);
+    if (int e = cache.get(p, &tmp)) return (shk_status)e;
+    SHK_CHECK(exact_adjoint(d_eigenvalues, p, d_eigenvalues, p, eta, d_grad_re, d_grad_im, !f64,
+                            d_grad_eigenvalues, tmp, d_grad_eta, s));
+    return (shk_status)add_into(d_grad_eigenvalues, tmp, p, s);
+}
+
 shk_status shk_el_karoui_fit_dev(const double *d_x, const double *d_g_re, const double *d_g_im,
                                  size_t m, double eta, double c, const double *d_atoms,
                                  const double *d_weights, size_t k, double *d_loss,

@@ -32,7 +32,8 @@ enum CacheSlot : int {
     CACHE_ADJ_PART = 4,      // adjoint sweeps: FP64 partial sums
     CACHE_EK_SCR = 5,        // El Karoui fit kernels: per-point scratch
     CACHE_EK_STEP = 6,       // fused fit step: sample transform and its gradient
-    kCacheSlots = 7
+    CACHE_ADJ_TMP = 7,       // all-points adjoint by two sweeps: the query gradient
+    kCacheSlots = 8
 };
 // Returns the slot's buffer (>= bytes) in *out, ordered on stream s.
 int cache_acquire(int slot, size_t bytes, hipStream_t s, void **out);
@@ -188,6 +189,38 @@ int widen_f32(const float *in, double *out, size_t n, hipStream_t stream);
 int exact_adjoint(const double *q, size_t nq, const double *ev, size_t p, double eta,
                   const double *gre, const double *gim, int f32c, double *grad_ev,
                   double *grad_q, double *grad_eta, hipStream_t stream, int accumulate_eta = 0);
+
+// Gradient of a real loss of the raw all-points sums S_i = sum_j
+// 1/((lam_i - lam_j) - i eta) with respect to every eigenvalue (query and
+// source roles together) into grad (p), and with respect to eta into
+// *grad_eta (may be NULL; accumulate_eta != 0 adds to it).  Symmetric
+// kernels (each unordered pair once; shk_symadj.hip), p <= kSymMaxP; f64 != 0
+// selects FP64, else FP32-compensated.
+int exact_adjoint_sym(const double *ev, size_t p, double eta, const double *gre,
+                      const double *gim, int f64, double *grad, double *grad_eta,
+                      hipStream_t stream, int accumulate_eta = 0);
+
+// Smallest p for which the all-points adjoint entry uses the symmetric
+// kernels (below, the two sweeps of exact_adjoint are added); the FP32-
+// compensated kernel also needs eta >= kSymAdjMinEta, so that r^2 = 1/D^2
+// stays within f32 range.
+#ifndef SHK_SYMADJ_MIN_P
+#define SHK_SYMADJ_MIN_P 8192
+#endif
+#ifndef SHK_SYMADJ64_MIN_P
+#define SHK_SYMADJ64_MIN_P 8192
+#endif
+constexpr size_t kSymAdjMinP = SHK_SYMADJ_MIN_P;
+constexpr size_t kSymAdj64MinP = SHK_SYMADJ64_MIN_P;
+constexpr double kSymAdjMinEta = 1.0e-9;
+
+// *out = sum(v[0, n)) in a fixed order (one workgroup); accumulate != 0 adds
+// to *out.
+int adjoint_ordered_sum(const double *v, size_t n, double *out, int accumulate,
+                        hipStream_t stream);
+
+// out[i] += in[i] (one kernel); n = 0 is a no-op.
+int add_into(double *out, const double *in, size_t n, hipStream_t stream);
 
 // Writes n zeros with a kernel (graph-capturable); n = 0 is a no-op.
 int zero_fill(double *p, size_t n, hipStream_t stream);

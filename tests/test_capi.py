@@ -143,3 +143,18 @@ def test_many_argument_checks(shk):
                                        None) == _lib.SHK_ERR_UNSUPPORTED   # fft5
     assert lib.shk_all_points_many_dev(None, None, 0, None, 2, 1, 1, None, None, 2, None,
                                        None) == _lib.SHK_OK
+
+
+def test_all_points_adjoint_argument_checks(shk):
+    """shk_exact_adjoint_all_points_dev rejects bad arguments before any
+    device work; with no output requested it does nothing."""
+    from shrinkers_amd import _lib
+    lib = _lib.load()
+    dummy = ctypes.c_void_p(16)  # never dereferenced: the checks fail first
+    f = lib.shk_exact_adjoint_all_points_dev
+    assert f(dummy, 5, 0.1, dummy, dummy, _lib.PREC_F64, None, None, None) == _lib.SHK_OK
+    assert f(None, 5, 0.1, dummy, dummy, _lib.PREC_F64, dummy, None, None) == _lib.SHK_ERR_BAD_ARG
+    assert f(dummy, 5, 0.1, None, dummy, _lib.PREC_F64, dummy, None, None) == _lib.SHK_ERR_BAD_ARG
+    assert f(dummy, 5, -0.1, dummy, dummy, _lib.PREC_F64, dummy, None, None) == _lib.SHK_ERR_BAD_ETA
+    assert f(dummy, 5, 0.1, dummy, dummy, _lib.PREC_F32, dummy, None, None) == _lib.SHK_ERR_BAD_ARG
+    assert "precision" in _lib.last_error()

@@ -111,6 +111,142 @@ def test_adjoint_edge_shapes(shk, orc, precision, p, nq, eta):
     assert abs(ge - re_) <= tol * abs(re_)
 
 
+def _all_points_adjoint(ev, eta, gre, gim, precision, want_grad=True, want_eta=True):
+    """shk_exact_adjoint_all_points_dev through the C ABI."""
+    import torch
+    from shrinkers_amd import _lib
+    from shrinkers_amd.stieltjes import _PRECISIONS, _stream_handle
+    tev, tre, tim = _t(ev), _t(gre), _t(gim)
+    gev = torch.full_like(tev, np.nan) if want_grad else None
+    ge = torch.full((), np.nan, dtype=torch.float64, device="cuda") if want_eta else None
+    _lib.check(_lib.load().shk_exact_adjoint_all_points_dev(
+        tev.data_ptr(), ev.size, float(eta), tre.data_ptr(), tim.data_ptr(),
+        _PRECISIONS[precision], gev.data_ptr() if want_grad else None,
+        ge.data_ptr() if want_eta else None, _stream_handle(None)))
+    torch.cuda.synchronize()
+    return (gev.cpu().numpy() if want_grad else None,
+            float(ge.cpu()) if want_eta else None)
+
+
+def _two_sweeps(ev, eta, gre, gim, precision):
+    """shk_exact_adjoint_dev at queries = eigenvalues: (dL/dlam as source,
+    dL/dx as query, dL/deta)."""
+    import torch
+    from shrinkers_amd import _lib
+    from shrinkers_amd.stieltjes import _PRECISIONS, _stream_handle
+    tev, tre, tim = _t(ev), _t(gre), _t(gim)
+    gev, gq = torch.empty_like(tev), torch.empty_like(tev)
+    ge = torch.empty((), dtype=torch.float64, device="cuda")
+    _lib.check(_lib.load().shk_exact_adjoint_dev(
+        tev.data_ptr(), ev.size, tev.data_ptr(), ev.size, float(eta), tre.data_ptr(),
+        tim.data_ptr(), _PRECISIONS[precision], gev.data_ptr(), gq.data_ptr(), ge.data_ptr(),
+        _stream_handle(None)))
+    torch.cuda.synchronize()
+    return gev.cpu().numpy(), gq.cpu().numpy(), float(ge.cpu())
+
+
+def _hard_spectrum(case, p):
+    from shrinkers_amd import spectra
+    rng = np.random.default_rng(p)
+    if case == "mp_spikes":
+        return spectra.mp_inverse_cdf_spectrum(p, c=0.3, seed=4)  # outliers 12, 7, 4
+    if case == "duplicates":
+        return np.repeat(np.linspace(0.5, 2.0, p // 9), 9)
+    if case == "unsorted":
+        return rng.permutation(spectra.mp_inverse_cdf_spectrum(p, c=0.5, seed=5))
+    raise ValueError(case)
+
+
+@pytest.mark.parametrize("precision,tol", [("f64", 1e-12), ("f32c", 5e-6)])
+@pytest.mark.parametrize("p", [8192, 8193, 9001])
+def test_symmetric_all_points_adjoint_matches_oracle(shk, orc, precision, tol, p):
+    """Each unordered pair once (p >= 8192): the total eigenvalue gradient
+    (query and source roles) and the eta gradient against the oracle's two
+    FP64 sweeps."""
+    ev = _hard_spectrum("mp_spikes", p)
+    eta = 1.0 / np.sqrt(p)
+    rng = np.random.default_rng(p + 11)
+    gre, gim = rng.standard_normal(p), rng.standard_normal(p)
+    gev, ge = _all_points_adjoint(ev, eta, gre, gim, precision)
+    rev, rq, re_ = orc.adjoint_at_points(ev, ev, eta, gre, gim)
+    e_ev, e_eta = _rel(gev, rev + rq), abs(ge - re_) / abs(re_)
+    print(f"symmetric adjoint {precision} p={p}: lam {e_ev:.3e} eta {e_eta:.3e}", flush=True)
+    assert e_ev <= tol and e_eta <= tol
+
+
+@pytest.mark.parametrize("precision,tol", [("f64", 1e-12), ("f32c", 5e-6)])
+@pytest.mark.parametrize("case", ["duplicates", "unsorted"])
+def test_symmetric_all_points_adjoint_hard_inputs(shk, orc, precision, tol, case):
+    """Ties (pairs at d = 0 off the diagonal) and unsorted input."""
+    ev = _hard_spectrum(case, 9216)
+    eta = 0.5 / np.sqrt(ev.size)
+    rng = np.random.default_rng(3)
+    gre, gim = rng.standard_normal(ev.size), rng.standard_normal(ev.size)
+    gev, ge = _all_points_adjoint(ev, eta, gre, gim, precision)
+    rev, rq, re_ = orc.adjoint_at_points(ev, ev, eta, gre, gim)
+    e_ev, e_eta = _rel(gev, rev + rq), abs(ge - re_) / abs(re_)
+    print(f"symmetric adjoint {precision} {case}: lam {e_ev:.3e} eta {e_eta:.3e}", flush=True)
+    assert e_ev <= tol and e_eta <= tol
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32c"])
+def test_symmetric_all_points_adjoint_large(shk, precision):
+    """p = 50 000 (the transform's headline size): against the two GPU FP64
+    sweeps; deterministic; each output alone equals the joint call."""
+    ev = _hard_spectrum("mp_spikes", 50_000)
+    eta = 1.0 / np.sqrt(ev.size)
+    rng = np.random.default_rng(9)
+    gre, gim = rng.standard_normal(ev.size), rng.standard_normal(ev.size)
+    gev, ge = _all_points_adjoint(ev, eta, gre, gim, precision)
+    rev, rq, re_ = _two_sweeps(ev, eta, gre, gim, "f64")
+    tol = 1e-12 if precision == "f64" else 5e-6
+    e_ev, e_eta = _rel(gev, rev + rq), abs(ge - re_) / abs(re_)
+    print(f"symmetric adjoint {precision} p=50000: lam {e_ev:.3e} eta {e_eta:.3e}", flush=True)
+    assert e_ev <= tol and e_eta <= tol
+    gev2, ge2 = _all_points_adjoint(ev, eta, gre, gim, precision)
+    assert np.array_equal(gev, gev2) and ge == ge2
+    only_grad, _ = _all_points_adjoint(ev, eta, gre, gim, precision, want_eta=False)
+    assert np.array_equal(only_grad, gev)
+    _, only_eta = _all_points_adjoint(ev, eta, gre, gim, precision, want_grad=False)
+    assert abs(only_eta - ge) <= tol * abs(ge)   # the eta-only call runs the query sweep
+
+
+@pytest.mark.parametrize("precision,p,eta", [("f64", 3000, 0.02), ("f32c", 3000, 0.02),
+                                             ("f32c", 8192, 1e-10)])
+def test_all_points_adjoint_two_sweep_sizes(shk, precision, p, eta):
+    """Below the symmetric range, and FP32-compensated with eta < 1e-9, the
+    entry adds the two sweeps: bit for bit their sum."""
+    ev = _hard_spectrum("mp_spikes", p)
+    rng = np.random.default_rng(p)
+    gre, gim = rng.standard_normal(p), rng.standard_normal(p)
+    gev, ge = _all_points_adjoint(ev, eta, gre, gim, precision)
+    rev, rq, re_ = _two_sweeps(ev, eta, gre, gim, precision)
+    assert np.array_equal(gev, rev + rq)
+    assert ge == re_
+
+
+def test_all_points_autograd_symmetric(shk, orc):
+    """stieltjes_all_points (forward and backward through the all-points
+    entries) at p = 8200 in FP64, scaled: eigenvalue and eta gradients
+    against the oracle."""
+    import torch
+    from shrinkers_amd import autograd as ag
+    p = 8200
+    ev = _hard_spectrum("mp_spikes", p)
+    eta = 1.0 / np.sqrt(p)
+    rng = np.random.default_rng(21)
+    gre, gim = rng.standard_normal(p), rng.standard_normal(p)
+    tev, te = _t(ev, True), _t(np.float64(eta).reshape(()), True)
+    re, im = ag.stieltjes_all_points(tev, te, "f64", scaled=True)
+    ((re * _t(gre)).sum() + (im * _t(gim)).sum()).backward()
+    torch.cuda.synchronize()
+    rr, ri = orc.at_points(ev, ev, eta)
+    assert rel_l2(re.detach().cpu().numpy(), im.detach().cpu().numpy(), rr / p, ri / p) <= 1e-13
+    rev, rq, re_ = orc.adjoint_at_points(ev, ev, eta, gre / p, gim / p)
+    assert _rel(tev.grad.cpu().numpy(), rev + rq) <= 1e-12
+    assert abs(float(te.grad.cpu()) - re_) <= 1e-12 * abs(re_)
+
+
 def _fit_inputs(seed=0):
     from shrinkers_amd import autograd as ag
     ev, grid, atoms, eta = ag.bulk_fit_problem()
