@@ -58,7 +58,6 @@ struct ChebTree {
     double theta = 0, theta_sq = 0;
     int64_t leaf_cap = 0;
     int f32_far = 0;
-    int n_levels = 0;
     double *sorted = nullptr;  // p ascending
     int32_t *perm = nullptr;   // p, or null when the input was ascending
     float *s_hi = nullptr, *s_lo = nullptr;
@@ -199,11 +198,11 @@ __global__ void hint_scatter_kernel(const double *__restrict__ sorted, int p, in
 //      2(b - a) - 1 nodes, and the frontier ranges are disjoint), then the
 //      subtree sizes bottom-up.
 //   C  one workgroup: sizes of the top levels, pre-order numbers of the top
-//      levels and the frontier roots, and the per-depth node lists (the top
-//      levels in BFS order, then for each deeper depth the subtrees' levels
-//      in frontier order).
+//      levels and the frontier roots, and each subtree's block of the node
+//      list (the list names every node slot once: the top slots, then the
+//      subtrees' nodes below their roots, subtree by subtree).
 //   D  one workgroup per frontier node: pre-order numbers inside its subtree
-//      and its nodes' places in the depth lists.
+//      and its nodes' entries of the node list.
 // Every split is the reference's (midpoint value split with index-midpoint
 // fallback, leaf if count <= leaf_cap or depth >= 48, chebcode.rs:347-388).
 
@@ -214,11 +213,9 @@ constexpr int kMaxFrontier = 2 * kFrontier;  // the frontier level is smaller
 constexpr int kLv = kMaxDepth + 2;           // levels per subtree (root level 0)
 
 // info[] layout (ints), copied to the host after the topology.
-constexpr int kInfoN = 0, kInfoLevels = 1, kInfoOverflow = 2, kInfoUnsorted = 3;
+constexpr int kInfoN = 0, kInfoOverflow = 2, kInfoUnsorted = 3;
 constexpr int kInfoT = 4, kInfoF = 5, kInfoD = 6, kInfoFront = 7;
-constexpr int kInfoDepthOff = 8;                          // kLv + 1 entries
-constexpr int kInfoDepthCnt = kInfoDepthOff + kLv + 1;    // kLv + 1 entries
-constexpr int kInfoTopLevel = kInfoDepthCnt + kLv + 1;    // kMaxDepth + 3 entries
+constexpr int kInfoTopLevel = 8;                          // kMaxDepth + 3 entries
 constexpr int kInfoInts = kInfoTopLevel + kMaxDepth + 3;
 
 struct TopoCtx {
@@ -459,7 +456,7 @@ __global__ __launch_bounds__(kSubThreads) void topo_sub_kernel(TopoCtx c,
     }
 }
 
-// C: top sizes and pre-order numbers, then the per-depth node lists.
+// C: top sizes and pre-order numbers, then the node list's blocks.
 __global__ __launch_bounds__(kTopoThreads) void topo_mid_kernel(
     TopoCtx c, int *__restrict__ info, const int *__restrict__ lv_cnt, int *__restrict__ lv_off,
     int *__restrict__ list) {
@@ -480,33 +477,21 @@ __global__ __launch_bounds__(kTopoThreads) void topo_mid_kernel(
         __syncthreads();
     }
     for (int i = tid; i < T; i += kTopoThreads) list[i] = i;
-    // depth lists: top depths are BFS ranges; deeper ones are filled by D
-    const int top_levels = (F > 0) ? D + 1 : D;
-    for (int d = tid; d < top_levels; d += kTopoThreads) {
-        info[kInfoDepthOff + d] = lvl[d];
-        info[kInfoDepthCnt + d] = lvl[d + 1] - lvl[d];
+    // subtree f's nodes below its root take list[lv_off[f], + count): one
+    // scan over the subtrees' node counts (their level counts past level 0;
+    // B zeroes the counts past a subtree's last level)
+    int mine = 0;
+    if (tid < F) {
+#pragma unroll 8
+        for (int l = 1; l < kLv; ++l) mine += lv_cnt[tid * kLv + l];
     }
-    int n = T, levels = top_levels;
-    for (int l = 1; F > 0 && l < kLv && D + l <= kMaxDepth; ++l) {
-        const int mine = (tid < F) ? lv_cnt[tid * kLv + l] : 0;
-        int tot;
-        const int off = block_exclusive_scan<kTopoThreads>(mine, s_wave, &tot);
-        if (tot == 0) break;
-        if (tid < F) lv_off[tid * kLv + l] = n + off;
-        if (tid == 0) {
-            info[kInfoDepthOff + D + l] = n;
-            info[kInfoDepthCnt + D + l] = tot;
-        }
-        n += tot;
-        levels = D + l + 1;
-    }
-    if (tid == 0) {
-        info[kInfoN] = n;
-        info[kInfoLevels] = levels;
-    }
+    int tot;
+    const int off = block_exclusive_scan<kTopoThreads>(mine, s_wave, &tot);
+    if (tid < F) lv_off[tid * kLv] = T + off;
+    if (tid == 0) info[kInfoN] = T + tot;
 }
 
-// D: pre-order numbers inside the subtree and its entries of the depth lists.
+// D: pre-order numbers inside the subtree and its entries of the node list.
 __global__ __launch_bounds__(kSubThreads) void topo_fin_kernel(TopoCtx c,
                                                                const int *__restrict__ info,
                                                                const int *__restrict__ lv_start,
@@ -520,9 +505,11 @@ __global__ __launch_bounds__(kSubThreads) void topo_fin_kernel(TopoCtx c,
         preorder_level<kSubThreads>(c, lb, lb + lv_cnt[f * kLv + l]);
         __syncthreads();
     }
+    int o = lv_off[f * kLv];
     for (int l = 1; l < kLv && lv_cnt[f * kLv + l] > 0; ++l) {
-        const int lb = lv_start[f * kLv + l], n = lv_cnt[f * kLv + l], o = lv_off[f * kLv + l];
+        const int lb = lv_start[f * kLv + l], n = lv_cnt[f * kLv + l];
         for (int i = threadIdx.x; i < n; i += kSubThreads) list[o + i] = lb + i;
+        o += n;
     }
 }
 
@@ -1364,7 +1351,7 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
     const int cap_top = (int)std::min<size_t>(2 * p - 1, (size_t)(kMaxDepth + 3) * kFrontier);
     const size_t slots = (size_t)cap_top + 2 * p;
 
-    // Scratch: info words, slot arrays, depth lists, subtree levels, node/
+    // Scratch: info words, slot arrays, node list, subtree levels, node/
     // weight tables, sort temporaries.
     const int n_info = kInfoInts;
     Carver sc;
@@ -1493,7 +1480,6 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
     }
     const int N = info[kInfoN];
     t->N = (size_t)N;
-    t->n_levels = info[kInfoLevels];
 
     // Arena part 2: node records and panels.
     Carver nr;
