@@ -154,22 +154,45 @@ __device__ __forceinline__ int hint_bucket(double v, const Hint &h) {
     return (f > 0.0) ? (int)f : 0;
 }
 
-__global__ void hint_clear_kernel(int *__restrict__ bstart, int n) {
+// gap table: gap[0] = number of long gaps recorded, then kMaxGaps entries
+// int4 {first bucket, last bucket, index of the first source after the gap}
+// from gap + 4.
+constexpr int kMaxGaps = 64;
+constexpr int kGapInts = 4 + 4 * kMaxGaps;
+
+__global__ void hint_clear_kernel(int *__restrict__ bstart, int n, int *__restrict__ gap) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) bstart[i] = -1;
+    if (i == 0) gap[0] = 0;
 }
 
 // bstart[0, R]: element i fills the buckets (bucket(x[i-1]), bucket(x[i])]
 // with i, the last element also (bucket(x[p-1]), R] with p.  Runs longer
-// than kHintGap buckets (gaps in the spectrum) stay -1 and the search falls
-// back to the node's own range there.
+// than kHintGap buckets (gaps in the spectrum) stay -1 and go to the gap
+// table instead (first bucket, last bucket, i): a midpoint in a gap splits
+// at i, since every source before the gap is below it and every source from
+// i on above it.  Gaps past the table's capacity fall back to a search of
+// the node's own range.
 constexpr int kHintGap = 64;
+// Buckets per source.  Measured on MI355X (FAST build + evaluate, ms, at
+// p = 5e4 / 2e5 / 1e6): 1 per source 0.271 / 0.295 / 0.439, 2: 0.270 /
+// 0.290 / 0.435, 4: 0.268 / 0.290 / 0.437, 8: 0.266 / 0.290 / 0.457.
+#ifndef SHK_HINT_PER_SOURCE
+#define SHK_HINT_PER_SOURCE 2
+#endif
+constexpr int kHintPerSource = SHK_HINT_PER_SOURCE;
+constexpr int64_t kHintMaxBuckets = int64_t(1) << 26;  // past p = 2^25, one per source
+
+__device__ __forceinline__ void record_gap(int *__restrict__ gap, int lo, int hi, int idx) {
+    const int g = atomicAdd(gap, 1);
+    if (g < kMaxGaps) reinterpret_cast<int4 *>(gap + 4)[g] = make_int4(lo, hi, idx, 0);
+}
 
 // The same pass writes the f32 hi/lo split of each source for the FAST near
 // field (hi = f32(x), lo = f32(x - hi)).
 __global__ void hint_scatter_kernel(const double *__restrict__ sorted, int p, int R,
-                                    int *__restrict__ bstart, float *__restrict__ s_hi,
-                                    float *__restrict__ s_lo) {
+                                    int *__restrict__ bstart, int *__restrict__ gap,
+                                    float *__restrict__ s_hi, float *__restrict__ s_lo) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p) return;
     const double x = sorted[i];
@@ -181,8 +204,14 @@ __global__ void hint_scatter_kernel(const double *__restrict__ sorted, int p, in
     const int bp = (i == 0) ? -1 : hint_bucket(sorted[i - 1], h);
     if (bi - bp <= kHintGap)
         for (int k = bp + 1; k <= bi; ++k) bstart[k] = i;
-    if (i == p - 1 && R - bi <= kHintGap)
-        for (int k = bi + 1; k <= R; ++k) bstart[k] = p;
+    else
+        record_gap(gap, bp + 1, bi, i);
+    if (i == p - 1) {
+        if (R - bi <= kHintGap)
+            for (int k = bi + 1; k <= R; ++k) bstart[k] = p;
+        else
+            record_gap(gap, bi + 1, R, p);
+    }
 }
 
 // ---- topology ---------------------------------------------------------------
@@ -224,6 +253,8 @@ struct TopoCtx {
     const int *bstart;
     int R;   // hint buckets (bstart holds R + 1 entries)
     Hint h;  // set on the device at kernel entry (hint_of)
+    const int *gap;  // gap table (hint_scatter_kernel)
+    int ng;          // its usable entries, set on the device at kernel entry
     int *b_lo, *b_hi, *b_left, *b_size, *b_pre;
 };
 
@@ -260,12 +291,25 @@ __device__ int expand_level(const TopoCtx &c, int lb, int le, int depth, int all
         }
 #pragma unroll
         for (int k = 0; k < kTopoK; ++k) mid[k] = xa[k] + 0.5 * (xb[k] - xa[k]);
-        int hl[kTopoK], hr[kTopoK];
+        int hl[kTopoK], hr[kTopoK], kb[kTopoK];
+        bool gap_lookup = false;
 #pragma unroll
         for (int k = 0; k < kTopoK; ++k) {
-            const int kb = hint_bucket(mid[k], c.h);  // in [0, R-1]
-            hl[k] = c.bstart[kb];
-            hr[k] = c.bstart[kb + 1];
+            kb[k] = hint_bucket(mid[k], c.h);  // in [0, R-1]
+            hl[k] = c.bstart[kb[k]];
+            hr[k] = c.bstart[kb[k] + 1];
+            gap_lookup |= need[k] && (hl[k] < 0 || hr[k] < 0);
+        }
+        if (gap_lookup) {  // a bucket in a long gap: the gap table gives its split
+            const int4 *gt = reinterpret_cast<const int4 *>(c.gap + 4);
+            for (int g = 0; g < c.ng; ++g) {
+                const int4 e = gt[g];
+#pragma unroll
+                for (int k = 0; k < kTopoK; ++k) {
+                    if (hl[k] < 0 && e.x <= kb[k] && kb[k] <= e.y) hl[k] = e.z;
+                    if (hr[k] < 0 && e.x <= kb[k] + 1 && kb[k] + 1 <= e.y) hr[k] = e.z;
+                }
+            }
         }
 #pragma unroll
         for (int k = 0; k < kTopoK; ++k) {
@@ -385,6 +429,7 @@ __global__ __launch_bounds__(kTopoThreads) void topo_top_kernel(TopoCtx c, int c
     __shared__ int s_wave[kTopoThreads / 64];
     const int tid = threadIdx.x;
     c.h = hint_of(c.sorted, c.p, c.R);
+    c.ng = min(c.gap[0], kMaxGaps);
     if (tid == 0) {
         c.b_lo[0] = 0;
         c.b_hi[0] = c.p;
@@ -415,12 +460,14 @@ __global__ __launch_bounds__(kTopoThreads) void topo_top_kernel(TopoCtx c, int c
 __global__ __launch_bounds__(kSubThreads) void topo_sub_kernel(TopoCtx c,
                                                                const int *__restrict__ info,
                                                                int *__restrict__ lv_start,
-                                                               int *__restrict__ lv_cnt) {
+                                                               int *__restrict__ lv_cnt,
+                                                               int *__restrict__ sub_nodes) {
     __shared__ int s_wave[kSubThreads / 64];
     __shared__ int s_lb[kLv], s_le[kLv];
     const int f = blockIdx.x;
     if (f >= info[kInfoF]) return;
     c.h = hint_of(c.sorted, c.p, c.R);
+    c.ng = min(c.gap[0], kMaxGaps);
     const int T = info[kInfoT], D = info[kInfoD];
     const int v = info[kInfoFront] + f;
     const int a = c.b_lo[v], b = c.b_hi[v];
@@ -454,37 +501,71 @@ __global__ __launch_bounds__(kSubThreads) void topo_sub_kernel(TopoCtx c,
         lv_start[f * kLv + l] = (l < n_lv) ? s_lb[l] : 0;
         lv_cnt[f * kLv + l] = (l < n_lv) ? s_le[l] - s_lb[l] : 0;
     }
+    // the levels below the root fill slots [s_lb[1], s_le[n_lv - 1]) contiguously
+    if (threadIdx.x == 0) sub_nodes[f] = (n_lv > 1) ? s_le[n_lv - 1] - s_lb[1] : 0;
 }
 
-// C: top sizes and pre-order numbers, then the node list's blocks.
+// C: top sizes and pre-order numbers, then the node list's blocks.  With at
+// most kMidLds top slots both sweeps run in LDS (a top slot's children are
+// top slots; the frontier roots' sizes come from B), otherwise on the slot
+// arrays in global memory.
+constexpr int kMidLds = 4096;
+
 __global__ __launch_bounds__(kTopoThreads) void topo_mid_kernel(
-    TopoCtx c, int *__restrict__ info, const int *__restrict__ lv_cnt, int *__restrict__ lv_off,
+    TopoCtx c, int *__restrict__ info, const int *__restrict__ sub_nodes, int *__restrict__ lv_off,
     int *__restrict__ list) {
     __shared__ int s_wave[kTopoThreads / 64];
+    __shared__ int s_left[kMidLds], s_size[kMidLds], s_pre[kMidLds];
     const int tid = threadIdx.x;
     const int T = info[kInfoT], F = info[kInfoF], D = info[kInfoD];
     const int *lvl = info + kInfoTopLevel;  // top level d = BFS slots [lvl[d], lvl[d+1])
     // Expanded top levels are 0..D-1; with a frontier (F > 0) level D holds
     // the subtree roots, whose sizes come from B.
-    for (int d = D - 1; d >= 0; --d) {
-        sizes_level<kTopoThreads>(c, lvl[d], lvl[d + 1]);
+    if (T <= kMidLds) {
+        const int front = (F > 0) ? info[kInfoFront] : T;
+        for (int v = tid; v < T; v += kTopoThreads) {
+            s_left[v] = c.b_left[v];
+            s_size[v] = (v >= front) ? c.b_size[v] : 0;
+        }
         __syncthreads();
-    }
-    if (tid == 0) c.b_pre[0] = 0;
-    __syncthreads();
-    for (int d = 0; d < D; ++d) {
-        preorder_level<kTopoThreads>(c, lvl[d], lvl[d + 1]);
+        for (int d = D - 1; d >= 0; --d) {
+            for (int v = lvl[d] + tid; v < lvl[d + 1]; v += kTopoThreads) {
+                const int l = s_left[v];
+                s_size[v] = (l < 0) ? 1 : 1 + s_size[l] + s_size[l + 1];
+            }
+            __syncthreads();
+        }
+        if (tid == 0) s_pre[0] = 0;
         __syncthreads();
+        for (int d = 0; d < D; ++d) {
+            for (int v = lvl[d] + tid; v < lvl[d + 1]; v += kTopoThreads) {
+                const int l = s_left[v];
+                if (l >= 0) {
+                    s_pre[l] = s_pre[v] + 1;
+                    s_pre[l + 1] = s_pre[v] + 1 + s_size[l];
+                }
+            }
+            __syncthreads();
+        }
+        for (int v = tid; v < T; v += kTopoThreads) {
+            c.b_size[v] = s_size[v];
+            c.b_pre[v] = s_pre[v];
+        }
+    } else {
+        for (int d = D - 1; d >= 0; --d) {
+            sizes_level<kTopoThreads>(c, lvl[d], lvl[d + 1]);
+            __syncthreads();
+        }
+        if (tid == 0) c.b_pre[0] = 0;
+        __syncthreads();
+        for (int d = 0; d < D; ++d) {
+            preorder_level<kTopoThreads>(c, lvl[d], lvl[d + 1]);
+            __syncthreads();
+        }
     }
     for (int i = tid; i < T; i += kTopoThreads) list[i] = i;
-    // subtree f's nodes below its root take list[lv_off[f], + count): one
-    // scan over the subtrees' node counts (their level counts past level 0;
-    // B zeroes the counts past a subtree's last level)
-    int mine = 0;
-    if (tid < F) {
-#pragma unroll 8
-        for (int l = 1; l < kLv; ++l) mine += lv_cnt[tid * kLv + l];
-    }
+    // subtree f's nodes below its root take list[lv_off[f * kLv], + sub_nodes[f])
+    const int mine = (tid < F) ? sub_nodes[tid] : 0;
     int tot;
     const int off = block_exclusive_scan<kTopoThreads>(mine, s_wave, &tot);
     if (tid < F) lv_off[tid * kLv] = T + off;
@@ -1358,11 +1439,16 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
     (void)sc.take<int>(n_info);
     (void)sc.take<int>(6 * slots);
     (void)sc.take<int2>(slots);
-    (void)sc.take<int>(3 * (size_t)kMaxFrontier * kLv);
+    (void)sc.take<int>(3 * (size_t)kMaxFrontier * kLv + kMaxFrontier);
     (void)sc.take<double>(2 * (size_t)n);
     (void)sc.take<int32_t>(p);
-    const int R = ip;  // hint buckets: about one element each on average
+    // hint buckets: kHintPerSource per source, so that a split search in the
+    // bulk of a spectrum with outliers (which stretch the value range) meets
+    // few sources per bucket
+    const int R = (int)std::max<int64_t>(ip, std::min<int64_t>((int64_t)kHintPerSource * ip,
+                                                                kHintMaxBuckets));
     (void)sc.take<int>((size_t)R + 1);
+    (void)sc.take<int>(kGapInts);
     size_t sort_bytes = 0;
     SHK_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, d_ev, (double *)nullptr,
                                       (const int32_t *)nullptr, (int32_t *)nullptr, p, 0, 64,
@@ -1376,15 +1462,17 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
     int *d_info = sc.take<int>(n_info);
     int *b_buf = sc.take<int>(6 * slots);
     int2 *d_mrec = sc.take<int2>(slots);
-    int *lv_buf = sc.take<int>(3 * (size_t)kMaxFrontier * kLv);
+    int *lv_buf = sc.take<int>(3 * (size_t)kMaxFrontier * kLv + kMaxFrontier);
     double *d_tab = sc.take<double>(2 * (size_t)n);
     int32_t *d_iota = sc.take<int32_t>(p);
     int *d_hint = sc.take<int>((size_t)R + 1);
+    int *d_gap = sc.take<int>(kGapInts);
     void *d_sort_tmp = sc.take<char>(sort_bytes);
     int *b_lo = b_buf, *b_hi = b_buf + slots, *b_left = b_buf + 2 * slots,
         *b_size = b_buf + 3 * slots, *b_pre = b_buf + 4 * slots, *d_list = b_buf + 5 * slots;
     int *lv_start = lv_buf, *lv_cnt = lv_buf + (size_t)kMaxFrontier * kLv,
-        *lv_off = lv_buf + 2 * (size_t)kMaxFrontier * kLv;
+        *lv_off = lv_buf + 2 * (size_t)kMaxFrontier * kLv,
+        *sub_nodes = lv_buf + 3 * (size_t)kMaxFrontier * kLv;
 
     // Arena part 1 (sizes known before the topology): sources.
     Carver ar;
@@ -1423,21 +1511,21 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
     // (every split lies strictly inside its node), it is only wasted work.
     auto topology = [&]() -> int {
         hipLaunchKernelGGL(hint_clear_kernel, dim3(blocks_for((size_t)R + 1, 256)), dim3(256), 0,
-                           stream, d_hint, R + 1);
+                           stream, d_hint, R + 1, d_gap);
         SHK_HIP(hipGetLastError());
         hipLaunchKernelGGL(hint_scatter_kernel, dim3(blocks_for(p, 256)), dim3(256), 0, stream,
-                           t->sorted, ip, R, d_hint, t->s_hi, t->s_lo);
+                           t->sorted, ip, R, d_hint, d_gap, t->s_hi, t->s_lo);
         SHK_HIP(hipGetLastError());
-        const TopoCtx c{t->sorted, ip, leaf_cap, d_hint, R, Hint{}, b_lo, b_hi, b_left, b_size,
-                        b_pre};
+        const TopoCtx c{t->sorted, ip,  leaf_cap, d_hint, R,      Hint{}, d_gap,
+                        0,         b_lo, b_hi,     b_left, b_size, b_pre};
         hipLaunchKernelGGL(topo_top_kernel, dim3(1), dim3(kTopoThreads), 0, stream, c, cap_top,
                            d_info);
         SHK_HIP(hipGetLastError());
         hipLaunchKernelGGL(topo_sub_kernel, dim3(kMaxFrontier), dim3(kSubThreads), 0, stream, c,
-                           d_info, lv_start, lv_cnt);
+                           d_info, lv_start, lv_cnt, sub_nodes);
         SHK_HIP(hipGetLastError());
         hipLaunchKernelGGL(topo_mid_kernel, dim3(1), dim3(kTopoThreads), 0, stream, c, d_info,
-                           lv_cnt, lv_off, d_list);
+                           sub_nodes, lv_off, d_list);
         SHK_HIP(hipGetLastError());
         hipLaunchKernelGGL(topo_fin_kernel, dim3(kMaxFrontier), dim3(kSubThreads), 0, stream, c,
                            d_info, lv_start, lv_cnt, lv_off, d_list);
