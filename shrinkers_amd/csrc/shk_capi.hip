@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <atomic>
 #include <cmath>
 #include <cstring>
@@ -414,6 +415,38 @@ int cache_free_all() {
         c->p = nullptr;
         c->cap = 0;
         c->used = false;
+    }
+    return 0;
+}
+
+int host_mailbox(int **mail) {
+    // per device id; no destructor (the HIP runtime may already be gone at
+    // thread exit), the pinned pages go with the process
+    thread_local std::vector<int *> boxes;
+    int d = 0;
+    SHK_HIP(hipGetDevice(&d));
+    if ((size_t)d >= boxes.size()) boxes.resize((size_t)d + 1, nullptr);
+    if (!boxes[(size_t)d]) {
+        void *p = nullptr;
+        SHK_HIP(hipHostMalloc(&p, sizeof(int) * kMailInts, hipHostMallocCoherent));
+        std::memset(p, 0, sizeof(int) * kMailInts);
+        boxes[(size_t)d] = static_cast<int *>(p);
+    }
+    *mail = boxes[(size_t)d];
+    return 0;
+}
+
+int mailbox_wait(volatile int *mail, int seq, hipStream_t stream, int spin_us) {
+    const auto t0 = std::chrono::steady_clock::now();
+    while (__atomic_load_n(const_cast<int *>(mail), __ATOMIC_ACQUIRE) != seq) {
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(spin_us)) {
+            SHK_HIP(hipStreamSynchronize(stream));
+            if (__atomic_load_n(const_cast<int *>(mail), __ATOMIC_ACQUIRE) != seq) {
+                set_error("host mailbox: the kernel finished without writing it");
+                return SHK_ERR_HIP;
+            }
+            break;
+        }
     }
     return 0;
 }

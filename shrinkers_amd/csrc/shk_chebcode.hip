@@ -246,6 +246,7 @@ constexpr int kInfoN = 0, kInfoOverflow = 2, kInfoUnsorted = 3;
 constexpr int kInfoT = 4, kInfoF = 5, kInfoD = 6, kInfoFront = 7;
 constexpr int kInfoTopLevel = 8;                          // kMaxDepth + 3 entries
 constexpr int kInfoInts = kInfoTopLevel + kMaxDepth + 3;
+static_assert(kMailData + kInfoInts <= kMailInts, "the info words fit in the host mailbox");
 
 struct TopoCtx {
     const double *sorted;
@@ -513,7 +514,7 @@ constexpr int kMidLds = 4096;
 
 __global__ __launch_bounds__(kTopoThreads) void topo_mid_kernel(
     TopoCtx c, int *__restrict__ info, const int *__restrict__ sub_nodes, int *__restrict__ lv_off,
-    int *__restrict__ list) {
+    int *__restrict__ list, int *__restrict__ mail, int seq) {
     __shared__ int s_wave[kTopoThreads / 64];
     __shared__ int s_left[kMidLds], s_size[kMidLds], s_pre[kMidLds];
     const int tid = threadIdx.x;
@@ -570,6 +571,12 @@ __global__ __launch_bounds__(kTopoThreads) void topo_mid_kernel(
     const int off = block_exclusive_scan<kTopoThreads>(mine, s_wave, &tot);
     if (tid < F) lv_off[tid * kLv] = T + off;
     if (tid == 0) info[kInfoN] = T + tot;
+    // the info words to the host mailbox, then its sequence number
+    __syncthreads();
+    for (int i = tid; i < kInfoInts; i += kTopoThreads) mail[kMailData + i] = info[i];
+    __threadfence_system();
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(mail, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // D: pre-order numbers inside the subtree and its entries of the node list.
@@ -1484,6 +1491,9 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
 
     // 1. sortedness, then the sorted copy (+ permutation)
     std::vector<int> info(n_info, 0);
+    int *mail = nullptr;
+    if (int st = host_mailbox(&mail)) return st;
+    thread_local int mail_seq = 0;
     // The topology needs the sorted values; its node count sizes the rest of
     // the arena, so the arena is allocated after a first pass with the
     // sources in scratch-independent storage: allocate the source part now.
@@ -1524,15 +1534,17 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
         hipLaunchKernelGGL(topo_sub_kernel, dim3(kMaxFrontier), dim3(kSubThreads), 0, stream, c,
                            d_info, lv_start, lv_cnt, sub_nodes);
         SHK_HIP(hipGetLastError());
+        const int seq = ++mail_seq;
         hipLaunchKernelGGL(topo_mid_kernel, dim3(1), dim3(kTopoThreads), 0, stream, c, d_info,
-                           sub_nodes, lv_off, d_list);
+                           sub_nodes, lv_off, d_list, mail, seq);
         SHK_HIP(hipGetLastError());
         hipLaunchKernelGGL(topo_fin_kernel, dim3(kMaxFrontier), dim3(kSubThreads), 0, stream, c,
                            d_info, lv_start, lv_cnt, lv_off, d_list);
         SHK_HIP(hipGetLastError());
-        SHK_HIP(hipMemcpyAsync(info.data(), d_info, sizeof(int) * n_info,
-                               hipMemcpyDeviceToHost, stream));
-        SHK_HIP(hipStreamSynchronize(stream));
+        // the host continues as soon as C has published the info words; D
+        // and everything after it follow in stream order
+        if (int st = mailbox_wait(mail, seq, stream)) return st;
+        std::copy(mail + kMailData, mail + kMailData + n_info, info.begin());
         return 0;
     };
     // Chebyshev nodes of the second kind s_j = cos(j*pi/(n-1)) and

@@ -179,6 +179,18 @@ int exact_many(const double *ev, const SegSpectrum *d_table, int n_table, int n_
 // float -> double widening on device (for the f32 entry points).
 int widen_f32(const float *in, double *out, size_t n, hipStream_t stream);
 
+// Per-thread, per-device pinned host mailbox (coherent host memory that
+// kernels write directly): mail[0] is a sequence number a kernel stores last,
+// with system-scope release, after the payload in mail[kMailData, ...).
+// Lives until the process ends.
+constexpr int kMailData = 16;
+constexpr int kMailInts = 256;
+int host_mailbox(int **mail);
+// Waits until mail[0] == seq: spins on the host for up to spin_us, then
+// synchronises `stream` and checks again (a launch that failed or faulted
+// reports its error there).
+int mailbox_wait(volatile int *mail, int seq, hipStream_t stream, int spin_us = 2000);
+
 // ---- reverse mode (shk_adjoint.hip) -----------------------------------------
 
 // Gradients of a real loss of the raw at-points sums, given the upstream
