@@ -93,11 +93,6 @@ __device__ __forceinline__ double rcp_f64(double x) {
     return __builtin_fma(r, e, r);
 }
 
-__global__ void check_sorted_kernel(const double *__restrict__ v, int p, int *__restrict__ bad) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-    if (i < p && !(v[i - 1] <= v[i])) atomicOr(bad, 1);
-}
-
 __global__ void iota_kernel(int32_t *__restrict__ v, int p) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < p) v[i] = i;
@@ -158,12 +153,17 @@ __device__ __forceinline__ int hint_bucket(double v, const Hint &h) {
 // int4 {first bucket, last bucket, index of the first source after the gap}
 // from gap + 4.
 constexpr int kMaxGaps = 64;
+constexpr int kInfoUnsortedFlag = 3;  // info[] word set by ingest_kernel
 constexpr int kGapInts = 4 + 4 * kMaxGaps;
 
-__global__ void hint_clear_kernel(int *__restrict__ bstart, int n, int *__restrict__ gap) {
+// Also clears the info words the checks below set (count, overflow flag,
+// unsortedness flag).
+__global__ void hint_clear_kernel(int *__restrict__ bstart, int n, int *__restrict__ gap,
+                                  int *__restrict__ info) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) bstart[i] = -1;
     if (i == 0) gap[0] = 0;
+    if (i < 4) info[i] = 0;
 }
 
 // bstart[0, R]: element i fills the buckets (bucket(x[i-1]), bucket(x[i])]
@@ -188,20 +188,18 @@ __device__ __forceinline__ void record_gap(int *__restrict__ gap, int lo, int hi
     if (g < kMaxGaps) reinterpret_cast<int4 *>(gap + 4)[g] = make_int4(lo, hi, idx, 0);
 }
 
-// The same pass writes the f32 hi/lo split of each source for the FAST near
-// field (hi = f32(x), lo = f32(x - hi)).
-__global__ void hint_scatter_kernel(const double *__restrict__ sorted, int p, int R,
-                                    int *__restrict__ bstart, int *__restrict__ gap,
-                                    float *__restrict__ s_hi, float *__restrict__ s_lo) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= p) return;
-    const double x = sorted[i];
+// Source i of the hint: its buckets (or its gap), and the f32 hi/lo split
+// of x = v[i] for the FAST near field (hi = f32(x), lo = f32(x - hi)); xp is
+// v[i - 1] for i > 0.
+__device__ __forceinline__ void hint_source(int i, double x, double xp, const Hint &h, int p,
+                                            int R, int *__restrict__ bstart,
+                                            int *__restrict__ gap, float *__restrict__ s_hi,
+                                            float *__restrict__ s_lo) {
     const float xh = (float)x;
     s_hi[i] = xh;
     s_lo[i] = (float)(x - (double)xh);
-    const Hint h = hint_of(sorted, p, R);
     const int bi = hint_bucket(x, h);
-    const int bp = (i == 0) ? -1 : hint_bucket(sorted[i - 1], h);
+    const int bp = (i == 0) ? -1 : hint_bucket(xp, h);
     if (bi - bp <= kHintGap)
         for (int k = bp + 1; k <= bi; ++k) bstart[k] = i;
     else
@@ -212,6 +210,40 @@ __global__ void hint_scatter_kernel(const double *__restrict__ sorted, int p, in
         else
             record_gap(gap, bi + 1, R, p);
     }
+}
+
+__global__ void hint_scatter_kernel(const double *__restrict__ sorted, int p, int R,
+                                    int *__restrict__ bstart, int *__restrict__ gap,
+                                    float *__restrict__ s_hi, float *__restrict__ s_lo) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p) return;
+    const double xp = (i == 0) ? 0.0 : sorted[i - 1];
+    hint_source(i, sorted[i], xp, hint_of(sorted, p, R), p, R, bstart, gap, s_hi, s_lo);
+}
+
+// Chebyshev nodes and barycentric weights (node_kernel and the merges read
+// them from device memory), passed by value.
+struct ChebTab {
+    double v[2 * kMaxN];
+};
+
+// First pass over the input ev, taken to be ascending: the tree's copy of
+// the sources, the ascending check (info[kInfoUnsorted]), the hint and the
+// hi/lo split, as hint_scatter_kernel does on the copy; threads below 2n
+// also store the table.
+__global__ void ingest_kernel(const double *__restrict__ ev, int p, int R,
+                              double *__restrict__ sorted, int *__restrict__ bstart,
+                              int *__restrict__ gap, float *__restrict__ s_hi,
+                              float *__restrict__ s_lo, int *__restrict__ info, ChebTab tab,
+                              int n_tab, double *__restrict__ d_tab) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_tab) d_tab[i] = tab.v[i];
+    if (i >= p) return;
+    const double x = ev[i];
+    sorted[i] = x;
+    const double xp = (i == 0) ? 0.0 : ev[i - 1];
+    if (i > 0 && !(xp <= x)) atomicOr(info + kInfoUnsortedFlag, 1);
+    hint_source(i, x, xp, hint_of(ev, p, R), p, R, bstart, gap, s_hi, s_lo);
 }
 
 // ---- topology ---------------------------------------------------------------
@@ -242,7 +274,7 @@ constexpr int kMaxFrontier = 2 * kFrontier;  // the frontier level is smaller
 constexpr int kLv = kMaxDepth + 2;           // levels per subtree (root level 0)
 
 // info[] layout (ints), copied to the host after the topology.
-constexpr int kInfoN = 0, kInfoOverflow = 2, kInfoUnsorted = 3;
+constexpr int kInfoN = 0, kInfoOverflow = 2, kInfoUnsorted = kInfoUnsortedFlag;
 constexpr int kInfoT = 4, kInfoF = 5, kInfoD = 6, kInfoFront = 7;
 constexpr int kInfoTopLevel = 8;                          // kMaxDepth + 3 entries
 constexpr int kInfoInts = kInfoTopLevel + kMaxDepth + 3;
@@ -1513,18 +1545,31 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
     t->s_lo = ar.take<float>(p);
     t->arena = src_arena;  // owned from here on (freed by cheb_destroy)
 
+    // Chebyshev nodes of the second kind s_j = cos(j*pi/(n-1)) and
+    // barycentric weights (-1)^j halved at both ends (chebcode.rs:242-261).
+    ChebTab tab{};
+    for (int j = 0; j < n; ++j) {
+        tab.v[j] = std::cos((double)j * kPi / (double)(n - 1));
+        const double sign = (j % 2 == 0) ? 1.0 : -1.0;
+        tab.v[n + j] = (j == 0 || j == n - 1) ? 0.5 * sign : sign;
+    }
     // 1-2. Speculate that the input is ascending (eigenvalues usually are):
     // copy it, check it and build the topology on the copy, with one host
     // synchronisation for the check and the node count together.  An input
     // that is not ascending is then sorted (+ permutation) and the topology
     // rebuilt; on unsorted values the first topology still stays in bounds
     // (every split lies strictly inside its node), it is only wasted work.
-    auto topology = [&]() -> int {
+    auto topology = [&](bool first) -> int {
         hipLaunchKernelGGL(hint_clear_kernel, dim3(blocks_for((size_t)R + 1, 256)), dim3(256), 0,
-                           stream, d_hint, R + 1, d_gap);
+                           stream, d_hint, R + 1, d_gap, d_info);
         SHK_HIP(hipGetLastError());
-        hipLaunchKernelGGL(hint_scatter_kernel, dim3(blocks_for(p, 256)), dim3(256), 0, stream,
-                           t->sorted, ip, R, d_hint, d_gap, t->s_hi, t->s_lo);
+        if (first)  // copy, check, hint and split of the input, and the table
+            hipLaunchKernelGGL(ingest_kernel, dim3(blocks_for(std::max<size_t>(p, 2 * n), 256)),
+                               dim3(256), 0, stream, d_ev, ip, R, t->sorted, d_hint, d_gap,
+                               t->s_hi, t->s_lo, d_info, tab, 2 * n, d_tab);
+        else  // the sorted copy
+            hipLaunchKernelGGL(hint_scatter_kernel, dim3(blocks_for(p, 256)), dim3(256), 0,
+                               stream, t->sorted, ip, R, d_hint, d_gap, t->s_hi, t->s_lo);
         SHK_HIP(hipGetLastError());
         const TopoCtx c{t->sorted, ip,  leaf_cap, d_hint, R,      Hint{}, d_gap,
                         0,         b_lo, b_hi,     b_left, b_size, b_pre};
@@ -1547,32 +1592,14 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
         std::copy(mail + kMailData, mail + kMailData + n_info, info.begin());
         return 0;
     };
-    // Chebyshev nodes of the second kind s_j = cos(j*pi/(n-1)) and
-    // barycentric weights (-1)^j halved at both ends (chebcode.rs:242-261).
-    std::vector<double> tab(2 * n);
-    for (int j = 0; j < n; ++j) {
-        tab[j] = std::cos((double)j * kPi / (double)(n - 1));
-        const double sign = (j % 2 == 0) ? 1.0 : -1.0;
-        tab[n + j] = (j == 0 || j == n - 1) ? 0.5 * sign : sign;
-    }
-    SHK_HIP(hipMemcpyAsync(d_tab, tab.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice,
-                           stream));
-    int zero4[4] = {0, 0, 0, 0};
-    SHK_HIP(hipMemcpyAsync(d_info, zero4, sizeof(zero4), hipMemcpyHostToDevice, stream));
-    SHK_HIP(hipMemcpyAsync(t->sorted, d_ev, sizeof(double) * p, hipMemcpyDeviceToDevice, stream));
-    if (p > 1) {
-        hipLaunchKernelGGL(check_sorted_kernel, dim3(blocks_for(p, 256)), dim3(256), 0, stream,
-                           d_ev, ip, d_info + 3);
-        SHK_HIP(hipGetLastError());
-    }
-    if (int st = topology()) return st;
+    if (int st = topology(true)) return st;
     if (info[kInfoUnsorted] != 0) {
         t->perm = perm_buf;
         hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(p, 256)), dim3(256), 0, stream, d_iota, ip);
         SHK_HIP(hipGetLastError());
         SHK_HIP(rocprim::radix_sort_pairs(d_sort_tmp, sort_bytes, d_ev, t->sorted, d_iota, t->perm,
                                           p, 0, 64, stream));
-        if (int st = topology()) return st;
+        if (int st = topology(false)) return st;
     }
     if (info[kInfoOverflow]) {
         set_error("chebcode build: node capacity exceeded");
