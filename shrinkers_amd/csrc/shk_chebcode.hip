@@ -668,6 +668,42 @@ __device__ __forceinline__ void wave_sync() {
 constexpr int kBaryLanes = 64;
 constexpr int kWaccPerLane = (kMaxN + kBaryLanes - 1) / kBaryLanes;
 
+// Adds rows [r0, r0 + cnt) of the tables into acc in row order: w_j =
+// fma(m_r, q_rj, w_j) for a regular row (hit -1), w_j += m_r for a row that
+// hit node j, nothing otherwise.  Eight rows' entries are read before they
+// are added, and the update is a selection between the candidates (quotients
+// of other rows are never used), so the row loop has no branches and no
+// per-row LDS round trip; the arithmetic and its order are the sequential
+// ones.
+__device__ __forceinline__ double bary_rows(double acc, int r0, int cnt, int j, int n,
+                                            const double *__restrict__ sq,
+                                            const double *__restrict__ sm,
+                                            const int *__restrict__ shit) {
+    int v = 0;
+    for (; v + 8 <= cnt; v += 8) {
+        int h[8];
+        double m[8], q[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            h[u] = shit[r0 + v + u];
+            m[u] = sm[r0 + v + u];
+            q[u] = sq[(r0 + v + u) * n + j];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const double f = __builtin_fma(m[u], q[u], acc), a = acc + m[u];
+            acc = (h[u] == -1) ? f : ((h[u] == j) ? a : acc);
+        }
+    }
+    for (; v < cnt; ++v) {
+        const int h = shit[r0 + v];
+        const double m = sm[r0 + v], q = sq[(r0 + v) * n + j];
+        const double f = __builtin_fma(m, q, acc), a = acc + m;
+        acc = (h == -1) ? f : ((h == j) ? a : acc);
+    }
+    return acc;
+}
+
 template <int NT, class RowFn>
 __device__ void bary_accumulate(RowFn row, int nrows, const double *__restrict__ t,
                                 const double *__restrict__ lam, double *__restrict__ w,
@@ -728,15 +764,7 @@ __device__ void bary_accumulate(RowFn row, int nrows, const double *__restrict__
 #pragma unroll
         for (int u = 0; u < kWaccPerLane; ++u) {
             const int j = lane + u * kBaryLanes;
-            if (j < n) {
-                double acc = wacc[u];
-                for (int rr = 0; rr < rows; ++rr) {
-                    const int h = shit[rr];
-                    if (h == -1) acc = __builtin_fma(sm[rr], sq[rr * n + j], acc);
-                    else if (h == j) acc += sm[rr];
-                }
-                wacc[u] = acc;
-            }
+            if (j < n) wacc[u] = bary_rows(wacc[u], 0, rows, j, n, sq, sm, shit);
         }
         wave_sync();  // the next rows overwrite sq, sm, shit
     }
@@ -905,14 +933,7 @@ __device__ __forceinline__ void merge_group(int lb, int base, int cnt,
     const int src = min(ga, G - 1) * R;  // first lane of group ga
     const int kx = __shfl(m.x, src, kBaryLanes), ky = __shfl(m.y, src, kBaryLanes);
     if (ga < G && ky >= 0) {
-        const int r0 = ga * R;
-        double acc = 0.0;
-#pragma unroll
-        for (int u = 0; u < R; ++u) {
-            const int h = bl.shit[r0 + u];
-            if (h == -1) acc = __builtin_fma(bl.sm[r0 + u], bl.sq[(r0 + u) * NT + j], acc);
-            else if (h == j) acc += bl.sm[r0 + u];
-        }
+        const double acc = bary_rows(0.0, ga * R, R, j, NT, bl.sq, bl.sm, bl.shit);
         t.w[(size_t)kx * NT + j] = acc;
         panel32[(size_t)kx * 2 * NT + NT + j] = (float)acc;
     }
@@ -997,6 +1018,116 @@ __global__ __launch_bounds__(kMergeMaxThreads) void merge_top_kernel(
     for (int d = levels - 1; d >= 0; --d) {
         merge_level(n, lvl[d], lvl[d + 1] - lvl[d], wv, waves, mrec, lam, t, panel32, mine);
         __syncthreads();
+    }
+}
+
+// LDS form of the top merges for the presets' orders.  The slots of levels
+// [0, levels] (level `levels` holds finished children) are staged in LDS once
+// with their left-child slots, pre-order numbers, positions and weights;
+// each level then reads and writes LDS only (weights also go to the tree and
+// the f32 panel) behind an LDS-only barrier.  Per node the arithmetic and
+// its order are merge_group's, so the weights are the same bit for bit.
+constexpr int kTopLdsWaves = 4;
+constexpr size_t kTopLdsMax = 65536;
+
+__host__ __device__ inline size_t top_lds_bytes(int slots, int n, int waves) {
+    return (size_t)slots * (2 * sizeof(int) + 2 * (size_t)n * sizeof(double)) +
+           (size_t)waves * sizeof(double) * (size_t)bary_lds_doubles(n);
+}
+
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <int NT>
+__device__ __forceinline__ void merge_group_lds(int lb, int base, int cnt,
+                                                const int *__restrict__ s_left,
+                                                const int *__restrict__ s_k, const double *s_pos,
+                                                double *s_w, const double (&lr)[NT],
+                                                const TreeArrays &t, float *__restrict__ panel32,
+                                                double *lds_wave) {
+    constexpr int R = 2 * NT, G = kBaryLanes / R;
+    const int lane = threadIdx.x & (kBaryLanes - 1);
+    const int g = lane / R, r = lane - g * R;
+    const BaryLds bl = bary_lds(lds_wave, NT);
+    const int u = lb + base + g;
+    const bool ok = g < G && base + g < cnt && s_left[u] >= 0;
+    int hit = -2;  // as in bary_accumulate
+    double mm = 0.0;
+    if (ok) {
+        double tr[NT];
+#pragma unroll
+        for (int j = 0; j < NT; ++j) tr[j] = s_pos[u * NT + j];
+        const int ch = s_left[u] + (r < NT ? 0 : 1), jj = (r < NT) ? r : r - NT;
+        const double x = s_pos[ch * NT + jj], mass = s_w[ch * NT + jj];
+        if (mass != 0.0) {
+            double sum = 0.0;
+            hit = -1;
+#pragma unroll
+            for (int j = 0; j < NT; ++j) {
+                const double d = x - tr[j];
+                const double q = lr[j] / d;
+                bl.sq[lane * NT + j] = q;
+                sum += q;
+                hit = (hit < 0 && d == 0.0) ? j : hit;
+            }
+            mm = (hit >= 0) ? mass : mass * (1.0 / sum);
+        }
+    }
+    bl.sm[lane] = mm;
+    bl.shit[lane] = hit;
+    wave_sync();
+    const int ga = lane / NT, j = lane - ga * NT;
+    const int ua = lb + base + ga;
+    if (ga < G && base + ga < cnt && s_left[ua] >= 0) {
+        const double acc = bary_rows(0.0, ga * R, R, j, NT, bl.sq, bl.sm, bl.shit);
+        s_w[ua * NT + j] = acc;
+        const int k = s_k[ua];
+        t.w[(size_t)k * NT + j] = acc;
+        panel32[(size_t)k * 2 * NT + NT + j] = (float)acc;
+    }
+    wave_sync();  // the next group overwrites sq, sm, shit
+}
+
+template <int NT>
+__global__ __launch_bounds__(kTopLdsWaves * kBaryLanes) void merge_top_lds_kernel(
+    int levels, const int *__restrict__ info, const int *__restrict__ b_left,
+    const int2 *__restrict__ mrec, const double *__restrict__ lam, TreeArrays t,
+    float *__restrict__ panel32) {
+    extern __shared__ double lds[];
+    constexpr int G = kBaryLanes / (2 * NT);
+    const int *lvl = info + kInfoTopLevel;
+    const int S = lvl[levels + 1], fin = lvl[levels];
+    const int waves = blockDim.x / kBaryLanes, wv = threadIdx.x / kBaryLanes;
+    double *s_pos = lds;
+    double *s_w = s_pos + (size_t)S * NT;
+    double *bary = s_w + (size_t)S * NT;
+    int *s_left = reinterpret_cast<int *>(bary + (size_t)waves * bary_lds_doubles(NT));
+    int *s_k = s_left + S;
+    double lr[NT];  // barycentric weights, loaded once
+#pragma unroll
+    for (int j = 0; j < NT; ++j) lr[j] = lam[j];
+    for (int v = threadIdx.x; v < S; v += blockDim.x) {
+        const int2 m = mrec[v];
+        s_k[v] = m.x;
+        s_left[v] = (v < fin && m.y >= 0) ? b_left[v] : -1;  // children of slot v: slots
+    }
+    lds_barrier();
+    for (int e = threadIdx.x; e < S * NT; e += blockDim.x) {
+        const int v = e / NT, j = e - v * NT;
+        const size_t o = (size_t)s_k[v] * NT + j;
+        s_pos[e] = t.nodes[o];
+        s_w[e] = t.w[o];  // finished weights; merged nodes overwrite theirs
+    }
+    lds_barrier();
+    for (int d = levels - 1; d >= 0; --d) {
+        const int lb = lvl[d], cnt = lvl[d + 1] - lvl[d];
+        for (int c = wv * G; c < cnt; c += waves * G)
+            merge_group_lds<NT>(lb, c, cnt, s_left, s_k, s_pos, s_w, lr, t, panel32,
+                                bary + (size_t)wv * bary_lds_doubles(NT));
+        lds_barrier();
     }
 }
 
@@ -1667,13 +1798,33 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
     const int group = merge_group_size(n);
     auto groups = [&](int d) { return (lvl[d + 1] - lvl[d] + group - 1) / group; };
     int small = 0;  // leading levels merged together in one workgroup
-    while (small < top && groups(small) <= kMergeTopRounds * top_waves) ++small;
+    // LDS form (the presets' orders): as many top levels as fit, with their
+    // finished children
+    const bool top_lds = (n == 8 || n == 11 || n == 12);
+    if (top_lds)
+        while (small < top && top_lds_bytes(lvl[small + 2], n, kTopLdsWaves) <= kTopLdsMax)
+            ++small;
+    else
+        while (small < top && groups(small) <= kMergeTopRounds * top_waves) ++small;
     for (int d = top - 1; d >= small; --d) {
         hipLaunchKernelGGL(merge_range_kernel, dim3(groups(d)), dim3(kBaryLanes), lds, stream, n,
                            lvl[d], lvl[d + 1] - lvl[d], d_mrec, d_tab + n, ta, t->panel32);
         SHK_HIP(hipGetLastError());
     }
-    if (small > 0) {
+    if (small > 0 && top_lds) {
+        const size_t bytes = top_lds_bytes(lvl[small + 1], n, kTopLdsWaves);
+        const dim3 blk(kTopLdsWaves * kBaryLanes);
+        if (n == 8)
+            hipLaunchKernelGGL(merge_top_lds_kernel<8>, dim3(1), blk, bytes, stream, small, d_info,
+                               b_left, d_mrec, d_tab + n, ta, t->panel32);
+        else if (n == 11)
+            hipLaunchKernelGGL(merge_top_lds_kernel<11>, dim3(1), blk, bytes, stream, small,
+                               d_info, b_left, d_mrec, d_tab + n, ta, t->panel32);
+        else
+            hipLaunchKernelGGL(merge_top_lds_kernel<12>, dim3(1), blk, bytes, stream, small,
+                               d_info, b_left, d_mrec, d_tab + n, ta, t->panel32);
+        SHK_HIP(hipGetLastError());
+    } else if (small > 0) {
         hipLaunchKernelGGL(merge_top_kernel, dim3(1), dim3(top_waves * kBaryLanes),
                            top_waves * lds, stream, n, small, d_info, d_mrec, d_tab + n, ta,
                            t->panel32);
