@@ -1186,11 +1186,15 @@ __device__ __forceinline__ void leaf_f64(const double *__restrict__ xs, int len,
 // the FAST f32 test with the OUTSIDE_THETA2 factor for panels that do not
 // contain the query (chebcode.rs:776-783).  Shared by the evaluation and the
 // work-counting kernels so both make the same decisions.
-__device__ __forceinline__ bool accept_f64(const EvalView &v, int node, double x, double eta2) {
-    const double lo = v.lo[node], hi = v.hi[node];
+__device__ __forceinline__ bool accept_f64_of(double lo, double hi, double acc, double x,
+                                              double eta2) {
     const double cl = fmax(fmax(lo - x, x - hi), 0.0);
     const double d_sq = __builtin_fma(cl, cl, eta2);
-    return v.acc[node] < d_sq;
+    return acc < d_sq;
+}
+
+__device__ __forceinline__ bool accept_f64(const EvalView &v, int node, double x, double eta2) {
+    return accept_f64_of(v.lo[node], v.hi[node], v.acc[node], x, eta2);
 }
 
 __device__ __forceinline__ bool accept_fast(const float4 r, float x, float eta2) {
@@ -1218,7 +1222,10 @@ __device__ __forceinline__ bool accept_fast(const float4 r, float x, float eta2)
 //
 // kWave = false: each lane walks on its own (sparse queries, such as a grid
 // spanning the spectrum, whose near fields differ).
-template <bool kWave>
+// NT > 0 (the wave walk at order NT = n): a visit loads the node's link,
+// interval, acceptance radius and panel together, one round trip instead of
+// three dependent ones; the arithmetic and its order are unchanged.
+template <bool kWave, int NT = 0>
 __global__ __launch_bounds__(256) void eval_f64_kernel(EvalView v, const double *__restrict__ query,
                                                        int nq, int queries_are_sources, double eta,
                                                        double scale, double *__restrict__ out_re,
@@ -1232,9 +1239,21 @@ __global__ __launch_bounds__(256) void eval_f64_kernel(EvalView v, const double 
     double re = 0.0, im = 0.0;
     int skip = valid ? 0 : v.N;
     int node = 0;
+    constexpr int kP = (NT > 0) ? NT : 1;
     while (node < v.N) {
         if constexpr (kWave) node = __builtin_amdgcn_readfirstlane(node);
         const int4 lk = v.link[node];  // right, esc, lo_idx, hi_idx
+        double rlo = 0.0, rhi = 0.0, racc = 0.0, pts[kP], pws[kP];
+        if constexpr (NT > 0) {
+            rlo = v.lo[node];
+            rhi = v.hi[node];
+            racc = v.acc[node];
+#pragma unroll
+            for (int j = 0; j < NT; ++j) {
+                pts[j] = v.nodes[(size_t)node * NT + j];
+                pws[j] = v.w[(size_t)node * NT + j];
+            }
+        }
         const bool active = !kWave || node >= skip;
         if (lk.x < 0) {
             if (active) {
@@ -1246,8 +1265,31 @@ __global__ __launch_bounds__(256) void eval_f64_kernel(EvalView v, const double 
             node = lk.y;
             continue;
         }
-        const bool accept = accept_f64(v, node, x, eta2);
-        if (active && accept) {
+        const bool accept = (NT > 0) ? accept_f64_of(rlo, rhi, racc, x, eta2)
+                                     : accept_f64(v, node, x, eta2);
+        if (active && accept && NT > 0) {
+            double afr0 = 0.0, afr1 = 0.0, afi0 = 0.0, afi1 = 0.0;
+#pragma unroll
+            for (int j = 0; j + 2 <= kP; j += 2) {
+                const double d0 = x - pts[j], d1 = x - pts[j + 1];
+                const double q0 = pws[j] * rcp_f64(__builtin_fma(d0, d0, eta2));
+                const double q1 = pws[j + 1] * rcp_f64(__builtin_fma(d1, d1, eta2));
+                afr0 = __builtin_fma(q0, d0, afr0);
+                afr1 = __builtin_fma(q1, d1, afr1);
+                afi0 += q0;
+                afi1 += q1;
+            }
+            double fr = afr0 + afr1, fi = afi0 + afi1;
+            if (kP % 2) {
+                const double dj = x - pts[kP - 1];
+                const double q = pws[kP - 1] * rcp_f64(dj * dj + eta2);
+                fr += q * dj;
+                fi += q;
+            }
+            re += fr;
+            im += eta * fi;
+            skip = lk.y;
+        } else if (active && accept) {
             const double *ts = v.nodes + (size_t)node * n;
             const double *ws = v.w + (size_t)node * n;
             double afr0 = 0.0, afr1 = 0.0, afi0 = 0.0, afi1 = 0.0;
@@ -1283,7 +1325,8 @@ __global__ __launch_bounds__(256) void eval_f64_kernel(EvalView v, const double 
     out_im[o] = im * scale;
 }
 
-template <bool kWave>
+// NT > 0 as in eval_f64_kernel: record and f32 panel loaded with the link.
+template <bool kWave, int NT = 0>
 __global__ __launch_bounds__(256) void eval_fast_kernel(EvalView v, const double *__restrict__ query,
                                                         int nq, int queries_are_sources, double eta,
                                                         double scale, double *__restrict__ out_re,
@@ -1302,9 +1345,20 @@ __global__ __launch_bounds__(256) void eval_fast_kernel(EvalView v, const double
     double leaf_re = 0.0, leaf_inv = 0.0;
     int skip = valid ? 0 : v.N;
     int node = 0;
+    constexpr int kP = (NT > 0) ? NT : 1;
     while (node < v.N) {
         if constexpr (kWave) node = __builtin_amdgcn_readfirstlane(node);
         const int4 lk = v.link[node];
+        float4 rec = make_float4(0.f, 0.f, 0.f, 0.f);
+        float pts[kP], pws[kP];
+        if constexpr (NT > 0) {
+            rec = v.rec32[node];
+#pragma unroll
+            for (int j = 0; j < NT; ++j) {
+                pts[j] = v.panel32[(size_t)node * 2 * NT + j];
+                pws[j] = v.panel32[(size_t)node * 2 * NT + NT + j];
+            }
+        }
         const bool active = !kWave || node >= skip;
         if (lk.x < 0) {
             if (active) {
@@ -1346,8 +1400,18 @@ __global__ __launch_bounds__(256) void eval_fast_kernel(EvalView v, const double
             node = lk.y;
             continue;
         }
-        const bool accept = accept_fast(v.rec32[node], x, eta2);
-        if (active && accept) {
+        const bool accept = accept_fast((NT > 0) ? rec : v.rec32[node], x, eta2);
+        if (active && accept && NT > 0) {
+            // term j goes to accumulator j mod 4 (static indices)
+#pragma unroll
+            for (int j = 0; j < kP; ++j) {
+                const float d = x - pts[j];
+                const float q = pws[j] * __builtin_amdgcn_rcpf(__builtin_fmaf(d, d, eta2));
+                far_re[j % 4] = __builtin_fmaf(q, d, far_re[j % 4]);
+                far_im[j % 4] = far_im[j % 4] + q;
+            }
+            skip = lk.y;
+        } else if (active && accept) {
             const float *ts = v.panel32 + (size_t)node * 2 * n;
             const float *ws = ts + n;
             // term j goes to accumulator j mod 4 (static indices)
@@ -1890,12 +1954,21 @@ int cheb_eval(const ChebTree *t, const double *d_query, size_t nq, double eta, d
     const bool wave = queries_are_sources || nq >= t->p;
     const dim3 grid(blocks_for(nq, 256)), block(256);
     ProfileMark m = profile_begin(stream);
-    if (t->f32_far && wave)
+    if (t->f32_far && wave && t->n == 8)
+        hipLaunchKernelGGL((eval_fast_kernel<true, 8>), grid, block, 0, stream, v, d_query,
+                           (int)nq, queries_are_sources, eta, scale, d_out_re, d_out_im);
+    else if (t->f32_far && wave)
         hipLaunchKernelGGL(eval_fast_kernel<true>, grid, block, 0, stream, v, d_query, (int)nq,
                            queries_are_sources, eta, scale, d_out_re, d_out_im);
     else if (t->f32_far)
         hipLaunchKernelGGL(eval_fast_kernel<false>, grid, block, 0, stream, v, d_query, (int)nq,
                            queries_are_sources, eta, scale, d_out_re, d_out_im);
+    else if (wave && t->n == 11)
+        hipLaunchKernelGGL((eval_f64_kernel<true, 11>), grid, block, 0, stream, v, d_query,
+                           (int)nq, queries_are_sources, eta, scale, d_out_re, d_out_im);
+    else if (wave && t->n == 12)
+        hipLaunchKernelGGL((eval_f64_kernel<true, 12>), grid, block, 0, stream, v, d_query,
+                           (int)nq, queries_are_sources, eta, scale, d_out_re, d_out_im);
     else if (wave)
         hipLaunchKernelGGL(eval_f64_kernel<true>, grid, block, 0, stream, v, d_query, (int)nq,
                            queries_are_sources, eta, scale, d_out_re, d_out_im);
