@@ -43,12 +43,17 @@ def test_tree_identical_to_oracle(shk, orc, preset, p):
 
 
 @pytest.mark.parametrize("preset", ["chebcode_fast", "chebcode"])
-@pytest.mark.parametrize("case", ["mp200k_c05", "mp1m_c01", "geometric_depth_cap", "duplicates"])
+@pytest.mark.parametrize("case", ["mp200k_c05", "mp1m_c01", "geometric_depth_cap", "duplicates",
+                                  "clusters"])
 def test_tree_topology_large_and_degenerate(shk, orc, preset, case):
     """Topology, intervals and positions equal the oracle's tree on large MP
     spectra, on a geometric spectrum whose midpoint splits peel one value per
-    level until the depth cap (48, chebcode.rs:95), and on heavy duplicates
-    (the index-midpoint fallback, chebcode.rs:373-381)."""
+    level until the depth cap (48, chebcode.rs:95), on heavy duplicates (the
+    index-midpoint fallback, chebcode.rs:373-381), and on 200 separated
+    clusters of geometric runs: 199 long gaps (more than the split hint's gap
+    table holds), a tree that ends inside the breadth-first top walk with
+    ~9 600 top slots (the top sizes and pre-order numbers outside LDS), and
+    merge levels past the LDS budget."""
     from shrinkers_amd import spectra as S
     if case == "mp200k_c05":
         ev = S.mp_inverse_cdf_spectrum(200_000, c=0.5)
@@ -56,6 +61,9 @@ def test_tree_topology_large_and_degenerate(shk, orc, preset, case):
         ev = S.mp_inverse_cdf_spectrum(1_000_000, c=0.1)
     elif case == "geometric_depth_cap":
         ev = np.sort(1.3 ** (-np.arange(2000.0)))
+    elif case == "clusters":
+        centres = np.linspace(1.0, 50.0, 200)
+        ev = np.sort(np.concatenate([c + 0.01 * 1.3 ** (-np.arange(100.0)) for c in centres]))
     else:
         ev = np.sort(np.repeat(np.linspace(0.5, 2.0, 700), 37))
     theta, n, leaf, _ = shk.CHEB_PRESETS[preset]
