@@ -1131,6 +1131,65 @@ __global__ __launch_bounds__(kTopLdsWaves * kBaryLanes) void merge_top_lds_kerne
     }
 }
 
+// LDS form of the subtree merges for the presets' orders: subtree f (its root
+// and the contiguous slots of its levels below) staged in LDS as in
+// merge_top_lds_kernel, local node 0 being the root and slot v >= base1 the
+// local node v - base1 + 1.  A subtree with more than `cap` nodes takes the
+// global path in the same workgroup.
+template <int NT>
+__global__ __launch_bounds__(kMergeSubWaves * kBaryLanes) void merge_sub_lds_kernel(
+    const int *__restrict__ lv_start, const int *__restrict__ lv_cnt,
+    const int *__restrict__ sub_nodes, const int *__restrict__ b_left,
+    const int2 *__restrict__ mrec, const double *__restrict__ lam, TreeArrays t,
+    float *__restrict__ panel32, int cap) {
+    extern __shared__ double lds[];
+    constexpr int G = kBaryLanes / (2 * NT);
+    const int f = blockIdx.x;
+    const int waves = blockDim.x / kBaryLanes, wv = threadIdx.x / kBaryLanes;
+    int n_lv = 1;
+    while (n_lv < kLv && lv_cnt[f * kLv + n_lv] > 0) ++n_lv;
+    const int S = 1 + sub_nodes[f];
+    double *s_pos = lds;
+    double *s_w = s_pos + (size_t)cap * NT;
+    double *bary = s_w + (size_t)cap * NT;
+    double *mine = bary + (size_t)wv * bary_lds_doubles(NT);
+    if (S > cap) {  // uniform per workgroup
+        for (int l = n_lv - 2; l >= 0; --l) {
+            merge_level(NT, lv_start[f * kLv + l], lv_cnt[f * kLv + l], wv, waves, mrec, lam, t,
+                        panel32, mine);
+            __syncthreads();
+        }
+        return;
+    }
+    int *s_left = reinterpret_cast<int *>(bary + (size_t)waves * bary_lds_doubles(NT));
+    int *s_k = s_left + cap;
+    const int root = lv_start[f * kLv], base1 = (n_lv > 1) ? lv_start[f * kLv + 1] : 0;
+    double lr[NT];  // barycentric weights, loaded once
+#pragma unroll
+    for (int j = 0; j < NT; ++j) lr[j] = lam[j];
+    for (int u = threadIdx.x; u < S; u += blockDim.x) {
+        const int v = (u == 0) ? root : base1 + u - 1;
+        const int2 m = mrec[v];
+        s_k[u] = m.x;
+        s_left[u] = (m.y >= 0) ? b_left[v] - base1 + 1 : -1;
+    }
+    lds_barrier();
+    for (int e = threadIdx.x; e < S * NT; e += blockDim.x) {
+        const int u = e / NT, j = e - u * NT;
+        const size_t o = (size_t)s_k[u] * NT + j;
+        s_pos[e] = t.nodes[o];
+        s_w[e] = t.w[o];  // leaf weights; merged nodes overwrite theirs
+    }
+    lds_barrier();
+    for (int l = n_lv - 2; l >= 0; --l) {
+        const int lb = (l == 0) ? 0 : lv_start[f * kLv + l] - base1 + 1;
+        const int cnt = lv_cnt[f * kLv + l];
+        for (int c = wv * G; c < cnt; c += waves * G)
+            merge_group_lds<NT>(lb, c, cnt, s_left, s_k, s_pos, s_w, lr, t, panel32, mine);
+        lds_barrier();
+    }
+}
+
 // ---- evaluation --------------------------------------------------------------
 
 struct EvalView {
@@ -1848,7 +1907,23 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
     // their LDS within 64 KiB.
     const int lds_waves = (int)std::max<size_t>(1, 65536 / lds);
     const int F = info[kInfoF], D = info[kInfoD];
-    if (F > 0) {
+    if (F > 0 && (n == 8 || n == 11 || n == 12)) {
+        // LDS form: subtrees of up to `cap` nodes in a 64 KiB workgroup
+        const size_t fixed = (size_t)kMergeSubWaves * lds;
+        const int cap = (int)((kTopLdsMax - fixed) / (2 * sizeof(int) + 2 * sizeof(double) * n));
+        const size_t bytes = fixed + (size_t)cap * (2 * sizeof(int) + 2 * sizeof(double) * n);
+        const dim3 blk(kMergeSubWaves * kBaryLanes);
+        if (n == 8)
+            hipLaunchKernelGGL(merge_sub_lds_kernel<8>, dim3(F), blk, bytes, stream, lv_start,
+                               lv_cnt, sub_nodes, b_left, d_mrec, d_tab + n, ta, t->panel32, cap);
+        else if (n == 11)
+            hipLaunchKernelGGL(merge_sub_lds_kernel<11>, dim3(F), blk, bytes, stream, lv_start,
+                               lv_cnt, sub_nodes, b_left, d_mrec, d_tab + n, ta, t->panel32, cap);
+        else
+            hipLaunchKernelGGL(merge_sub_lds_kernel<12>, dim3(F), blk, bytes, stream, lv_start,
+                               lv_cnt, sub_nodes, b_left, d_mrec, d_tab + n, ta, t->panel32, cap);
+        SHK_HIP(hipGetLastError());
+    } else if (F > 0) {
         const int waves = std::min(lds_waves, kMergeSubWaves);
         hipLaunchKernelGGL(merge_sub_kernel, dim3(F), dim3(waves * kBaryLanes), waves * lds, stream,
                            n, lv_start, lv_cnt, d_mrec, d_tab + n, ta, t->panel32);
