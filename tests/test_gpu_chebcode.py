@@ -298,3 +298,27 @@ def test_cached_buffers_across_streams_and_release(shk):
         want = ref[k] if k >= 0 else g_ref
         np.testing.assert_array_equal(re.cpu().numpy(), want[0])
         np.testing.assert_array_equal(im.cpu().numpy(), want[1])
+
+
+def test_build_queued_behind_long_work(shk):
+    """A one-shot ChebCode transform enqueued behind ~4.5 ms of other work (an
+    FP64 all-points transform at p = 150 000) on the same stream: the host's
+    wait for the topology's node count outlasts its 2 ms spin and falls back
+    to a stream synchronisation. The result equals a run
+    on an idle stream bit for bit, and so does the long transform's."""
+    import torch
+    from shrinkers_amd import spectra
+    big = torch.from_numpy(spectra.mp_inverse_cdf_spectrum(150_000, c=0.3)).cuda()
+    ev = torch.from_numpy(spectra.mp_inverse_cdf_spectrum(40_000, c=0.25)).cuda()
+    eta_b, eta = 1.0 / np.sqrt(big.numel()), 1.0 / np.sqrt(ev.numel())
+    want_big = [t.cpu().numpy() for t in shk.all_points_dev(big, eta_b, "blocked", precision="f64")]
+    want = [t.cpu().numpy() for t in shk.all_points_dev(ev, eta, "chebcode_fast")]
+    torch.cuda.synchronize()
+    for _ in range(2):
+        got_big = shk.all_points_dev(big, eta_b, "blocked", precision="f64")  # queued first
+        got = shk.all_points_dev(ev, eta, "chebcode_fast")
+        torch.cuda.synchronize()
+        for g, w in zip(got, want):
+            np.testing.assert_array_equal(g.cpu().numpy(), w)
+        for g, w in zip(got_big, want_big):
+            np.testing.assert_array_equal(g.cpu().numpy(), w)
