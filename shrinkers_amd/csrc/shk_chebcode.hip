@@ -98,9 +98,19 @@ __global__ void iota_kernel(int32_t *__restrict__ v, int p) {
     if (i < p) v[i] = i;
 }
 
+// Workgroup barrier that orders LDS only: it does not wait for this
+// workgroup's global stores (s_waitcnt lgkmcnt(0); s_barrier).
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Workgroup exclusive scan of one int per thread (THREADS threads):
 // wavefront scans by shuffles, then the wavefront totals through LDS.
-template <int THREADS>
+// LDS_ONLY: the barriers order LDS only (callers with global stores in
+// flight that no other thread reads before the kernel ends).
+template <int THREADS, bool LDS_ONLY = false>
 __device__ int block_exclusive_scan(int v, int *s_wave, int *total) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     int x = v;
@@ -110,7 +120,8 @@ __device__ int block_exclusive_scan(int v, int *s_wave, int *total) {
         if (lane >= o) x += y;
     }
     if (lane == 63) s_wave[wid] = x;
-    __syncthreads();
+    if (LDS_ONLY) lds_barrier();
+    else __syncthreads();
     int before = 0, all = 0;
 #pragma unroll
     for (int w = 0; w < THREADS / 64; ++w) {
@@ -118,7 +129,8 @@ __device__ int block_exclusive_scan(int v, int *s_wave, int *total) {
         before += (w < wid) ? t : 0;
         all += t;
     }
-    __syncthreads();  // s_wave is reused by the next call
+    if (LDS_ONLY) lds_barrier();  // s_wave is reused by the next call
+    else __syncthreads();
     *total = all;
     return before + x - v;
 }
@@ -457,27 +469,95 @@ __device__ void preorder_level(const TopoCtx &c, int lb, int le) {
 }
 
 // A: breadth-first from the root while levels are smaller than kFrontier.
+// A level (fewer than kFrontier nodes, so one per thread) keeps its nodes'
+// ranges and end values in LDS, double-buffered with the level it creates:
+// a node's midpoint is known at once, the split search then fetches the
+// children's end values sorted[split - 1] and sorted[split], and the level
+// ends on an LDS-only barrier (the slot arrays in global memory are read
+// only by later kernels).  Splits are expand_level's.
+constexpr int kTopCap = 2 * kFrontier;  // children of a level below kFrontier
+
 __global__ __launch_bounds__(kTopoThreads) void topo_top_kernel(TopoCtx c, int cap_top,
                                                                 int *__restrict__ info) {
+    static_assert(kFrontier <= kTopoThreads, "one node of a top level per thread");
     __shared__ int s_wave[kTopoThreads / 64];
+    __shared__ int s_a[2][kTopCap], s_b[2][kTopCap];
+    __shared__ double s_xa[2][kTopCap], s_xb[2][kTopCap];
     const int tid = threadIdx.x;
     c.h = hint_of(c.sorted, c.p, c.R);
     c.ng = min(c.gap[0], kMaxGaps);
     if (tid == 0) {
         c.b_lo[0] = 0;
         c.b_hi[0] = c.p;
+        s_a[0][0] = 0;
+        s_b[0][0] = c.p;
+        s_xa[0][0] = c.sorted[0];
+        s_xb[0][0] = c.sorted[c.p - 1];
         info[kInfoTopLevel + 0] = 0;
         info[kInfoTopLevel + 1] = 1;
     }
-    __syncthreads();
-    int lb = 0, le = 1, depth = 0, overflow = 0;
+    lds_barrier();
+    const int4 *gt = reinterpret_cast<const int4 *>(c.gap + 4);
+    int lb = 0, le = 1, depth = 0, overflow = 0, cur = 0;
     while (lb < le && le - lb < kFrontier && !overflow) {
-        const int used = expand_level<kTopoThreads>(c, lb, le, depth, le, cap_top, s_wave,
-                                                    &overflow);
-        __syncthreads();
+        const int cnt = le - lb, nxt = cur ^ 1;
+        const bool mine_node = tid < cnt;
+        const int a = mine_node ? s_a[cur][tid] : 0, b = mine_node ? s_b[cur][tid] : 0;
+        const double xa = mine_node ? s_xa[cur][tid] : 0.0, xb = mine_node ? s_xb[cur][tid] : 0.0;
+        const bool need = mine_node && (b - a > c.leaf_cap) && (depth < kMaxDepth);
+        const double mid = xa + 0.5 * (xb - xa);
+        const int kb = hint_bucket(mid, c.h);  // in [0, R-1]
+        int hl = need ? c.bstart[kb] : 0, hr = need ? c.bstart[kb + 1] : 0;
+        if (need && (hl < 0 || hr < 0)) {  // a bucket in a long gap
+            for (int g = 0; g < c.ng; ++g) {
+                const int4 e = gt[g];
+                if (hl < 0 && e.x <= kb && kb <= e.y) hl = e.z;
+                if (hr < 0 && e.x <= kb + 1 && kb + 1 <= e.y) hr = e.z;
+            }
+        }
+        const bool ok = hl >= 0 && hr >= 0;
+        // upper_bound(mid) in [hl, hr], or partition_point in [a, b)
+        int l = need ? (ok ? hl : a) : 0, r = need ? (ok ? hr : b) : 0;
+        while (l < r) {
+            const int m = l + (r - l) / 2;
+            if (c.sorted[m] <= mid) l = m + 1;
+            else r = m;
+        }
+        int split = 0;
+        double xs0 = 0.0, xs1 = 0.0;
+        if (need) {
+            const int u = min(max(l, a), b);
+            split = (u == a || u == b) ? (a + b) / 2 : u;
+            xs0 = c.sorted[split - 1];  // a < split < b
+            xs1 = c.sorted[split];
+        }
+        int added;
+        const int off = block_exclusive_scan<kTopoThreads, true>(need ? 2 : 0, s_wave, &added);
+        if (le + added > cap_top) {
+            overflow = 1;  // uniform: every thread sees the same total
+            break;
+        }
+        if (mine_node) c.b_left[lb + tid] = need ? le + off : -1;
+        if (need) {
+            const int ch = le + off;
+            c.b_lo[ch] = a;
+            c.b_hi[ch] = split;
+            c.b_lo[ch + 1] = split;
+            c.b_hi[ch + 1] = b;
+            s_a[nxt][off] = a;
+            s_b[nxt][off] = split;
+            s_xa[nxt][off] = xa;
+            s_xb[nxt][off] = xs0;
+            s_a[nxt][off + 1] = split;
+            s_b[nxt][off + 1] = b;
+            s_xa[nxt][off + 1] = xs1;
+            s_xb[nxt][off + 1] = xb;
+        }
+        lds_barrier();
         lb = le;
-        le += used;
+        le += added;
         ++depth;
+        cur = nxt;
         if (tid == 0) info[kInfoTopLevel + depth + 1] = le;
     }
     if (tid == 0) {
@@ -1033,12 +1113,6 @@ constexpr size_t kTopLdsMax = 65536;
 __host__ __device__ inline size_t top_lds_bytes(int slots, int n, int waves) {
     return (size_t)slots * (2 * sizeof(int) + 2 * (size_t)n * sizeof(double)) +
            (size_t)waves * sizeof(double) * (size_t)bary_lds_doubles(n);
-}
-
-__device__ __forceinline__ void lds_barrier() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 template <int NT>
