@@ -281,6 +281,7 @@ __global__ void ingest_kernel(const double *__restrict__ ev, int p, int R,
 
 constexpr int kTopoK = 4;        // nodes per thread per pass (interleaved searches)
 constexpr int kSubThreads = 256; // workgroup of the subtree kernels
+constexpr int kSubCap = 1024;    // subtree level held in LDS (4 nodes per thread)
 constexpr int kFrontier = 512;   // top walk stops at a level this large
 constexpr int kMaxFrontier = 2 * kFrontier;  // the frontier level is smaller
 constexpr int kLv = kMaxDepth + 2;           // levels per subtree (root level 0)
@@ -468,13 +469,126 @@ __device__ void preorder_level(const TopoCtx &c, int lb, int le) {
     }
 }
 
-// A: breadth-first from the root while levels are smaller than kFrontier.
-// A level (fewer than kFrontier nodes, so one per thread) keeps its nodes'
-// ranges and end values in LDS, double-buffered with the level it creates:
+// One breadth-first level whose nodes' ranges and end values sit in LDS
+// (arrays ca/cb/cxa/cxb, level-local index; thread t holds nodes tK..tK+K-1):
 // a node's midpoint is known at once, the split search then fetches the
-// children's end values sorted[split - 1] and sorted[split], and the level
-// ends on an LDS-only barrier (the slot arrays in global memory are read
-// only by later kernels).  Splits are expand_level's.
+// children's end values sorted[split - 1] and sorted[split], and the
+// children go to the slot arrays at alloc, alloc + 1, ... in node order and,
+// when they number at most `cap`, to the next level's LDS arrays (*fits).
+// The splits are expand_level's.  Returns the children count (uniform), or
+// sets *overflow when they would pass `limit`.  Ends on an LDS-only barrier:
+// a caller that reads the slot arrays next must first wait for the stores.
+template <int THREADS, int K>
+__device__ int lds_level(const TopoCtx &c, int lb, int cnt, int depth, int alloc, int limit,
+                         int cap, const int *ca, const int *cb, const double *cxa,
+                         const double *cxb, int *na, int *nb, double *nxa, double *nxb,
+                         int *s_wave, int *overflow, bool *fits) {
+    const int tid = threadIdx.x;
+    const int4 *gt = reinterpret_cast<const int4 *>(c.gap + 4);
+    int a[K], b[K], split[K];
+    bool need[K];
+    double xa[K], xb[K], xs0[K], xs1[K];
+    int mine = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i = tid * K + k;
+        const bool node = i < cnt;
+        a[k] = node ? ca[i] : 0;
+        b[k] = node ? cb[i] : 0;
+        xa[k] = node ? cxa[i] : 0.0;
+        xb[k] = node ? cxb[i] : 0.0;
+        need[k] = node && (b[k] - a[k] > c.leaf_cap) && (depth < kMaxDepth);
+        mine += need[k] ? 2 : 0;
+    }
+    int l[K], r[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const double mid = xa[k] + 0.5 * (xb[k] - xa[k]);
+        const int kb = hint_bucket(mid, c.h);  // in [0, R-1]
+        int hl = need[k] ? c.bstart[kb] : 0, hr = need[k] ? c.bstart[kb + 1] : 0;
+        if (need[k] && (hl < 0 || hr < 0)) {  // a bucket in a long gap
+            for (int g = 0; g < c.ng; ++g) {
+                const int4 e = gt[g];
+                if (hl < 0 && e.x <= kb && kb <= e.y) hl = e.z;
+                if (hr < 0 && e.x <= kb + 1 && kb + 1 <= e.y) hr = e.z;
+            }
+        }
+        const bool ok = hl >= 0 && hr >= 0;
+        // upper_bound(mid) in [hl, hr], or partition_point in [a, b)
+        l[k] = need[k] ? (ok ? hl : a[k]) : 0;
+        r[k] = need[k] ? (ok ? hr : b[k]) : 0;
+    }
+    // partition_point(x <= mid), the K searches in lockstep
+    bool busy = false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) busy |= l[k] < r[k];
+    while (busy) {
+        int m[K];
+        double xm[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            m[k] = l[k] + (r[k] - l[k]) / 2;
+            xm[k] = c.sorted[min(m[k], c.p - 1)];
+        }
+        busy = false;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const double mid = xa[k] + 0.5 * (xb[k] - xa[k]);
+            const bool act = l[k] < r[k];
+            const bool le_mid = xm[k] <= mid;
+            l[k] = (act && le_mid) ? m[k] + 1 : l[k];
+            r[k] = (act && !le_mid) ? m[k] : r[k];
+            busy |= l[k] < r[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        split[k] = 0;
+        xs0[k] = xs1[k] = 0.0;
+        if (need[k]) {
+            const int u = min(max(l[k], a[k]), b[k]);
+            split[k] = (u == a[k] || u == b[k]) ? (a[k] + b[k]) / 2 : u;
+            xs0[k] = c.sorted[split[k] - 1];  // a < split < b
+            xs1[k] = c.sorted[split[k]];
+        }
+    }
+    int added;
+    const int off = block_exclusive_scan<THREADS, true>(mine, s_wave, &added);
+    if (alloc + added > limit) {
+        *overflow = 1;  // uniform: every thread sees the same total
+        return 0;
+    }
+    *fits = added <= cap;
+    int o = off;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i = tid * K + k;
+        if (i < cnt) c.b_left[lb + i] = need[k] ? alloc + o : -1;
+        if (need[k]) {
+            const int ch = alloc + o;
+            c.b_lo[ch] = a[k];
+            c.b_hi[ch] = split[k];
+            c.b_lo[ch + 1] = split[k];
+            c.b_hi[ch + 1] = b[k];
+            if (*fits) {
+                na[o] = a[k];
+                nb[o] = split[k];
+                nxa[o] = xa[k];
+                nxb[o] = xs0[k];
+                na[o + 1] = split[k];
+                nb[o + 1] = b[k];
+                nxa[o + 1] = xs1[k];
+                nxb[o + 1] = xb[k];
+            }
+            o += 2;
+        }
+    }
+    lds_barrier();
+    return added;
+}
+
+// A: breadth-first from the root while levels are smaller than kFrontier,
+// one node per thread, each level in LDS (lds_level), double-buffered.
 constexpr int kTopCap = 2 * kFrontier;  // children of a level below kFrontier
 
 __global__ __launch_bounds__(kTopoThreads) void topo_top_kernel(TopoCtx c, int cap_top,
@@ -497,63 +611,14 @@ __global__ __launch_bounds__(kTopoThreads) void topo_top_kernel(TopoCtx c, int c
         info[kInfoTopLevel + 1] = 1;
     }
     lds_barrier();
-    const int4 *gt = reinterpret_cast<const int4 *>(c.gap + 4);
     int lb = 0, le = 1, depth = 0, overflow = 0, cur = 0;
+    bool fits = true;
     while (lb < le && le - lb < kFrontier && !overflow) {
-        const int cnt = le - lb, nxt = cur ^ 1;
-        const bool mine_node = tid < cnt;
-        const int a = mine_node ? s_a[cur][tid] : 0, b = mine_node ? s_b[cur][tid] : 0;
-        const double xa = mine_node ? s_xa[cur][tid] : 0.0, xb = mine_node ? s_xb[cur][tid] : 0.0;
-        const bool need = mine_node && (b - a > c.leaf_cap) && (depth < kMaxDepth);
-        const double mid = xa + 0.5 * (xb - xa);
-        const int kb = hint_bucket(mid, c.h);  // in [0, R-1]
-        int hl = need ? c.bstart[kb] : 0, hr = need ? c.bstart[kb + 1] : 0;
-        if (need && (hl < 0 || hr < 0)) {  // a bucket in a long gap
-            for (int g = 0; g < c.ng; ++g) {
-                const int4 e = gt[g];
-                if (hl < 0 && e.x <= kb && kb <= e.y) hl = e.z;
-                if (hr < 0 && e.x <= kb + 1 && kb + 1 <= e.y) hr = e.z;
-            }
-        }
-        const bool ok = hl >= 0 && hr >= 0;
-        // upper_bound(mid) in [hl, hr], or partition_point in [a, b)
-        int l = need ? (ok ? hl : a) : 0, r = need ? (ok ? hr : b) : 0;
-        while (l < r) {
-            const int m = l + (r - l) / 2;
-            if (c.sorted[m] <= mid) l = m + 1;
-            else r = m;
-        }
-        int split = 0;
-        double xs0 = 0.0, xs1 = 0.0;
-        if (need) {
-            const int u = min(max(l, a), b);
-            split = (u == a || u == b) ? (a + b) / 2 : u;
-            xs0 = c.sorted[split - 1];  // a < split < b
-            xs1 = c.sorted[split];
-        }
-        int added;
-        const int off = block_exclusive_scan<kTopoThreads, true>(need ? 2 : 0, s_wave, &added);
-        if (le + added > cap_top) {
-            overflow = 1;  // uniform: every thread sees the same total
-            break;
-        }
-        if (mine_node) c.b_left[lb + tid] = need ? le + off : -1;
-        if (need) {
-            const int ch = le + off;
-            c.b_lo[ch] = a;
-            c.b_hi[ch] = split;
-            c.b_lo[ch + 1] = split;
-            c.b_hi[ch + 1] = b;
-            s_a[nxt][off] = a;
-            s_b[nxt][off] = split;
-            s_xa[nxt][off] = xa;
-            s_xb[nxt][off] = xs0;
-            s_a[nxt][off + 1] = split;
-            s_b[nxt][off + 1] = b;
-            s_xa[nxt][off + 1] = xs1;
-            s_xb[nxt][off + 1] = xb;
-        }
-        lds_barrier();
+        const int nxt = cur ^ 1;
+        const int added = lds_level<kTopoThreads, 1>(
+            c, lb, le - lb, depth, le, cap_top, kTopCap, s_a[cur], s_b[cur], s_xa[cur],
+            s_xb[cur], s_a[nxt], s_b[nxt], s_xa[nxt], s_xb[nxt], s_wave, &overflow, &fits);
+        if (overflow) break;
         lb = le;
         le += added;
         ++depth;
@@ -577,6 +642,8 @@ __global__ __launch_bounds__(kSubThreads) void topo_sub_kernel(TopoCtx c,
                                                                int *__restrict__ sub_nodes) {
     __shared__ int s_wave[kSubThreads / 64];
     __shared__ int s_lb[kLv], s_le[kLv];
+    __shared__ int s_a[2][kSubCap], s_b[2][kSubCap];
+    __shared__ double s_xa[2][kSubCap], s_xb[2][kSubCap];
     const int f = blockIdx.x;
     if (f >= info[kInfoF]) return;
     c.h = hint_of(c.sorted, c.p, c.R);
@@ -584,16 +651,37 @@ __global__ __launch_bounds__(kSubThreads) void topo_sub_kernel(TopoCtx c,
     const int T = info[kInfoT], D = info[kInfoD];
     const int v = info[kInfoFront] + f;
     const int a = c.b_lo[v], b = c.b_hi[v];
-    int lb = v, le = v + 1, alloc = T + 2 * a, lev = 0, overflow = 0;
+    int lb = v, le = v + 1, alloc = T + 2 * a, lev = 0, overflow = 0, cur = 0;
     const int limit = T + 2 * b;
     if (threadIdx.x == 0) {
         s_lb[0] = lb;
         s_le[0] = le;
+        s_a[0][0] = a;
+        s_b[0][0] = b;
+        s_xa[0][0] = c.sorted[a];
+        s_xb[0][0] = c.sorted[b - 1];
     }
+    lds_barrier();
+    // levels in LDS while they fit (kSubCap nodes), then on the slot arrays
+    bool in_lds = true;
     while (true) {
-        const int used = expand_level<kSubThreads>(c, lb, le, D + lev, alloc, limit, s_wave,
-                                                   &overflow);
-        __syncthreads();
+        int used;
+        if (in_lds) {
+            const int nxt = cur ^ 1;
+            bool fits = true;
+            used = lds_level<kSubThreads, kSubCap / kSubThreads>(
+                c, lb, le - lb, D + lev, alloc, limit, kSubCap, s_a[cur], s_b[cur], s_xa[cur],
+                s_xb[cur], s_a[nxt], s_b[nxt], s_xa[nxt], s_xb[nxt], s_wave, &overflow, &fits);
+            cur = nxt;
+            if (!fits) {
+                in_lds = false;
+                __syncthreads();  // the next level reads the slot arrays
+            }
+        } else {
+            used = expand_level<kSubThreads>(c, lb, le, D + lev, alloc, limit, s_wave,
+                                             &overflow);
+            __syncthreads();
+        }
         if (used == 0 || overflow) break;
         lb = alloc;
         le = alloc + used;
