@@ -367,3 +367,25 @@ def test_fused_fit_step_matches_reference_and_autograd(shk, orc, precision, tol)
     assert _rel(gev, a_gev) <= 1e-12
     assert abs(ge - a_ge) <= 1e-12 * abs(a_ge)
     assert _rel(glog, a_gl) <= 1e-12
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32c"])
+def test_all_points_adjoint_tiny(shk, orc, precision):
+    """p = 0 writes a zero eta gradient; p = 1 and 2 (two-sweep sizes) match
+    the oracle, the single eigenvalue's own term cancelling in the total."""
+    import torch
+    from shrinkers_amd import _lib
+    from shrinkers_amd.stieltjes import _PRECISIONS, _stream_handle
+    ge = torch.full((), np.nan, dtype=torch.float64, device="cuda")
+    _lib.check(_lib.load().shk_exact_adjoint_all_points_dev(
+        None, 0, 0.1, None, None, _PRECISIONS[precision], None, ge.data_ptr(),
+        _stream_handle(None)))
+    torch.cuda.synchronize()
+    assert float(ge.cpu()) == 0.0
+    for ev in (np.array([1.3]), np.array([0.7, 1.9])):
+        gre, gim = np.linspace(0.5, 1.5, ev.size), np.linspace(-1.0, 2.0, ev.size)
+        gev, g_eta = _all_points_adjoint(ev, 0.05, gre, gim, precision)
+        rev, rq, re_ = orc.adjoint_at_points(ev, ev, 0.05, gre, gim)
+        tol = 1e-12 if precision == "f64" else 5e-6
+        assert np.allclose(gev, rev + rq, rtol=tol, atol=tol * np.max(np.abs(rev)))
+        assert abs(g_eta - re_) <= tol * abs(re_)
