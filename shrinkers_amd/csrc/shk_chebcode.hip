@@ -1672,6 +1672,124 @@ __global__ __launch_bounds__(256) void eval_fast_kernel(EvalView v, const double
     out_im[o] = im * scale;
 }
 
+// Sparse queries (a grid across the spectrum, fewer than kSparseMax points):
+// one wavefront per query.  The wavefront walks its query's path in the
+// reference's visit order; the lanes split a leaf's sources (lane l takes
+// sources l, l + 64, ...) and a panel's terms (lane j takes term j), each
+// into its own accumulators, and a fixed butterfly adds the 64 lanes at the
+// end.  The per-lane walk leaves 63 of 64 lanes of a wavefront waiting on
+// the one whose near field is largest, with a few hundred queries on a few
+// CUs; this form spreads them over the chip.  Only the summation order
+// differs from the per-lane walk (both within the presets' tolerances).
+constexpr int kSparseMax = 16384;
+constexpr int kSparseWaves = 4;  // queries per workgroup
+
+__device__ __forceinline__ double wave_sum_fixed(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double w = __shfl_xor(v, o, 64);
+        v = ((threadIdx.x & o) ? w + v : v + w);  // the same operand order on both lanes
+    }
+    return v;
+}
+
+__global__ __launch_bounds__(kSparseWaves * 64) void eval_f64_sparse_kernel(
+    EvalView v, const double *__restrict__ query, int nq, double eta, double scale,
+    double *__restrict__ out_re, double *__restrict__ out_im) {
+    const int q = blockIdx.x * kSparseWaves + (int)(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (q >= nq) return;  // wavefront-uniform
+    const double x = query[q];
+    const double eta2 = eta * eta;
+    const int n = v.n;
+    double lr = 0.0, li = 0.0, fr = 0.0, fi = 0.0;
+    int node = 0;
+    while (node < v.N) {
+        node = __builtin_amdgcn_readfirstlane(node);
+        const int4 lk = v.link[node];
+        if (lk.x < 0) {
+            for (int k = lk.z + lane; k < lk.w; k += 64) {
+                const double d = x - v.sorted[k];
+                const double inv = rcp_f64(__builtin_fma(d, d, eta2));
+                lr = __builtin_fma(d, inv, lr);
+                li += inv;
+            }
+            node = lk.y;
+            continue;
+        }
+        if (accept_f64(v, node, x, eta2)) {
+            for (int j = lane; j < n; j += 64) {
+                const double d = x - v.nodes[(size_t)node * n + j];
+                const double qv = v.w[(size_t)node * n + j] * rcp_f64(__builtin_fma(d, d, eta2));
+                fr = __builtin_fma(qv, d, fr);
+                fi += qv;
+            }
+            node = lk.y;
+        } else {
+            node = node + 1;
+        }
+    }
+    const double re = wave_sum_fixed(lr + fr), im = wave_sum_fixed(li + fi);
+    if (lane == 0) {
+        out_re[q] = re * scale;
+        out_im[q] = (eta * im) * scale;
+    }
+}
+
+__global__ __launch_bounds__(kSparseWaves * 64) void eval_fast_sparse_kernel(
+    EvalView v, const double *__restrict__ query, int nq, double eta, double scale,
+    double *__restrict__ out_re, double *__restrict__ out_im) {
+    const int q = blockIdx.x * kSparseWaves + (int)(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (q >= nq) return;  // wavefront-uniform
+    const double xd = query[q];
+    const float x = (float)xd;
+    const float x_lo = (float)(xd - (double)x);
+    const float e32 = (float)eta;
+    const float eta2 = e32 * e32;
+    const float eta2n = (float)(eta * eta);
+    const int n = v.n;
+    double leaf_re = 0.0, leaf_inv = 0.0;
+    float far_re = 0.f, far_im = 0.f;
+    int node = 0;
+    while (node < v.N) {
+        node = __builtin_amdgcn_readfirstlane(node);
+        const int4 lk = v.link[node];
+        if (lk.x < 0) {
+            // f32 near field on the hi/lo split, folded into f64 per leaf
+            float ar = 0.f, ai = 0.f;
+            for (int k = lk.z + lane; k < lk.w; k += 64) {
+                const float d = (x - v.s_hi[k]) + (x_lo - v.s_lo[k]);
+                const float r = __builtin_amdgcn_rcpf(__builtin_fmaf(d, d, eta2n));
+                ar = __builtin_fmaf(d, r, ar);
+                ai += r;
+            }
+            leaf_re += (double)ar;
+            leaf_inv += (double)ai;
+            node = lk.y;
+            continue;
+        }
+        if (accept_fast(v.rec32[node], x, eta2)) {
+            const float *ts = v.panel32 + (size_t)node * 2 * n;
+            for (int j = lane; j < n; j += 64) {
+                const float d = x - ts[j];
+                const float qv = ts[n + j] * __builtin_amdgcn_rcpf(__builtin_fmaf(d, d, eta2));
+                far_re = __builtin_fmaf(qv, d, far_re);
+                far_im = far_im + qv;
+            }
+            node = lk.y;
+        } else {
+            node = node + 1;
+        }
+    }
+    const double re = wave_sum_fixed(leaf_re + (double)far_re);
+    const double li = wave_sum_fixed(leaf_inv), fi = wave_sum_fixed((double)far_im);
+    if (lane == 0) {
+        out_re[q] = re * scale;
+        out_im[q] = (eta * li + eta * fi) * scale;
+    }
+}
+
 // Work of the traversal, without arithmetic: near-field pair-evaluations
 // (sum of the sizes of the leaves a query visits) and far-field panel terms
 // (n per accepted panel), with the evaluation kernels' decisions; one query
@@ -2197,6 +2315,10 @@ int cheb_eval(const ChebTree *t, const double *d_query, size_t nq, double eta, d
     else if (t->f32_far && wave)
         hipLaunchKernelGGL(eval_fast_kernel<true>, grid, block, 0, stream, v, d_query, (int)nq,
                            queries_are_sources, eta, scale, d_out_re, d_out_im);
+    else if (t->f32_far && nq <= (size_t)kSparseMax)
+        hipLaunchKernelGGL(eval_fast_sparse_kernel,
+                           dim3(blocks_for(nq, kSparseWaves)), dim3(kSparseWaves * 64), 0,
+                           stream, v, d_query, (int)nq, eta, scale, d_out_re, d_out_im);
     else if (t->f32_far)
         hipLaunchKernelGGL(eval_fast_kernel<false>, grid, block, 0, stream, v, d_query, (int)nq,
                            queries_are_sources, eta, scale, d_out_re, d_out_im);
@@ -2209,6 +2331,10 @@ int cheb_eval(const ChebTree *t, const double *d_query, size_t nq, double eta, d
     else if (wave)
         hipLaunchKernelGGL(eval_f64_kernel<true>, grid, block, 0, stream, v, d_query, (int)nq,
                            queries_are_sources, eta, scale, d_out_re, d_out_im);
+    else if (nq <= (size_t)kSparseMax)
+        hipLaunchKernelGGL(eval_f64_sparse_kernel, dim3(blocks_for(nq, kSparseWaves)),
+                           dim3(kSparseWaves * 64), 0, stream, v, d_query, (int)nq, eta, scale,
+                           d_out_re, d_out_im);
     else
         hipLaunchKernelGGL(eval_f64_kernel<false>, grid, block, 0, stream, v, d_query, (int)nq,
                            queries_are_sources, eta, scale, d_out_re, d_out_im);
