@@ -11,6 +11,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <initializer_list>
 #include <mutex>
 #include <string>
 #include <utility>
@@ -253,12 +254,21 @@ hipStream_t host_stream() {
     return s;
 }
 
-// RAII device buffer freed on the stream.
+// Device copies of a host-array call's inputs and outputs, in buffers cached
+// per thread: the k-th buffer a call takes is cache slot CACHE_HOST_0 + k, so
+// a call allocates nothing once the buffers are large enough.  A call's
+// buffers are released together when it returns.
+thread_local int g_host_slots = 0;
+
 struct DevBuf {
     void *p = nullptr;
     hipStream_t s = nullptr;
+    int slot = -1;
     ~DevBuf() {
-        if (p) (void)hipFreeAsync(p, s);
+        if (slot >= 0) {
+            (void)cache_release(slot, s);
+            --g_host_slots;
+        }
     }
     template <class T>
     T *as() const {
@@ -266,9 +276,114 @@ struct DevBuf {
     }
 };
 
+// Pinned host staging for a host-array call: the caller's arrays are copied
+// into it (and out of it) with memcpy, and the device copies run from pinned
+// memory.  One buffer per thread, grown to the largest call; a call uses it
+// only between its start and its final stream synchronisation.
+int stage_acquire(size_t bytes, char **out) {
+    thread_local char *base = nullptr;
+    thread_local size_t cap = 0;
+    if (cap < bytes) {
+        if (base) SHK_HIP(hipHostFree(base));
+        base = nullptr;
+        cap = 0;
+        const size_t want = bytes + bytes / 4 + 4096;
+        void *p = nullptr;
+        SHK_HIP(hipHostMalloc(&p, want, hipHostMallocDefault));
+        base = static_cast<char *>(p);
+        cap = want;
+    }
+    *out = base;
+    return 0;
+}
+
+inline size_t stage_round(size_t b) { return (b + 255) / 256 * 256; }
+
+// The host<->device copies of one host-array call, run through the thread's
+// pinned staging buffer.  `reserve` sizes the buffer for every upload and
+// download of the call at once (it must not move while copies are queued);
+// `finish` synchronises the stream and copies the downloads out to the
+// caller's arrays.  A call that returns early still waits for its queued
+// copies, so the next call never reuses staging memory a copy is using.
+class HostStage {
+  public:
+    explicit HostStage(hipStream_t s) : s_(s) {}
+    HostStage(const HostStage &) = delete;
+    HostStage &operator=(const HostStage &) = delete;
+    ~HostStage() {
+        if (queued_) (void)hipStreamSynchronize(s_);
+    }
+    int reserve(std::initializer_list<size_t> sizes) {
+        size_t total = 0;
+        for (size_t b : sizes) total += stage_round(b);
+        cap_ = total;
+        used_ = 0;
+        return stage_acquire(total, &base_);
+    }
+    int upload(void *dev, const void *host, size_t bytes) {
+        char *at = take(bytes);
+        if (!at) return overflow();
+        std::memcpy(at, host, bytes);
+        queued_ = true;
+        SHK_HIP(hipMemcpyAsync(dev, at, bytes, hipMemcpyHostToDevice, s_));
+        return 0;
+    }
+    // With host == nullptr the bytes stay in staging; `staged` (if given)
+    // points at them, valid after `finish` until the call returns.
+    int download(void *host, const void *dev, size_t bytes, const char **staged = nullptr) {
+        char *at = take(bytes);
+        if (!at || n_out_ == kMaxOut) return overflow();
+        queued_ = true;
+        SHK_HIP(hipMemcpyAsync(at, dev, bytes, hipMemcpyDeviceToHost, s_));
+        out_[n_out_++] = {host, at, bytes};
+        if (staged) *staged = at;
+        return 0;
+    }
+    int finish() {
+        SHK_HIP(hipStreamSynchronize(s_));
+        queued_ = false;
+        for (int k = 0; k < n_out_; ++k)
+            if (out_[k].host) std::memcpy(out_[k].host, out_[k].staged, out_[k].bytes);
+        n_out_ = 0;
+        return 0;
+    }
+
+  private:
+    static constexpr int kMaxOut = 8;
+    struct Out {
+        void *host;
+        const char *staged;
+        size_t bytes;
+    };
+    char *take(size_t bytes) {
+        const size_t r = stage_round(bytes);
+        if (!base_ || used_ + r > cap_) return nullptr;
+        char *at = base_ + used_;
+        used_ += r;
+        return at;
+    }
+    static int overflow() {
+        set_error("host-array call: staging copies exceed the reserved size");
+        return SHK_ERR_HIP;
+    }
+    hipStream_t s_;
+    char *base_ = nullptr;
+    size_t cap_ = 0, used_ = 0;
+    Out out_[kMaxOut];
+    int n_out_ = 0;
+    bool queued_ = false;
+};
+
 int dev_alloc(DevBuf &b, size_t bytes, hipStream_t s) {
+    if (g_host_slots >= kCacheHostSlots) {
+        set_error("host-array call: more device buffers than cache slots");
+        return SHK_ERR_HIP;
+    }
+    const int slot = CACHE_HOST_0 + g_host_slots;
+    if (int st = cache_acquire(slot, bytes ? bytes : 8, s, &b.p)) return st;
     b.s = s;
-    SHK_HIP(hipMallocAsync(&b.p, bytes ? bytes : 8, s));
+    b.slot = slot;
+    ++g_host_slots;
     return 0;
 }
 
@@ -601,12 +716,15 @@ shk_status shk_compute_all_stieltjes(const double *eigenvalues, size_t p, double
     SHK_CHECK(dev_alloc(ev, sizeof(double) * p, s));
     SHK_CHECK(dev_alloc(re, sizeof(double) * p, s));
     SHK_CHECK(dev_alloc(im, sizeof(double) * p, s));
-    SHK_HIP_S(hipMemcpyAsync(ev.p, eigenvalues, sizeof(double) * p, hipMemcpyHostToDevice, s));
+    const size_t bp = sizeof(double) * p;
+    HostStage hs(s);
+    SHK_CHECK(hs.reserve({bp, bp, bp}));
+    SHK_CHECK(hs.upload(ev.p, eigenvalues, bp));
     SHK_CHECK(run_allpoints(ev.as<double>(), p, eta, method, cutoff_ratio, precision,
                             1.0 / (double)p, re.as<double>(), im.as<double>(), s));
-    SHK_HIP_S(hipMemcpyAsync(out_re, re.p, sizeof(double) * p, hipMemcpyDeviceToHost, s));
-    SHK_HIP_S(hipMemcpyAsync(out_im, im.p, sizeof(double) * p, hipMemcpyDeviceToHost, s));
-    SHK_HIP_S(hipStreamSynchronize(s));
+    SHK_CHECK(hs.download(out_re, re.p, bp));
+    SHK_CHECK(hs.download(out_im, im.p, bp));
+    SHK_CHECK(hs.finish());
     return SHK_OK;
 }
 
@@ -627,7 +745,10 @@ shk_status shk_compute_all_stieltjes_f32(const float *eigenvalues, size_t p, flo
     DevBuf ev32, ev, re, im, re32, im32;
     SHK_CHECK(dev_alloc(ev32, sizeof(float) * p, s));
     SHK_CHECK(dev_alloc(ev, sizeof(double) * p, s));
-    SHK_HIP_S(hipMemcpyAsync(ev32.p, eigenvalues, sizeof(float) * p, hipMemcpyHostToDevice, s));
+    const size_t b32 = sizeof(float) * p, b64 = sizeof(double) * p;
+    HostStage hs(s);
+    SHK_CHECK(hs.reserve({b32, b64, b64}));
+    SHK_CHECK(hs.upload(ev32.p, eigenvalues, b32));
     SHK_CHECK(widen_f32(ev32.as<float>(), ev.as<double>(), p, s));
     const double eta_d = (double)eta;
     const double scale = 1.0 / (double)p;
@@ -638,9 +759,9 @@ shk_status shk_compute_all_stieltjes_f32(const float *eigenvalues, size_t p, flo
         SHK_CHECK(dev_alloc(im32, sizeof(float) * p, s));
         SHK_CHECK(exact_f32c(ev.as<double>(), p, ev.as<double>(), p, eta_d, scale, re32.p,
                              im32.p, 1, s));
-        SHK_HIP_S(hipMemcpyAsync(out_re, re32.p, sizeof(float) * p, hipMemcpyDeviceToHost, s));
-        SHK_HIP_S(hipMemcpyAsync(out_im, im32.p, sizeof(float) * p, hipMemcpyDeviceToHost, s));
-        SHK_HIP_S(hipStreamSynchronize(s));
+        SHK_CHECK(hs.download(out_re, re32.p, b32));
+        SHK_CHECK(hs.download(out_im, im32.p, b32));
+        SHK_CHECK(hs.finish());
         return SHK_OK;
     }
     // Other methods: the f64 path, converted (mod.rs:832-849).
@@ -648,10 +769,12 @@ shk_status shk_compute_all_stieltjes_f32(const float *eigenvalues, size_t p, flo
     SHK_CHECK(dev_alloc(im, sizeof(double) * p, s));
     SHK_CHECK(run_allpoints(ev.as<double>(), p, eta_d, method, cutoff_ratio, SHK_PREC_F64, scale,
                             re.as<double>(), im.as<double>(), s));
-    std::vector<double> hr(p), hi(p);
-    SHK_HIP_S(hipMemcpyAsync(hr.data(), re.p, sizeof(double) * p, hipMemcpyDeviceToHost, s));
-    SHK_HIP_S(hipMemcpyAsync(hi.data(), im.p, sizeof(double) * p, hipMemcpyDeviceToHost, s));
-    SHK_HIP_S(hipStreamSynchronize(s));
+    const char *sr = nullptr, *si = nullptr;
+    SHK_CHECK(hs.download(nullptr, re.p, b64, &sr));
+    SHK_CHECK(hs.download(nullptr, im.p, b64, &si));
+    SHK_CHECK(hs.finish());
+    const double *hr = reinterpret_cast<const double *>(sr);
+    const double *hi = reinterpret_cast<const double *>(si);
     for (size_t i = 0; i < p; ++i) {
         out_re[i] = (float)hr[i];
         out_im[i] = (float)hi[i];
@@ -683,13 +806,16 @@ shk_status shk_compute_stieltjes_at_points(const double *query_points, size_t nq
     SHK_CHECK(dev_alloc(ev, sizeof(double) * p, s));
     SHK_CHECK(dev_alloc(re, sizeof(double) * nq, s));
     SHK_CHECK(dev_alloc(im, sizeof(double) * nq, s));
-    SHK_HIP_S(hipMemcpyAsync(q.p, query_points, sizeof(double) * nq, hipMemcpyHostToDevice, s));
-    SHK_HIP_S(hipMemcpyAsync(ev.p, eigenvalues, sizeof(double) * p, hipMemcpyHostToDevice, s));
+    const size_t bq = sizeof(double) * nq, bp = sizeof(double) * p;
+    HostStage hs(s);
+    SHK_CHECK(hs.reserve({bq, bp, bq, bq}));
+    SHK_CHECK(hs.upload(q.p, query_points, bq));
+    SHK_CHECK(hs.upload(ev.p, eigenvalues, bp));
     SHK_CHECK(run_atpoints(q.as<double>(), nq, ev.as<double>(), p, eta, method, cutoff_ratio,
                            re.as<double>(), im.as<double>(), s));
-    SHK_HIP_S(hipMemcpyAsync(out_re, re.p, sizeof(double) * nq, hipMemcpyDeviceToHost, s));
-    SHK_HIP_S(hipMemcpyAsync(out_im, im.p, sizeof(double) * nq, hipMemcpyDeviceToHost, s));
-    SHK_HIP_S(hipStreamSynchronize(s));
+    SHK_CHECK(hs.download(out_re, re.p, bq));
+    SHK_CHECK(hs.download(out_im, im.p, bq));
+    SHK_CHECK(hs.finish());
     return SHK_OK;
 }
 
@@ -708,15 +834,17 @@ shk_status shk_compute_all_stieltjes_with_deriv(const double *eigenvalues, size_
     DevBuf ev, o;
     SHK_CHECK(dev_alloc(ev, sizeof(double) * p, s));
     SHK_CHECK(dev_alloc(o, sizeof(double) * 4 * p, s));
-    SHK_HIP_S(hipMemcpyAsync(ev.p, eigenvalues, sizeof(double) * p, hipMemcpyHostToDevice, s));
+    const size_t bp = sizeof(double) * p;
+    HostStage hs(s);
+    SHK_CHECK(hs.reserve({bp, bp, bp, bp, bp}));
+    SHK_CHECK(hs.upload(ev.p, eigenvalues, bp));
     double *base = o.as<double>();
     double *outs[4] = {base, base + p, base + 2 * p, base + 3 * p};
     SHK_CHECK(exact_f64(ev.as<double>(), p, ev.as<double>(), p, eta, EXACT_DERIV, 0.0, 1.0, outs,
                         s));
     double *host[4] = {val_re, val_im, der_re, der_im};
-    for (int k = 0; k < 4; ++k)
-        SHK_HIP_S(hipMemcpyAsync(host[k], outs[k], sizeof(double) * p, hipMemcpyDeviceToHost, s));
-    SHK_HIP_S(hipStreamSynchronize(s));
+    for (int k = 0; k < 4; ++k) SHK_CHECK(hs.download(host[k], outs[k], bp));
+    SHK_CHECK(hs.finish());
     return SHK_OK;
 }
 
@@ -734,13 +862,16 @@ shk_status shk_compute_stieltjes_scaled_ray(const double *eigenvalues, size_t p,
     SHK_CHECK(dev_alloc(ev, sizeof(double) * p, s));
     SHK_CHECK(dev_alloc(re, sizeof(double) * p, s));
     SHK_CHECK(dev_alloc(im, sizeof(double) * p, s));
-    SHK_HIP_S(hipMemcpyAsync(ev.p, eigenvalues, sizeof(double) * p, hipMemcpyHostToDevice, s));
+    const size_t bp = sizeof(double) * p;
+    HostStage hs(s);
+    SHK_CHECK(hs.reserve({bp, bp, bp}));
+    SHK_CHECK(hs.upload(ev.p, eigenvalues, bp));
     double *outs[2] = {re.as<double>(), im.as<double>()};
     SHK_CHECK(exact_f64(ev.as<double>(), p, ev.as<double>(), p, scale, EXACT_RAY, 0.0, 1.0, outs,
                         s));
-    SHK_HIP_S(hipMemcpyAsync(out_re, re.p, sizeof(double) * p, hipMemcpyDeviceToHost, s));
-    SHK_HIP_S(hipMemcpyAsync(out_im, im.p, sizeof(double) * p, hipMemcpyDeviceToHost, s));
-    SHK_HIP_S(hipStreamSynchronize(s));
+    SHK_CHECK(hs.download(out_re, re.p, bp));
+    SHK_CHECK(hs.download(out_im, im.p, bp));
+    SHK_CHECK(hs.finish());
     return SHK_OK;
 }
 
@@ -1071,6 +1202,7 @@ shk_status shk_chebcode_build(const double *eigenvalues, size_t p, double theta,
     if (!s) return fail(SHK_ERR_HIP, "could not create a HIP stream");
     DevBuf ev;
     SHK_CHECK(dev_alloc(ev, sizeof(double) * p, s));
+    // A plain copy: staging it would make the call wait for the whole build.
     SHK_HIP_S(hipMemcpyAsync(ev.p, eigenvalues, sizeof(double) * p, hipMemcpyHostToDevice, s));
     ChebParams prm{theta, n, (int64_t)leaf_cap, f32_far ? 1 : 0};
     ChebTree *t = nullptr;
@@ -1114,9 +1246,11 @@ shk_status shk_chebcode_evaluate(const shk_chebcode *h, double eta, double *out_
     SHK_CHECK(dev_alloc(re, sizeof(double) * p, s));
     SHK_CHECK(dev_alloc(im, sizeof(double) * p, s));
     SHK_CHECK(cheb_eval(h->tree, nullptr, p, eta, 1.0, 1, re.as<double>(), im.as<double>(), s));
-    SHK_HIP_S(hipMemcpyAsync(out_re, re.p, sizeof(double) * p, hipMemcpyDeviceToHost, s));
-    SHK_HIP_S(hipMemcpyAsync(out_im, im.p, sizeof(double) * p, hipMemcpyDeviceToHost, s));
-    SHK_HIP_S(hipStreamSynchronize(s));
+    HostStage hs(s);
+    SHK_CHECK(hs.reserve({sizeof(double) * p, sizeof(double) * p}));
+    SHK_CHECK(hs.download(out_re, re.p, sizeof(double) * p));
+    SHK_CHECK(hs.download(out_im, im.p, sizeof(double) * p));
+    SHK_CHECK(hs.finish());
     return SHK_OK;
 }
 
@@ -1134,12 +1268,15 @@ shk_status shk_chebcode_evaluate_points(const shk_chebcode *h, const double *que
     SHK_CHECK(dev_alloc(q, sizeof(double) * nq, s));
     SHK_CHECK(dev_alloc(re, sizeof(double) * nq, s));
     SHK_CHECK(dev_alloc(im, sizeof(double) * nq, s));
-    SHK_HIP_S(hipMemcpyAsync(q.p, query, sizeof(double) * nq, hipMemcpyHostToDevice, s));
+    const size_t bq = sizeof(double) * nq;
+    HostStage hs(s);
+    SHK_CHECK(hs.reserve({bq, bq, bq}));
+    SHK_CHECK(hs.upload(q.p, query, bq));
     SHK_CHECK(cheb_eval(h->tree, q.as<double>(), nq, eta, 1.0, 0, re.as<double>(),
                         im.as<double>(), s));
-    SHK_HIP_S(hipMemcpyAsync(out_re, re.p, sizeof(double) * nq, hipMemcpyDeviceToHost, s));
-    SHK_HIP_S(hipMemcpyAsync(out_im, im.p, sizeof(double) * nq, hipMemcpyDeviceToHost, s));
-    SHK_HIP_S(hipStreamSynchronize(s));
+    SHK_CHECK(hs.download(out_re, re.p, bq));
+    SHK_CHECK(hs.download(out_im, im.p, bq));
+    SHK_CHECK(hs.finish());
     return SHK_OK;
 }
 
@@ -1159,16 +1296,16 @@ shk_status shk_chebcode_evaluate_many(const shk_chebcode *h, const double *etas,
         const double e0 = etas[k];
         const double e1 = (k + 1 < n_eta) ? etas[k + 1] : e0;
         SHK_CHECK(cheb_eval_x2(h->tree, e0, e1, b, b + p, b + 2 * p, b + 3 * p, s));
-        SHK_HIP_S(hipMemcpyAsync(out_re + k * p, b, sizeof(double) * p, hipMemcpyDeviceToHost, s));
-        SHK_HIP_S(hipMemcpyAsync(out_im + k * p, b + p, sizeof(double) * p, hipMemcpyDeviceToHost,
-                                 s));
+        const size_t bp = sizeof(double) * p;
+        HostStage hs(s);
+        SHK_CHECK(hs.reserve({bp, bp, bp, bp}));
+        SHK_CHECK(hs.download(out_re + k * p, b, bp));
+        SHK_CHECK(hs.download(out_im + k * p, b + p, bp));
         if (k + 1 < n_eta) {
-            SHK_HIP_S(hipMemcpyAsync(out_re + (k + 1) * p, b + 2 * p, sizeof(double) * p,
-                                     hipMemcpyDeviceToHost, s));
-            SHK_HIP_S(hipMemcpyAsync(out_im + (k + 1) * p, b + 3 * p, sizeof(double) * p,
-                                     hipMemcpyDeviceToHost, s));
+            SHK_CHECK(hs.download(out_re + (k + 1) * p, b + 2 * p, bp));
+            SHK_CHECK(hs.download(out_im + (k + 1) * p, b + 3 * p, bp));
         }
-        SHK_HIP_S(hipStreamSynchronize(s));
+        SHK_CHECK(hs.finish());
     }
     return SHK_OK;
 }

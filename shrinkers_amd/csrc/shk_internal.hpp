@@ -33,7 +33,9 @@ enum CacheSlot : int {
     CACHE_EK_SCR = 5,        // El Karoui fit kernels: per-point scratch
     CACHE_EK_STEP = 6,       // fused fit step: sample transform and its gradient
     CACHE_ADJ_TMP = 7,       // all-points adjoint by two sweeps: the query gradient
-    kCacheSlots = 8
+    CACHE_HOST_0 = 8,        // host-array calls: the k-th device copy of a call
+    kCacheHostSlots = 6,     //   takes slot CACHE_HOST_0 + k
+    kCacheSlots = CACHE_HOST_0 + kCacheHostSlots
 };
 // Returns the slot's buffer (>= bytes) in *out, ordered on stream s.
 int cache_acquire(int slot, size_t bytes, hipStream_t s, void **out);
