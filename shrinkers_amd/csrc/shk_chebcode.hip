@@ -74,7 +74,7 @@ struct ChebTree {
     // (CACHE_CHEB_SRC / CACHE_CHEB_NODES), released at destruction on `stream`
     bool cached_src = false, cached_nodes = false;
     hipStream_t stream = nullptr;
-    hipEvent_t ready = nullptr;  // recorded when the build's last kernel is enqueued
+    hipEvent_t ready = nullptr;  // recorded after the build's last kernel (none if transient)
 };
 
 namespace {
@@ -1028,6 +1028,132 @@ __global__ __launch_bounds__(kBaryLanes) void node_kernel(
     };
     bary_any(row, b - a, pos, tab + n, t.w + (size_t)k * n,
                     panel32 + (size_t)k * 2 * n + n, n, bl.sq, bl.sm, bl.shit);
+}
+
+// node_kernel for the presets' orders with kNodeGroup nodes per wavefront:
+// lane group g (kGroupLanes lanes) writes node list[G*blockIdx + g]'s
+// records and positions and, for a leaf of up to kGroupRowsMax sources,
+// forms its rows kGroupLanes at a time; lane g*kGroupLanes + j then adds
+// w_j over those rows in row order.  A larger leaf is weighted by the whole
+// wavefront afterwards.  Per node the arithmetic and its order are those of
+// node_kernel, so records and weights are the same bit for bit; the gain is
+// a quarter of the workgroups, each one dependent-load chain.
+constexpr int kGroupLanes = 16;
+constexpr int kNodeGroup = kBaryLanes / kGroupLanes;
+constexpr int kGroupRowsMax = 2 * kGroupLanes;
+
+__host__ __device__ inline int node_group_lds_doubles(int n) {
+    return bary_lds_doubles(n) + kNodeGroup * n;
+}
+
+template <int NT>
+__global__ __launch_bounds__(kBaryLanes) void node_group_kernel(
+    const double *__restrict__ sorted, int count, double theta_sq, const int *__restrict__ list,
+    const int *__restrict__ b_lo, const int *__restrict__ b_hi, const int *__restrict__ b_left,
+    const int *__restrict__ b_size, const int *__restrict__ b_pre, const double *__restrict__ tab,
+    TreeArrays t, int2 *__restrict__ mrec, float4 *__restrict__ rec32, int4 *__restrict__ link,
+    float *__restrict__ panel32) {
+    static_assert(NT <= kGroupLanes, "one lane per weight in a lane group");
+    extern __shared__ double lds[];
+    const int lane = threadIdx.x;
+    const int g = lane / kGroupLanes, r = lane - g * kGroupLanes;
+    const int slot = blockIdx.x * kNodeGroup + g;
+    const BaryLds bl = bary_lds(lds, NT);
+    double *pos = lds + bary_lds_doubles(NT);  // kNodeGroup x NT positions
+    int k = 0, a = 0, cnt = 0;
+    bool leaf = false;
+    if (slot < count) {
+        const int v = list[slot];
+        k = b_pre[v];
+        a = b_lo[v];
+        const int b = b_hi[v], l = b_left[v], size = b_size[v];
+        const int right = (l < 0) ? -1 : b_pre[l + 1];
+        const double lo = sorted[a], hi = sorted[b - 1];
+        const double hw = 0.5 * (hi - lo);
+        const double acc = (hw * hw) / theta_sq;
+        if (r == 0) {
+            t.lo[k] = lo;
+            t.hi[k] = hi;
+            t.acc[k] = acc;
+            t.lo_idx[k] = a;
+            t.hi_idx[k] = b;
+            t.right[k] = right;
+            t.esc[k] = k + size;
+            mrec[v] = make_int2(k, right);
+            rec32[k] = make_float4((float)lo, (float)hi, (float)acc, 0.0f);
+            link[k] = make_int4(right, k + size, a, b);
+        }
+        const double c = 0.5 * (lo + hi), hl = 0.5 * (hi - lo);
+        if (r < NT) {
+            const double x = c + hl * tab[r];
+            t.nodes[(size_t)k * NT + r] = x;
+            panel32[(size_t)k * 2 * NT + r] = (float)x;
+            pos[g * NT + r] = x;
+        }
+        leaf = right < 0;
+        cnt = b - a;
+    }
+    const bool small = leaf && cnt <= kGroupRowsMax;
+    const unsigned long long any_small = __ballot(small);
+    const unsigned long long big = __ballot(leaf && !small);
+    if (!any_small && !big) return;
+    wave_sync();
+    double lr[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) lr[j] = tab[NT + j];
+    if (any_small) {
+        double tr[NT];
+#pragma unroll
+        for (int j = 0; j < NT; ++j) tr[j] = pos[g * NT + j];
+        const int chunks = __ballot(small && cnt > kGroupLanes) ? 2 : 1;
+        double wacc = 0.0;
+        for (int ch = 0; ch < chunks; ++ch) {
+            const int rr = ch * kGroupLanes + r;
+            int hit = -2;  // as in bary_accumulate
+            double m = 0.0;
+            if (small && rr < cnt) {
+                const double x = sorted[a + rr];
+                const double mass = 1.0;
+                double s = 0.0;
+                hit = -1;
+#pragma unroll
+                for (int j = 0; j < NT; ++j) {
+                    const double d = x - tr[j];
+                    const double q = lr[j] / d;
+                    bl.sq[lane * NT + j] = q;
+                    s += q;
+                    hit = (hit < 0 && d == 0.0) ? j : hit;
+                }
+                m = (hit >= 0) ? mass : mass * (1.0 / s);
+            }
+            bl.sm[lane] = m;
+            bl.shit[lane] = hit;
+            wave_sync();
+            const int rows = small ? min(kGroupLanes, cnt - ch * kGroupLanes) : 0;
+            if (r < NT && rows > 0)
+                wacc = bary_rows(wacc, g * kGroupLanes, rows, r, NT, bl.sq, bl.sm, bl.shit);
+            wave_sync();  // the next chunk overwrites sq, sm, shit
+        }
+        if (small && r < NT) {
+            t.w[(size_t)k * NT + r] = wacc;
+            panel32[(size_t)k * 2 * NT + NT + r] = (float)wacc;
+        }
+    }
+    // Leaves above kGroupRowsMax sources: the whole wavefront, one at a time.
+    for (int gg = 0; gg < kNodeGroup; ++gg) {
+        if (!((big >> (gg * kGroupLanes)) & 1ull)) continue;
+        const int src = gg * kGroupLanes;
+        const int kg = __shfl(k, src, kBaryLanes), ag = __shfl(a, src, kBaryLanes);
+        const int cg = __shfl(cnt, src, kBaryLanes);
+        const double *xs = sorted + ag;
+        auto row = [xs](int rr, double &x, double &mass) {
+            x = xs[rr];
+            mass = 1.0;
+        };
+        bary_accumulate<NT>(row, cg, pos + gg * NT, tab + NT, t.w + (size_t)kg * NT,
+                            panel32 + (size_t)kg * 2 * NT + NT, NT, bl.sq, bl.sm, bl.shit);
+        wave_sync();  // the next leaf overwrites sq, sm, shit
+    }
 }
 
 // Weights of the node in slot v from its children's: the 2n rows are the
@@ -2179,9 +2305,19 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
     // 3. node records and mirrors, leaf weights: one wavefront per node
     TreeArrays ta{t->lo, t->hi, t->acc, t->nodes, t->w, t->lo_idx, t->hi_idx, t->right, t->esc};
     const size_t lds = bary_lds_bytes(n);
-    hipLaunchKernelGGL(node_kernel, dim3(N), dim3(kBaryLanes), lds + sizeof(double) * n, stream,
-                       t->sorted, n, t->theta_sq, d_list, b_lo, b_hi, b_left, b_size, b_pre, d_tab,
-                       ta, d_mrec, t->rec32, t->link, t->panel32);
+    if (n == 8 || n == 11 || n == 12) {
+        const dim3 grid((unsigned)((N + kNodeGroup - 1) / kNodeGroup));
+        const size_t bytes = sizeof(double) * (size_t)node_group_lds_doubles(n);
+        auto kern = (n == 8) ? node_group_kernel<8>
+                             : (n == 11) ? node_group_kernel<11> : node_group_kernel<12>;
+        hipLaunchKernelGGL(kern, grid, dim3(kBaryLanes), bytes, stream, t->sorted, N,
+                           t->theta_sq, d_list, b_lo, b_hi, b_left, b_size, b_pre, d_tab, ta,
+                           d_mrec, t->rec32, t->link, t->panel32);
+    } else {
+        hipLaunchKernelGGL(node_kernel, dim3(N), dim3(kBaryLanes), lds + sizeof(double) * n,
+                           stream, t->sorted, n, t->theta_sq, d_list, b_lo, b_hi, b_left, b_size,
+                           b_pre, d_tab, ta, d_mrec, t->rec32, t->link, t->panel32);
+    }
     SHK_HIP(hipGetLastError());
     // 4. parents, deepest first.  Workgroups of several wavefronts keep
     // their LDS within 64 KiB.
@@ -2249,9 +2385,12 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
                            t->panel32);
         SHK_HIP(hipGetLastError());
     }
-    // Evaluations on any stream wait for this event (cheb_eval*).
-    SHK_HIP(hipEventCreateWithFlags(&t->ready, hipEventDisableTiming));
-    SHK_HIP(hipEventRecord(t->ready, stream));
+    // Evaluations on any stream wait for this event (cheb_eval*).  A
+    // transient tree is evaluated on its build stream only and needs none.
+    if (!transient) {
+        SHK_HIP(hipEventCreateWithFlags(&t->ready, hipEventDisableTiming));
+        SHK_HIP(hipEventRecord(t->ready, stream));
+    }
     *out = t.release();
     return 0;
 }
@@ -2270,7 +2409,7 @@ int cheb_params(const ChebTree *t, ChebParams *prm) {
 int cheb_export(const ChebTree *t, double *lo, double *hi, int64_t *lo_idx, int64_t *hi_idx,
                 int32_t *right, double *nodes, double *w, hipStream_t stream) {
     const size_t N = t->N, n = t->n;
-    SHK_HIP(hipStreamWaitEvent(stream, t->ready, 0));
+    if (t->ready) SHK_HIP(hipStreamWaitEvent(stream, t->ready, 0));
     if (lo) SHK_HIP(hipMemcpyAsync(lo, t->lo, sizeof(double) * N, hipMemcpyDeviceToHost, stream));
     if (hi) SHK_HIP(hipMemcpyAsync(hi, t->hi, sizeof(double) * N, hipMemcpyDeviceToHost, stream));
     if (right)
@@ -2302,7 +2441,7 @@ int cheb_eval(const ChebTree *t, const double *d_query, size_t nq, double eta, d
         set_error("chebcode eval: too many query points");
         return 6;
     }
-    SHK_HIP(hipStreamWaitEvent(stream, t->ready, 0));
+    if (t->ready) SHK_HIP(hipStreamWaitEvent(stream, t->ready, 0));
     EvalView v = view_of(t);
     // Dense queries (the tree's own sources, or at least as many points as
     // sources) share near fields within a wavefront: walk together.
@@ -2353,7 +2492,7 @@ int cheb_count(const ChebTree *t, const double *d_query, size_t nq, double eta,
         set_error("chebcode count: too many query points");
         return 6;
     }
-    SHK_HIP(hipStreamWaitEvent(stream, t->ready, 0));
+    if (t->ready) SHK_HIP(hipStreamWaitEvent(stream, t->ready, 0));
     unsigned long long *d_counts = nullptr;
     SHK_HIP(hipMallocAsync((void **)&d_counts, 2 * sizeof(unsigned long long), stream));
     EvalView v = view_of(t);
@@ -2382,7 +2521,7 @@ int cheb_count(const ChebTree *t, const double *d_query, size_t nq, double eta,
 
 int cheb_eval_x2(const ChebTree *t, double eta0, double eta1, double *d_re0, double *d_im0,
                  double *d_re1, double *d_im1, hipStream_t stream) {
-    SHK_HIP(hipStreamWaitEvent(stream, t->ready, 0));
+    if (t->ready) SHK_HIP(hipStreamWaitEvent(stream, t->ready, 0));
     EvalView v = view_of(t);
     const size_t nq = t->p;
     hipLaunchKernelGGL(eval_x2_kernel, dim3(blocks_for(nq, 256)), dim3(256), 0, stream, v, (int)nq,

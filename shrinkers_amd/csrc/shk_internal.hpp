@@ -268,9 +268,10 @@ struct ChebParams {
 };
 
 // Builds the tree on the GPU from device eigenvalues (any order).  A
-// transient tree (one evaluated and destroyed by the same thread before its
-// next build) keeps its arrays in that thread's buffer cache instead of
-// allocating them; its destruction then needs no synchronisation.
+// transient tree (one evaluated on its build stream and destroyed by the same
+// thread before its next build) keeps its arrays in that thread's buffer
+// cache instead of allocating them; its destruction then needs no
+// synchronisation, and its evaluations no event.
 int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **out,
                hipStream_t stream, int transient = 0);
 void cheb_destroy(ChebTree *t);
