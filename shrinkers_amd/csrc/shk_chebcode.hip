@@ -2305,7 +2305,9 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
     // 3. node records and mirrors, leaf weights: one wavefront per node
     TreeArrays ta{t->lo, t->hi, t->acc, t->nodes, t->w, t->lo_idx, t->hi_idx, t->right, t->esc};
     const size_t lds = bary_lds_bytes(n);
-    if (n == 8 || n == 11 || n == 12) {
+    // Grouped at the presets' orders when leaves fit a lane group's rows;
+    // wide leaves (the grid path's leaf_cap) keep one wavefront per node.
+    if ((n == 8 || n == 11 || n == 12) && prm.leaf_cap <= kGroupRowsMax) {
         const dim3 grid((unsigned)((N + kNodeGroup - 1) / kNodeGroup));
         const size_t bytes = sizeof(double) * (size_t)node_group_lds_doubles(n);
         auto kern = (n == 8) ? node_group_kernel<8>
