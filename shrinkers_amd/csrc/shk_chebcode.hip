@@ -43,6 +43,7 @@
 //   difference within one f32 rounding so the leaf error (~1e-7) stays far
 //   below the preset's interpolation error (~2e-5).
 #include "shk_internal.hpp"
+#include "shk_pair.hpp"
 
 #include <rocprim/device/device_radix_sort.hpp>
 
@@ -84,14 +85,6 @@ constexpr int kTopoThreads = 1024;
 constexpr int kMaxN = 120;       // largest interpolation order (LDS of the build)
 constexpr float kOutsideTheta2 = 1.69f;  // chebcode.rs:108
 constexpr double kPi = 3.14159265358979323846;  // std::f64::consts::PI
-
-__device__ __forceinline__ double rcp_f64(double x) {
-    double r = __builtin_amdgcn_rcp(x);
-    double e = __builtin_fma(-x, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    e = __builtin_fma(-x, r, 1.0);
-    return __builtin_fma(r, e, r);
-}
 
 __global__ void iota_kernel(int32_t *__restrict__ v, int p) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1616,13 +1609,21 @@ __global__ __launch_bounds__(256) void eval_f64_kernel(EvalView v, const double 
                                      : accept_f64(v, node, x, eta2);
         if (active && accept && NT > 0) {
             double afr0 = 0.0, afr1 = 0.0, afi0 = 0.0, afi1 = 0.0;
+            // the terms' reciprocals two per v_rcp_f64 (rcp_pairs_f64)
+            constexpr int kPair = (kP >= 2) ? (kP & ~1) : 2;  // (NT = 0 never runs this)
+            double dd[kPair], den[kPair], inv[kPair];
+#pragma unroll
+            for (int j = 0; j < kPair; ++j) {
+                dd[j] = x - pts[j];
+                den[j] = __builtin_fma(dd[j], dd[j], eta2);
+            }
+            rcp_pairs_f64<kPair>(den, inv);
 #pragma unroll
             for (int j = 0; j + 2 <= kP; j += 2) {
-                const double d0 = x - pts[j], d1 = x - pts[j + 1];
-                const double q0 = pws[j] * rcp_f64(__builtin_fma(d0, d0, eta2));
-                const double q1 = pws[j + 1] * rcp_f64(__builtin_fma(d1, d1, eta2));
-                afr0 = __builtin_fma(q0, d0, afr0);
-                afr1 = __builtin_fma(q1, d1, afr1);
+                const double q0 = pws[j] * inv[j];
+                const double q1 = pws[j + 1] * inv[j + 1];
+                afr0 = __builtin_fma(q0, dd[j], afr0);
+                afr1 = __builtin_fma(q1, dd[j + 1], afr1);
                 afi0 += q0;
                 afi1 += q1;
             }
