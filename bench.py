@@ -296,6 +296,17 @@ def chebcode_leg(shk, sync) -> dict:
                          "nodes": b.node_count, "near_pairs": near, "far_terms": far,
                          "eval_tflops": flops / (ms_eval * 1e-3) / 1e12,
                          "near_field_tflops_lower_bound": 7.0 * near / (ms_eval * 1e-3) / 1e12}
+            if name in ("chebcode_fast", "chebcode"):
+                # the one-shot transform (compute_all_stieltjes with the preset):
+                # build, evaluate and release in one call
+                for _ in range(3):
+                    shk.all_points_dev(d, eta, name)
+                sync()
+                t0 = time.perf_counter()
+                for _ in range(10):
+                    shk.all_points_dev(d, eta, name)
+                sync()
+                row[name]["ms_transform_one_shot"] = (time.perf_counter() - t0) / 10 * 1e3
             if name == "chebcode":
                 # evaluate_many: four etas, two per traversal (host outputs)
                 etas = [eta * f for f in (0.5, 1.0, 2.0, 4.0)]
