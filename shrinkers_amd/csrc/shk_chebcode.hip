@@ -1314,8 +1314,9 @@ __global__ __launch_bounds__(kMergeMaxThreads) void merge_top_kernel(
 // each level then reads and writes LDS only (weights also go to the tree and
 // the f32 panel) behind an LDS-only barrier.  Per node the arithmetic and
 // its order are merge_group's, so the weights are the same bit for bit.
-constexpr int kTopLdsWaves = 4;
-constexpr size_t kTopLdsMax = 65536;
+constexpr int kTopLdsWaves = 16;          // the top-level merge workgroup
+constexpr size_t kTopLdsMax = 65536;      // LDS of a subtree merge workgroup (two per CU)
+constexpr size_t kTopLdsWgMax = 163840;   // LDS of the one top-level merge workgroup
 
 __host__ __device__ inline size_t top_lds_bytes(int slots, int n, int waves) {
     return (size_t)slots * (2 * sizeof(int) + 2 * (size_t)n * sizeof(double)) +
@@ -2322,8 +2323,9 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
                            b_pre, d_tab, ta, d_mrec, t->rec32, t->link, t->panel32);
     }
     SHK_HIP(hipGetLastError());
-    // 4. parents, deepest first.  Workgroups of several wavefronts keep
-    // their LDS within 64 KiB.
+    // 4. parents, deepest first.  The subtree workgroups keep their LDS
+    // within 64 KiB (two per CU); the one top-level workgroup may use the
+    // whole 160 KiB of a CU.
     const int lds_waves = (int)std::max<size_t>(1, 65536 / lds);
     const int F = info[kInfoF], D = info[kInfoD];
     if (F > 0 && (n == 8 || n == 11 || n == 12)) {
@@ -2360,7 +2362,8 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
     // finished children
     const bool top_lds = (n == 8 || n == 11 || n == 12);
     if (top_lds)
-        while (small < top && top_lds_bytes(lvl[small + 2], n, kTopLdsWaves) <= kTopLdsMax)
+        while (small < top && top_lds_bytes(lvl[small + 2], n, kTopLdsWaves) <= kTopLdsWgMax &&
+               groups(small) <= kMergeTopRounds * kTopLdsWaves)
             ++small;
     else
         while (small < top && groups(small) <= kMergeTopRounds * top_waves) ++small;
