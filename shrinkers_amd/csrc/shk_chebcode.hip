@@ -1259,7 +1259,8 @@ __device__ __forceinline__ void merge_level(int n, int lb, int cnt, int wv, int 
 }
 
 constexpr int kMergeMaxThreads = 1024;
-constexpr int kMergeSubWaves = 4;   // wavefronts per subtree workgroup
+constexpr int kMergeSubWaves = 8;   // wavefronts per subtree workgroup
+constexpr size_t kSubLdsMax = 81920;  // LDS of a subtree workgroup (two per CU)
 constexpr int kMergeTopRounds = 2;  // a top level of at most this many groups per wavefront
                                     // merges in the one-workgroup kernel
 
@@ -1315,7 +1316,6 @@ __global__ __launch_bounds__(kMergeMaxThreads) void merge_top_kernel(
 // the f32 panel) behind an LDS-only barrier.  Per node the arithmetic and
 // its order are merge_group's, so the weights are the same bit for bit.
 constexpr int kTopLdsWaves = 16;          // the top-level merge workgroup
-constexpr size_t kTopLdsMax = 65536;      // LDS of a subtree merge workgroup (two per CU)
 constexpr size_t kTopLdsWgMax = 163840;   // LDS of the one top-level merge workgroup
 
 __host__ __device__ inline size_t top_lds_bytes(int slots, int n, int waves) {
@@ -2324,14 +2324,14 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
     }
     SHK_HIP(hipGetLastError());
     // 4. parents, deepest first.  The subtree workgroups keep their LDS
-    // within 64 KiB (two per CU); the one top-level workgroup may use the
+    // within 80 KiB (two per CU); the one top-level workgroup may use the
     // whole 160 KiB of a CU.
     const int lds_waves = (int)std::max<size_t>(1, 65536 / lds);
     const int F = info[kInfoF], D = info[kInfoD];
     if (F > 0 && (n == 8 || n == 11 || n == 12)) {
-        // LDS form: subtrees of up to `cap` nodes in a 64 KiB workgroup
+        // LDS form: subtrees of up to `cap` nodes in an 80 KiB workgroup
         const size_t fixed = (size_t)kMergeSubWaves * lds;
-        const int cap = (int)((kTopLdsMax - fixed) / (2 * sizeof(int) + 2 * sizeof(double) * n));
+        const int cap = (int)((kSubLdsMax - fixed) / (2 * sizeof(int) + 2 * sizeof(double) * n));
         const size_t bytes = fixed + (size_t)cap * (2 * sizeof(int) + 2 * sizeof(double) * n);
         const dim3 blk(kMergeSubWaves * kBaryLanes);
         if (n == 8)
