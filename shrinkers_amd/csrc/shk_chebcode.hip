@@ -1474,6 +1474,17 @@ __global__ __launch_bounds__(kMergeSubWaves * kBaryLanes) void merge_sub_lds_ker
 
 // ---- evaluation --------------------------------------------------------------
 
+// Passes x through an asm move whose other inputs are loaded node data:
+// the loads then complete before x's use (the leaf test of a visit), and
+// the compiler cannot defer them behind that test into a second round trip.
+// The asm has no side effects and touches no memory.
+template <class T>
+__device__ __forceinline__ int hold2(int x, T a, T b) {
+    int y;
+    asm("s_mov_b32 %0, %1" : "=s"(y) : "s"(x), "s"(a), "s"(b));
+    return y;
+}
+
 struct EvalView {
     const double *sorted;
     const float *s_hi, *s_lo;
@@ -1566,7 +1577,11 @@ __device__ __forceinline__ bool accept_fast(const float4 r, float x, float eta2)
 // NT > 0 (the wave walk at order NT = n): a visit loads the node's link,
 // interval, acceptance radius and panel together, one round trip instead of
 // three dependent ones; the arithmetic and its order are unchanged.
-template <bool kWave, int NT = 0>
+// kHold (NT > 0): asm moves that consume every loaded value keep the
+// compiler from issuing the panel loads behind the leaf test, so a visit is
+// one round trip.  That pays where the walk is latency-bound (fewer than
+// about two wavefronts per SIMD) and costs 2-5 % where it is issue-bound.
+template <bool kWave, int NT = 0, bool kHold = false>
 __global__ __launch_bounds__(256) void eval_f64_kernel(EvalView v, const double *__restrict__ query,
                                                        int nq, int queries_are_sources, double eta,
                                                        double scale, double *__restrict__ out_re,
@@ -1583,7 +1598,7 @@ __global__ __launch_bounds__(256) void eval_f64_kernel(EvalView v, const double 
     constexpr int kP = (NT > 0) ? NT : 1;
     while (node < v.N) {
         if constexpr (kWave) node = __builtin_amdgcn_readfirstlane(node);
-        const int4 lk = v.link[node];  // right, esc, lo_idx, hi_idx
+        int4 lk = v.link[node];  // right, esc, lo_idx, hi_idx
         double rlo = 0.0, rhi = 0.0, racc = 0.0, pts[kP], pws[kP];
         if constexpr (NT > 0) {
             rlo = v.lo[node];
@@ -1593,6 +1608,13 @@ __global__ __launch_bounds__(256) void eval_f64_kernel(EvalView v, const double 
             for (int j = 0; j < NT; ++j) {
                 pts[j] = v.nodes[(size_t)node * NT + j];
                 pws[j] = v.w[(size_t)node * NT + j];
+            }
+            if constexpr (kHold) {
+                int leaf_bits = hold2(lk.x, rlo, rhi);
+                leaf_bits = hold2(leaf_bits, racc, racc);
+#pragma unroll
+                for (int j = 0; j < NT; ++j) leaf_bits = hold2(leaf_bits, pts[j], pws[j]);
+                lk.x = leaf_bits;
             }
         }
         const bool active = !kWave || node >= skip;
@@ -1675,7 +1697,7 @@ __global__ __launch_bounds__(256) void eval_f64_kernel(EvalView v, const double 
 }
 
 // NT > 0 as in eval_f64_kernel: record and f32 panel loaded with the link.
-template <bool kWave, int NT = 0>
+template <bool kWave, int NT = 0, bool kHold = false>
 __global__ __launch_bounds__(256) void eval_fast_kernel(EvalView v, const double *__restrict__ query,
                                                         int nq, int queries_are_sources, double eta,
                                                         double scale, double *__restrict__ out_re,
@@ -1697,7 +1719,7 @@ __global__ __launch_bounds__(256) void eval_fast_kernel(EvalView v, const double
     constexpr int kP = (NT > 0) ? NT : 1;
     while (node < v.N) {
         if constexpr (kWave) node = __builtin_amdgcn_readfirstlane(node);
-        const int4 lk = v.link[node];
+        int4 lk = v.link[node];
         float4 rec = make_float4(0.f, 0.f, 0.f, 0.f);
         float pts[kP], pws[kP];
         if constexpr (NT > 0) {
@@ -1706,6 +1728,13 @@ __global__ __launch_bounds__(256) void eval_fast_kernel(EvalView v, const double
             for (int j = 0; j < NT; ++j) {
                 pts[j] = v.panel32[(size_t)node * 2 * NT + j];
                 pws[j] = v.panel32[(size_t)node * 2 * NT + NT + j];
+            }
+            if constexpr (kHold) {
+                int leaf_bits = hold2(lk.x, rec.x, rec.y);
+                leaf_bits = hold2(leaf_bits, rec.z, rec.z);
+#pragma unroll
+                for (int j = 0; j < NT; ++j) leaf_bits = hold2(leaf_bits, pts[j], pws[j]);
+                lk.x = leaf_bits;
             }
         }
         const bool active = !kWave || node >= skip;
@@ -2440,6 +2469,9 @@ int cheb_export(const ChebTree *t, double *lo, double *hi, int64_t *lo_idx, int6
     return 0;
 }
 
+// Wave walks over fewer queries than this hold each visit's loads together.
+constexpr int kEvalHoldMax = 1 << 19;
+
 int cheb_eval(const ChebTree *t, const double *d_query, size_t nq, double eta, double scale,
               int queries_are_sources, double *d_out_re, double *d_out_im, hipStream_t stream) {
     if (nq == 0) return 0;
@@ -2454,7 +2486,11 @@ int cheb_eval(const ChebTree *t, const double *d_query, size_t nq, double eta, d
     const bool wave = queries_are_sources || nq >= t->p;
     const dim3 grid(blocks_for(nq, 256)), block(256);
     ProfileMark m = profile_begin(stream);
-    if (t->f32_far && wave && t->n == 8)
+    const bool hold = nq < (size_t)kEvalHoldMax;
+    if (t->f32_far && wave && t->n == 8 && hold)
+        hipLaunchKernelGGL((eval_fast_kernel<true, 8, true>), grid, block, 0, stream, v, d_query,
+                           (int)nq, queries_are_sources, eta, scale, d_out_re, d_out_im);
+    else if (t->f32_far && wave && t->n == 8)
         hipLaunchKernelGGL((eval_fast_kernel<true, 8>), grid, block, 0, stream, v, d_query,
                            (int)nq, queries_are_sources, eta, scale, d_out_re, d_out_im);
     else if (t->f32_far && wave)
@@ -2467,8 +2503,14 @@ int cheb_eval(const ChebTree *t, const double *d_query, size_t nq, double eta, d
     else if (t->f32_far)
         hipLaunchKernelGGL(eval_fast_kernel<false>, grid, block, 0, stream, v, d_query, (int)nq,
                            queries_are_sources, eta, scale, d_out_re, d_out_im);
+    else if (wave && t->n == 11 && hold)
+        hipLaunchKernelGGL((eval_f64_kernel<true, 11, true>), grid, block, 0, stream, v, d_query,
+                           (int)nq, queries_are_sources, eta, scale, d_out_re, d_out_im);
     else if (wave && t->n == 11)
         hipLaunchKernelGGL((eval_f64_kernel<true, 11>), grid, block, 0, stream, v, d_query,
+                           (int)nq, queries_are_sources, eta, scale, d_out_re, d_out_im);
+    else if (wave && t->n == 12 && hold)
+        hipLaunchKernelGGL((eval_f64_kernel<true, 12, true>), grid, block, 0, stream, v, d_query,
                            (int)nq, queries_are_sources, eta, scale, d_out_re, d_out_im);
     else if (wave && t->n == 12)
         hipLaunchKernelGGL((eval_f64_kernel<true, 12>), grid, block, 0, stream, v, d_query,
