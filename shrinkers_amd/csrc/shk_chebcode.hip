@@ -1574,13 +1574,15 @@ __device__ __forceinline__ bool accept_fast(const float4 r, float x, float eta2)
 //
 // kWave = false: each lane walks on its own (sparse queries, such as a grid
 // spanning the spectrum, whose near fields differ).
-// NT > 0 (the wave walk at order NT = n): a visit loads the node's link,
-// interval, acceptance radius and panel together, one round trip instead of
-// three dependent ones; the arithmetic and its order are unchanged.
-// kHold (NT > 0): asm moves that consume every loaded value keep the
-// compiler from issuing the panel loads behind the leaf test, so a visit is
-// one round trip.  That pays where the walk is latency-bound (fewer than
-// about two wavefronts per SIMD) and costs 2-5 % where it is issue-bound.
+// NT > 0 (the wave walk at order NT = n): a visit's link, interval,
+// acceptance radius and panel are loaded at its start, from the node index
+// alone, instead of three dependent loads; the arithmetic and its order are
+// unchanged.  The compiler still issues the panel loads behind the leaf
+// test, a second round trip for an internal node.
+// kHold (NT > 0): asm moves that consume every loaded value keep the panel
+// loads ahead of the leaf test, so a visit is one round trip.  That pays
+// where the walk is latency-bound (fewer than about two wavefronts per SIMD)
+// and costs 2-5 % where it is issue-bound.
 template <bool kWave, int NT = 0, bool kHold = false>
 __global__ __launch_bounds__(256) void eval_f64_kernel(EvalView v, const double *__restrict__ query,
                                                        int nq, int queries_are_sources, double eta,
