@@ -330,3 +330,37 @@ def test_symmetric_f64_paired_fallback(shk):
     big = np.ldexp(1.0, 300)
     rs, is_ = shk.compute_all_stieltjes(ev * big, eta * big, "blocked")
     assert_close(rs * big, is_ * big, r, i, 1e-13, "symmetric f64, all pairs on the fallback")
+
+
+def test_host_calls_from_several_threads(shk):
+    """Host-array entry points called from four threads at once give the
+    one-thread results bit for bit. Each calling thread has its own stream,
+    cached device buffers, pinned staging buffer and (ChebCode) host mailbox;
+    ctypes releases the GIL, so the calls overlap on the device. Sizes span the
+    split-tile and symmetric exact kernels, a grid, and a one-shot ChebCode
+    build; f32 covers the float32 entry."""
+    from concurrent.futures import ThreadPoolExecutor
+    from shrinkers_amd import spectra as S
+    cases = []
+    for k, p in enumerate((3000, 9000, 30000, 50000)):
+        ev = S.mp_inverse_cdf_spectrum(p)
+        cases.append((ev, 1.0 / np.sqrt(p), np.linspace(ev.min(), ev.max(), 200 + k)))
+
+    def run(case):
+        ev, eta, grid = case
+        outs = [shk.compute_all_stieltjes(ev, eta, "blocked", precision="f32c"),
+                shk.compute_all_stieltjes(ev, eta, "blocked"),
+                shk.compute_stieltjes_at_points(grid, ev, eta, "blocked"),
+                shk.compute_all_stieltjes(ev, eta, "chebcode_fast"),
+                shk.compute_all_stieltjes_f32(ev, eta, "blocked")]
+        return [np.asarray(o) for o in outs]
+
+    want = [run(c) for c in cases]
+    with ThreadPoolExecutor(4) as pool:
+        for rep in range(3):
+            got = list(pool.map(run, cases[rep % 2:] + cases[:rep % 2]))
+            order = cases[rep % 2:] + cases[:rep % 2]
+            for case, g in zip(order, got):
+                w = want[next(i for i, c in enumerate(cases) if c is case)]
+                for a, b in zip(w, g):
+                    assert a.dtype == b.dtype and np.array_equal(a, b), f"rep {rep}, p={case[0].size}"
