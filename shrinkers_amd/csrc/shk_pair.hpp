@@ -64,6 +64,20 @@ __device__ __forceinline__ void rcp_pairs_f64(const double (&t)[R], double (&r)[
     }
 }
 
+// Tile pair (a, b), a <= b, of the symmetric kernels' workgroup `id` in
+// [0, nt(nt+1)/2), in row-major order (row a holds b = a .. nt-1).  A
+// one-dimensional grid of exactly these pairs dispatches no workgroup that
+// would only exit, as the lower half of an nt x nt grid would.
+__device__ __forceinline__ void tile_pair_of(int id, int nt, int &a, int &b) {
+    auto start = [nt](int x) { return x * (2 * nt - x + 1) / 2; };  // first id of row x
+    const double w = 2.0 * nt + 1.0;
+    int r = (int)(0.5 * (w - sqrt(w * w - 8.0 * id)));
+    while (r > 0 && start(r) > id) --r;
+    while (r + 1 < nt && start(r + 1) <= id) ++r;
+    a = r;
+    b = r + (id - start(r));
+}
+
 constexpr int kF32Run = 32;  // f32 terms per FP64 fold
 
 template <int FORM>

@@ -240,8 +240,8 @@ __device__ __forceinline__ void sym_sweep_f64(const double *__restrict__ s_src, 
 __global__ __launch_bounds__(kSL * kSW) void sym_f64_kernel(const double *__restrict__ ev, int p,
                                                             double eta, int nt,
                                                             double *__restrict__ part) {
-    const int b = blockIdx.x, a = blockIdx.y;
-    if (b < a) return;  // upper triangle of tile pairs only (uniform per workgroup)
+    int a, b;  // query tile a, source tile b >= a
+    tile_pair_of((int)blockIdx.x, nt, a, b);
     __shared__ double s_src[kSW][kSQ];
     __shared__ double s_jbuf[kSW * 2 * kSB64 * kJRow];  // j-side lane sums / i-side combine
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -273,8 +273,8 @@ static_assert(kSW * 2 * kSB64 * kJRow >= 2 * kST, "FP64 combine buffer fits in s
 __global__ __launch_bounds__(kSL * kSW) void sym_f32c_kernel(const double *__restrict__ ev, int p,
                                                              double eta, int nt,
                                                              double *__restrict__ part) {
-    const int b = blockIdx.x, a = blockIdx.y;
-    if (b < a) return;  // upper triangle of tile pairs only (uniform per workgroup)
+    int a, b;  // query tile a, source tile b >= a
+    tile_pair_of((int)blockIdx.x, nt, a, b);
     __shared__ double s_src[kSW][kSQ];                      // 8 KB
     __shared__ double s_jbuf[kSW * 2 * kSB * kJRow / 2];    // j-side lane sums / i-side combine
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -356,10 +356,10 @@ int exact_sym(const double *ev, size_t p, double eta, double scale, int f64, dou
     if (int e = cache.get(2 * (size_t)nt * p, &part)) return e;
     ProfileMark m = profile_begin(stream);
     if (f64)
-        hipLaunchKernelGGL(sym_f64_kernel, dim3(nt, nt), dim3(kSL * kSW), 0, stream, ev, (int)p,
+        hipLaunchKernelGGL(sym_f64_kernel, dim3(nt * (nt + 1) / 2), dim3(kSL * kSW), 0, stream, ev, (int)p,
                            eta, nt, part);
     else
-        hipLaunchKernelGGL(sym_f32c_kernel, dim3(nt, nt), dim3(kSL * kSW), 0, stream, ev, (int)p,
+        hipLaunchKernelGGL(sym_f32c_kernel, dim3(nt * (nt + 1) / 2), dim3(kSL * kSW), 0, stream, ev, (int)p,
                            eta, nt, part);
     profile_end(m, stream);
     int st = 0;

@@ -279,8 +279,8 @@ static_assert(kJQ % kAB == 0 && kJQ % kAB64 == 0, "batches tile the wave's sourc
 __global__ __launch_bounds__(kJL * kJW) void symadj_f32c_kernel(
     const double *__restrict__ ev, const double *__restrict__ g_re,
     const double *__restrict__ g_im, int p, double eta, int nt, double *__restrict__ part) {
-    const int b = blockIdx.x, a = blockIdx.y;
-    if (b < a) return;  // upper triangle of tile pairs only (uniform per workgroup)
+    int a, b;  // query tile a, source tile b >= a
+    tile_pair_of((int)blockIdx.x, nt, a, b);
     __shared__ double s_src[kJW][kJQ];
     __shared__ float s_gr[kJW][kJQ], s_gi[kJW][kJQ];
     __shared__ double s_jbuf[kJbufBytes / 8];
@@ -353,8 +353,8 @@ __global__ __launch_bounds__(kJL * kJW) void symadj_f32c_kernel(
 __global__ __launch_bounds__(kJL * kJW) void symadj_f64_kernel(
     const double *__restrict__ ev, const double *__restrict__ g_re,
     const double *__restrict__ g_im, int p, double eta, int nt, double *__restrict__ part) {
-    const int b = blockIdx.x, a = blockIdx.y;
-    if (b < a) return;  // upper triangle of tile pairs only (uniform per workgroup)
+    int a, b;  // query tile a, source tile b >= a
+    tile_pair_of((int)blockIdx.x, nt, a, b);
     __shared__ double s_src[kJW][kJQ];
     __shared__ double s_gr[kJW][kJQ], s_gi[kJW][kJQ];
     __shared__ double s_jbuf[kJbufBytes / 8];
@@ -408,10 +408,10 @@ int exact_adjoint_sym(const double *ev, size_t p, double eta, const double *gre,
     double *eta_term = part + 2 * (size_t)nt * p;
     ProfileMark m = profile_begin(stream);
     if (f64)
-        hipLaunchKernelGGL(symadj_f64_kernel, dim3(nt, nt), dim3(kJL * kJW), 0, stream, ev, gre,
+        hipLaunchKernelGGL(symadj_f64_kernel, dim3(nt * (nt + 1) / 2), dim3(kJL * kJW), 0, stream, ev, gre,
                            gim, (int)p, eta, nt, part);
     else
-        hipLaunchKernelGGL(symadj_f32c_kernel, dim3(nt, nt), dim3(kJL * kJW), 0, stream, ev, gre,
+        hipLaunchKernelGGL(symadj_f32c_kernel, dim3(nt * (nt + 1) / 2), dim3(kJL * kJW), 0, stream, ev, gre,
                            gim, (int)p, eta, nt, part);
     profile_end(m, stream);
     const hipError_t le = hipGetLastError();
