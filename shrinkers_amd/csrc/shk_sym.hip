@@ -356,11 +356,11 @@ int exact_sym(const double *ev, size_t p, double eta, double scale, int f64, dou
     if (int e = cache.get(2 * (size_t)nt * p, &part)) return e;
     ProfileMark m = profile_begin(stream);
     if (f64)
-        hipLaunchKernelGGL(sym_f64_kernel, dim3(nt * (nt + 1) / 2), dim3(kSL * kSW), 0, stream, ev, (int)p,
-                           eta, nt, part);
+        hipLaunchKernelGGL(sym_f64_kernel, dim3(nt * (nt + 1) / 2), dim3(kSL * kSW), 0, stream,
+                           ev, (int)p, eta, nt, part);
     else
-        hipLaunchKernelGGL(sym_f32c_kernel, dim3(nt * (nt + 1) / 2), dim3(kSL * kSW), 0, stream, ev, (int)p,
-                           eta, nt, part);
+        hipLaunchKernelGGL(sym_f32c_kernel, dim3(nt * (nt + 1) / 2), dim3(kSL * kSW), 0, stream,
+                           ev, (int)p, eta, nt, part);
     profile_end(m, stream);
     int st = 0;
     const hipError_t le = hipGetLastError();
