@@ -408,11 +408,11 @@ int exact_adjoint_sym(const double *ev, size_t p, double eta, const double *gre,
     double *eta_term = part + 2 * (size_t)nt * p;
     ProfileMark m = profile_begin(stream);
     if (f64)
-        hipLaunchKernelGGL(symadj_f64_kernel, dim3(nt * (nt + 1) / 2), dim3(kJL * kJW), 0, stream, ev, gre,
-                           gim, (int)p, eta, nt, part);
+        hipLaunchKernelGGL(symadj_f64_kernel, dim3(nt * (nt + 1) / 2), dim3(kJL * kJW), 0,
+                           stream, ev, gre, gim, (int)p, eta, nt, part);
     else
-        hipLaunchKernelGGL(symadj_f32c_kernel, dim3(nt * (nt + 1) / 2), dim3(kJL * kJW), 0, stream, ev, gre,
-                           gim, (int)p, eta, nt, part);
+        hipLaunchKernelGGL(symadj_f32c_kernel, dim3(nt * (nt + 1) / 2), dim3(kJL * kJW), 0,
+                           stream, ev, gre, gim, (int)p, eta, nt, part);
     profile_end(m, stream);
     const hipError_t le = hipGetLastError();
     if (le != hipSuccess) {
