@@ -132,11 +132,11 @@ int exact_reduce_plain(const double *part, int n_split, size_t nq, double eta, d
 // ends); partials of 16 * nt * p bytes, nt = ceil(p/512).  The FP32-
 // compensated one is used by the all-points entry for kSymMinP <= p <=
 // kSymMaxP.
-// Below about 2.4e4 the split-tile kernel is faster (measured on MI355X:
-// equal at p = 2e4, split ahead by 20 % at 1.6e4, symmetric ahead by 9 % at
-// 3e4 and 23 % at 5e4).
+// With the one-dimensional grid of tile pairs the symmetric kernel is ahead
+// from about 1e4 (measured on MI355X, all-points entry: 4.5 % at 1e4, 7 % at
+// 1.4e4, 10 % at 2.2e4, 7 % at 2.4e4; equal at 3e4 where it already ran).
 #ifndef SHK_SYM_MIN_P
-#define SHK_SYM_MIN_P 24576
+#define SHK_SYM_MIN_P 8192
 #endif
 constexpr size_t kSymMinP = SHK_SYM_MIN_P;
 constexpr size_t kSymMaxP = 262144;

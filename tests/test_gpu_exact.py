@@ -250,10 +250,10 @@ def test_f64_paired_reciprocal_fallback(shk, orc):
     assert_close(r, i, rr / dup.size, ri / dup.size, 1e-13, "underflowing products")
 
 
-@pytest.mark.parametrize("p", [24576, 24577, 25087, 50000, 65536 + 511])
+@pytest.mark.parametrize("p", [8192, 8193, 10000, 24576, 24577, 25087, 50000, 65536 + 511])
 def test_symmetric_f32c_kernel(shk, orc, p):
     """The all-points FP32-compensated entry takes the symmetric kernel for
-    24 576 <= p <= 262 144 (each unordered pair once, credited to both
+    8 192 <= p <= 262 144 (each unordered pair once, credited to both
     ends; tiles of 512).  Against the FP64 oracle within 1e-6 normwise, and
     per element within 1e-6 of max|S|; against the split-tile kernel (the
     spectrum as its own query points) within 1e-7.  Sizes: whole tiles, one
@@ -299,7 +299,7 @@ def test_symmetric_f32c_hard_inputs(shk, orc):
 @pytest.mark.parametrize("p", [24576, 24577, 50000])
 def test_symmetric_f64_kernel(shk, orc, p):
     """The FP64 all-points entry takes the symmetric FP64 kernel for
-    24 576 <= p <= 262 144: within 1e-13 of the FP64 oracle (normwise and per
+    18 432 <= p <= 262 144: within 1e-13 of the FP64 oracle (normwise and per
     element against max|S|), and within 1e-13 of the split-tile FP64 kernel
     (the spectrum as its own query points)."""
     import torch
