@@ -31,10 +31,6 @@ METRIC = "Stieltjes pair-evals/sec at p=50k: FP32-comp (≤1e-6) vs FP64 exact, 
 FLOPS_PER_PAIR = 7          # sub, fma(2), rcp, fma(2), add
 PEAK_FP32_TFLOPS = 157.3    # MI355X FP32 vector (published)
 PEAK_FP64_TFLOPS = 78.6     # MI355X FP64 vector (published)
-# The all-points FP32-compensated entry runs the symmetric kernel (each unordered pair
-# once) for SYM_MIN_P <= p <= SYM_MAX_P and the split-tile kernel otherwise (the
-# library's rule, shk_internal.hpp).
-SYM_MIN_P, SYM_MAX_P = 24576, 262144
 # HBM bytes per launch of the kernel the headline runs at p = 50 000 (sym_f32c_kernel),
 # from one rocprofv3 pass per counter (FETCH_SIZE, WRITE_SIZE) scaled by the readings for a
 # 256 MiB device copy (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000): 1681.0 KiB fetched (the
@@ -518,6 +514,9 @@ def main() -> int:
         return float(np.sqrt(num / np.sum(b[0] ** 2 + b[1] ** 2)))
 
     pairs = float(p) * float(p)
+    # the range in which the all-points FP32-compensated entry takes the symmetric
+    # kernel (each unordered pair once); the split-tile kernel runs outside it
+    sym_lo, sym_hi = shk.symmetric_range("f32c")
     ms32 = t32 / args.steps * 1e3
     ms64 = t64 / steps64 * 1e3
     value = pairs * args.steps / t32
@@ -560,7 +559,7 @@ def main() -> int:
             "frac": achieved / PEAK_FP32_TFLOPS,
             "traffic": TRAFFIC_P50000 if p == 50_000 else None,
             "traffic_unit": "bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
-            "kernel": "sym_f32c_kernel" if SYM_MIN_P <= p <= SYM_MAX_P else "exact_f32c_kernel",
+            "kernel": "sym_f32c_kernel" if sym_lo <= p <= sym_hi else "exact_f32c_kernel",
             "kernel_avg_ms": k32,
             "flops_per_launch": FLOPS_PER_PAIR * pairs,
             "peak_box": box_peaks(),

@@ -97,8 +97,18 @@ typedef enum shk_method {
  *                   hi/lo parts relative to a per-tile centre, FP32 partial
  *                   sums over short runs folded into FP64 accumulators, FP64
  *                   cross-block reduction.  Normwise relative error ~1e-7.
+ *                   Range: 1e-18 <= eta <= 1e18 (the diagonal term 1/eta^2
+ *                   and its f32 runs stay in f32's normal range) and
+ *                   |eigenvalue| <= 1e18.  Forward calls with eta outside
+ *                   that range run the FP64 kernels instead; host-array calls
+ *                   also do so for larger eigenvalues, while device-pointer
+ *                   calls take the magnitude bound as a precondition.  The
+ *                   reverse-mode entries use FP32-compensated arithmetic for
+ *                   1e-9 <= eta <= 1e18 only (their terms are G/D^2), and
+ *                   FP64 otherwise.
  *   SHK_PREC_F32  : the reference's f32 arm (input and output float,
- *                   src/stieltjes/mod.rs:805-851); runs the F32C kernel. */
+ *                   src/stieltjes/mod.rs:805-851); runs the F32C kernel
+ *                   (FP64 outside the range above). */
 typedef enum shk_precision {
     SHK_PREC_F64 = 0,
     SHK_PREC_F32C = 1,
@@ -128,6 +138,15 @@ const char *shk_method_name(int method);
 
 /* The reference's default eta = 0.1/sqrt(p) (src/stieltjes/mod.rs:105-108). */
 double shk_default_eta(size_t p);
+
+/* Eigenvalue counts [*min_p, *max_p] for which the all-points entries
+ * (shk_compute_all_stieltjes, shk_all_points_dev) take the symmetric kernel,
+ * which evaluates each unordered pair once, for precision SHK_PREC_F64 or
+ * SHK_PREC_F32C (with the exact family, no cutoff).  No counterpart in the
+ * reference (its sequential sweep is always symmetric and its parallel path
+ * never, cacheblock.rs:841-954, 1454-1513); for callers and benchmarks that
+ * report which kernel ran. */
+shk_status shk_symmetric_range(int precision, size_t *min_p, size_t *max_p);
 
 /* Resolve the auto presets (src/config.rs:543-555) for this library:
  * Auto/SpeedAuto -> CHEBCODE_FAST, AccuracyAuto -> BLOCKED; a concrete
@@ -185,10 +204,12 @@ shk_status shk_all_points_dev(const double *d_eigenvalues, size_t p, double eta,
  * Each spectrum's result is its all-points transform for the method and
  * precision (scaled by 1/p_s when scaled != 0), written at the same offsets
  * of d_out_re / d_out_im.  For the exact family it is bit for bit the
- * split-tile single-spectrum kernel's: shk_all_points_dev's, except that
- * shk_all_points_dev takes the symmetric kernel for FP32-compensated
- * spectra of 24 576 to 262 144 eigenvalues (same as
- * shk_exact_at_points_dev with the spectrum as its own query points, raw).
+ * split-tile single-spectrum kernel's: shk_all_points_dev's, except where
+ * shk_all_points_dev takes the symmetric kernel (shk_symmetric_range: 8 192
+ * to 262 144 FP32-compensated eigenvalues, 18 432 to 262 144 in FP64) (same
+ * as shk_exact_at_points_dev with the spectrum as its own query points, raw).
+ * If precision is SHK_PREC_F32C and some eta lies outside [1e-18, 1e18], the
+ * whole call runs in FP64.
  * n_streams = 0 (recommended): for the exact family, ONE segmented launch on
  * `stream` covers every spectrum (one workgroup per spectrum and 128-query
  * tile, largest spectra first; no partial-sum scratch); for ChebCode, as
@@ -356,8 +377,13 @@ shk_status shk_exact_adjoint_dev(const double *d_query, size_t nq, const double 
  * eigenvalue and query gradients at queries = eigenvalues) and dL/deta (one
  * device double; NULL skips it).  A pair (i, j) adds T to dL/dlam_i and -T to
  * dL/dlam_j, so for 8192 <= p <= 262144 each unordered pair is evaluated
- * once (symmetric kernels; the FP32-compensated one for eta >= 1e-9); other
- * sizes add the two sweeps of shk_exact_adjoint_dev.  Deterministic.  The
+ * once (symmetric kernels); other sizes add the two sweeps of
+ * shk_exact_adjoint_dev in FP64.  SHK_PREC_F32C gives FP32-compensated
+ * arithmetic only in the symmetric range with 1e-9 <= eta <= 1e18, and FP64
+ * otherwise: in the two sweeps the diagonal pair's terms -G_re/eta^2 and
+ * +G_re/eta^2 would cancel only to f32 rounding (an eta gradient alone,
+ * d_grad_eigenvalues = NULL, keeps FP32 arithmetic in that eta range).
+ * Deterministic.  The
  * reference differentiates with respect to the query only (autovec.rs:38-59)
  * and has no counterpart. */
 shk_status shk_exact_adjoint_all_points_dev(const double *d_eigenvalues, size_t p, double eta,

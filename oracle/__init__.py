@@ -78,6 +78,7 @@ def lib() -> ctypes.CDLL:
                                        _f64p, _f64p]
     L.orc_cheb_count_work.argtypes = [ctypes.c_void_p, _f64p, _i64, _d, ctypes.c_int, _lp, _lp]
     L.orc_grid_leaf_cap.argtypes = [_i64, _i64, _i64, _i64]
+    L.orc_adjoint_all_points.argtypes = [_f64p, _i64, _d, _f64p, _f64p, _f64p, _dp]
     L.orc_grid_leaf_cap.restype = _i64
     L.orc_max_threads.restype = ctypes.c_int
     L.orc_set_outside_theta2.argtypes = [ctypes.c_float]
@@ -213,6 +214,17 @@ def adjoint_at_points(q, ev, eta, g_re, g_im):
     lib().orc_adjoint_at_points(q, q.size, ev, ev.size, float(eta), g_re, g_im, gev, gq,
                                 ctypes.byref(ge))
     return gev, gq, ge.value
+
+
+def adjoint_all_points(ev, eta, g_re, g_im):
+    """Gradients of sum_i g_re[i] Re S_i + g_im[i] Im S_i (raw all-points
+    sums) with respect to each eigenvalue (both roles; the self pair, whose
+    terms cancel exactly, left out) and eta."""
+    ev, g_re, g_im = _c(ev), _c(g_re), _c(g_im)
+    g = np.empty_like(ev)
+    ge = _d()
+    lib().orc_adjoint_all_points(ev, ev.size, float(eta), g_re, g_im, g, ctypes.byref(ge))
+    return g, ge.value
 
 
 def ek_residual(x, g_re, g_im, eta, c, atoms, weights):

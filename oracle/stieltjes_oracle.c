@@ -44,6 +44,9 @@
  *                               gradients is 0); homogeneity of degree -1
  *                               (lam.grad + x.grad + eta.grad = -L); eta part
  *                               against orc_with_deriv (dS/deta = -i dS/dx)
+ *   orc_adjoint_all_points    grad_ev + grad_q of orc_adjoint_at_points at
+ *                               moderate eta; exact rational sums at tiny eta;
+ *                               central FD of the all-points loss
  *   orc_ek_residual           a(z) and z/a against the reference golden master
  *                               (population Im m = Im(a g), w_re = Re(z/a));
  *                               r = 0 on the Marchenko-Pastur law (H = delta_1,
@@ -438,6 +441,38 @@ EXPORT void orc_adjoint_at_points(const double *q, int64_t nq, const double *ev,
         }
         if (grad_eta) *grad_eta = ge;
     }
+}
+
+/* The all-points form of orc_adjoint_at_points (queries = eigenvalues):
+ * grad[i] is the total gradient with respect to lam_i through both of its
+ * roles, i.e. grad_ev + grad_q of orc_adjoint_at_points(ev, ev, ...), and
+ * grad_eta the same eta gradient.  The pair (i, i) contributes -G_re,i/eta^2
+ * to the source role and +G_re,i/eta^2 to the query role; they cancel
+ * exactly, so grad[i] sums over j != i only and does not pay the rounding
+ * of 1/eta^2 (which swamps the neighbours' terms at small eta).  For j != i,
+ * with d = lam_i - lam_j, u = d/D, v = eta/D:
+ *   grad[i] += (G_re,j - G_re,i)(u^2 - v^2) - 2uv (G_im,i + G_im,j).
+ * grad_eta keeps every ordered pair, the diagonal's -G_im,i/eta^2 included.
+ * No reference counterpart (autovec.rs:38-59 differentiates the query only).
+ * Either output may be NULL. */
+EXPORT void orc_adjoint_all_points(const double *ev, int64_t p, double eta, const double *gre,
+                                   const double *gim, double *grad, double *grad_eta) {
+    const double e2 = eta * eta;
+    if (grad) {
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < p; i++) {
+            double acc = 0.0;
+            for (int64_t j = 0; j < p; j++) {
+                if (j == i) continue;
+                const double d = ev[i] - ev[j];
+                const double den = d * d + e2;
+                const double u = d / den, v = eta / den;
+                acc += (gre[j] - gre[i]) * (u * u - v * v) - 2.0 * u * v * (gim[i] + gim[j]);
+            }
+            grad[i] = acc;
+        }
+    }
+    if (grad_eta) orc_adjoint_at_points(ev, p, ev, p, eta, gre, gim, NULL, NULL, grad_eta);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -23,6 +23,7 @@ from .stieltjes import (  # noqa: F401
     parse_method,
     stieltjes_transform,
     stieltjes_transform_with_deriv,
+    symmetric_range,
 )
 from ._lib import ShkError  # noqa: F401
 

@@ -6,7 +6,7 @@ all spectra (``n_streams=0``, the default), or as the single-spectrum kernels
 spread over a few library streams (``n_streams`` 1..8, which can also return
 per-spectrum device start/end times).  Each result equals the split-tile
 single-spectrum kernel bit for bit (the all-points entry's own result outside
-its symmetric range of 24 576 to 262 144 FP32-compensated eigenvalues).
+the range in which it takes the symmetric kernel, ``symmetric_range()``).
 
 ``BatchedStieltjes``: a fixed set of spectra kept on the device with its
 spectrum table; the segmented launch is captured in a hipGraph and replayed

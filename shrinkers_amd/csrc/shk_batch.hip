@@ -151,6 +151,12 @@ shk_status shk_batch_create(const double *d_eigenvalues, const int64_t *offsets,
         if (!(std::isfinite(etas[s]) && etas[s] > 0.0))
             return bfail(SHK_ERR_BAD_ETA, "eta must be a positive finite number, got " +
                                               std::to_string(etas[s]));
+        // the prepared batch runs the FP32-compensated kernel only
+        if (!f32c_eta_ok(etas[s]))
+            return bfail(SHK_ERR_BAD_ETA,
+                         "eta outside the FP32-compensated range [1e-18, 1e18] (use "
+                         "shk_all_points_many_dev, which runs FP64 there), got " +
+                             std::to_string(etas[s]));
     }
     if (const int dv = device_ready()) return (shk_status)dv;
     shk_batch *b = new shk_batch();

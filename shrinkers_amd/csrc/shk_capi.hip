@@ -179,6 +179,12 @@ bool all_finite(const double *v, size_t n) {
     return true;
 }
 
+double max_abs(const double *v, size_t n) {
+    double m = 0.0;
+    for (size_t i = 0; i < n; ++i) m = std::max(m, std::fabs(v[i]));
+    return m;
+}
+
 shk_status check_eta(double eta) {
     if (!(std::isfinite(eta) && eta > 0.0))
         return fail(SHK_ERR_BAD_ETA, "eta must be a positive finite number, got " +
@@ -422,7 +428,7 @@ int run_allpoints(const double *d_ev, size_t p, double eta, int method, double c
         return st;
     }
     const ExactKind kind = allpoints_kind(method, cutoff_ratio);
-    if (precision == SHK_PREC_F32C && kind == EXACT_PLAIN) {
+    if (precision == SHK_PREC_F32C && kind == EXACT_PLAIN && f32c_eta_ok(eta)) {
         if (p >= kSymMinP && p <= kSymMaxP)
             return exact_sym(d_ev, p, eta, scale, 0, d_re, d_im, s);
         return exact_f32c(d_ev, p, d_ev, p, eta, scale, d_re, d_im, 0, s);
@@ -683,6 +689,15 @@ const char *shk_method_name(int method) {
 
 double shk_default_eta(size_t p) { return 0.1 / std::sqrt((double)p); }
 
+shk_status shk_symmetric_range(int precision, size_t *min_p, size_t *max_p) {
+    if (!min_p || !max_p) return fail(SHK_ERR_BAD_ARG, "NULL pointer");
+    if (precision != SHK_PREC_F64 && precision != SHK_PREC_F32C)
+        return fail(SHK_ERR_BAD_ARG, "precision must be SHK_PREC_F64 or SHK_PREC_F32C");
+    *min_p = precision == SHK_PREC_F64 ? kSym64MinP : kSymMinP;
+    *max_p = kSymMaxP;
+    return SHK_OK;
+}
+
 int shk_resolve_auto_method(int method, int parallel, size_t p) {
     (void)parallel;
     (void)p;
@@ -709,6 +724,8 @@ shk_status shk_compute_all_stieltjes(const double *eigenvalues, size_t p, double
         return fail(SHK_ERR_BAD_ARG, "precision must be SHK_PREC_F64 or SHK_PREC_F32C here");
     method = shk_resolve_auto_method(method, parallel, p);
     SHK_CHECK(check_method(method));
+    if (precision == SHK_PREC_F32C && max_abs(eigenvalues, p) > kF32cMaxAbs)
+        precision = SHK_PREC_F64;
     SHK_CHECK(ensure_device());
     hipStream_t s = host_stream();
     if (!s) return fail(SHK_ERR_HIP, "could not create a HIP stream");
@@ -753,7 +770,10 @@ shk_status shk_compute_all_stieltjes_f32(const float *eigenvalues, size_t p, flo
     const double eta_d = (double)eta;
     const double scale = 1.0 / (double)p;
     const ExactKind kind = allpoints_kind(method, cutoff_ratio);
-    if (family_of(method) == FAM_EXACT && kind == EXACT_PLAIN) {
+    float amax = 0.0f;
+    for (size_t i = 0; i < p; ++i) amax = std::max(amax, std::fabs(eigenvalues[i]));
+    if (family_of(method) == FAM_EXACT && kind == EXACT_PLAIN && f32c_eta_ok(eta_d) &&
+        (double)amax <= kF32cMaxAbs) {
         // Blocked/BlockedTiled/Auto arm (mod.rs:817-830): compensated f32.
         SHK_CHECK(dev_alloc(re32, sizeof(float) * p, s));
         SHK_CHECK(dev_alloc(im32, sizeof(float) * p, s));
@@ -764,7 +784,8 @@ shk_status shk_compute_all_stieltjes_f32(const float *eigenvalues, size_t p, flo
         SHK_CHECK(hs.finish());
         return SHK_OK;
     }
-    // Other methods: the f64 path, converted (mod.rs:832-849).
+    // Other methods, and eta or magnitudes outside the FP32-compensated
+    // range: the f64 path, converted (mod.rs:832-849).
     SHK_CHECK(dev_alloc(re, sizeof(double) * p, s));
     SHK_CHECK(dev_alloc(im, sizeof(double) * p, s));
     SHK_CHECK(run_allpoints(ev.as<double>(), p, eta_d, method, cutoff_ratio, SHK_PREC_F64, scale,
@@ -909,6 +930,9 @@ shk_status shk_all_points_many_dev(const double *d_eigenvalues, const int64_t *o
     }
     if (precision != SHK_PREC_F64 && precision != SHK_PREC_F32C)
         return fail(SHK_ERR_BAD_ARG, "precision must be SHK_PREC_F64 or SHK_PREC_F32C");
+    if (precision == SHK_PREC_F32C)
+        for (size_t k = 0; k < n_spectra; ++k)
+            if (!f32c_eta_ok(etas[k])) precision = SHK_PREC_F64;
     method = shk_resolve_auto_method(method, 0, (size_t)offsets[n_spectra]);
     SHK_CHECK(check_method(method));
     SHK_CHECK(ensure_device());
@@ -1079,7 +1103,7 @@ shk_status shk_exact_at_points_dev(const double *d_query, size_t nq, const doubl
         return fail(SHK_ERR_BAD_ARG, "precision must be SHK_PREC_F64 or SHK_PREC_F32C");
     SHK_CHECK(ensure_device());
     hipStream_t s = (hipStream_t)stream;
-    if (precision == SHK_PREC_F32C)
+    if (precision == SHK_PREC_F32C && f32c_eta_ok(eta))
         return (shk_status)exact_f32c(d_query, nq, d_eigenvalues, p, eta, 1.0, d_out_re, d_out_im,
                                       0, s);
     double *outs[2] = {d_out_re, d_out_im};
@@ -1110,8 +1134,9 @@ shk_status shk_exact_adjoint_dev(const double *d_query, size_t nq, const double 
         return fail(SHK_ERR_BAD_ARG, "precision must be SHK_PREC_F64 or SHK_PREC_F32C");
     SHK_CHECK(ensure_device());
     return (shk_status)exact_adjoint(d_query, nq, d_eigenvalues, p, eta, d_grad_re, d_grad_im,
-                                     precision == SHK_PREC_F32C, d_grad_eigenvalues, d_grad_query,
-                                     d_grad_eta, (hipStream_t)stream);
+                                     precision == SHK_PREC_F32C && adj_f32c_eta_ok(eta),
+                                     d_grad_eigenvalues, d_grad_query, d_grad_eta,
+                                     (hipStream_t)stream);
 }
 
 shk_status shk_exact_adjoint_all_points_dev(const double *d_eigenvalues, size_t p, double eta,
@@ -1130,9 +1155,17 @@ shk_status shk_exact_adjoint_all_points_dev(const double *d_eigenvalues, size_t 
         return fail(SHK_ERR_BAD_ARG, "precision must be SHK_PREC_F64 or SHK_PREC_F32C");
     SHK_CHECK(ensure_device());
     hipStream_t s = (hipStream_t)stream;
-    const bool f64 = precision == SHK_PREC_F64;
-    const bool sym = d_grad_eigenvalues && p >= (f64 ? kSymAdj64MinP : kSymAdjMinP) &&
-                     p <= kSymMaxP && (f64 || eta >= kSymAdjMinEta);
+    // The eigenvalue gradient through the symmetric kernels wherever they run
+    // (p <= kSymMaxP): they leave the pair (i, i) out of the gradient sums
+    // (its T is 0).  In the two sweeps the diagonal pair adds -G_re/eta^2 to
+    // the source sweep and +G_re/eta^2 to the query sweep, each among its
+    // neighbours' terms, and the two cancel only to the rounding of 1/eta^2:
+    // in f32 runs that swamps the result at small eta, so beyond kSymMaxP the
+    // two sweeps run in FP64.  (The eta gradient alone has no such
+    // cancellation: its diagonal term -G_im/eta^2 is part of the result.)
+    const bool sym = d_grad_eigenvalues && p <= kSymMaxP;
+    const bool f64 = precision == SHK_PREC_F64 || !adj_f32c_eta_ok(eta) ||
+                     (d_grad_eigenvalues && !sym);
     if (sym)
         return (shk_status)exact_adjoint_sym(d_eigenvalues, p, eta, d_grad_re, d_grad_im, f64,
                                              d_grad_eigenvalues, d_grad_eta, s);
@@ -1184,8 +1217,9 @@ shk_status shk_el_karoui_fit_step_dev(const double *d_eigenvalues, size_t p, con
         return fail(SHK_ERR_BAD_ARG, "precision must be SHK_PREC_F64 or SHK_PREC_F32C");
     SHK_CHECK(ensure_device());
     return (shk_status)ek_fit_step(d_eigenvalues, p, d_x, m, eta, c, d_atoms, d_weights, k,
-                                   precision == SHK_PREC_F32C, d_loss, d_grad_eigenvalues,
-                                   d_grad_eta, d_grad_weights, (hipStream_t)stream);
+                                   precision == SHK_PREC_F32C && adj_f32c_eta_ok(eta), d_loss,
+                                   d_grad_eigenvalues, d_grad_eta, d_grad_weights,
+                                   (hipStream_t)stream);
 }
 
 shk_status shk_chebcode_build(const double *eigenvalues, size_t p, double theta, int n,

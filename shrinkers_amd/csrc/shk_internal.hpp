@@ -95,6 +95,22 @@ void profile_end(const ProfileMark &m, hipStream_t stream);
 
 // ---- exact family (shk_exact.hip) -------------------------------------------
 
+// Range of eta in which the FP32-compensated kernels are used.  They form
+// eta^2 and 1/(d^2 + eta^2) in f32 and add runs of up to 32 such terms, so
+// the diagonal term 1/eta^2 and its runs must stay inside f32's normal range
+// (eta^2 >= 1e-36 > 1.2e-38, 32/eta^2 <= 3.2e37 < 3.4e38).  Outside it the
+// entry points run the FP64 kernels instead (more accurate, never worse).
+// The adjoint's FP32 terms are G/D^2, so its floor is higher: eta >=
+// kSymAdjMinEta (below).
+constexpr double kF32cMinEta = 1.0e-18;
+constexpr double kF32cMaxEta = 1.0e18;
+// Largest |eigenvalue| the FP32-compensated kernels accept: pair differences
+// are formed in FP64 relative to a tile centre and rounded to f32, and d^2
+// must not overflow f32.  Host-array entry points check it and run FP64
+// beyond it; device entry points document it as a precondition.
+constexpr double kF32cMaxAbs = 1.0e18;
+inline bool f32c_eta_ok(double eta) { return eta >= kF32cMinEta && eta <= kF32cMaxEta; }
+
 // Which accumulation the exact kernels perform.
 enum ExactKind : int {
     EXACT_PLAIN = 0,   // re = sum d*inv, im = eta * sum inv
@@ -214,19 +230,11 @@ int exact_adjoint_sym(const double *ev, size_t p, double eta, const double *gre,
                       const double *gim, int f64, double *grad, double *grad_eta,
                       hipStream_t stream, int accumulate_eta = 0);
 
-// Smallest p for which the all-points adjoint entry uses the symmetric
-// kernels (below, the two sweeps of exact_adjoint are added); the FP32-
-// compensated kernel also needs eta >= kSymAdjMinEta, so that r^2 = 1/D^2
-// stays within f32 range.
-#ifndef SHK_SYMADJ_MIN_P
-#define SHK_SYMADJ_MIN_P 8192
-#endif
-#ifndef SHK_SYMADJ64_MIN_P
-#define SHK_SYMADJ64_MIN_P 8192
-#endif
-constexpr size_t kSymAdjMinP = SHK_SYMADJ_MIN_P;
-constexpr size_t kSymAdj64MinP = SHK_SYMADJ64_MIN_P;
+// The FP32-compensated adjoint kernels need eta >= kSymAdjMinEta, so that
+// r^2 = 1/D^2 stays within f32 range.
 constexpr double kSymAdjMinEta = 1.0e-9;
+// eta range of the FP32-compensated adjoint kernels (terms G/D^2 in f32).
+inline bool adj_f32c_eta_ok(double eta) { return eta >= kSymAdjMinEta && eta <= kF32cMaxEta; }
 
 // *out = sum(v[0, n)) in a fixed order (one workgroup); accumulate != 0 adds
 // to *out.

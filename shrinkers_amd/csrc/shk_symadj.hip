@@ -15,7 +15,9 @@
 //   beta  = 2 eta d / D^2          (antisymmetric),
 // the pair adds T = alpha Gr_j - beta Gi_j - alpha Gr_i - beta Gi_i to
 // dL/dlam_i and -T to dL/dlam_j (i as query of source j and as source of
-// query j; the diagonal gives T = 0), and the ordered pair (i, j) adds
+// query j; the diagonal gives T = 0 and is left out of the gradient sums,
+// where its alpha = -1/eta^2 would otherwise cancel only to rounding), and
+// the ordered pair (i, j) adds
 // Gi_i alpha - Gr_i beta to dL/deta.  So each end needs only three sums over
 // its partners: for end i (sums over j)
 //   P = sum alpha Gr_j - beta' Gi'_j,  A = sum alpha,  B = sum beta'
@@ -80,7 +82,7 @@ __device__ __forceinline__ void adj_sweep_f32c(
     const float (&ql)[kJR], const float (&gr)[kJR], const float (&gi)[kJR], float eta2f,
     float m2eta2f, double (&P)[kJR], double (&A)[kJR], double (&B)[kJR], float *__restrict__ s_jb,
     const double *__restrict__ g_re, const double *__restrict__ g_im, double two_eta,
-    double *__restrict__ j_grad, double *__restrict__ j_eta, int lane) {
+    double *__restrict__ j_grad, double *__restrict__ j_eta, int lane, int qoff) {
     for (int j0 = 0; j0 < kJQ; j0 += kAB) {  // same trip count in every wave
         float rp[kJR], ra[kJR], rb[kJR];
 #pragma unroll
@@ -98,8 +100,16 @@ __device__ __forceinline__ void adj_sweep_f32c(
                     const float d = pair_diff<FORM>(qh[k], ql[k], sh, sl);
                     const float r = __builtin_amdgcn_rcpf(__builtin_fmaf(d, d, eta2f));
                     const float r2 = r * r;
-                    const float al = __builtin_fmaf(m2eta2f, r2, r);  // (d^2 - eta^2) / D^2
-                    const float be = d * r2;
+                    float al = __builtin_fmaf(m2eta2f, r2, r);  // (d^2 - eta^2) / D^2
+                    float be = d * r2;
+                    if (!JSIDE) {
+                        // the pair (i, i): T = 0, so it adds nothing.  Left in,
+                        // its alpha = -1/eta^2 would enter P and A and cancel
+                        // only in P - Gr_i A, to the rounding of 1/eta^2.
+                        const bool self = qoff + k == j;
+                        al = self ? 0.0f : al;
+                        be = self ? 0.0f : be;
+                    }
                     rp[k] = __builtin_fmaf(al, grj, rp[k]);
                     rp[k] = __builtin_fmaf(-be, gij, rp[k]);
                     ra[k] += al;
@@ -162,7 +172,7 @@ __device__ __forceinline__ void adj_sweep_f64(
     const double (&gr)[kJR], const double (&gi)[kJR], double e2, double m2e2, double (&P)[kJR],
     double (&A)[kJR], double (&B)[kJR], double *__restrict__ s_jb,
     const double *__restrict__ g_re, const double *__restrict__ g_im, double two_eta,
-    double *__restrict__ j_grad, double *__restrict__ j_eta, int lane) {
+    double *__restrict__ j_grad, double *__restrict__ j_eta, int lane, int qoff) {
     for (int j0 = 0; j0 < kJQ; j0 += kAB64) {  // same trip count in every wave
         for (int u = 0; u < kAB64; ++u) {
             const int j = j0 + u;
@@ -179,8 +189,13 @@ __device__ __forceinline__ void adj_sweep_f64(
 #pragma unroll
                 for (int k = 0; k < kJR; ++k) {
                     const double r2 = r[k] * r[k];
-                    const double al = __builtin_fma(m2e2, r2, r[k]);  // (d^2 - eta^2) / D^2
-                    const double be = d[k] * r2;
+                    double al = __builtin_fma(m2e2, r2, r[k]);  // (d^2 - eta^2) / D^2
+                    double be = d[k] * r2;
+                    if (!JSIDE) {  // the pair (i, i) adds nothing (as in the FP32 form)
+                        const bool self = qoff + k == j;
+                        al = self ? 0.0 : al;
+                        be = self ? 0.0 : be;
+                    }
                     P[k] = __builtin_fma(al, grj, P[k]);
                     P[k] = __builtin_fma(-be, gij, P[k]);
                     A[k] += al;
@@ -227,13 +242,15 @@ __device__ __forceinline__ void adj_sweep_f64(
 }
 
 // i-side finish: waves 1..3 hand their sums to wave 0 in turn (fixed order,
-// through `buf`, 3 * kJT doubles); wave 0 writes part[.][b][i].
+// through `buf`, 3 * kJT doubles); wave 0 writes part[.][b][i].  self_alpha
+// is the pair (i, i)'s alpha, -1/eta^2, in a diagonal tile (0 otherwise): it
+// was left out of the sums, and its eta term Gi_i alpha belongs to dL/deta.
 __device__ __forceinline__ void adj_finish(double (&P)[kJR], double (&A)[kJR], double (&B)[kJR],
                                            double *__restrict__ buf, int w, int lane,
                                            const double *__restrict__ g_re,
                                            const double *__restrict__ g_im, double two_eta,
                                            double *__restrict__ part, int nt, int p, int b,
-                                           int q0) {
+                                           int q0, double self_alpha) {
     for (int r = 1; r < kJW; ++r) {
         __syncthreads();
         if (w == r) {
@@ -263,7 +280,7 @@ __device__ __forceinline__ void adj_finish(double (&P)[kJR], double (&A)[kJR], d
             if (i < p) {
                 const double g_r = g_re[i], g_i = g_im[i];
                 i_grad[i] = (P[k] - g_r * A[k]) - (two_eta * g_i) * B[k];
-                i_eta[i] = g_i * A[k] - (two_eta * g_r) * B[k];
+                i_eta[i] = g_i * (A[k] + self_alpha) - (two_eta * g_r) * B[k];
             }
         }
     }
@@ -336,7 +353,8 @@ __global__ __launch_bounds__(kJL * kJW) void symadj_f32c_kernel(
 #define SHK_ADJ_SWEEP(F, J)                                                                     \
     adj_sweep_f32c<F, J>(s_src[w], s_gr[w], s_gi[w], n_src, c, qh, ql, gr, gi, eta2f, m2eta2f,  \
                          P, A, B,                                                               \
-                         s_jb, g_re + js, g_im + js, two_eta, j_grad + js, j_eta + js, lane)
+                         s_jb, g_re + js, g_im + js, two_eta, j_grad + js, j_eta + js, lane,     \
+                         q0 - js)
     if (a < b) {
         if (form == 0) SHK_ADJ_SWEEP(0, true);
         else if (form == 1) SHK_ADJ_SWEEP(1, true);
@@ -347,7 +365,8 @@ __global__ __launch_bounds__(kJL * kJW) void symadj_f32c_kernel(
         else SHK_ADJ_SWEEP(2, false);
     }
 #undef SHK_ADJ_SWEEP
-    adj_finish(P, A, B, s_jbuf, w, lane, g_re, g_im, two_eta, part, nt, p, b, q0);
+    adj_finish(P, A, B, s_jbuf, w, lane, g_re, g_im, two_eta, part, nt, p, b, q0,
+               a == b ? -1.0 / (eta * eta) : 0.0);
 }
 
 __global__ __launch_bounds__(kJL * kJW) void symadj_f64_kernel(
@@ -384,11 +403,14 @@ __global__ __launch_bounds__(kJL * kJW) void symadj_f64_kernel(
     const double e2 = eta * eta, m2e2 = -2.0 * e2;
     if (a < b)
         adj_sweep_f64<true>(s_src[w], s_gr[w], s_gi[w], n_src, x, gr, gi, e2, m2e2, P, A, B, s_jb,
-                            g_re + js, g_im + js, two_eta, j_grad + js, j_eta + js, lane);
+                            g_re + js, g_im + js, two_eta, j_grad + js, j_eta + js, lane,
+                            q0 - js);
     else
         adj_sweep_f64<false>(s_src[w], s_gr[w], s_gi[w], n_src, x, gr, gi, e2, m2e2, P, A, B, s_jb,
-                             g_re + js, g_im + js, two_eta, j_grad + js, j_eta + js, lane);
-    adj_finish(P, A, B, s_jbuf, w, lane, g_re, g_im, two_eta, part, nt, p, b, q0);
+                             g_re + js, g_im + js, two_eta, j_grad + js, j_eta + js, lane,
+                             q0 - js);
+    adj_finish(P, A, B, s_jbuf, w, lane, g_re, g_im, two_eta, part, nt, p, b, q0,
+               a == b ? -1.0 / (eta * eta) : 0.0);
 }
 
 }  // namespace

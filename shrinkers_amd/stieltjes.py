@@ -80,6 +80,15 @@ def default_eta(p: int) -> float:
     return 0.1 / math.sqrt(p)
 
 
+def symmetric_range(precision: str = "f32c"):
+    """``(min_p, max_p)``: the eigenvalue counts for which the all-points
+    entries take the symmetric kernel (each unordered pair evaluated once)."""
+    lo, hi = ctypes.c_size_t(), ctypes.c_size_t()
+    check(_lib.load().shk_symmetric_range(_PRECISIONS[precision], ctypes.byref(lo),
+                                          ctypes.byref(hi)))
+    return lo.value, hi.value
+
+
 def default_eta_bulk(p: int) -> float:
     """eta = 0.4/sqrt(p): src/stieltjes/mod.rs:149-152."""
     return 0.4 / math.sqrt(p)

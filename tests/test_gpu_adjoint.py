@@ -211,18 +211,31 @@ def test_symmetric_all_points_adjoint_large(shk, precision):
     assert abs(only_eta - ge) <= tol * abs(ge)   # the eta-only call runs the query sweep
 
 
-@pytest.mark.parametrize("precision,p,eta", [("f64", 3000, 0.02), ("f32c", 3000, 0.02),
-                                             ("f32c", 8192, 1e-10)])
-def test_all_points_adjoint_two_sweep_sizes(shk, precision, p, eta):
-    """Below the symmetric range, and FP32-compensated with eta < 1e-9, the
-    entry adds the two sweeps: bit for bit their sum."""
+@pytest.mark.parametrize("precision", ["f64", "f32c"])
+@pytest.mark.parametrize("p,eta", [(3000, 0.02), (3000, 1e-7), (8192, 1e-10), (700, 1e-12)])
+def test_all_points_adjoint_small_eta(shk, orc, precision, p, eta):
+    """The all-points adjoint runs the symmetric kernels at every size up to
+    their maximum, with the pair (i, i) left out of the gradient sums (its
+    terms +-G_re/eta^2 cancel exactly).  Against the oracle's all-points
+    gradient, which leaves it out too, at sizes below the old two-sweep
+    threshold and at eta down to 1e-12, where 1/eta^2 = 1e24 would swamp the
+    result of any cancellation.  FP32-compensated arithmetic runs for
+    eta >= 1e-9 only; below, the call runs FP64 and equals the FP64 call
+    bit for bit."""
     ev = _hard_spectrum("mp_spikes", p)
     rng = np.random.default_rng(p)
     gre, gim = rng.standard_normal(p), rng.standard_normal(p)
     gev, ge = _all_points_adjoint(ev, eta, gre, gim, precision)
-    rev, rq, re_ = _two_sweeps(ev, eta, gre, gim, precision)
-    assert np.array_equal(gev, rev + rq)
-    assert ge == re_
+    rg, re_ = orc.adjoint_all_points(ev, eta, gre, gim)
+    f32 = precision == "f32c" and eta >= 1e-9
+    tol = 5e-6 if f32 else 1e-12
+    e_ev, e_eta = _rel(gev, rg), abs(ge - re_) / abs(re_)
+    print(f"all-points adjoint {precision} p={p} eta={eta:g}: lam {e_ev:.3e} eta {e_eta:.3e}",
+          flush=True)
+    assert e_ev <= tol and e_eta <= tol
+    if precision == "f32c" and not f32:
+        fev, fe = _all_points_adjoint(ev, eta, gre, gim, "f64")
+        assert np.array_equal(gev, fev) and ge == fe
 
 
 def test_all_points_autograd_symmetric(shk, orc):

@@ -222,23 +222,66 @@ def test_grid_path_vs_exact(shk, orc):
     assert rel_l2(r, i, orr, ori) <= 1e-13
 
 
-@pytest.mark.parametrize("p,c", [(200_000, 0.1), (200_000, 0.5), (1_000_000, 0.1)])
+# Config 3: (theta, n, leaf, f32 far field) and the bar against the GPU FP64
+# exact kernel.  FAST and DEFAULT are the reference presets
+# (chebcode.rs:942-973); chebcode_n12 is the build-defined f64 point of
+# order 12.  The bars sit above the documented classes (~2e-5, ~5e-10)
+# because at c = 0.5 the reference algorithm itself lands above them
+# (tests/test_oracle.py pins 2.81e-5 / 3.26e-9 at p = 2e5 on the CPU).
+CONFIG3 = (("chebcode_fast", (1.0, 8, 32, True), 6e-5),
+           ("chebcode", (0.5, 11, 32, False), 1e-8),
+           ("chebcode_n12", (0.5, 12, 32, False), 5e-9))
+CONFIG3_SIZES = [(200_000, 0.1), (200_000, 0.5), (1_000_000, 0.1), (1_000_000, 0.5)]
+
+
+@pytest.mark.parametrize("p,c", CONFIG3_SIZES)
 def test_large_p_against_gpu_exact(shk, p, c):
-    """Config 3: ChebCode at p = 2e5 / 1e6 checked against the GPU FP64 exact kernel."""
+    """Config 3: ChebCode FAST, DEFAULT and the f64 n = 12 preset at p = 2e5
+    and 1e6, c = 0.1 and 0.5, against the GPU FP64 exact kernel; the n = 12
+    preset is more accurate than DEFAULT."""
     import torch
     from shrinkers_amd import spectra as S
     ev = S.mp_inverse_cdf_spectrum(p, c=c)
     eta = 1.0 / np.sqrt(p)
     d = torch.from_numpy(ev).cuda()
-    er, ei = shk.all_points_dev(d, eta, "blocked")
+    er, ei = shk.all_points_dev(d, eta, "blocked", scaled=False)
     torch.cuda.synchronize()
     er, ei = er.cpu().numpy(), ei.cpu().numpy()
-    for m, tol in (("chebcode_fast", 6e-5), ("chebcode", 1e-8)):
-        r, i = shk.all_points_dev(d, eta, m)
+    errs = {}
+    for name, (theta, n, leaf, f32), tol in CONFIG3:
+        b = shk.ChebCodeBatch.build_dev(d, theta, n, leaf, f32)
+        r, i = b.evaluate_dev(eta)
         torch.cuda.synchronize()
         err = rel_l2(r.cpu().numpy(), i.cpu().numpy(), er, ei)
-        print(f"{m} p={p} c={c}: rel-L2 vs GPU exact {err:.3e}", flush=True)
-        assert err <= tol
+        errs[name] = err
+        print(f"{name} p={p} c={c}: rel-L2 vs GPU exact {err:.3e}", flush=True)
+        assert err <= tol, name
+        if name in shk.CHEB_PRESETS:  # the one-shot transform gives the same bits
+            r1, i1 = shk.all_points_dev(d, eta, name, scaled=False)
+            torch.cuda.synchronize()
+            assert torch.equal(r1, r) and torch.equal(i1, i)
+    assert errs["chebcode_n12"] < errs["chebcode"]
+
+
+@pytest.mark.parametrize("p,c", CONFIG3_SIZES)
+def test_large_p_against_oracle_chebcode(shk, orc, p, c):
+    """Config 3 at its own sizes against the oracle's restatement of the
+    reference ChebCode (build, FAST and f64 traversals; O(p log p) on the
+    CPU): the f64 presets within 1e-13 normwise, FAST within 1e-6 (its near
+    field is f32 hi/lo on the GPU, f64 in the reference)."""
+    import torch
+    from shrinkers_amd import spectra as S
+    ev = S.mp_inverse_cdf_spectrum(p, c=c)
+    eta = 1.0 / np.sqrt(p)
+    d = torch.from_numpy(ev).cuda()
+    for name, (theta, n, leaf, f32), _ in CONFIG3:
+        b = shk.ChebCodeBatch.build_dev(d, theta, n, leaf, f32)
+        r, i = b.evaluate_dev(eta)
+        torch.cuda.synchronize()
+        orr, ori = orc.chebcode_all(ev, eta, theta, n, leaf, f32)
+        err = rel_l2(r.cpu().numpy(), i.cpu().numpy(), orr, ori)
+        print(f"{name} p={p} c={c}: rel-L2 vs oracle ChebCode {err:.3e}", flush=True)
+        assert err <= (1e-6 if f32 else 1e-13), name
 
 
 @pytest.mark.parametrize("preset", ["chebcode_fast", "chebcode", "chebcode_xtreme"])

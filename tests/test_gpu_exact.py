@@ -276,16 +276,22 @@ def test_symmetric_f32c_kernel(shk, orc, p):
     assert rel_l2(re, im, sr, si) <= 1e-7
 
 
-def test_symmetric_f32c_hard_inputs(shk, orc):
+@pytest.mark.parametrize("p", [10000, 30000])
+def test_symmetric_f32c_hard_inputs(shk, orc, p):
     """Unsorted input (wide per-lane query spans: the compensated forms),
-    heavy duplicates with a small eta, and a log-spaced spectrum, all above
-    the symmetric kernel's threshold."""
+    heavy duplicates with a small eta (1e-4 and 1e-12: d = 0 exactly for the
+    tied pairs, terms of 1/eta^2 = 1e24 in f32), and a log-spaced spectrum,
+    at sizes inside the symmetric kernel's range (10 000 lies in the part
+    below 24 576 that it took over from the split-tile kernel)."""
     from shrinkers_amd import spectra as S
     rng = np.random.default_rng(5)
+    lo, hi = shk.symmetric_range("f32c")
+    assert lo <= p <= hi
     cases = {
-        "unsorted": (orc.shuffled(S.mp_inverse_cdf_spectrum(30000), 3), 1.0 / np.sqrt(30000)),
-        "duplicates": (np.sort(np.repeat(rng.uniform(0.5, 2.0, 3200), 8)), 1e-4),
-        "log": (S.log_spectrum(30000), 1.0 / np.sqrt(30000)),
+        "unsorted": (orc.shuffled(S.mp_inverse_cdf_spectrum(p), 3), 1.0 / np.sqrt(p)),
+        "duplicates": (np.sort(np.repeat(rng.uniform(0.5, 2.0, p // 8), 8)), 1e-4),
+        "duplicates_tiny_eta": (np.sort(np.repeat(rng.uniform(0.5, 2.0, p // 8), 8)), 1e-12),
+        "log": (S.log_spectrum(p), 1.0 / np.sqrt(p)),
     }
     for name, (ev, eta) in cases.items():
         r, i = shk.compute_all_stieltjes(ev, eta, "blocked", precision="f32c")
@@ -296,7 +302,7 @@ def test_symmetric_f32c_hard_inputs(shk, orc):
         assert err <= 1e-6, name
 
 
-@pytest.mark.parametrize("p", [24576, 24577, 50000])
+@pytest.mark.parametrize("p", [18432, 20000, 24576, 24577, 50000])
 def test_symmetric_f64_kernel(shk, orc, p):
     """The FP64 all-points entry takes the symmetric FP64 kernel for
     18 432 <= p <= 262 144: within 1e-13 of the FP64 oracle (normwise and per
@@ -364,3 +370,96 @@ def test_host_calls_from_several_threads(shk):
                 w = want[next(i for i, c in enumerate(cases) if c is case)]
                 for a, b in zip(w, g):
                     assert a.dtype == b.dtype and np.array_equal(a, b), f"rep {rep}, p={case[0].size}"
+
+
+@pytest.mark.parametrize("p", [3000, 10000])
+def test_f32c_outside_eta_range_runs_fp64(shk, orc, p):
+    """The FP32-compensated entries form eta^2 and 1/eta^2 in f32; outside
+    1e-18 <= eta <= 1e18 they run the FP64 kernels instead.  At eta = 1e-20
+    (1/eta^2 = 1e40 overflows f32) and eta = 1e20 (eta^2 overflows), with
+    duplicated eigenvalues, every f32c entry returns the FP64 result: the
+    all-points entry (split-tile size 3000, symmetric size 10000), the
+    at-points entry and the float32 entry, all within the FP64 bar of the
+    oracle."""
+    import torch
+    from shrinkers_amd import spectra as S
+    from shrinkers_amd import _lib
+    ev = np.sort(np.concatenate([S.mp_inverse_cdf_spectrum(p - 100), S.mp_inverse_cdf_spectrum(p)[:100]]))
+    for eta in (1e-20, 1e20):
+        rr, ri = orc.tiled_parallel(ev, eta)
+        r, i = shk.compute_all_stieltjes(ev, eta, "blocked", precision="f32c")
+        assert np.all(np.isfinite(r)) and np.all(np.isfinite(i))
+        assert_close(r, i, rr / p, ri / p, 1e-13, f"f32c routed to FP64, eta={eta:g}")
+        d = torch.from_numpy(ev).cuda()
+        r2, i2 = shk.all_points_dev(d, eta, "blocked", precision="f32c", scaled=False)
+        sr, si = torch.empty_like(d), torch.empty_like(d)
+        assert _lib.load().shk_exact_at_points_dev(d.data_ptr(), p, d.data_ptr(), p, eta, 1,
+                                                   sr.data_ptr(), si.data_ptr(),
+                                                   None) == _lib.SHK_OK
+        torch.cuda.synchronize()
+        assert_close(r2.cpu().numpy(), i2.cpu().numpy(), rr, ri, 1e-13, "all_points_dev")
+        assert_close(sr.cpu().numpy(), si.cpu().numpy(), rr, ri, 1e-13, "exact_at_points_dev")
+    # the float32 entry at eta = 1e-20: the FP64 path on the widened input
+    ev32 = ev.astype(np.float32)
+    r3, i3 = shk.compute_all_stieltjes_f32(ev32, 1e-20, "blocked")
+    rr, ri = orc.tiled_parallel(ev32.astype(np.float64), float(np.float32(1e-20)))
+    assert np.all(np.isfinite(r3)) and np.all(np.isfinite(i3))
+    assert rel_l2(r3, i3, rr / p, ri / p) <= 1e-6
+
+
+def test_f32c_inside_eta_range_small_eta(shk, orc):
+    """Just inside the range (eta = 1e-15, duplicates: diagonal terms of
+    1e30 in f32) the FP32-compensated kernels still meet their 1e-6 bar."""
+    from shrinkers_amd import spectra as S
+    base = S.mp_inverse_cdf_spectrum(4000)
+    for ev in (np.sort(np.repeat(base[:500], 6)), np.sort(np.repeat(base[:1500], 6))):
+        eta = 1e-15
+        r, i = shk.compute_all_stieltjes(ev, eta, "blocked", precision="f32c")
+        rr, ri = orc.tiled_parallel(ev, eta)
+        assert rel_l2(r, i, rr / ev.size, ri / ev.size) <= 1e-6, ev.size
+
+
+def test_f32c_batch_outside_eta_range(shk):
+    """The prepared batch runs FP32-compensated only: it refuses an eta
+    outside [1e-18, 1e18] with SHK_ERR_BAD_ETA.  The one-call batch runs
+    such a set in FP64 (bit for bit the FP64 call)."""
+    import torch
+    from shrinkers_amd import _lib
+    from shrinkers_amd import spectra as S
+    from shrinkers_amd.batch import BatchedStieltjes, all_points_many_dev
+    sp = [S.mp_inverse_cdf_spectrum(p) for p in (700, 1300)]
+    etas = [1e-20, 0.03]
+    with pytest.raises(_lib.ShkError) as e:
+        BatchedStieltjes(sp, etas)
+    assert e.value.status == _lib.SHK_ERR_BAD_ETA
+    ev = torch.from_numpy(np.concatenate(sp)).cuda()
+    off = np.array([0, 700, 2000], dtype=np.int64)
+    r32, i32 = all_points_many_dev(ev, off, etas, precision="f32c")
+    r64, i64 = all_points_many_dev(ev, off, etas, precision="f64")
+    torch.cuda.synchronize()
+    assert torch.equal(r32, r64) and torch.equal(i32, i64)
+
+
+def test_config1_spiked_wishart(shk, orc):
+    """Config 1: eigenvalues of X^T X / n for X of shape 4096 x 1024 with
+    i.i.d. N(0, 1) entries scaled by sqrt(Sigma), Sigma = diag(12, 7, 4, 1,
+    ..., 1) (seed 0), eta = 1/sqrt(p), z_i = lam_i + i eta for all i.  The
+    FP64 GPU transform within 1e-13 of the oracle's tiled and symmetric
+    restatements, the FP32-compensated one within 1e-6, and the ChebCode
+    presets within 1e-13 (f64) / 1e-6 (FAST) of the oracle's ChebCode."""
+    from shrinkers_amd import spectra as S
+    ev = S.spiked_wishart_spectrum(4096, 1024)
+    p = ev.size
+    eta = 1.0 / np.sqrt(p)
+    rr, ri = orc.tiled_parallel(ev, eta)
+    sr, si = orc.symmetric_sweep(ev, eta)
+    r, i = shk.compute_all_stieltjes(ev, eta, "blocked")
+    assert_close(r, i, rr / p, ri / p, 1e-13, "config 1 f64 vs tiled oracle")
+    assert_close(r, i, sr / p, si / p, 1e-13, "config 1 f64 vs symmetric oracle")
+    r, i = shk.compute_all_stieltjes(ev, eta, "blocked", precision="f32c")
+    assert rel_l2(r, i, rr / p, ri / p) <= 1e-6
+    for preset in ("chebcode", "chebcode_fast", "chebcode_xtreme", "chebcode_balanced"):
+        r, i = shk.compute_all_stieltjes(ev, eta, preset)
+        orr, ori = orc.chebcode_preset(ev, eta, preset)
+        tol = 1e-6 if preset == "chebcode_fast" else 1e-13
+        assert rel_l2(r, i, orr / p, ori / p) <= tol, preset

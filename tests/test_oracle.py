@@ -273,3 +273,41 @@ def test_f32_reference_bound(orc):
     err = max(np.max(np.abs(r64 - r32) / np.maximum(np.abs(r64), 1e-12)),
               np.max(np.abs(i64 - i32) / np.maximum(np.abs(i64), 1e-12)))
     assert err < 5e-2
+
+
+def test_chebcode_error_at_config3_is_the_algorithms(orc):
+    """At p = 2e5, MP c = 0.5 with the three outliers, eta = 1/sqrt(p), the
+    reference algorithm as restated here (build and traversals) lands above
+    the documented classes: FAST 2.81e-5 (class ~2e-5) and DEFAULT 3.26e-9
+    (class ~5e-10) against the exact sums.  The GPU measured the same two
+    figures against its FP64 exact kernel (FAST 2.811e-5, DEFAULT 3.259e-9),
+    so the excess is the algorithm's on this input, not the GPU's: pinned
+    here to 2 digits."""
+    from shrinkers_amd import spectra as S
+    p = 200_000
+    ev = S.mp_inverse_cdf_spectrum(p, c=0.5)
+    eta = 1.0 / np.sqrt(p)
+    er, ei = orc.tiled_parallel(ev, eta)
+    for preset, want in (("chebcode_fast", 2.811e-5), ("chebcode", 3.259e-9)):
+        r, i = orc.chebcode_preset(ev, eta, preset)
+        err = rel_l2(r, i, er, ei)
+        print(f"oracle {preset} at p=2e5 c=0.5: {err:.4e}", flush=True)
+        assert abs(err / want - 1.0) < 0.01, (preset, err, want)
+
+
+def test_config1_spiked_wishart_generator():
+    """Config 1's spectrum (4096 x 1024 Wishart, Sigma = diag(12, 7, 4, 1, ...))
+    carries three outliers whose inverse-BBP images (spiked/estimation.rs:
+    10-11 inverted) recover the spikes within 5 %, and a bulk inside the
+    Marchenko-Pastur support [(1 - sqrt c)^2, (1 + sqrt c)^2] up to edge
+    fluctuations (the reference's sanity figure for this setup,
+    README.md:42-49, is a 3-spike detection within ~2 %)."""
+    from shrinkers_amd import spectra as S
+    ev = S.spiked_wishart_spectrum(4096, 1024)
+    c = 1024 / 4096
+    top = ev[-3:][::-1]
+    b = top + 1.0 - c
+    ell = 0.5 * (b + np.sqrt(b * b - 4.0 * top))
+    assert np.all(np.abs(ell / np.array([12.0, 7.0, 4.0]) - 1.0) < 0.05), ell
+    lo, hi = (1 - np.sqrt(c)) ** 2, (1 + np.sqrt(c)) ** 2
+    assert ev[0] > 0.9 * lo and ev[-4] < 1.1 * hi

@@ -172,3 +172,65 @@ def test_ek_fit_gradients_match_central_differences(orc):
     assert _rel(ggi, fd_gi) < 1e-6
     assert _rel(gw, fd_w) < 1e-6
     assert abs(ge - fd_e) / abs(fd_e) < 1e-6
+
+
+def _all_points_loss(orc, ev, eta, gre, gim):
+    sr, si = orc.at_points(ev, ev, eta)
+    return float(np.dot(gre, sr) + np.dot(gim, si))
+
+
+def test_all_points_adjoint_matches_two_roles_and_fd(orc):
+    """orc_adjoint_all_points at a moderate eta: the sum of the two roles of
+    orc_adjoint_at_points at queries = eigenvalues (where leaving the self
+    pair out or in differs only by rounding), central finite differences of
+    the all-points loss with every eigenvalue moving in both roles, and the
+    same eta gradient."""
+    rng = np.random.default_rng(7)
+    ev = np.sort(rng.uniform(0.5, 2.0, 60))
+    eta = 0.05
+    gre, gim = rng.standard_normal(60), rng.standard_normal(60)
+    g, ge = orc.adjoint_all_points(ev, eta, gre, gim)
+    gev, gq, ge2 = orc.adjoint_at_points(ev, ev, eta, gre, gim)
+    assert _rel(g, gev + gq) <= 1e-13
+    assert ge == ge2
+    h = 1e-6
+    fd = np.empty_like(ev)
+    for i in range(ev.size):
+        e1, e0 = ev.copy(), ev.copy()
+        e1[i] += h
+        e0[i] -= h
+        fd[i] = (_all_points_loss(orc, e1, eta, gre, gim)
+                 - _all_points_loss(orc, e0, eta, gre, gim)) / (2 * h)
+    assert _rel(g, fd) < 1e-6
+    # translation invariance of the all-points sums: the gradient sums to zero
+    assert abs(g.sum()) <= 1e-12 * np.sum(np.abs(g))
+
+
+def test_all_points_adjoint_exact_at_tiny_eta(orc):
+    """At eta = 1e-10 the two roles' self terms are +-G_re/eta^2 = 1e20: the
+    oracle's all-points gradient equals the exact rational sum over j != i
+    (Python fractions on the same doubles) to 1e-15, while adding the two
+    roles of orc_adjoint_at_points loses most of its digits to the rounding
+    of 1e20 (the failure the all-points form avoids)."""
+    from fractions import Fraction as F
+    rng = np.random.default_rng(11)
+    p = 24
+    ev = np.sort(rng.uniform(0.5, 2.0, p))
+    eta = 1e-10
+    gre, gim = rng.standard_normal(p), rng.standard_normal(p)
+    g, _ = orc.adjoint_all_points(ev, eta, gre, gim)
+    fe = F(eta)
+    exact = []
+    for i in range(p):
+        acc = F(0)
+        for j in range(p):
+            if j == i:
+                continue
+            d = F(ev[i]) - F(ev[j])
+            den = d * d + fe * fe
+            acc += ((F(gre[j]) - F(gre[i])) * (d * d - fe * fe) - 2 * fe * d * (F(gim[i]) + F(gim[j]))) / (den * den)
+        exact.append(float(acc))
+    exact = np.array(exact)
+    assert _rel(g, exact) <= 1e-15
+    gev, gq, _ = orc.adjoint_at_points(ev, ev, eta, gre, gim)
+    assert _rel(gev + gq, exact) > 1e-9
