@@ -148,6 +148,15 @@ double shk_default_eta(size_t p);
  * report which kernel ran. */
 shk_status shk_symmetric_range(int precision, size_t *min_p, size_t *max_p);
 
+/* Caps the device scratch that the symmetric kernels' partial sums may take
+ * (forward and adjoint; default 512 MiB, bytes = 0 restores it).  When a
+ * transform's partials (16 * ceil(p/512) * p bytes) exceed the cap, the
+ * triangle of tile pairs runs in bands of rows whose partials fit, each band
+ * followed by an in-order fold; results are the same bits for every cap.
+ * Process-wide; calls in flight keep the cap they started with.  No
+ * counterpart in the reference (its kernels keep no partial sums). */
+shk_status shk_set_symmetric_scratch_limit(size_t bytes);
+
 /* Resolve the auto presets (src/config.rs:543-555) for this library:
  * Auto/SpeedAuto -> CHEBCODE_FAST, AccuracyAuto -> BLOCKED; a concrete
  * method is returned unchanged. */
@@ -206,7 +215,7 @@ shk_status shk_all_points_dev(const double *d_eigenvalues, size_t p, double eta,
  * of d_out_re / d_out_im.  For the exact family it is bit for bit the
  * split-tile single-spectrum kernel's: shk_all_points_dev's, except where
  * shk_all_points_dev takes the symmetric kernel (shk_symmetric_range: 8 192
- * to 262 144 FP32-compensated eigenvalues, 18 432 to 262 144 in FP64) (same
+ * to 4 194 304 FP32-compensated eigenvalues, 18 432 to 4 194 304 in FP64) (same
  * as shk_exact_at_points_dev with the spectrum as its own query points, raw).
  * If precision is SHK_PREC_F32C and some eta lies outside [1e-18, 1e18], the
  * whole call runs in FP64.
@@ -376,13 +385,12 @@ shk_status shk_exact_adjoint_dev(const double *d_query, size_t nq, const double 
  * eigenvalue as query and as source: the sum of shk_exact_adjoint_dev's
  * eigenvalue and query gradients at queries = eigenvalues) and dL/deta (one
  * device double; NULL skips it).  A pair (i, j) adds T to dL/dlam_i and -T to
- * dL/dlam_j, so for 8192 <= p <= 262144 each unordered pair is evaluated
- * once (symmetric kernels); other sizes add the two sweeps of
- * shk_exact_adjoint_dev in FP64.  SHK_PREC_F32C gives FP32-compensated
- * arithmetic only in the symmetric range with 1e-9 <= eta <= 1e18, and FP64
- * otherwise: in the two sweeps the diagonal pair's terms -G_re/eta^2 and
- * +G_re/eta^2 would cancel only to f32 rounding (an eta gradient alone,
- * d_grad_eigenvalues = NULL, keeps FP32 arithmetic in that eta range).
+ * dL/dlam_j, so each unordered pair is evaluated once (symmetric kernels,
+ * p <= 4 194 304; the pair (i, i), whose T is 0, is left out of the
+ * gradient sums, so no 1/eta^2 terms cancel).  Larger p add the two sweeps
+ * of shk_exact_adjoint_dev in FP64.  SHK_PREC_F32C gives FP32-compensated
+ * arithmetic for 1e-9 <= eta <= 1e18 and FP64 otherwise (an eta gradient
+ * alone, d_grad_eigenvalues = NULL, runs the query sweep).
  * Deterministic.  The
  * reference differentiates with respect to the query only (autovec.rs:38-59)
  * and has no counterpart. */

@@ -52,6 +52,7 @@ SIGNATURES = {
     "shk_default_eta": (_d, [_sz]),
     "shk_resolve_auto_method": (_i, [_i, _i, _sz]),
     "shk_symmetric_range": (_i, [_i, _szp, _szp]),
+    "shk_set_symmetric_scratch_limit": (_i, [_sz]),
     "shk_compute_all_stieltjes": (_i, [_dp, _sz, _d, _i, _d, _i, _i, _dp, _dp]),
     "shk_compute_all_stieltjes_f32": (_i, [_fp, _sz, _f, _i, _d, _fp, _fp]),
     "shk_compute_stieltjes_at_points": (_i, [_dp, _sz, _dp, _sz, _d, _i, _d, _i, _dp, _dp]),

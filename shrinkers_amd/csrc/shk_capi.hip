@@ -21,6 +21,8 @@ namespace shk {
 
 namespace {
 thread_local std::string g_err;
+// Scratch budget of the symmetric kernels' partial sums (bytes).
+std::atomic<size_t> g_sym_budget{shk::kSymScratchBudget};
 
 std::atomic<int> g_prof_on{0};
 std::mutex g_prof_mu;
@@ -459,6 +461,7 @@ int run_atpoints(const double *d_q, size_t nq, const double *d_ev, size_t p, dou
 
 namespace shk {
 int device_ready() { return ensure_device(); }
+size_t sym_scratch_budget() { return g_sym_budget.load(std::memory_order_relaxed); }
 }  // namespace shk
 
 // ---- per-thread device buffer cache ----------------------------------------
@@ -688,6 +691,11 @@ const char *shk_method_name(int method) {
 }
 
 double shk_default_eta(size_t p) { return 0.1 / std::sqrt((double)p); }
+
+shk_status shk_set_symmetric_scratch_limit(size_t bytes) {
+    g_sym_budget.store(bytes ? bytes : kSymScratchBudget, std::memory_order_relaxed);
+    return SHK_OK;
+}
 
 shk_status shk_symmetric_range(int precision, size_t *min_p, size_t *max_p) {
     if (!min_p || !max_p) return fail(SHK_ERR_BAD_ARG, "NULL pointer");

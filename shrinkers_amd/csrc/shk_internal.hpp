@@ -155,11 +155,62 @@ int exact_reduce_plain(const double *part, int n_split, size_t nq, double eta, d
 #define SHK_SYM_MIN_P 8192
 #endif
 constexpr size_t kSymMinP = SHK_SYM_MIN_P;
-constexpr size_t kSymMaxP = 262144;
+// Upper end of the symmetric kernels' range.  Their partials are bounded by
+// running the triangle of tile pairs in bands (SymBand): one band when the
+// whole triangle's partials fit in kSymScratchBudget bytes, otherwise bands of
+// R rows whose partials fit, folded in slot order after each band.
+constexpr size_t kSymMaxP = (size_t)1 << 22;
+constexpr size_t kSymScratchBudget = (size_t)512 << 20;
+// The current budget: kSymScratchBudget unless shk_set_symmetric_scratch_limit
+// set another (process-wide).
+size_t sym_scratch_budget();
+// Tiles of the symmetric kernels (forward and adjoint).
+constexpr int kSymTile = 512;
+
+// One band of rows [r0, r1) of a symmetric kernel's triangle of tile pairs
+// (a, b), a <= b, and where its partials go.  Tile t's slot m holds the
+// partial of the pair {t, m} at t's points.  Band tiles (r0 <= t < r1) get
+// their slots m in [r0, nt) in A (the j-sides of band rows m < t and the
+// i-sides of row t); tiles beyond the band (t >= r1) get the band rows'
+// slots m in [r0, r1) in B.  Slots m < r0 were folded by earlier bands.
+// With r0 = 0, r1 = nt, A is the single-launch layout part[c][m][i].
+struct SymBand {
+    double *A;  // [2][nt - r0][na]: points [r0 T, r0 T + na)
+    double *B;  // [2][r1 - r0][nb]: points [r1 T, r1 T + nb)
+    int r0, r1, nt, na, nb;
+    // Component c's partial for tile t, slot m: pointer at the tile's first point.
+    __host__ __device__ double *slot(int c, int t, int m) const {
+        if (t < r1)
+            return A + (size_t)(c * (nt - r0) + (m - r0)) * na + (size_t)(t - r0) * kSymTile;
+        return B + (size_t)(c * (r1 - r0) + (m - r0)) * nb + (size_t)(t - r1) * kSymTile;
+    }
+};
+
+// Rows per band for p eigenvalues with `avail` doubles of scratch: nt (one
+// band) when the whole triangle's partials fit (2 nt p doubles), else the
+// largest R whose band partials and running sums fit, 0 if none does.
+int sym_band_rows(size_t p, size_t avail);
+// Scratch doubles a run with R rows per band uses.
+size_t sym_scratch_doubles(size_t p, int R);
+// Launches one band's kernel: grid = the band's tile pairs, id0 = the index
+// of its first pair in the row-major triangle.
+using SymLaunch = int (*)(const SymBand &bd, int grid, int id0, void *ctx);
+// Runs the triangle band by band on `scratch` (avail doubles): each band's
+// launch, then an in-order fold of its partials -- final outputs for the band
+// tiles, out0 = s0 * scale and out1 = (mul * s1) * scale, running sums for
+// the tiles beyond.  The additions per point happen in slot order 0..nt-1
+// for every band count, so results are the same bits for every budget.
+int sym_run_bands(size_t p, double *scratch, size_t avail, double mul, double scale,
+                  double *out0, double *out1, SymLaunch launch, void *ctx, hipStream_t stream);
+
 // f64 != 0: the FP64 kernel (pair reciprocals as in rcp_pairs_f64), used for
 // kSym64MinP <= p <= kSymMaxP.
 int exact_sym(const double *ev, size_t p, double eta, double scale, int f64, double *out_re,
               double *out_im, hipStream_t stream);
+// The same on caller-provided scratch of scratch_doubles doubles (at least
+// sym_scratch_doubles(p, 1)); the band height follows from its size.
+int exact_sym_on(const double *ev, size_t p, double eta, double scale, int f64, double *out_re,
+                 double *out_im, double *scratch, size_t scratch_doubles, hipStream_t stream);
 // FP64 crossover (measured on MI355X): split ahead by 19 % at 1.64e4,
 // symmetric ahead by 9 % at 2e4, 52 % at 5e4 and 57 % at 1e5.
 #ifndef SHK_SYM64_MIN_P

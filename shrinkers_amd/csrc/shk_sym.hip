@@ -22,10 +22,14 @@
 //   a = b: the full square of tile a, i-side only (the diagonal term
 //   included, d = 0 exactly).
 //
-// Each query i of tile t receives exactly one FP64 partial per tile m:
-// part[.][m][i] holds the i-side of workgroup (t, m) for m >= t and the
-// j-side of workgroup (m, t) for m < t.  reduce_kernel adds the nt partials
-// in m order (deterministic) and applies eta and the 1/p scale.
+// Each query i of tile t receives exactly one FP64 partial per tile m (its
+// slot m): the i-side of workgroup (t, m) for m >= t and the j-side of
+// workgroup (m, t) for m < t.  The slots are added in m order (deterministic)
+// and eta and the 1/p scale applied once (sym_band_reduce_kernel).  When the
+// triangle's partials (16 nt p bytes) exceed the scratch budget, the rows of
+// the triangle run in bands: after each band the band tiles are finished and
+// the tiles beyond it carry running sums of their slots so far, so the
+// additions per point keep the order 0..nt-1 (SymBand, sym_run_bands).
 #include "shk_internal.hpp"
 #include "shk_pair.hpp"
 
@@ -137,11 +141,12 @@ __device__ __forceinline__ void sym_sweep(const double *__restrict__ s_src, int 
 }
 
 // i-side finish: waves 1..3 hand their partial sums to wave 0 in turn (fixed
-// order, through `buf`, 2 * kST doubles), and wave 0 writes part[.][b][i].
+// order, through `buf`, 2 * kST doubles), and wave 0 writes tile a's slot b
+// (i_re / i_im point at the tile's first point).
 __device__ __forceinline__ void sym_finish(double (&re)[kSR], double (&iv)[kSR],
                                            double *__restrict__ buf, int w, int lane,
-                                           double *__restrict__ part, int nt, int p, int b,
-                                           int q0) {
+                                           double *__restrict__ i_re, double *__restrict__ i_im,
+                                           int p, int q0) {
     for (int r = 1; r < kSW; ++r) {
         __syncthreads();
         if (w == r) {
@@ -161,14 +166,12 @@ __device__ __forceinline__ void sym_finish(double (&re)[kSR], double (&iv)[kSR],
         }
     }
     if (w == 0) {
-        double *i_re = part + (size_t)b * p;          // part[0][b][i]
-        double *i_im = part + ((size_t)nt + b) * p;   // part[1][b][i]
+        const int l0 = lane * kSR;  // this lane's first query within the tile
 #pragma unroll
         for (int k = 0; k < kSR; ++k) {
-            const int i = q0 + k;
-            if (i < p) {
-                i_re[i] = re[k];
-                i_im[i] = iv[k];
+            if (q0 + k < p) {
+                i_re[l0 + k] = re[k];
+                i_im[l0 + k] = iv[k];
             }
         }
     }
@@ -238,10 +241,9 @@ __device__ __forceinline__ void sym_sweep_f64(const double *__restrict__ s_src, 
 }
 
 __global__ __launch_bounds__(kSL * kSW) void sym_f64_kernel(const double *__restrict__ ev, int p,
-                                                            double eta, int nt,
-                                                            double *__restrict__ part) {
-    int a, b;  // query tile a, source tile b >= a
-    tile_pair_of((int)blockIdx.x, nt, a, b);
+                                                            double eta, SymBand bd, int id0) {
+    int a, b;  // query tile a (in the band), source tile b >= a
+    tile_pair_of((int)blockIdx.x + id0, bd.nt, a, b);
     __shared__ double s_src[kSW][kSQ];
     __shared__ double s_jbuf[kSW * 2 * kSB64 * kJRow];  // j-side lane sums / i-side combine
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -258,23 +260,22 @@ __global__ __launch_bounds__(kSL * kSW) void sym_f64_kernel(const double *__rest
     for (int u = lane; u < n_src; u += kSL) s_src[w][u] = ev[js + u];
     __syncthreads();
     double *s_jb = s_jbuf + w * (2 * kSB64 * kJRow);
-    double *j_re = part + (size_t)a * p + js;         // part[0][a][j]
-    double *j_im = part + ((size_t)nt + a) * p + js;  // part[1][a][j]
+    double *j_re = bd.slot(0, b, a) + w * kSQ;  // tile b, slot a, this wave's sources
+    double *j_im = bd.slot(1, b, a) + w * kSQ;
     const double e2 = eta * eta;
     if (a < b)
         sym_sweep_f64<true>(s_src[w], n_src, x, e2, re, iv, s_jb, j_re, j_im, lane);
     else
         sym_sweep_f64<false>(s_src[w], n_src, x, e2, re, iv, s_jb, j_re, j_im, lane);
-    sym_finish(re, iv, s_jbuf, w, lane, part, nt, p, b, q0);
+    sym_finish(re, iv, s_jbuf, w, lane, bd.slot(0, a, b), bd.slot(1, a, b), p, q0);
 }
 
 static_assert(kSW * 2 * kSB64 * kJRow >= 2 * kST, "FP64 combine buffer fits in s_jbuf");
 
 __global__ __launch_bounds__(kSL * kSW) void sym_f32c_kernel(const double *__restrict__ ev, int p,
-                                                             double eta, int nt,
-                                                             double *__restrict__ part) {
-    int a, b;  // query tile a, source tile b >= a
-    tile_pair_of((int)blockIdx.x, nt, a, b);
+                                                             double eta, SymBand bd, int id0) {
+    int a, b;  // query tile a (in the band), source tile b >= a
+    tile_pair_of((int)blockIdx.x + id0, bd.nt, a, b);
     __shared__ double s_src[kSW][kSQ];                      // 8 KB
     __shared__ double s_jbuf[kSW * 2 * kSB * kJRow / 2];    // j-side lane sums / i-side combine
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -316,8 +317,8 @@ __global__ __launch_bounds__(kSL * kSW) void sym_f32c_kernel(const double *__res
     __syncthreads();
 
     float *s_jb = reinterpret_cast<float *>(s_jbuf) + w * (2 * kSB * kJRow);
-    double *j_re = part + (size_t)a * p + js;                  // part[0][a][j]
-    double *j_im = part + ((size_t)nt + a) * p + js;           // part[1][a][j]
+    double *j_re = bd.slot(0, b, a) + w * kSQ;  // tile b, slot a, this wave's sources
+    double *j_im = bd.slot(1, b, a) + w * kSQ;
     const float eta2f = (float)(eta * eta);
     if (jside) {
         if (form == 0)
@@ -335,13 +336,138 @@ __global__ __launch_bounds__(kSL * kSW) void sym_f32c_kernel(const double *__res
             sym_sweep<2, false>(s_src[w], n_src, c, qh, ql, eta2f, re, iv, s_jb, j_re, j_im, lane);
     }
 
-    sym_finish(re, iv, s_jbuf, w, lane, part, nt, p, b, q0);
+    sym_finish(re, iv, s_jbuf, w, lane, bd.slot(0, a, b), bd.slot(1, a, b), p, q0);
 }
 
 static_assert(kSW * 2 * kSB * kJRow / 2 >= 2 * kST, "i-side combine buffer fits in s_jbuf");
 static_assert(kSQ % kSB == 0, "batches tile the wave's sources");
+static_assert(kST == kSymTile, "the band layout's tile");
+
+// The in-order fold of one band's partials, one thread per point of
+// [r0 T, p): band points add their running sums (none in the first band) and
+// slots r0..nt-1 and store out0 = s0 * scale, out1 = (mul * s1) * scale;
+// points beyond the band add slots r0..r1-1 to their running sums
+// (acc[0, p) and acc[p, 2p)).  Loads run kFoldAhead slots ahead of the
+// additions.  A group of 64 points never straddles a tile.
+constexpr int kFoldWG = 64;
+constexpr int kFoldAhead = 8;
+
+__global__ __launch_bounds__(kFoldWG) void sym_band_reduce_kernel(SymBand bd, int p, double mul,
+                                                                 double scale,
+                                                                 double *__restrict__ acc,
+                                                                 double *__restrict__ o0,
+                                                                 double *__restrict__ o1) {
+    const int i = bd.r0 * kSymTile + (int)(blockIdx.x * kFoldWG + threadIdx.x);
+    if (i >= p) return;
+    const int t = i / kSymTile, loc = i - t * kSymTile;
+    const bool band = t < bd.r1;
+    const int n = (band ? bd.nt : bd.r1) - bd.r0;
+    const size_t stride = band ? (size_t)bd.na : (size_t)bd.nb;
+    const double *c0 = bd.slot(0, t, bd.r0) + loc;
+    const double *c1 = bd.slot(1, t, bd.r0) + loc;
+    double s0 = 0.0, s1 = 0.0;
+    if (bd.r0 > 0) {
+        s0 = acc[i];
+        s1 = acc[(size_t)p + i];
+    }
+    int m = 0;
+    for (; m + kFoldAhead <= n; m += kFoldAhead) {
+        double v0[kFoldAhead], v1[kFoldAhead];
+#pragma unroll
+        for (int u = 0; u < kFoldAhead; ++u) {
+            v0[u] = c0[(size_t)(m + u) * stride];
+            v1[u] = c1[(size_t)(m + u) * stride];
+        }
+#pragma unroll
+        for (int u = 0; u < kFoldAhead; ++u) {
+            s0 += v0[u];
+            s1 += v1[u];
+        }
+    }
+    for (; m < n; ++m) {
+        s0 += c0[(size_t)m * stride];
+        s1 += c1[(size_t)m * stride];
+    }
+    if (band) {
+        o0[i] = s0 * scale;
+        o1[i] = (mul * s1) * scale;
+    } else {
+        acc[i] = s0;
+        acc[(size_t)p + i] = s1;
+    }
+}
+
+// index of row x's first tile pair in the row-major triangle of nt tiles
+inline int64_t row_start(int64_t x, int64_t nt) { return x * (2 * nt - x + 1) / 2; }
+
+struct SymCtx {
+    const double *ev;
+    int p;
+    double eta;
+    int f64;
+    hipStream_t stream;
+};
+
+int sym_launch(const SymBand &bd, int grid, int id0, void *vctx) {
+    const SymCtx &c = *static_cast<const SymCtx *>(vctx);
+    ProfileMark pm = profile_begin(c.stream);
+    if (c.f64)
+        hipLaunchKernelGGL(sym_f64_kernel, dim3(grid), dim3(kSL * kSW), 0, c.stream, c.ev, c.p,
+                           c.eta, bd, id0);
+    else
+        hipLaunchKernelGGL(sym_f32c_kernel, dim3(grid), dim3(kSL * kSW), 0, c.stream, c.ev, c.p,
+                           c.eta, bd, id0);
+    profile_end(pm, c.stream);
+    const hipError_t le = hipGetLastError();
+    if (le != hipSuccess) {
+        set_error(std::string("symmetric kernel launch: ") + hipGetErrorString(le));
+        return 7;
+    }
+    return 0;
+}
 
 }  // namespace
+
+int sym_band_rows(size_t p, size_t avail) {
+    const size_t nt = (p + kSymTile - 1) / kSymTile;
+    if (2 * nt * p <= avail) return (int)nt;
+    const size_t per_row = 2 * nt * kSymTile + 2 * p;  // A and B per band row
+    if (avail < 2 * p + per_row) return 0;
+    return (int)std::min(nt, (avail - 2 * p) / per_row);
+}
+
+size_t sym_scratch_doubles(size_t p, int R) {
+    const size_t nt = (p + kSymTile - 1) / kSymTile;
+    if ((size_t)R >= nt) return 2 * nt * p;
+    return 2 * nt * std::min(p, (size_t)R * kSymTile) + 2 * (size_t)R * p + 2 * p;
+}
+
+int sym_run_bands(size_t p, double *scratch, size_t avail, double mul, double scale,
+                  double *out0, double *out1, SymLaunch launch, void *ctx, hipStream_t stream) {
+    if (p == 0) return 0;
+    const int R = sym_band_rows(p, avail);
+    if (R <= 0) {
+        set_error("symmetric kernel: scratch too small for one band");
+        return 6;
+    }
+    const int nt = (int)((p + kSymTile - 1) / kSymTile);
+    // A first (the largest band's), then B, then the running sums
+    const size_t a_max = 2 * (size_t)nt * std::min(p, (size_t)R * kSymTile);
+    double *A = scratch, *B = scratch + a_max;
+    double *acc = B + 2 * (size_t)R * p;
+    for (int r0 = 0; r0 < nt; r0 += R) {
+        const int r1 = std::min(nt, r0 + R);
+        const size_t lo = (size_t)r0 * kSymTile, mid = std::min(p, (size_t)r1 * kSymTile);
+        const SymBand bd{A, B, r0, r1, nt, (int)(mid - lo), (int)(p - mid)};
+        const int64_t id0 = row_start(r0, nt), id1 = row_start(r1, nt);
+        if (int st = launch(bd, (int)(id1 - id0), (int)id0, ctx)) return st;
+        hipLaunchKernelGGL(sym_band_reduce_kernel,
+                           dim3((unsigned)((p - lo + kFoldWG - 1) / kFoldWG)), dim3(kFoldWG), 0,
+                           stream, bd, (int)p, mul, scale, acc, out0, out1);
+        SHK_HIP(hipGetLastError());
+    }
+    return 0;
+}
 
 int exact_sym(const double *ev, size_t p, double eta, double scale, int f64, double *out_re,
               double *out_im, hipStream_t stream) {
@@ -350,26 +476,24 @@ int exact_sym(const double *ev, size_t p, double eta, double scale, int f64, dou
         set_error("symmetric kernel: p above its maximum");
         return 6;
     }
-    const int nt = (int)((p + kST - 1) / kST);
+    const int R = std::max(1, sym_band_rows(p, sym_scratch_budget() / sizeof(double)));
+    const size_t need = sym_scratch_doubles(p, R);
     double *part = nullptr;
     CachedBuf cache(CACHE_EXACT_PART, stream);
-    if (int e = cache.get(2 * (size_t)nt * p, &part)) return e;
-    ProfileMark m = profile_begin(stream);
-    if (f64)
-        hipLaunchKernelGGL(sym_f64_kernel, dim3(nt * (nt + 1) / 2), dim3(kSL * kSW), 0, stream,
-                           ev, (int)p, eta, nt, part);
-    else
-        hipLaunchKernelGGL(sym_f32c_kernel, dim3(nt * (nt + 1) / 2), dim3(kSL * kSW), 0, stream,
-                           ev, (int)p, eta, nt, part);
-    profile_end(m, stream);
-    int st = 0;
-    const hipError_t le = hipGetLastError();
-    if (le != hipSuccess) {
-        set_error(std::string("symmetric kernel launch: ") + hipGetErrorString(le));
-        st = 7;
+    if (int e = cache.get(need, &part)) return e;
+    return exact_sym_on(ev, p, eta, scale, f64, out_re, out_im, part, need, stream);
+}
+
+int exact_sym_on(const double *ev, size_t p, double eta, double scale, int f64, double *out_re,
+                 double *out_im, double *scratch, size_t scratch_doubles, hipStream_t stream) {
+    if (p == 0) return 0;
+    if (p > (size_t)kSymMaxP) {
+        set_error("symmetric kernel: p above its maximum");
+        return 6;
     }
-    if (!st) st = exact_reduce_plain(part, nt, p, eta, scale, ev, out_re, out_im, stream);
-    return st;
+    SymCtx ctx{ev, (int)p, eta, f64, stream};
+    return sym_run_bands(p, scratch, scratch_doubles, eta, scale, out_re, out_im, sym_launch,
+                         &ctx, stream);
 }
 
 }  // namespace shk

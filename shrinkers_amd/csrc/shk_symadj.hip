@@ -32,7 +32,7 @@
 // lane, and the four waves split source tile b.  a < b: i-side sums in the
 // lanes, j-side lane sums of each source through LDS, reduced per batch in a
 // fixed order.  a = b: the full square of tile a, i-side only.  Each end
-// gets one (gradient, eta term) partial per tile m: part[.][m][i], the i-side
+// gets one (gradient, eta term) partial per tile m (its slot m), the i-side
 // of workgroup (t, m) for m >= t and the j-side of (m, t) for m < t; the
 // in-order reduction and an ordered sum of the eta terms finish.  Every sum
 // runs in a fixed order: results are bitwise reproducible.
@@ -242,15 +242,17 @@ __device__ __forceinline__ void adj_sweep_f64(
 }
 
 // i-side finish: waves 1..3 hand their sums to wave 0 in turn (fixed order,
-// through `buf`, 3 * kJT doubles); wave 0 writes part[.][b][i].  self_alpha
+// through `buf`, 3 * kJT doubles); wave 0 writes tile a's slot b (i_grad /
+// i_eta point at the tile's first point).  self_alpha
 // is the pair (i, i)'s alpha, -1/eta^2, in a diagonal tile (0 otherwise): it
 // was left out of the sums, and its eta term Gi_i alpha belongs to dL/deta.
 __device__ __forceinline__ void adj_finish(double (&P)[kJR], double (&A)[kJR], double (&B)[kJR],
                                            double *__restrict__ buf, int w, int lane,
                                            const double *__restrict__ g_re,
                                            const double *__restrict__ g_im, double two_eta,
-                                           double *__restrict__ part, int nt, int p, int b,
-                                           int q0, double self_alpha) {
+                                           double *__restrict__ i_grad,
+                                           double *__restrict__ i_eta, int p, int q0,
+                                           double self_alpha) {
     for (int r = 1; r < kJW; ++r) {
         __syncthreads();
         if (w == r) {
@@ -272,15 +274,14 @@ __device__ __forceinline__ void adj_finish(double (&P)[kJR], double (&A)[kJR], d
         }
     }
     if (w == 0) {
-        double *i_grad = part + (size_t)b * p;          // part[0][b][i]
-        double *i_eta = part + ((size_t)nt + b) * p;    // part[1][b][i]
+        const int l0 = lane * kJR;  // this lane's first query within the tile
 #pragma unroll
         for (int k = 0; k < kJR; ++k) {
             const int i = q0 + k;
             if (i < p) {
                 const double g_r = g_re[i], g_i = g_im[i];
-                i_grad[i] = (P[k] - g_r * A[k]) - (two_eta * g_i) * B[k];
-                i_eta[i] = g_i * (A[k] + self_alpha) - (two_eta * g_r) * B[k];
+                i_grad[l0 + k] = (P[k] - g_r * A[k]) - (two_eta * g_i) * B[k];
+                i_eta[l0 + k] = g_i * (A[k] + self_alpha) - (two_eta * g_r) * B[k];
             }
         }
     }
@@ -295,9 +296,9 @@ static_assert(kJQ % kAB == 0 && kJQ % kAB64 == 0, "batches tile the wave's sourc
 
 __global__ __launch_bounds__(kJL * kJW) void symadj_f32c_kernel(
     const double *__restrict__ ev, const double *__restrict__ g_re,
-    const double *__restrict__ g_im, int p, double eta, int nt, double *__restrict__ part) {
-    int a, b;  // query tile a, source tile b >= a
-    tile_pair_of((int)blockIdx.x, nt, a, b);
+    const double *__restrict__ g_im, int p, double eta, SymBand bd, int id0) {
+    int a, b;  // query tile a (in the band), source tile b >= a
+    tile_pair_of((int)blockIdx.x + id0, bd.nt, a, b);
     __shared__ double s_src[kJW][kJQ];
     __shared__ float s_gr[kJW][kJQ], s_gi[kJW][kJQ];
     __shared__ double s_jbuf[kJbufBytes / 8];
@@ -346,14 +347,14 @@ __global__ __launch_bounds__(kJL * kJW) void symadj_f32c_kernel(
     __syncthreads();
 
     float *s_jb = reinterpret_cast<float *>(s_jbuf) + w * (kComp * kAB * kARow);
-    double *j_grad = part + (size_t)a * p;          // part[0][a][j]
-    double *j_eta = part + ((size_t)nt + a) * p;    // part[1][a][j]
+    double *j_grad = bd.slot(0, b, a) + w * kJQ;  // tile b, slot a, this wave's sources
+    double *j_eta = bd.slot(1, b, a) + w * kJQ;
     const float eta2f = (float)(eta * eta);
     const float m2eta2f = -2.0f * eta2f;
 #define SHK_ADJ_SWEEP(F, J)                                                                     \
     adj_sweep_f32c<F, J>(s_src[w], s_gr[w], s_gi[w], n_src, c, qh, ql, gr, gi, eta2f, m2eta2f,  \
                          P, A, B,                                                               \
-                         s_jb, g_re + js, g_im + js, two_eta, j_grad + js, j_eta + js, lane,     \
+                         s_jb, g_re + js, g_im + js, two_eta, j_grad, j_eta, lane,     \
                          q0 - js)
     if (a < b) {
         if (form == 0) SHK_ADJ_SWEEP(0, true);
@@ -365,15 +366,15 @@ __global__ __launch_bounds__(kJL * kJW) void symadj_f32c_kernel(
         else SHK_ADJ_SWEEP(2, false);
     }
 #undef SHK_ADJ_SWEEP
-    adj_finish(P, A, B, s_jbuf, w, lane, g_re, g_im, two_eta, part, nt, p, b, q0,
-               a == b ? -1.0 / (eta * eta) : 0.0);
+    adj_finish(P, A, B, s_jbuf, w, lane, g_re, g_im, two_eta, bd.slot(0, a, b),
+               bd.slot(1, a, b), p, q0, a == b ? -1.0 / (eta * eta) : 0.0);
 }
 
 __global__ __launch_bounds__(kJL * kJW) void symadj_f64_kernel(
     const double *__restrict__ ev, const double *__restrict__ g_re,
-    const double *__restrict__ g_im, int p, double eta, int nt, double *__restrict__ part) {
-    int a, b;  // query tile a, source tile b >= a
-    tile_pair_of((int)blockIdx.x, nt, a, b);
+    const double *__restrict__ g_im, int p, double eta, SymBand bd, int id0) {
+    int a, b;  // query tile a (in the band), source tile b >= a
+    tile_pair_of((int)blockIdx.x + id0, bd.nt, a, b);
     __shared__ double s_src[kJW][kJQ];
     __shared__ double s_gr[kJW][kJQ], s_gi[kJW][kJQ];
     __shared__ double s_jbuf[kJbufBytes / 8];
@@ -398,19 +399,47 @@ __global__ __launch_bounds__(kJL * kJW) void symadj_f64_kernel(
     }
     __syncthreads();
     double *s_jb = s_jbuf + w * (kComp * kAB64 * kARow);
-    double *j_grad = part + (size_t)a * p;
-    double *j_eta = part + ((size_t)nt + a) * p;
+    double *j_grad = bd.slot(0, b, a) + w * kJQ;
+    double *j_eta = bd.slot(1, b, a) + w * kJQ;
     const double e2 = eta * eta, m2e2 = -2.0 * e2;
     if (a < b)
         adj_sweep_f64<true>(s_src[w], s_gr[w], s_gi[w], n_src, x, gr, gi, e2, m2e2, P, A, B, s_jb,
-                            g_re + js, g_im + js, two_eta, j_grad + js, j_eta + js, lane,
+                            g_re + js, g_im + js, two_eta, j_grad, j_eta, lane,
                             q0 - js);
     else
         adj_sweep_f64<false>(s_src[w], s_gr[w], s_gi[w], n_src, x, gr, gi, e2, m2e2, P, A, B, s_jb,
-                             g_re + js, g_im + js, two_eta, j_grad + js, j_eta + js, lane,
+                             g_re + js, g_im + js, two_eta, j_grad, j_eta, lane,
                              q0 - js);
-    adj_finish(P, A, B, s_jbuf, w, lane, g_re, g_im, two_eta, part, nt, p, b, q0,
-               a == b ? -1.0 / (eta * eta) : 0.0);
+    adj_finish(P, A, B, s_jbuf, w, lane, g_re, g_im, two_eta, bd.slot(0, a, b),
+               bd.slot(1, a, b), p, q0, a == b ? -1.0 / (eta * eta) : 0.0);
+}
+
+static_assert(kJT == kSymTile, "the band layout's tile");
+
+struct AdjCtx {
+    const double *ev, *gre, *gim;
+    int p;
+    double eta;
+    int f64;
+    hipStream_t stream;
+};
+
+int symadj_launch(const SymBand &bd, int grid, int id0, void *vctx) {
+    const AdjCtx &c = *static_cast<const AdjCtx *>(vctx);
+    ProfileMark pm = profile_begin(c.stream);
+    if (c.f64)
+        hipLaunchKernelGGL(symadj_f64_kernel, dim3(grid), dim3(kJL * kJW), 0, c.stream, c.ev,
+                           c.gre, c.gim, c.p, c.eta, bd, id0);
+    else
+        hipLaunchKernelGGL(symadj_f32c_kernel, dim3(grid), dim3(kJL * kJW), 0, c.stream, c.ev,
+                           c.gre, c.gim, c.p, c.eta, bd, id0);
+    profile_end(pm, c.stream);
+    const hipError_t le = hipGetLastError();
+    if (le != hipSuccess) {
+        set_error(std::string("symmetric adjoint launch: ") + hipGetErrorString(le));
+        return 7;
+    }
+    return 0;
 }
 
 }  // namespace
@@ -423,26 +452,17 @@ int exact_adjoint_sym(const double *ev, size_t p, double eta, const double *gre,
         set_error("symmetric adjoint: p above its maximum");
         return 6;
     }
-    const int nt = (int)((p + kJT - 1) / kJT);
+    // partials (banded within the budget) and the per-end eta terms
+    const int R = std::max(1, sym_band_rows(p, sym_scratch_budget() / sizeof(double)));
+    const size_t need = sym_scratch_doubles(p, R);
     double *part = nullptr;
     CachedBuf cache(CACHE_ADJ_PART, stream);
-    if (int e = cache.get(2 * (size_t)nt * p + p, &part)) return e;
-    double *eta_term = part + 2 * (size_t)nt * p;
-    ProfileMark m = profile_begin(stream);
-    if (f64)
-        hipLaunchKernelGGL(symadj_f64_kernel, dim3(nt * (nt + 1) / 2), dim3(kJL * kJW), 0,
-                           stream, ev, gre, gim, (int)p, eta, nt, part);
-    else
-        hipLaunchKernelGGL(symadj_f32c_kernel, dim3(nt * (nt + 1) / 2), dim3(kJL * kJW), 0,
-                           stream, ev, gre, gim, (int)p, eta, nt, part);
-    profile_end(m, stream);
-    const hipError_t le = hipGetLastError();
-    if (le != hipSuccess) {
-        set_error(std::string("symmetric adjoint launch: ") + hipGetErrorString(le));
-        return 7;
-    }
-    // in-order sums over the nt tiles: gradient and per-end eta terms
-    if (int st = exact_reduce_plain(part, nt, p, 1.0, 1.0, nullptr, grad, eta_term, stream))
+    if (int e = cache.get(need + p, &part)) return e;
+    double *eta_term = part + need;
+    AdjCtx ctx{ev, gre, gim, (int)p, eta, f64, stream};
+    // in-order sums over the nt slots: gradient and per-end eta terms
+    if (int st = sym_run_bands(p, part, need, 1.0, 1.0, grad, eta_term, symadj_launch, &ctx,
+                               stream))
         return st;
     if (grad_eta) return adjoint_ordered_sum(eta_term, p, grad_eta, accumulate_eta, stream);
     return 0;

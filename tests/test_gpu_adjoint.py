@@ -402,3 +402,24 @@ def test_all_points_adjoint_tiny(shk, orc, precision):
         tol = 1e-12 if precision == "f64" else 5e-6
         assert np.allclose(gev, rev + rq, rtol=tol, atol=tol * np.max(np.abs(rev)))
         assert abs(g_eta - re_) <= tol * abs(re_)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32c"])
+def test_symmetric_adjoint_banded_equals_single_band(shk, precision):
+    """The all-points adjoint with its partial scratch capped at 2 MB (bands of
+    a few rows at p = 20 000) equals the one-band run bit for bit, gradient and
+    eta gradient."""
+    from shrinkers_amd import _lib
+    p = 20_000
+    ev = _hard_spectrum("mp_spikes", p)
+    eta = 1.0 / np.sqrt(p)
+    rng = np.random.default_rng(4)
+    gre, gim = rng.standard_normal(p), rng.standard_normal(p)
+    want = _all_points_adjoint(ev, eta, gre, gim, precision)
+    lib = _lib.load()
+    _lib.check(lib.shk_set_symmetric_scratch_limit(2 << 20))
+    try:
+        got = _all_points_adjoint(ev, eta, gre, gim, precision)
+    finally:
+        _lib.check(lib.shk_set_symmetric_scratch_limit(0))
+    assert np.array_equal(got[0], want[0]) and got[1] == want[1]

@@ -253,7 +253,7 @@ def test_f64_paired_reciprocal_fallback(shk, orc):
 @pytest.mark.parametrize("p", [8192, 8193, 10000, 24576, 24577, 25087, 50000, 65536 + 511])
 def test_symmetric_f32c_kernel(shk, orc, p):
     """The all-points FP32-compensated entry takes the symmetric kernel for
-    8 192 <= p <= 262 144 (each unordered pair once, credited to both
+    8 192 <= p <= 4 194 304 (each unordered pair once, credited to both
     ends; tiles of 512).  Against the FP64 oracle within 1e-6 normwise, and
     per element within 1e-6 of max|S|; against the split-tile kernel (the
     spectrum as its own query points) within 1e-7.  Sizes: whole tiles, one
@@ -305,7 +305,7 @@ def test_symmetric_f32c_hard_inputs(shk, orc, p):
 @pytest.mark.parametrize("p", [18432, 20000, 24576, 24577, 50000])
 def test_symmetric_f64_kernel(shk, orc, p):
     """The FP64 all-points entry takes the symmetric FP64 kernel for
-    18 432 <= p <= 262 144: within 1e-13 of the FP64 oracle (normwise and per
+    18 432 <= p <= 4 194 304: within 1e-13 of the FP64 oracle (normwise and per
     element against max|S|), and within 1e-13 of the split-tile FP64 kernel
     (the spectrum as its own query points)."""
     import torch
@@ -463,3 +463,61 @@ def test_config1_spiked_wishart(shk, orc):
         orr, ori = orc.chebcode_preset(ev, eta, preset)
         tol = 1e-6 if preset == "chebcode_fast" else 1e-13
         assert rel_l2(r, i, orr / p, ori / p) <= tol, preset
+
+
+def _with_scratch_limit(nbytes, fn):
+    from shrinkers_amd import _lib
+    lib = _lib.load()
+    _lib.check(lib.shk_set_symmetric_scratch_limit(int(nbytes)))
+    try:
+        return fn()
+    finally:
+        _lib.check(lib.shk_set_symmetric_scratch_limit(0))
+
+
+@pytest.mark.parametrize("precision", ["f32c", "f64"])
+@pytest.mark.parametrize("p,limit_mb", [(30000, 3), (30000, 12), (20481, 1)])
+def test_symmetric_banded_equals_single_band(shk, orc, precision, p, limit_mb):
+    """With the partial scratch capped below the whole triangle's (16 nt p
+    bytes: 56 MB at 3e4, 13 MB at 20 481), the symmetric kernel runs the
+    triangle in bands of rows and folds each band in slot order: the result
+    equals the one-band run bit for bit (3 MB at 3e4: 30 bands of 2 rows;
+    12 MB: 5 bands of 12 rows; 1 MB at 20 481, a partial last tile: 41 bands
+    of 1 row)."""
+    import torch
+    from shrinkers_amd import spectra as S
+    ev = S.mp_inverse_cdf_spectrum(p, c=0.3)
+    eta = 1.0 / np.sqrt(p)
+    d = torch.from_numpy(ev).cuda()
+    want = shk.all_points_dev(d, eta, "blocked", precision=precision, scaled=False)
+    torch.cuda.synchronize()
+    got = _with_scratch_limit(limit_mb << 20, lambda: shk.all_points_dev(
+        d, eta, "blocked", precision=precision, scaled=False))
+    torch.cuda.synchronize()
+    assert torch.equal(got[0], want[0]) and torch.equal(got[1], want[1])
+    rr, ri = orc.tiled_parallel(ev, eta)
+    tol = 1e-6 if precision == "f32c" else 1e-13
+    assert rel_l2(got[0].cpu().numpy(), got[1].cpu().numpy(), rr, ri) <= tol
+
+
+def test_symmetric_above_old_maximum(shk):
+    """p = 300 000 (above the 262 144 the unbanded partials allowed: 2.8 GB of
+    them) runs the symmetric kernels in bands within the 512 MiB budget:
+    FP64 within 1e-13 of the split-tile FP64 kernel (the spectrum as its own
+    query points), FP32-compensated within 1e-6 of FP64."""
+    import torch
+    from shrinkers_amd import _lib
+    from shrinkers_amd import spectra as S
+    p = 300_000
+    ev = S.mp_inverse_cdf_spectrum(p, c=0.2)
+    eta = 1.0 / np.sqrt(p)
+    d = torch.from_numpy(ev).cuda()
+    r64, i64 = shk.all_points_dev(d, eta, "blocked", precision="f64", scaled=False)
+    r32, i32 = shk.all_points_dev(d, eta, "blocked", precision="f32c", scaled=False)
+    sr, si = torch.empty_like(d), torch.empty_like(d)
+    assert _lib.load().shk_exact_at_points_dev(d.data_ptr(), p, d.data_ptr(), p, eta, 0,
+                                               sr.data_ptr(), si.data_ptr(), None) == _lib.SHK_OK
+    torch.cuda.synchronize()
+    r64, i64, r32, i32, sr, si = (t.cpu().numpy() for t in (r64, i64, r32, i32, sr, si))
+    assert_close(r64, i64, sr, si, 1e-13, "banded symmetric f64 vs split f64 at 3e5")
+    assert rel_l2(r32, i32, r64, i64) <= 1e-6
