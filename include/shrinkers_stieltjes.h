@@ -206,6 +206,22 @@ shk_status shk_all_points_dev(const double *d_eigenvalues, size_t p, double eta,
                               double cutoff_ratio, int precision, int scaled,
                               double *d_out_re, double *d_out_im, void *stream);
 
+/* shk_all_points_dev for the exact family (no cutoff) without any
+ * allocation: no counterpart in the reference.  For callers that cannot
+ * allocate device memory -- code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  It calls no hipMalloc, hipFree or
+ * allocating library routine: the symmetric kernels (each unordered pair
+ * once) run band by band on a 64 MiB device array of the library, one per
+ * device, held from library load until the process ends.  p <= 262 144
+ * (SHK_ERR_BAD_ARG above).  precision SHK_PREC_F64 or SHK_PREC_F32C (FP64
+ * for eta outside [1e-18, 1e18]); scaled != 0 multiplies by 1/p.  For p in
+ * shk_symmetric_range the results equal shk_all_points_dev's bit for bit.
+ * Calls share their device's array: calls that overlap on one device (other
+ * streams, other threads) race and must be serialised by the caller. */
+shk_status shk_all_points_noalloc_dev(const double *d_eigenvalues, size_t p, double eta,
+                                      int precision, int scaled, double *d_out_re,
+                                      double *d_out_im, void *stream);
+
 /* Many independent spectra in one call (no counterpart in the reference,
  * which batches only across eta on one spectrum, chebcode.rs:1312).
  * d_eigenvalues holds the spectra back to back; offsets (host, n_spectra+1

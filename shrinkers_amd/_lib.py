@@ -59,6 +59,7 @@ SIGNATURES = {
     "shk_compute_all_stieltjes_with_deriv": (_i, [_dp, _sz, _d, _dp, _dp, _dp, _dp]),
     "shk_compute_stieltjes_scaled_ray": (_i, [_dp, _sz, _d, _dp, _dp]),
     "shk_all_points_dev": (_i, [_vp, _sz, _d, _i, _d, _i, _i, _vp, _vp, _vp]),
+    "shk_all_points_noalloc_dev": (_i, [_vp, _sz, _d, _i, _i, _vp, _vp, _vp]),
     "shk_at_points_dev": (_i, [_vp, _sz, _vp, _sz, _d, _i, _d, _vp, _vp, _vp]),
     "shk_all_points_many_dev": (_i, [_vp, _i64p, _sz, _dp, _i, _i, _i, _vp, _vp, _i,
                                      ctypes.POINTER(ctypes.c_float), _vp]),
