@@ -257,7 +257,7 @@ def chebcode_leg(shk, sync) -> dict:
     import torch
     from shrinkers_amd import spectra
     res = {}
-    for p, c in ((200_000, 0.1), (200_000, 0.5), (1_000_000, 0.1)):
+    for p, c in ((200_000, 0.1), (200_000, 0.5), (1_000_000, 0.1), (1_000_000, 0.5)):
         ev = spectra.mp_inverse_cdf_spectrum(p, c=c)
         eta = 1.0 / np.sqrt(p)
         d = torch.from_numpy(ev).cuda()
@@ -311,9 +311,44 @@ def chebcode_leg(shk, sync) -> dict:
                 for _ in range(3):
                     b.evaluate_many(etas)
                 row[name]["ms_evaluate_many_4eta_host"] = (time.perf_counter() - t0) / 3 * 1e3
+                # the same on device buffers (no host copies)
+                mo = b.evaluate_many_dev(etas)
+                sync()
+                t0 = time.perf_counter()
+                for _ in range(5):
+                    b.evaluate_many_dev(etas, out=mo)
+                sync()
+                row[name]["ms_evaluate_many_4eta_dev"] = (time.perf_counter() - t0) / 5 * 1e3
             del b
+        # CPU comparators on this host's cores: the oracle's restatements of
+        # the reference's ChebCode FAST (chebcode.rs:509-828) and of its
+        # coarse-grid NUFFT (nufft.rs, fft5.rs; the reference's auto pick
+        # above p = 50 000, pareto_autogen.rs:65-71), errors against the GPU
+        # FP64 exact sums
+        row["cpu"] = cpu_config3(ev, eta, er, ei)
         res[f"p={p},c={c}"] = row
     return res
+
+
+def cpu_config3(ev, eta, er, ei) -> dict:
+    import oracle
+    oracle.lib()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    den = np.sum(er ** 2 + ei ** 2)
+    out = {"cores": threads}
+    for name, fn in (("nufft", lambda: oracle.nufft_all(ev, eta, threads)),
+                     ("chebcode_fast", lambda: oracle.chebcode_preset(ev, eta, "chebcode_fast",
+                                                                       threads))):
+        r, i = fn()  # warm-up
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        out[name] = {"ms_per_transform": 1e3 * float(np.median(ts)),
+                     "rel_l2_vs_fp64_exact": float(np.sqrt(np.sum((r - er) ** 2 + (i - ei) ** 2)
+                                                           / den))}
+    return out
 
 
 def adjoint_leg(shk, sync, d_ev, eta) -> dict:

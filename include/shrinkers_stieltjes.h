@@ -310,6 +310,16 @@ shk_status shk_chebcode_build_dev(const double *d_eigenvalues, size_t p, double 
 shk_status shk_chebcode_evaluate_dev(const shk_chebcode *h, double eta, double scale,
                                      double *d_out_re, double *d_out_im, void *stream);
 
+/* ChebCodeBatch::evaluate_many (chebcode.rs:1312-1361) on device buffers:
+ * one RAW result row per eta (host array etas), d_out_re / d_out_im are
+ * n_eta*p row-major on the device, in build input order; two etas per
+ * traversal (acceptance at the smaller), enqueued on `stream` without
+ * synchronising.  Bit for bit shk_chebcode_evaluate_many without the host
+ * copies. */
+shk_status shk_chebcode_evaluate_many_dev(const shk_chebcode *h, const double *etas,
+                                          size_t n_eta, double *d_out_re, double *d_out_im,
+                                          void *stream);
+
 /* Work of one traversal pass, with the evaluation kernels' node decisions:
  * *near_pairs = sum over queries of the sizes of the leaves each visits
  * (exact pair evaluations), *far_terms = n per accepted panel.  d_query =

@@ -79,6 +79,9 @@ def lib() -> ctypes.CDLL:
     L.orc_cheb_count_work.argtypes = [ctypes.c_void_p, _f64p, _i64, _d, ctypes.c_int, _lp, _lp]
     L.orc_grid_leaf_cap.argtypes = [_i64, _i64, _i64, _i64]
     L.orc_adjoint_all_points.argtypes = [_f64p, _i64, _d, _f64p, _f64p, _f64p, _dp]
+    L.orc_nufft_grid_size.argtypes = [_d, _d, _d, _d, _i64]
+    L.orc_nufft_grid_size.restype = _i64
+    L.orc_nufft_at_points.argtypes = [_f64p, _i64, _f64p, _i64, _d, ctypes.c_int, _f64p, _f64p]
     L.orc_grid_leaf_cap.restype = _i64
     L.orc_max_threads.restype = ctypes.c_int
     L.orc_set_outside_theta2.argtypes = [ctypes.c_float]
@@ -330,6 +333,26 @@ def chebcode_preset(ev, eta, preset: str, threads: int = 0):
 
 def grid_leaf_cap(n, p, nq, preset_leaf):
     return int(lib().orc_grid_leaf_cap(n, p, nq, preset_leaf))
+
+
+def nufft_grid_size(raw_range: float, eta: float, pad_eta_mult: float = 2000.0,
+                    pad_range_frac: float = 0.75, m_override: int = 0) -> int:
+    """The coarse-grid NUFFT's grid size rule (nufft.rs:104-124)."""
+    return int(lib().orc_nufft_grid_size(float(raw_range), float(eta), float(pad_eta_mult),
+                                         float(pad_range_frac), int(m_override)))
+
+
+def nufft_at_points(ev, q, eta, threads: int = 0):
+    """Coarse-grid NUFFT raw sums at points q (eigenvalues sorted ascending)."""
+    ev, q = _c(ev), _c(q)
+    re, im = np.empty_like(q), np.empty_like(q)
+    lib().orc_nufft_at_points(ev, ev.size, q, q.size, float(eta), int(threads), re, im)
+    return re, im
+
+
+def nufft_all(ev, eta, threads: int = 0):
+    """Coarse-grid NUFFT raw sums at every eigenvalue (sorted ascending)."""
+    return nufft_at_points(ev, ev, eta, threads)
 
 
 def set_outside_theta2(v: float) -> None:

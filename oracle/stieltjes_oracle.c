@@ -44,6 +44,10 @@
  *                               gradients is 0); homogeneity of degree -1
  *                               (lam.grad + x.grad + eta.grad = -L); eta part
  *                               against orc_with_deriv (dS/deta = -i dS/dx)
+ *   orc_nufft_*               the reference's sizing test (nufft.rs:239-263),
+ *                               its accuracy test (nufft.rs:265-318) and its
+ *                               recorded errors on its benchmark distribution
+ *                               (docs/nufft_algorithms.md:141-147)
  *   orc_adjoint_all_points    grad_ev + grad_q of orc_adjoint_at_points at
  *                               moderate eta; exact rational sums at tiny eta;
  *                               central FD of the all-points loss
@@ -1008,4 +1012,204 @@ EXPORT int orc_max_threads(void) {
 #else
     return 1;
 #endif
+}
+
+/* ------------------------------------------------------------------------ */
+/* Coarse-grid NUFFT (the reference's CPU pick above p = 50 000)             */
+/* ------------------------------------------------------------------------ */
+
+/* Grid size rule, src/stieltjes/nufft.rs:104-124 with the defaults of
+ * :33-57 (pad 2000 eta or 0.75 range, 5 cells per eta, grid in
+ * [8192, 65536], power of two); m_override > 0 forces it (:109-111).
+ * Pinned by the reference's own sizing test (nufft.rs:239-263). */
+EXPORT int64_t orc_nufft_grid_size(double raw_range, double eta, double pad_eta_mult,
+                                   double pad_range_frac, int64_t m_override) {
+    const int64_t min_grid = 8192, max_grid_opt = 65536;
+    if (m_override > 0) return m_override < 2 ? 2 : m_override;
+    double pad = pad_eta_mult * eta;
+    if (pad_range_frac * raw_range > pad) pad = pad_range_frac * raw_range;
+    const double range = raw_range + 2.0 * pad;
+    int64_t max_grid = max_grid_opt > min_grid ? max_grid_opt : min_grid;
+    if (max_grid < 2) max_grid = 2;
+    double target = 5.0 * range / eta;
+    target = isfinite(target) ? (target < (double)max_grid ? target : (double)max_grid)
+                              : (double)max_grid;
+    int64_t t = (int64_t)ceil(target);
+    if (t < min_grid) t = min_grid;
+    int64_t m = 1;
+    while (m < t) m <<= 1;  /* next_pow2, mod.rs:531-537 */
+    return m < max_grid ? m : max_grid;
+}
+
+/* Lagrange weights on N uniform nodes at offset t in [0, 1) from node i0
+ * (fft5.rs:136-164): prefix/suffix products over (t - x_l), x_l = l - (N/2-1). */
+static void nufft_lagrange(int N, double t, double *w) {
+    const double q = (double)(N / 2);
+    double pref[8], suf[8], acc = 1.0;
+    for (int j = 0; j < N; j++) {
+        pref[j] = acc;
+        acc *= t - (double)j + q - 1.0;
+    }
+    acc = 1.0;
+    for (int j = N - 1; j >= 0; j--) {
+        suf[j] = acc;
+        acc *= t - (double)j + q - 1.0;
+    }
+    for (int j = 0; j < N; j++) {
+        double den = 1.0;
+        for (int l = 0; l < N; l++)
+            if (l != j) den *= (double)(j - l);
+        w[j] = pref[j] * suf[j] / den;
+    }
+}
+
+/* In-place complex DFT of length m (a power of two): iterative radix-2,
+ * X[k] = sum_j x[j] exp(sign * 2 pi i jk / m), unnormalised -- the
+ * convention of rustfft's Forward (sign -1) and Inverse (sign +1)
+ * (fftplan.rs:26-28).  Rounding differs from rustfft's mixed-radix plans
+ * (~1e-15 relative), far below the method's ~1e-5 error. */
+static void nufft_fft(double complex *x, int64_t m, int sign) {
+    for (int64_t i = 1, j = 0; i < m; i++) {
+        int64_t bit = m >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) {
+            double complex t = x[i];
+            x[i] = x[j];
+            x[j] = t;
+        }
+    }
+    for (int64_t len = 2; len <= m; len <<= 1) {
+        const double ang = sign * 2.0 * M_PI / (double)len;
+        for (int64_t k = 0; k < len / 2; k++) {
+            const double complex w = cos(ang * (double)k) + I * sin(ang * (double)k);
+            for (int64_t s = 0; s < m; s += len) {
+                const double complex u = x[s + k], v = x[s + k + len / 2] * w;
+                x[s + k] = u + v;
+                x[s + k + len / 2] = u - v;
+            }
+        }
+    }
+}
+
+/* NUFFT Stieltjes sums at nq query points (raw, as the reference returns
+ * them): nufft.rs:162-200 with NufftOptions::default (Cubic stencil, :80-91)
+ * over fft5::fft_convolution (fft5.rs:325-433) and interpolate_grid
+ * (:440-476): splat the eigenvalues (sorted) onto m cells of the padded
+ * range, one forward FFT of density + i*R_odd (R_odd(x) = x/(x^2+eta^2) on
+ * signed cell distances), unpack both spectra (fftplan.rs:43-54), the even
+ * kernel's spectrum analytically (pi/dx) r^|k| with r = exp(-2 pi eta /
+ * (m dx)) by recurrence, pack D K_even + i D R_odd (fftplan.rs:62-76), one
+ * inverse FFT, then a 4-point Lagrange interpolation of (Im, Re) / m at each
+ * query.  threads > 0: per-thread splat buffers added element-wise (the
+ * reference's parallel splat, fft5.rs:259-293) and parallel interpolation;
+ * the FFTs stay sequential, as in the reference.  Pinned by the reference's
+ * test bound (nufft.rs:265-318: < 1e-4 normwise on the log spectrum at p =
+ * 8192, all points and a 200-point grid), its sizing test, and its
+ * documented error against the exact sums (~2e-5 at p = 5e4, ~4e-5 at 1e6,
+ * nufft.rs:30-32). */
+EXPORT void orc_nufft_at_points(const double *ev, int64_t p, const double *q, int64_t nq,
+                                double eta, int threads, double *out_re, double *out_im) {
+    const int N = 4; /* Order::Cubic */
+    if (p == 0 || nq == 0) return;
+    const double lo_raw = ev[0], hi_raw = ev[p - 1], raw_range = hi_raw - lo_raw;
+    const int64_t m = orc_nufft_grid_size(raw_range, eta, 2000.0, 0.75, 0);
+    double pad = 2000.0 * eta;
+    if (0.75 * raw_range > pad) pad = 0.75 * raw_range;
+    const double lo = lo_raw - pad, hi = hi_raw + pad, range = hi - lo;
+    const double dx = range / (double)m;
+    const int64_t half = m / 2;
+    const int off0 = -N / 2 + 1;
+    double *density = calloc((size_t)m, sizeof(double));
+#ifdef _OPENMP
+    const int nthr = threads > 0 ? threads : 1;
+#else
+    const int nthr = 1;
+#endif
+    if (nthr > 1 && p >= 4096) {
+        const int64_t chunk = (p + nthr - 1) / nthr;
+        double *bufs = calloc((size_t)m * (size_t)nthr, sizeof(double));
+#pragma omp parallel for num_threads(nthr) schedule(static)
+        for (int w = 0; w < nthr; w++) {
+            double *d = bufs + (size_t)w * (size_t)m;
+            const int64_t a = (int64_t)w * chunk, b = a + chunk < p ? a + chunk : p;
+            for (int64_t s = a; s < b; s++) {
+                const double pos = (ev[s] - lo) / dx;
+                const int64_t i0 = (int64_t)floor(pos);
+                double wts[8];
+                nufft_lagrange(N, pos - (double)i0, wts);
+                for (int j = 0; j < N; j++) {
+                    int64_t cell = i0 + off0 + j;
+                    cell = cell < 0 ? 0 : (cell > m - 1 ? m - 1 : cell);
+                    d[cell] += wts[j];
+                }
+            }
+        }
+        /* reduce in worker order (the reference's reduce order is Rayon's) */
+        for (int w = 0; w < nthr; w++)
+            for (int64_t i = 0; i < m; i++) density[i] += bufs[(size_t)w * (size_t)m + i];
+        free(bufs);
+    } else {
+        for (int64_t s = 0; s < p; s++) {
+            const double pos = (ev[s] - lo) / dx;
+            const int64_t i0 = (int64_t)floor(pos);
+            double wts[8];
+            nufft_lagrange(N, pos - (double)i0, wts);
+            for (int j = 0; j < N; j++) {
+                int64_t cell = i0 + off0 + j;
+                cell = cell < 0 ? 0 : (cell > m - 1 ? m - 1 : cell);
+                density[cell] += wts[j];
+            }
+        }
+    }
+    double complex *packed = malloc((size_t)m * sizeof(double complex));
+    for (int64_t i = 0; i < m; i++) {
+        const double sd = i <= half ? (double)i : (double)i - (double)m;
+        const double x = sd * dx, den = x * x + eta * eta;
+        packed[i] = density[i] + I * (x / den);
+    }
+    free(density);
+    nufft_fft(packed, m, -1);
+    /* even kernel spectrum by recurrence (fft5.rs:383-405) */
+    double *ke = malloc((size_t)m * sizeof(double));
+    const double r = exp(-2.0 * M_PI * eta / ((double)m * dx)), ke0 = M_PI / dx;
+    double pw = 1.0;
+    for (int64_t k = 0; k <= half; k++) {
+        ke[k] = ke0 * pw;
+        pw *= r;
+    }
+    pw = r;
+    for (int64_t k = m - 1; k > half; k--) {
+        ke[k] = ke0 * pw;
+        pw *= r;
+    }
+    double complex *out = malloc((size_t)m * sizeof(double complex));
+    for (int64_t k = 0; k < m; k++) {
+        const double complex ck = packed[k], cnk = conj(packed[(m - k) % m]);
+        const double complex d = 0.5 * (ck + cnk);
+        const double complex ko = -0.5 * I * (ck - cnk);
+        const double complex im_hat = d * ke[k], re_hat = d * ko;
+        out[k] = (creal(im_hat) - cimag(re_hat)) + I * (cimag(im_hat) + creal(re_hat));
+    }
+    free(ke);
+    free(packed);
+    nufft_fft(out, m, +1);
+    const double inv_m = 1.0 / (double)m;
+#pragma omp parallel for num_threads(nthr) schedule(static) if (nthr > 1 && nq >= 4096)
+    for (int64_t k = 0; k < nq; k++) {
+        const double pos = (q[k] - lo) / dx;
+        const int64_t i0 = (int64_t)floor(pos);
+        double wts[8];
+        nufft_lagrange(N, pos - (double)i0, wts);
+        double re_acc = 0.0, im_acc = 0.0;
+        for (int j = 0; j < N; j++) {
+            int64_t cell = i0 + off0 + j;
+            cell = cell < 0 ? 0 : (cell > m - 1 ? m - 1 : cell);
+            im_acc += creal(out[cell]) * wts[j];
+            re_acc += cimag(out[cell]) * wts[j];
+        }
+        out_re[k] = re_acc * inv_m;
+        out_im[k] = im_acc * inv_m;
+    }
+    free(out);
 }

@@ -1352,6 +1352,26 @@ shk_status shk_chebcode_evaluate_many(const shk_chebcode *h, const double *etas,
     return SHK_OK;
 }
 
+shk_status shk_chebcode_evaluate_many_dev(const shk_chebcode *h, const double *etas,
+                                          size_t n_eta, double *d_out_re, double *d_out_im,
+                                          void *stream) {
+    if (!h || !h->tree) return fail(SHK_ERR_BAD_ARG, "NULL handle");
+    if (n_eta == 0) return SHK_OK;
+    if (!etas || !d_out_re || !d_out_im) return fail(SHK_ERR_BAD_ARG, "NULL pointer");
+    for (size_t k = 0; k < n_eta; ++k) SHK_CHECK(check_eta(etas[k]));
+    const size_t p = cheb_p(h->tree);
+    hipStream_t s = (hipStream_t)stream;
+    // two rows per traversal (chebcode.rs:1312-1361); an odd last eta is
+    // paired with itself and both lanes write the same row
+    for (size_t k = 0; k < n_eta; k += 2) {
+        const bool pair = k + 1 < n_eta;
+        double *r0 = d_out_re + k * p, *i0 = d_out_im + k * p;
+        double *r1 = pair ? r0 + p : r0, *i1 = pair ? i0 + p : i0;
+        SHK_CHECK(cheb_eval_x2(h->tree, etas[k], pair ? etas[k + 1] : etas[k], r0, i0, r1, i1, s));
+    }
+    return SHK_OK;
+}
+
 shk_status shk_chebcode_build_dev(const double *d_eigenvalues, size_t p, double theta, int n,
                                   size_t leaf_cap, int f32_far, void *stream, shk_chebcode **out) {
     if (!out) return fail(SHK_ERR_BAD_ARG, "NULL handle pointer");

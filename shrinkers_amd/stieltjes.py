@@ -342,6 +342,20 @@ class ChebCodeBatch:
                                                     _stream_handle(stream)))
         return out
 
+    def evaluate_many_dev(self, etas, out=None, stream=None):
+        """evaluate_many into CUDA float64 tensors of shape (len(etas), p),
+        enqueued on ``stream``; two etas per traversal.  Returns ``(re, im)``."""
+        import torch
+        e = np.ascontiguousarray(etas, dtype=np.float64)
+        if out is None:
+            out = (torch.empty((e.size, self.p), dtype=torch.float64, device="cuda"),
+                   torch.empty((e.size, self.p), dtype=torch.float64, device="cuda"))
+        if e.size:
+            check(_lib.load().shk_chebcode_evaluate_many_dev(
+                self._h, _lib.dptr(e), e.size, out[0].data_ptr(), out[1].data_ptr(),
+                _stream_handle(stream)))
+        return out
+
     def count_work(self, eta: float, query=None, stream=None):
         """``(near_pairs, far_terms)`` of one traversal pass: the summed sizes
         of the leaves each query visits and n per accepted panel, with the

@@ -365,3 +365,23 @@ def test_build_queued_behind_long_work(shk):
             np.testing.assert_array_equal(g.cpu().numpy(), w)
         for g, w in zip(got_big, want_big):
             np.testing.assert_array_equal(g.cpu().numpy(), w)
+
+
+@pytest.mark.parametrize("preset", ["chebcode", "chebcode_fast"])
+@pytest.mark.parametrize("n_eta", [3, 4])
+def test_evaluate_many_dev_equals_host(shk, preset, n_eta):
+    """shk_chebcode_evaluate_many_dev (one tree, many eta, device outputs) is
+    bit for bit the host-array evaluate_many (chebcode.rs:1312-1361), an odd
+    count included, at p = 2e5 with an unsorted input."""
+    import torch
+    from shrinkers_amd import spectra as S
+    p = 200_000
+    ev = np.random.default_rng(2).permutation(S.mp_inverse_cdf_spectrum(p, c=0.5))
+    eta = 1.0 / np.sqrt(p)
+    etas = [eta * f for f in (0.5, 1.0, 2.0, 4.0)[:n_eta]]
+    b = shk.ChebCodeBatch.build_preset(ev, preset)
+    hr, hi = b.evaluate_many(etas)
+    dr, di = b.evaluate_many_dev(etas)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(dr.cpu().numpy(), hr)
+    np.testing.assert_array_equal(di.cpu().numpy(), hi)

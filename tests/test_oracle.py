@@ -311,3 +311,56 @@ def test_config1_spiked_wishart_generator():
     assert np.all(np.abs(ell / np.array([12.0, 7.0, 4.0]) - 1.0) < 0.05), ell
     lo, hi = (1 - np.sqrt(c)) ** 2, (1 + np.sqrt(c)) ** 2
     assert ev[0] > 0.9 * lo and ev[-4] < 1.1 * hi
+
+
+def test_nufft_grid_size_rule(orc):
+    """The coarse-grid NUFFT's sizing rule against the reference's own test
+    (nufft.rs:239-263): the cap for a wide spectrum at small eta, 32 768 at
+    range 2 / eta 0.01, the floor without padding, the cap (not an overflow)
+    at eta = 1e-300, and an explicit override."""
+    assert orc.nufft_grid_size(4.0, 1e-4) == 65536
+    assert orc.nufft_grid_size(2.0, 0.01) == 32768
+    assert orc.nufft_grid_size(0.1, 1.0, pad_eta_mult=0.0, pad_range_frac=0.0) == 8192
+    assert orc.nufft_grid_size(2.0, 1e-300) == 65536
+    assert orc.nufft_grid_size(4.0, 1e-4, m_override=12345) == 12345
+
+
+def test_nufft_reference_bounds(orc):
+    """nufft.rs:265-318: on the log spectrum at p = 8192, eta = 1/sqrt(p), the
+    NUFFT is within 1e-4 normwise of the exact sums at all points and on a
+    200-point grid, and the parallel splat agrees with the sequential one to
+    1e-9."""
+    p = 8192
+    ev = orc.log_spectrum(p)
+    eta = 1.0 / np.sqrt(p)
+    er, ei = orc.tiled_parallel(ev, eta)
+    r, i = orc.nufft_all(ev, eta)
+    assert rel_l2(r, i, er, ei) < 1e-4
+    rp, ip = orc.nufft_all(ev, eta, threads=4)
+    assert max(np.max(np.abs(r - rp)), np.max(np.abs(i - ip))) < 1e-9
+    grid = np.array([ev[0] + (ev[-1] - ev[0]) * k / 199.0 for k in range(200)])
+    gr, gi = orc.at_points(grid, ev, eta)
+    nr, ni = orc.nufft_at_points(ev, grid, eta)
+    assert rel_l2(nr, ni, gr, gi) < 1e-4
+
+
+def test_nufft_recorded_error_class(orc):
+    """The reference's measured NUFFT error against the exact sums
+    (docs/nufft_algorithms.md:141-147: rel-L2 2.1e-5 at p = 5e4, 2.7e-5 at
+    1e5, eta = 1/sqrt(p)) on its benchmark spectrum mp_spectrum(p, 0.25, 7)
+    (benches/support/mod.rs:42-54: uniform on the MP support plus a uniform
+    jitter of 0.1, drawn from rand's StdRng).  The error is the kernel's
+    periodization floor, a property of the distribution, so a spectrum of
+    the same distribution from numpy's generator reproduces it to 2 digits."""
+    for p, want in ((50_000, 2.1e-5), (100_000, 2.7e-5)):
+        c = 0.25
+        lo, hi = max(1 - np.sqrt(c), 0.01) ** 2, (1 + np.sqrt(c)) ** 2
+        rng = np.random.default_rng(7)
+        t = lo + rng.random(p) * (hi - lo)
+        ev = np.sort(t + rng.random(p) * 0.1)
+        eta = 1.0 / np.sqrt(p)
+        er, ei = orc.tiled_parallel(ev, eta)
+        r, i = orc.nufft_all(ev, eta, threads=4)
+        err = rel_l2(r, i, er, ei)
+        print(f"nufft p={p}: {err:.3e} (recorded {want:.1e})", flush=True)
+        assert abs(err / want - 1.0) < 0.05, (p, err)
