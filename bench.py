@@ -37,6 +37,12 @@ PEAK_FP64_TFLOPS = 78.6     # MI355X FP64 vector (published)
 # eigenvalues, once per XCD L2) and 76562.5 KiB written (the FP64 per-tile partials,
 # 2 x 98 tiles x 50 000 x 8 B).
 TRAFFIC_P50000 = (1681.0 * 2.000 + 76562.5 * 1.000) * 1024
+# Fraction of the kernel's time the vector ALUs issue (rocprofv3 derived counter VALUBusy
+# = SQ_ACTIVE_INST_VALU / (CU_NUM * GRBM_GUI_ACTIVE)) for sym_f32c_kernel at p = 50 000,
+# from `rocprofv3 --pmc VALUBusy VALUUtilization -- python3 bench.py --kernels`; its
+# instruction mix is 4.22 VALU instructions per ordered pair (SQ_INSTS_VALU).
+VALU_BUSY_P50000 = 0.977
+VALU_INSTS_PER_PAIR_P50000 = 4.22
 
 
 def log(msg: str) -> None:
@@ -484,6 +490,10 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU timing")
     ap.add_argument("--quick", action="store_true",
                     help="only the headline workload (no ChebCode / grid / reverse-mode legs)")
+    ap.add_argument("--kernels", action="store_true",
+                    help="only run each profiled workload three times (for rocprofv3 counter "
+                         "passes): the headline f32c and FP64 transforms at p, then the one-shot "
+                         "ChebCode FAST and DEFAULT transforms at 5e4 and 1e6; prints no JSON")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="first copy an MB-sized buffer on the device 5 times (a plain copy "
                          "kernel of known size, to calibrate HBM byte counters)")
@@ -522,6 +532,20 @@ def main() -> int:
 
     def step64():
         shk.all_points_dev(d_ev, eta, "blocked", precision="f64", scaled=True, out=out64)
+
+    if args.kernels:
+        for _ in range(3):
+            step32()
+            step64()
+        sync()
+        for pc in (50_000, 1_000_000):
+            evc = torch.from_numpy(spectra.mp_inverse_cdf_spectrum(pc, c=0.25)).to(dev)
+            for m in ("chebcode_fast", "chebcode"):
+                for _ in range(3):
+                    shk.all_points_dev(evc, 1.0 / np.sqrt(pc), m)
+            sync()
+        log("kernel workloads done")
+        return 0
 
     log(f"p={p} eta={eta:.6g}: timing FP32-compensated ({args.steps} steps)")
     t32 = time_steps(step32, args.steps, args.warmup, sync)
@@ -594,6 +618,8 @@ def main() -> int:
             "frac": achieved / PEAK_FP32_TFLOPS,
             "traffic": TRAFFIC_P50000 if p == 50_000 else None,
             "traffic_unit": "bytes per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+            "valu_busy": VALU_BUSY_P50000 if p == 50_000 else None,
+            "valu_insts_per_ordered_pair": VALU_INSTS_PER_PAIR_P50000 if p == 50_000 else None,
             "kernel": "sym_f32c_kernel" if sym_lo <= p <= sym_hi else "exact_f32c_kernel",
             "kernel_avg_ms": k32,
             "flops_per_launch": FLOPS_PER_PAIR * pairs,
