@@ -320,6 +320,29 @@ shk_status shk_chebcode_evaluate_many_dev(const shk_chebcode *h, const double *e
                                           size_t n_eta, double *d_out_re, double *d_out_im,
                                           void *stream);
 
+/* Evaluation at the tree's own eigenvalues (input order, times scale) with
+ * local expansions instead of the per-query panel sums of
+ * FlatChebTree::contribution_f64 (chebcode.rs:607-711): no counterpart in
+ * the reference, whose notes leave a Chebyshev M2L to future work
+ * (docs/internals.md:580-599).  Target nodes are the topmost nodes no wider
+ * than eta/4; each walks the tree once with the reference's acceptance test
+ * taken at its query nearest to the panel (so a panel is used only where the
+ * reference would use it for every query of the target), evaluates the
+ * accepted panels at its n Chebyshev points (M2L), and each query
+ * interpolates them (L2P) and adds the exact sums of the leaves the walk
+ * reached.  A leaf wider than eta/4 shares one such walk among its queries
+ * and sums the accepted panels at each query; a target whose walk outgrows
+ * its lists (64 leaves, 512 panels, a 512-node front) keeps the reference's
+ * per-query walk.  Trees with an f64 far field and n = 11 or 12
+ * (SHK_ERR_BAD_ARG otherwise).  Error: that of the reference traversal or
+ * smaller, plus the interpolation error ~ (8 + sqrt 65)^-n (5e-14 at
+ * n = 11).  It pays off at about a million eigenvalues; below that the
+ * latency of one target's walk bounds the launch and the per-query
+ * evaluation is as fast or faster.  Enqueued on `stream` without
+ * synchronising. */
+shk_status shk_chebcode_evaluate_local_dev(const shk_chebcode *h, double eta, double scale,
+                                           double *d_out_re, double *d_out_im, void *stream);
+
 /* Work of one traversal pass, with the evaluation kernels' node decisions:
  * *near_pairs = sum over queries of the sizes of the leaves each visits
  * (exact pair evaluations), *far_terms = n per accepted panel.  d_query =

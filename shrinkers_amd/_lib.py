@@ -74,6 +74,7 @@ SIGNATURES = {
     "shk_chebcode_build_dev": (_i, [_vp, _sz, _d, _i, _sz, _i, _vp, ctypes.POINTER(_vp)]),
     "shk_chebcode_evaluate_dev": (_i, [_vp, _d, _d, _vp, _vp, _vp]),
     "shk_chebcode_evaluate_many_dev": (_i, [_vp, _dp, _sz, _vp, _vp, _vp]),
+    "shk_chebcode_evaluate_local_dev": (_i, [_vp, _d, _d, _vp, _vp, _vp]),
     "shk_chebcode_count_work": (_i, [_vp, _vp, _sz, _d, _i64p, _i64p, _vp]),
     "shk_batch_create": (_i, [_vp, _i64p, _sz, _dp, _i, ctypes.POINTER(_vp)]),
     "shk_batch_run": (_i, [_vp, _vp, _vp, _vp]),

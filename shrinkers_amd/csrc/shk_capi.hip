@@ -1352,6 +1352,14 @@ shk_status shk_chebcode_evaluate_many(const shk_chebcode *h, const double *etas,
     return SHK_OK;
 }
 
+shk_status shk_chebcode_evaluate_local_dev(const shk_chebcode *h, double eta, double scale,
+                                           double *d_out_re, double *d_out_im, void *stream) {
+    if (!h || !h->tree || !d_out_re || !d_out_im) return fail(SHK_ERR_BAD_ARG, "NULL pointer");
+    SHK_CHECK(check_eta(eta));
+    return (shk_status)cheb_eval_local(h->tree, eta, scale, d_out_re, d_out_im,
+                                       (hipStream_t)stream);
+}
+
 shk_status shk_chebcode_evaluate_many_dev(const shk_chebcode *h, const double *etas,
                                           size_t n_eta, double *d_out_re, double *d_out_im,
                                           void *stream) {

@@ -2064,6 +2064,383 @@ __global__ __launch_bounds__(256) void eval_x2_kernel(EvalView v, int nq, double
     im1[o] = ai[1];
 }
 
+// ---- local expansions (M2L / L2P) at the tree's own points ------------------
+//
+// No counterpart in the reference: its far field is the per-query panel sum
+// (M2P, chebcode.rs:607-711), and its notes leave a Chebyshev M2L to future
+// work (docs/internals.md:580-599).  Targets are the topmost nodes no wider
+// than kLocalWidth * eta, plus the leaves wider than that (which keep the
+// per-query walk).  One wavefront per target T walks the tree once, with the
+// reference's acceptance test (accept_f64_of) at T's gap to the panel, i.e.
+// at the query of T nearest to it: a panel accepted for T is accepted by the
+// reference for every query of T, so the M2P part is never less accurate.
+// Accepted panels are evaluated at T's n Chebyshev points y_k (M2L: n^2
+// terms per panel, shared by all of T's queries, instead of n per query);
+// leaves the walk reaches are listed and summed exactly per query, as the
+// reference sums every leaf it reaches.  Each query then interpolates T's
+// far field with the barycentric formula on the Chebyshev points of the
+// second kind (L2P, n terms) and adds the listed leaves' exact sums.  The far
+// field sum_j w_j / (x - t_j - i eta) is analytic within eta of the real
+// axis, so on an interval of width <= eta/4 its degree n-1 interpolant errs
+// by about rho^-n, rho = 8 + sqrt(65) (5e-14 at n = 11).  A target whose
+// near list overflows falls back to the per-query walk.
+constexpr double kLocalWidth = 0.25;
+constexpr int kLocalNear = 64;
+
+// The reference traversal for one query (the per-lane walk of
+// eval_f64_kernel<false>, same arithmetic and order).
+__device__ __forceinline__ void walk_one_f64(const EvalView &v, double x, double eta,
+                                             double eta2, double &re, double &im) {
+    const int n = v.n;
+    re = im = 0.0;
+    int node = 0;
+    while (node < v.N) {
+        const int4 lk = v.link[node];
+        if (lk.x < 0) {
+            double lr, li;
+            leaf_f64(v.sorted + lk.z, lk.w - lk.z, x, eta2, lr, li);
+            re += lr;
+            im += eta * li;
+            node = lk.y;
+            continue;
+        }
+        if (accept_f64(v, node, x, eta2)) {
+            const double *ts = v.nodes + (size_t)node * n;
+            const double *ws = v.w + (size_t)node * n;
+            double afr0 = 0.0, afr1 = 0.0, afi0 = 0.0, afi1 = 0.0;
+            int j = 0;
+            for (; j + 2 <= n; j += 2) {
+                const double d0 = x - ts[j], d1 = x - ts[j + 1];
+                const double q0 = ws[j] * rcp_f64(__builtin_fma(d0, d0, eta2));
+                const double q1 = ws[j + 1] * rcp_f64(__builtin_fma(d1, d1, eta2));
+                afr0 = __builtin_fma(q0, d0, afr0);
+                afr1 = __builtin_fma(q1, d1, afr1);
+                afi0 += q0;
+                afi1 += q1;
+            }
+            double fr = afr0 + afr1, fi = afi0 + afi1;
+            if (j < n) {
+                const double dj = x - ts[j];
+                const double q = ws[j] * rcp_f64(dj * dj + eta2);
+                fr += q * dj;
+                fi += q;
+            }
+            re += fr;
+            im += eta * fi;
+            node = lk.y;
+        } else {
+            node = node + 1;
+        }
+    }
+}
+
+__global__ void local_zero_kernel(int *count) {
+    if (threadIdx.x < 3 && blockIdx.x == 0) count[threadIdx.x] = 0;
+}
+
+// Target list: node T (code T) is a target when it is no wider than wmax and
+// its parent is wider (or T is the root); a leaf wider than wmax under such
+// parents is a fallback target (code ~T).  Every source lies in exactly one
+// target.  The order of the list does not matter: targets are independent.
+// Appends this lane's n entries (codes c0, c1) to a list with one atomic per
+// wavefront.
+__device__ __forceinline__ void wave_append(int n, int c0, int c1, int *__restrict__ list,
+                                            int *__restrict__ counter, int lane) {
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint64_t m1 = __ballot(n >= 1), m2 = __ballot(n >= 2);
+    const int tot = __popcll(m1) + __popcll(m2);
+    int base = 0;
+    if (tot > 0) {
+        if (lane == 0) base = atomicAdd(counter, tot);
+        base = __shfl(base, 0, 64);
+    }
+    const int at = base + __popcll(m1 & below) + __popcll(m2 & below);
+    if (n >= 1) list[at] = c0;
+    if (n >= 2) list[at + 1] = c1;
+}
+
+// Target lists: node T is a target (list, count[0]) when it is no wider than
+// wmax and its parent is wider (or T is the root); a leaf wider than wmax
+// under such parents is a wide target (wide_list, count[2]).  Every source
+// lies in exactly one target.  List order does not matter: targets are
+// independent.
+__global__ __launch_bounds__(256) void local_plan_kernel(EvalView v, double wmax,
+                                                        int *__restrict__ list,
+                                                        int *__restrict__ wide_list,
+                                                        int *__restrict__ count) {
+    const int node = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    int t[2] = {0, 0}, wd[2] = {0, 0}, nt = 0, nw = 0;
+    if (node < v.N) {
+        const int4 lk = v.link[node];
+        const double w = v.hi[node] - v.lo[node];
+        if (node == 0) {
+            if (w <= wmax) t[nt++] = 0;
+            else if (lk.x < 0) wd[nw++] = 0;
+        }
+        if (lk.x >= 0 && w > wmax) {
+            const int kids[2] = {node + 1, lk.x};
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const int k = kids[c];
+                if (v.hi[k] - v.lo[k] <= wmax) t[nt++] = k;
+                else if (v.link[k].x < 0) wd[nw++] = k;
+            }
+        }
+    }
+    wave_append(nt, t[0], t[1], list, count, lane);
+    wave_append(nw, wd[0], wd[1], wide_list, count + 2, lane);
+}
+
+// One wavefront per target (grid-stride over the list).  A leaf wider than
+// the target width sums the walk's accepted panels at each of its queries
+// instead (M2P; it has at most leaf_cap queries).  The walk is
+// breadth-first: each level's candidate nodes are spread over the lanes,
+// and ballots compact the outcomes -- accepted panels, reached leaves, the
+// children of rejected nodes (the next level) -- into LDS lists, so a target
+// costs one round of loads per tree level instead of one per visit.  A
+// target whose lists overflow falls back to the per-query walk.  Then lane l
+// evaluates the M2L terms u = l + 64 s (s < kS) of every accepted panel:
+// target point k = u / NT, panel point j = u % NT; the per-term sums are
+// added per k in j order.
+constexpr int kLocalFront = 512;
+constexpr int kLocalPanels = 512;
+
+template <int NT>
+__global__ __launch_bounds__(64) void local_f64_kernel(EvalView v, const int *__restrict__ list,
+                                                       const int *__restrict__ wide_list,
+                                                       int *__restrict__ count,
+                                                       int *__restrict__ fb_list, double eta,
+                                                       double scale, double *__restrict__ out_re,
+                                                       double *__restrict__ out_im) {
+    constexpr int kTerms = NT * NT;
+    constexpr int kS = (kTerms + 63) / 64;
+    __shared__ double s_fr[64 * kS], s_fi[64 * kS];
+    __shared__ double s_cr[NT], s_ci[NT], s_y[NT];
+    __shared__ int2 s_near[kLocalNear];
+    __shared__ int s_front[2][kLocalFront];
+    __shared__ int s_acc[kLocalPanels];
+    const int lane = threadIdx.x;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    // wide leaves first: each walks as long as a narrow target but is rarer,
+    // so starting them early keeps them off the tail of the launch
+    const int n_wide = count[2];
+    const int total = count[0] + n_wide;
+    const double eta2 = eta * eta;
+    for (int e = blockIdx.x; e < total; e += gridDim.x) {
+        const bool wide = e < n_wide;
+        const int T = wide ? wide_list[e] : list[e - n_wide];
+        const int4 lkT = v.link[T];
+        const int q0 = lkT.z, q1 = lkT.w;
+        bool local_ok = true;
+        int n_near = 0, n_acc = 0;
+        {
+            const double tlo = v.lo[T], thi = v.hi[T];
+            // breadth-first walk
+            if (lane == 0) s_front[0][0] = 0;
+            __syncthreads();
+            int fs = 1, cur = 0;
+            while (fs > 0 && local_ok) {
+                int nfs = 0;
+                for (int base = 0; base < fs; base += 64) {
+                    const int idx = base + lane;
+                    const bool valid = idx < fs;
+                    const int node = valid ? s_front[cur][idx] : 0;
+                    const int4 lk = v.link[node];
+                    const double plo = v.lo[node], phi = v.hi[node], pacc = v.acc[node];
+                    const bool leaf = valid && lk.x < 0;
+                    const bool acc =
+                        valid && !leaf &&
+                        accept_f64_of(plo, phi, pacc, fmin(fmax(0.5 * (plo + phi), tlo), thi), eta2);
+                    const bool desc = valid && !leaf && !acc;
+                    const uint64_t ml = __ballot(leaf), ma = __ballot(acc), md = __ballot(desc);
+                    if (leaf) {
+                        const int at = n_near + __popcll(ml & below);
+                        if (at < kLocalNear) s_near[at] = make_int2(lk.z, lk.w);
+                    }
+                    if (acc) {
+                        const int at = n_acc + __popcll(ma & below);
+                        if (at < kLocalPanels) s_acc[at] = node;
+                    }
+                    if (desc) {
+                        const int at = nfs + 2 * __popcll(md & below);
+                        if (at + 1 < kLocalFront) {
+                            s_front[cur ^ 1][at] = node + 1;
+                            s_front[cur ^ 1][at + 1] = lk.x;
+                        }
+                    }
+                    n_near += __popcll(ml);
+                    n_acc += __popcll(ma);
+                    nfs += 2 * __popcll(md);
+                }
+                if (n_near > kLocalNear || n_acc > kLocalPanels || nfs > kLocalFront)
+                    local_ok = false;
+                __syncthreads();
+                cur ^= 1;
+                fs = nfs;
+            }
+        }
+        if (wide) {
+          // a leaf wider than the target width: no local expansion
+          if (local_ok) {
+            // the accepted panels at each query (M2P), the walk shared
+            for (int c0 = q0; c0 < q1; c0 += 64) {
+                const int i = c0 + lane;
+                if (i >= q1) continue;
+                const double x = v.sorted[i];
+                double fr = 0.0, fi = 0.0;
+                for (int a = 0; a < n_acc; ++a) {
+                    const double *ts = v.nodes + (size_t)s_acc[a] * NT;
+                    const double *ws = v.w + (size_t)s_acc[a] * NT;
+#pragma unroll
+                    for (int j = 0; j < NT; ++j) {
+                        const double d = x - ts[j];
+                        const double q = ws[j] * rcp_f64(__builtin_fma(d, d, eta2));
+                        fr = __builtin_fma(q, d, fr);
+                        fi += q;
+                    }
+                }
+                for (int m = 0; m < n_near; ++m) {
+                    const int2 lf = s_near[m];
+                    for (int j = lf.x; j < lf.y; ++j) {
+                        const double d = x - v.sorted[j];
+                        const double inv = rcp_f64(__builtin_fma(d, d, eta2));
+                        fr = __builtin_fma(d, inv, fr);
+                        fi += inv;
+                    }
+                }
+                const int o = v.perm ? v.perm[i] : i;
+                out_re[o] = fr * scale;
+                out_im[o] = (eta * fi) * scale;
+            }
+          }
+        } else if (local_ok) {
+            // M2L over the accepted panels
+            double yk[kS], ar[kS], ai[kS];
+            int jk[kS];
+#pragma unroll
+            for (int s = 0; s < kS; ++s) {
+                const int u = lane + 64 * s;
+                const int k = (u < kTerms) ? u / NT : 0;
+                jk[s] = (u < kTerms) ? u % NT : 0;
+                yk[s] = v.nodes[(size_t)T * NT + k];
+                ar[s] = ai[s] = 0.0;
+            }
+            // four panels per step, their loads issued together
+            for (int a0 = 0; a0 < n_acc; a0 += 4) {
+                double tj[4][kS], wj[4][kS];
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const int node = s_acc[min(a0 + b, n_acc - 1)];
+#pragma unroll
+                    for (int s = 0; s < kS; ++s) {
+                        tj[b][s] = v.nodes[(size_t)node * NT + jk[s]];
+                        wj[b][s] = v.w[(size_t)node * NT + jk[s]];
+                    }
+                }
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    if (a0 + b >= n_acc) break;
+#pragma unroll
+                    for (int s = 0; s < kS; ++s) {
+                        if (lane + 64 * s < kTerms) {
+                            const double d = yk[s] - tj[b][s];
+                            const double q = wj[b][s] * rcp_f64(__builtin_fma(d, d, eta2));
+                            ar[s] = __builtin_fma(q, d, ar[s]);
+                            ai[s] += q;
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < kS; ++s) {
+                s_fr[lane + 64 * s] = ar[s];
+                s_fi[lane + 64 * s] = ai[s];
+            }
+            if (lane < NT) s_y[lane] = v.nodes[(size_t)T * NT + lane];
+            __syncthreads();
+            if (lane < NT) {
+                double cr = 0.0, ci = 0.0;
+#pragma unroll 1
+                for (int j = 0; j < NT; ++j) {
+                    cr += s_fr[lane * NT + j];
+                    ci += s_fi[lane * NT + j];
+                }
+                s_cr[lane] = cr;
+                s_ci[lane] = ci;
+            }
+            __syncthreads();
+            for (int c0 = q0; c0 < q1; c0 += 64) {
+                const int i = c0 + lane;
+                if (i >= q1) continue;
+                const double x = v.sorted[i];
+                // L2P: barycentric weights (-1)^k, halved at both ends
+                double fr = 0.0, fi = 0.0;
+                int hit = -1;
+                double num_r = 0.0, num_i = 0.0, den = 0.0;
+#pragma unroll 2
+                for (int k = 0; k < NT; ++k) {
+                    const double dx = x - s_y[k];
+                    if (dx == 0.0 && hit < 0) hit = k;
+                    const double lam =
+                        ((k & 1) ? -1.0 : 1.0) * ((k == 0 || k == NT - 1) ? 0.5 : 1.0);
+                    const double r = lam * rcp_f64(dx);
+                    num_r = __builtin_fma(r, s_cr[k], num_r);
+                    num_i = __builtin_fma(r, s_ci[k], num_i);
+                    den += r;
+                }
+                if (hit >= 0) {
+                    fr = s_cr[hit];
+                    fi = s_ci[hit];
+                } else {
+                    const double id = rcp_f64(den);
+                    fr = num_r * id;
+                    fi = num_i * id;
+                }
+                // the reached leaves, exactly
+                double nr = 0.0, ni = 0.0;
+                for (int m = 0; m < n_near; ++m) {
+                    const int2 lf = s_near[m];
+                    for (int j = lf.x; j < lf.y; ++j) {
+                        const double d = x - v.sorted[j];
+                        const double inv = rcp_f64(__builtin_fma(d, d, eta2));
+                        nr = __builtin_fma(d, inv, nr);
+                        ni += inv;
+                    }
+                }
+                const int o = v.perm ? v.perm[i] : i;
+                out_re[o] = (fr + nr) * scale;
+                out_im[o] = (eta * (fi + ni)) * scale;
+            }
+        }
+        __syncthreads();  // LDS is reused by the next target
+        if (!local_ok && lane == 0) fb_list[atomicAdd(count + 1, 1)] = T;
+    }
+}
+
+// The targets that keep the reference's per-query walk (leaves wider than
+// the target width, targets whose near list overflowed): one wavefront per
+// target, one query per lane.
+__global__ __launch_bounds__(64) void local_fallback_kernel(EvalView v,
+                                                            const int *__restrict__ fb_list,
+                                                            const int *__restrict__ count,
+                                                            double eta, double scale,
+                                                            double *__restrict__ out_re,
+                                                            double *__restrict__ out_im) {
+    const int total = count[1];
+    const double eta2 = eta * eta;
+    for (int e = blockIdx.x; e < total; e += gridDim.x) {
+        const int4 lkT = v.link[fb_list[e]];
+        for (int i = lkT.z + (int)threadIdx.x; i < lkT.w; i += 64) {
+            double re, im;
+            walk_one_f64(v, v.sorted[i], eta, eta2, re, im);
+            const int o = v.perm ? v.perm[i] : i;
+            out_re[o] = re * scale;
+            out_im[o] = im * scale;
+        }
+    }
+}
+
 EvalView view_of(const ChebTree *t) {
     EvalView v;
     v.sorted = t->sorted;
@@ -2566,6 +2943,44 @@ int cheb_count(const ChebTree *t, const double *d_query, size_t nq, double eta,
     }
     *near_pairs = (int64_t)h[0];
     *far_terms = (int64_t)h[1];
+    return 0;
+}
+
+int cheb_eval_local(const ChebTree *t, double eta, double scale, double *d_out_re,
+                    double *d_out_im, hipStream_t stream) {
+    if (t->f32_far) {
+        set_error("local-expansion evaluation needs a tree with an f64 far field");
+        return 6;
+    }
+    if (t->n != 11 && t->n != 12) {
+        set_error("local-expansion evaluation is built for orders 11 and 12");
+        return 6;
+    }
+    if (t->ready) SHK_HIP(hipStreamWaitEvent(stream, t->ready, 0));
+    EvalView v = view_of(t);
+    const int N = (int)t->N;
+    // counts (targets, fallbacks, wide leaves) and the three lists
+    int *buf = nullptr;
+    CachedBuf cache(CACHE_CHEB_LOCAL, stream);
+    if (int e = cache.get(3 * (size_t)N + 4, &buf)) return e;
+    int *count = buf, *list = buf + 4, *fb_list = list + N, *wide_list = fb_list + N;
+    hipLaunchKernelGGL(local_zero_kernel, dim3(1), dim3(64), 0, stream, count);
+    hipLaunchKernelGGL(local_plan_kernel, dim3(blocks_for((size_t)N, 256)), dim3(256), 0, stream,
+                       v, kLocalWidth * eta, list, wide_list, count);
+    SHK_HIP(hipGetLastError());
+    const unsigned grid = (unsigned)std::min(N, 32768);
+    ProfileMark m = profile_begin(stream);
+    if (t->n == 11)
+        hipLaunchKernelGGL(local_f64_kernel<11>, dim3(grid), dim3(64), 0, stream, v, list,
+                           wide_list, count, fb_list, eta, scale, d_out_re, d_out_im);
+    else
+        hipLaunchKernelGGL(local_f64_kernel<12>, dim3(grid), dim3(64), 0, stream, v, list,
+                           wide_list, count, fb_list, eta, scale, d_out_re, d_out_im);
+    profile_end(m, stream);
+    SHK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(local_fallback_kernel, dim3((unsigned)std::min(N, 8192)), dim3(64), 0,
+                       stream, v, fb_list, count, eta, scale, d_out_re, d_out_im);
+    SHK_HIP(hipGetLastError());
     return 0;
 }
 

@@ -33,7 +33,8 @@ enum CacheSlot : int {
     CACHE_EK_SCR = 5,        // El Karoui fit kernels: per-point scratch
     CACHE_EK_STEP = 6,       // fused fit step: sample transform and its gradient
     CACHE_ADJ_TMP = 7,       // all-points adjoint by two sweeps: the query gradient
-    CACHE_HOST_0 = 8,        // host-array calls: the k-th device copy of a call
+    CACHE_CHEB_LOCAL = 8,    // ChebCode local-expansion evaluation: target list
+    CACHE_HOST_0 = 9,        // host-array calls: the k-th device copy of a call
     kCacheHostSlots = 6,     //   takes slot CACHE_HOST_0 + k
     kCacheSlots = CACHE_HOST_0 + kCacheHostSlots
 };
@@ -350,6 +351,12 @@ int cheb_eval(const ChebTree *t, const double *d_query, size_t nq, double eta, d
 // near-field pair evaluations and far-field panel terms.  Synchronises.
 int cheb_count(const ChebTree *t, const double *d_query, size_t nq, double eta,
                int64_t *near_pairs, int64_t *far_terms, hipStream_t stream);
+// Local-expansion evaluation at the tree's own points (input order, times
+// scale): M2L into the Chebyshev points of target nodes no wider than eta/4,
+// L2P per query, exact sums over the leaves the target walks reach; f64 far
+// field, n = 11 or 12.
+int cheb_eval_local(const ChebTree *t, double eta, double scale, double *d_out_re,
+                    double *d_out_im, hipStream_t stream);
 // Two-eta traversal for evaluate_many: acceptance at min(eta0, eta1).
 int cheb_eval_x2(const ChebTree *t, double eta0, double eta1, double *d_re0, double *d_im0,
                  double *d_re1, double *d_im1, hipStream_t stream);

@@ -342,6 +342,20 @@ class ChebCodeBatch:
                                                     _stream_handle(stream)))
         return out
 
+    def evaluate_local_dev(self, eta: float, scale: float = 1.0, out=None, stream=None):
+        """Evaluate at the tree's own eigenvalues (build order) with local
+        expansions: M2L into the Chebyshev points of target nodes no wider than
+        eta/4 and L2P per query, instead of the per-query panel sums.  f64
+        trees of order 11 or 12.  Returns CUDA float64 ``(re, im)``."""
+        import torch
+        if out is None:
+            out = (torch.empty(self.p, dtype=torch.float64, device="cuda"),
+                   torch.empty(self.p, dtype=torch.float64, device="cuda"))
+        check(_lib.load().shk_chebcode_evaluate_local_dev(self._h, float(eta), float(scale),
+                                                          out[0].data_ptr(), out[1].data_ptr(),
+                                                          _stream_handle(stream)))
+        return out
+
     def evaluate_many_dev(self, etas, out=None, stream=None):
         """evaluate_many into CUDA float64 tensors of shape (len(etas), p),
         enqueued on ``stream``; two etas per traversal.  Returns ``(re, im)``."""

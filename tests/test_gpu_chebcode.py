@@ -385,3 +385,98 @@ def test_evaluate_many_dev_equals_host(shk, preset, n_eta):
     torch.cuda.synchronize()
     np.testing.assert_array_equal(dr.cpu().numpy(), hr)
     np.testing.assert_array_equal(di.cpu().numpy(), hi)
+
+
+def _local_vs_reference(shk, ev, eta, theta, n, leaf):
+    """(error of the local-expansion evaluation, error of the reference
+    traversal) against the GPU FP64 exact sums, and their difference."""
+    import torch
+    d = torch.from_numpy(np.ascontiguousarray(ev)).cuda()
+    er, ei = shk.all_points_dev(d, eta, "blocked", precision="f64", scaled=False)
+    b = shk.ChebCodeBatch.build_dev(d, theta, n, leaf, False)
+    rr, ri = b.evaluate_dev(eta)
+    lr, li = b.evaluate_local_dev(eta)
+    torch.cuda.synchronize()
+    er, ei, rr, ri, lr, li = (t.cpu().numpy() for t in (er, ei, rr, ri, lr, li))
+    return rel_l2(lr, li, er, ei), rel_l2(rr, ri, er, ei), rel_l2(lr, li, rr, ri)
+
+
+@pytest.mark.parametrize("preset", ["chebcode", "chebcode_n12", "chebcode_xtreme",
+                                    "chebcode_balanced"])
+@pytest.mark.parametrize("p,c", [(50_000, 0.25), (200_000, 0.5), (1_000_000, 0.1)])
+def test_local_expansion_within_reference_error(shk, preset, p, c):
+    """Local expansions (M2L into target nodes no wider than eta/4, L2P per
+    query) against the GPU FP64 exact sums: no worse than the reference
+    traversal's own error (its panels are accepted only where the reference
+    accepts them for every query of a target, and the interpolation adds
+    ~5e-14), and close to the reference traversal itself."""
+    from shrinkers_amd import spectra as S
+    theta, n, leaf, _ = {**shk.CHEB_PRESETS, **shk.stieltjes.EXTRA_CHEB_PRESETS}[preset]
+    ev = S.mp_inverse_cdf_spectrum(p, c=c)
+    e_loc, e_ref, e_diff = _local_vs_reference(shk, ev, 1.0 / np.sqrt(p), theta, n, leaf)
+    print(f"local {preset} p={p} c={c}: {e_loc:.3e} (reference traversal {e_ref:.3e}, "
+          f"difference {e_diff:.3e})", flush=True)
+    assert e_loc <= 1.05 * e_ref + 1e-13
+    assert e_diff <= 2.0 * e_ref + 1e-13
+
+
+@pytest.mark.parametrize("case", ["unsorted", "small_eta", "duplicates", "tiny"])
+def test_local_expansion_edge_cases(shk, case):
+    """Unsorted input (scatter through the permutation); eta = 1e-3/sqrt(p),
+    where leaves are wider than eta/4 and most targets fall back to the
+    per-query walk; heavy duplicates (zero-width targets: the interpolant is
+    the constant at the node); p = 1, 2, 3 (the root is the target)."""
+    from shrinkers_amd import spectra as S
+    theta, n, leaf, _ = shk.CHEB_PRESETS["chebcode"]
+    if case == "tiny":
+        for ev in (np.array([1.5]), np.array([0.5, 0.9]), np.array([2.0, 0.1, 0.7])):
+            e_loc, e_ref, _ = _local_vs_reference(shk, ev, 0.3, theta, n, leaf)
+            assert e_loc <= 1e-13, ev
+        return
+    p = 30_000
+    if case == "unsorted":
+        ev, eta = np.random.default_rng(3).permutation(S.mp_inverse_cdf_spectrum(p, c=0.3)), 1 / np.sqrt(p)
+    elif case == "small_eta":
+        ev, eta = S.mp_inverse_cdf_spectrum(p, c=0.3), 1e-3 / np.sqrt(p)
+    else:
+        ev, eta = np.repeat(np.linspace(0.5, 2.0, p // 20), 20), 1 / np.sqrt(p)
+    e_loc, e_ref, e_diff = _local_vs_reference(shk, ev, eta, theta, n, leaf)
+    print(f"local {case}: {e_loc:.3e} (reference {e_ref:.3e}, difference {e_diff:.3e})", flush=True)
+    assert e_loc <= 1.05 * e_ref + 1e-13
+
+
+def test_local_expansion_refuses_fast_and_other_orders(shk):
+    """The local evaluation is built for f64 trees of order 11 or 12."""
+    import torch
+    from shrinkers_amd import _lib
+    d = torch.linspace(0.5, 2.0, 5000, dtype=torch.float64, device="cuda")
+    for theta, n, leaf, f32 in ((1.0, 8, 32, True), (0.5, 9, 32, False)):
+        b = shk.ChebCodeBatch.build_dev(d, theta, n, leaf, f32)
+        with pytest.raises(_lib.ShkError) as e:
+            b.evaluate_local_dev(0.01)
+        assert e.value.status == _lib.SHK_ERR_BAD_ARG
+
+
+@pytest.mark.parametrize("preset", ["chebcode", "chebcode_balanced"])
+def test_local_expansion_against_oracle(shk, orc, preset):
+    """The local-expansion evaluation against the oracle on the CPU: its error
+    against the oracle's exact sums is no worse than the oracle ChebCode
+    traversal's own error, on the harness spectrum and an MP spectrum."""
+    import torch
+    from shrinkers_amd import spectra as S
+    theta, n, leaf, _ = shk.CHEB_PRESETS[preset]
+    p = 10_000
+    eta = 1.0 / np.sqrt(p)
+    for ev in (S.harness_spectrum(p), S.mp_inverse_cdf_spectrum(p, c=0.3)):
+        er, ei = orc.exact_stieltjes(ev, eta)
+        cr, ci = orc.chebcode_preset(ev, eta, preset)
+        b = shk.ChebCodeBatch.build_dev(torch.from_numpy(np.ascontiguousarray(ev)).cuda(),
+                                        theta, n, leaf, False)
+        lr, li = b.evaluate_local_dev(eta)
+        torch.cuda.synchronize()
+        lr, li = lr.cpu().numpy(), li.cpu().numpy()
+        e_loc, e_orc = rel_l2(lr, li, er, ei), rel_l2(cr, ci, er, ei)
+        print(f"local {preset} p={p}: vs oracle exact {e_loc:.3e} "
+              f"(oracle traversal {e_orc:.3e})", flush=True)
+        assert e_loc <= 1.05 * e_orc + 1e-13
+        assert rel_l2(lr, li, cr, ci) <= 2.0 * e_orc + 1e-13
