@@ -23,7 +23,14 @@
  * stream of the current device and return after the results are back in
  * host memory.  Device-pointer entry points (suffix _dev) enqueue work on the
  * given stream and return without synchronising; the caller keeps the
- * buffers alive until the stream has passed the work.
+ * buffers alive until the stream has passed the work.  The exception is a
+ * ChebCode build (shk_chebcode_build_dev, and the ChebCode methods of
+ * shk_all_points_dev, shk_at_points_dev and shk_all_points_many_dev): the
+ * tree's node count sizes the rest of its build, so the call waits once for
+ * its topology kernel, and with it for the work queued before it on the
+ * stream; a build cannot be captured in a HIP graph.  Evaluations of a
+ * built tree (shk_chebcode_evaluate_dev, shk_chebcode_evaluate_many_dev) can
+ * be, once its build has finished (SHK_ERR_BAD_ARG if it has not).
  *
  * Thread safety: calls on distinct streams may run concurrently; scratch
  * memory is stream-ordered per call (pool allocations, or the calling

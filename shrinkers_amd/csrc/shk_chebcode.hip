@@ -2812,6 +2812,28 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
 size_t cheb_node_count(const ChebTree *t) { return t ? t->N : 0; }
 size_t cheb_p(const ChebTree *t) { return t ? t->p : 0; }
 
+// Orders `stream` after the tree's build.  A graph being captured cannot
+// wait on work outside it, so under capture the build must already have
+// finished, which the host checks on the build's event.
+int wait_tree(const ChebTree *t, hipStream_t stream) {
+    if (!t->ready) return 0;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    SHK_HIP(hipStreamIsCapturing(stream, &cs));
+    if (cs == hipStreamCaptureStatusNone) {
+        SHK_HIP(hipStreamWaitEvent(stream, t->ready, 0));
+        return 0;
+    }
+    const hipError_t q = hipEventQuery(t->ready);
+    if (q == hipSuccess) return 0;
+    if (q == hipErrorNotReady) {
+        set_error("chebcode: an evaluation is being captured before its tree's build has "
+                  "finished; synchronise the build's stream first");
+        return 6;
+    }
+    SHK_HIP(q);
+    return 0;
+}
+
 int cheb_params(const ChebTree *t, ChebParams *prm) {
     prm->theta = t->theta;
     prm->n = t->n;
@@ -2823,7 +2845,7 @@ int cheb_params(const ChebTree *t, ChebParams *prm) {
 int cheb_export(const ChebTree *t, double *lo, double *hi, int64_t *lo_idx, int64_t *hi_idx,
                 int32_t *right, double *nodes, double *w, hipStream_t stream) {
     const size_t N = t->N, n = t->n;
-    if (t->ready) SHK_HIP(hipStreamWaitEvent(stream, t->ready, 0));
+    if (int st = wait_tree(t, stream)) return st;
     if (lo) SHK_HIP(hipMemcpyAsync(lo, t->lo, sizeof(double) * N, hipMemcpyDeviceToHost, stream));
     if (hi) SHK_HIP(hipMemcpyAsync(hi, t->hi, sizeof(double) * N, hipMemcpyDeviceToHost, stream));
     if (right)
@@ -2858,7 +2880,7 @@ int cheb_eval(const ChebTree *t, const double *d_query, size_t nq, double eta, d
         set_error("chebcode eval: too many query points");
         return 6;
     }
-    if (t->ready) SHK_HIP(hipStreamWaitEvent(stream, t->ready, 0));
+    if (int st = wait_tree(t, stream)) return st;
     EvalView v = view_of(t);
     // Dense queries (the tree's own sources, or at least as many points as
     // sources) share near fields within a wavefront: walk together.
@@ -2919,7 +2941,7 @@ int cheb_count(const ChebTree *t, const double *d_query, size_t nq, double eta,
         set_error("chebcode count: too many query points");
         return 6;
     }
-    if (t->ready) SHK_HIP(hipStreamWaitEvent(stream, t->ready, 0));
+    if (int st = wait_tree(t, stream)) return st;
     unsigned long long *d_counts = nullptr;
     SHK_HIP(hipMallocAsync((void **)&d_counts, 2 * sizeof(unsigned long long), stream));
     EvalView v = view_of(t);
@@ -2956,7 +2978,7 @@ int cheb_eval_local(const ChebTree *t, double eta, double scale, double *d_out_r
         set_error("local-expansion evaluation is built for orders 11 and 12");
         return 6;
     }
-    if (t->ready) SHK_HIP(hipStreamWaitEvent(stream, t->ready, 0));
+    if (int st = wait_tree(t, stream)) return st;
     EvalView v = view_of(t);
     const int N = (int)t->N;
     // counts (targets, fallbacks, wide leaves) and the three lists
@@ -2986,7 +3008,7 @@ int cheb_eval_local(const ChebTree *t, double eta, double scale, double *d_out_r
 
 int cheb_eval_x2(const ChebTree *t, double eta0, double eta1, double *d_re0, double *d_im0,
                  double *d_re1, double *d_im1, hipStream_t stream) {
-    if (t->ready) SHK_HIP(hipStreamWaitEvent(stream, t->ready, 0));
+    if (int st = wait_tree(t, stream)) return st;
     EvalView v = view_of(t);
     const size_t nq = t->p;
     hipLaunchKernelGGL(eval_x2_kernel, dim3(blocks_for(nq, 256)), dim3(256), 0, stream, v, (int)nq,

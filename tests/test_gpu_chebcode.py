@@ -480,3 +480,39 @@ def test_local_expansion_against_oracle(shk, orc, preset):
               f"(oracle traversal {e_orc:.3e})", flush=True)
         assert e_loc <= 1.05 * e_orc + 1e-13
         assert rel_l2(lr, li, cr, ci) <= 2.0 * e_orc + 1e-13
+
+
+@pytest.mark.parametrize("preset", ["chebcode", "chebcode_fast"])
+def test_evaluations_of_a_built_tree_captured_in_a_graph(shk, preset):
+    """A built tree's evaluate_dev and evaluate_many_dev captured in a HIP
+    graph (torch.cuda.graph, global capture mode) once the build has
+    finished, and replayed three times with eager work and synchronisations
+    between the replays: every replay equals the eager evaluations bit for
+    bit."""
+    import torch
+    from shrinkers_amd import spectra as S
+    theta, n, leaf, f32 = shk.CHEB_PRESETS[preset]
+    p = 20_000
+    ev = torch.from_numpy(np.ascontiguousarray(S.mp_inverse_cdf_spectrum(p, c=0.3))).cuda()
+    b = shk.ChebCodeBatch.build_dev(ev, theta, n, leaf, bool(f32))
+    eta = 1.0 / np.sqrt(p)
+    etas = [eta, 2.0 * eta, 0.5 * eta]
+    er, ei = b.evaluate_dev(eta, 1.0 / p)
+    mr, mi = b.evaluate_many_dev(etas)
+    torch.cuda.synchronize()
+    out = (torch.empty_like(er), torch.empty_like(ei))
+    mout = (torch.empty_like(mr), torch.empty_like(mi))
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        b.evaluate_dev(eta, 1.0 / p, out=out, stream=s)
+        b.evaluate_many_dev(etas, out=mout, stream=s)
+    for rep in range(3):
+        for t in out + mout:
+            t.fill_(float("nan"))
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for got, want in zip(out + mout, (er, ei, mr, mi)):
+            assert torch.equal(got, want), (preset, rep)
