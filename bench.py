@@ -298,6 +298,18 @@ def chebcode_leg(shk, sync) -> dict:
                          "nodes": b.node_count, "near_pairs": near, "far_terms": far,
                          "eval_tflops": flops / (ms_eval * 1e-3) / 1e12,
                          "near_field_tflops_lower_bound": 7.0 * near / (ms_eval * 1e-3) / 1e12}
+            if not f32:
+                # local expansions (M2L into target nodes, L2P per query)
+                # instead of the per-query panel sums
+                lr, li = b.evaluate_local_dev(eta)
+                e0.record()
+                for _ in range(5):
+                    b.evaluate_local_dev(eta, out=(lr, li))
+                e1.record()
+                sync()
+                row[name]["ms_eval_local"] = e0.elapsed_time(e1) / 5
+                row[name]["rel_l2_local_vs_fp64_exact"] = float(np.sqrt(
+                    np.sum((lr.cpu().numpy() - er) ** 2 + (li.cpu().numpy() - ei) ** 2) / den))
             if name in ("chebcode_fast", "chebcode"):
                 # the one-shot transform (compute_all_stieltjes with the preset):
                 # build, evaluate and release in one call

@@ -99,6 +99,14 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Makes a wavefront's LDS writes visible to its own lanes' reads (for LDS
+// that only this wavefront uses at the time).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Workgroup exclusive scan of one int per thread (THREADS threads):
 // wavefront scans by shuffles, then the wavefront totals through LDS.
 // LDS_ONLY: the barriers order LDS only (callers with global stores in
@@ -274,7 +282,9 @@ __global__ void ingest_kernel(const double *__restrict__ ev, int p, int R,
 
 constexpr int kTopoK = 4;        // nodes per thread per pass (interleaved searches)
 constexpr int kSubThreads = 256; // workgroup of the subtree kernels
-constexpr int kSubCap = 1024;    // subtree level held in LDS (4 nodes per thread)
+constexpr int kSubCap = 512;     // subtree level held in LDS (2 nodes per thread)
+constexpr int kSubStaged = 5120; // sources of a subtree staged in LDS (its searches run there)
+constexpr int kSubStageMin = 256; // smaller subtrees (a level or two) search the global array
 constexpr int kFrontier = 512;   // top walk stops at a level this large
 constexpr int kMaxFrontier = 2 * kFrontier;  // the frontier level is smaller
 constexpr int kLv = kMaxDepth + 2;           // levels per subtree (root level 0)
@@ -580,8 +590,304 @@ __device__ int lds_level(const TopoCtx &c, int lb, int cnt, int depth, int alloc
     return added;
 }
 
+// Subtree levels whose sources sorted[base, base + staged) sit in LDS (src):
+// the partition point of x <= mid is searched in the staged values, in the
+// bracket a local bucket index gives (StagedIndex), and the children's end
+// values are read there.  The search is over sorted values, so the
+// partition point in the node's range [a, b) is the clamp into [a, b) of the
+// subtree's one, which is what lds_level and expand_level compute from the
+// global array (the hint range always holds the global partition point):
+// the splits are theirs.
+//
+// StagedIndex: bucket(v) = min(floor((v - x0) * scale), nb - 1), v >= x0,
+// is monotone in v, so with kb = bucket(mid) every staged value of a lower
+// bucket is < mid and every one of a higher bucket is > mid: the partition
+// point lies in [first[kb], first[kb + 1]], first[k] being the first local
+// index whose bucket is >= k (first[nb] = the staged count).
+constexpr int kSubBuckets = 2048;
+struct StagedIndex {
+    const int *first;
+    double x0, scale;
+};
+__device__ __forceinline__ int staged_bucket(double v, const StagedIndex &ix) {
+    const double t = (v - ix.x0) * ix.scale;
+    return (t >= (double)(kSubBuckets - 1)) ? kSubBuckets - 1 : (t > 0.0 ? (int)t : 0);
+}
+
+template <int K>
+__device__ __forceinline__ void staged_splits(const double *src, int base, const StagedIndex &ix,
+                                              const int (&a)[K], const int (&b)[K],
+                                              const bool (&need)[K], const double (&mid)[K],
+                                              int (&split)[K]) {
+    int l[K], r[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int kb = staged_bucket(mid[k], ix);
+        const int lo = ix.first[kb] + base, hi = ix.first[kb + 1] + base;
+        // the bracket [lo, hi] clamped into [a, b]: the search below then
+        // returns the clamp of the subtree's partition point
+        l[k] = need[k] ? min(max(lo, a[k]), b[k]) : 0;
+        r[k] = need[k] ? max(min(hi, b[k]), l[k]) : 0;
+    }
+    bool busy = false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) busy |= l[k] < r[k];
+    while (busy) {
+        int m[K];
+        double xm[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            m[k] = l[k] + (r[k] - l[k]) / 2;
+            xm[k] = src[(l[k] < r[k]) ? m[k] - base : 0];
+        }
+        busy = false;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const bool act = l[k] < r[k];
+            const bool le_mid = xm[k] <= mid[k];
+            l[k] = (act && le_mid) ? m[k] + 1 : l[k];
+            r[k] = (act && !le_mid) ? m[k] : r[k];
+            busy |= l[k] < r[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        split[k] = 0;
+        if (need[k]) {
+            const int u = l[k];  // in [a, b]
+            split[k] = (u == a[k] || u == b[k]) ? (a[k] + b[k]) / 2 : u;
+        }
+    }
+}
+
+// lds_level with the subtree's sources staged in LDS (no hint loads; the
+// children's end values come from the staged sources).
+template <int THREADS, int K>
+__device__ __forceinline__ int lds_level_staged(const TopoCtx &c, int lb, int cnt, int depth,
+                                                int alloc, int limit, int cap, const int *ca,
+                                                const int *cb, const double *cxa,
+                                                const double *cxb, int *na, int *nb, double *nxa,
+                                                double *nxb, int *s_wave, int *overflow,
+                                                bool *fits, const double *src, int base,
+                                                const StagedIndex &ix) {
+    const int tid = threadIdx.x;
+    int a[K], b[K], split[K];
+    bool need[K];
+    double xa[K], xb[K], mid[K];
+    int mine = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i = tid * K + k;
+        const bool node = i < cnt;
+        a[k] = node ? ca[i] : 0;
+        b[k] = node ? cb[i] : 0;
+        xa[k] = node ? cxa[i] : 0.0;
+        xb[k] = node ? cxb[i] : 0.0;
+        need[k] = node && (b[k] - a[k] > c.leaf_cap) && (depth < kMaxDepth);
+        mid[k] = xa[k] + 0.5 * (xb[k] - xa[k]);
+        mine += need[k] ? 2 : 0;
+    }
+    staged_splits<K>(src, base, ix, a, b, need, mid, split);
+    int added;
+    const int off = block_exclusive_scan<THREADS, true>(mine, s_wave, &added);
+    if (alloc + added > limit) {
+        *overflow = 1;  // uniform: every thread sees the same total
+        return 0;
+    }
+    *fits = added <= cap;
+    int o = off;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int i = tid * K + k;
+        if (i < cnt) c.b_left[lb + i] = need[k] ? alloc + o : -1;
+        if (need[k]) {
+            const int ch = alloc + o;
+            c.b_lo[ch] = a[k];
+            c.b_hi[ch] = split[k];
+            c.b_lo[ch + 1] = split[k];
+            c.b_hi[ch + 1] = b[k];
+            if (*fits) {
+                na[o] = a[k];
+                nb[o] = split[k];
+                nxa[o] = xa[k];
+                nxb[o] = src[split[k] - 1 - base];  // a < split < b
+                na[o + 1] = split[k];
+                nb[o + 1] = b[k];
+                nxa[o + 1] = src[split[k] - base];
+                nxb[o + 1] = xb[k];
+            }
+            o += 2;
+        }
+    }
+    lds_barrier();
+    return added;
+}
+
+// expand_level with the subtree's sources staged in LDS.
+template <int THREADS>
+__device__ __forceinline__ int expand_level_staged(const TopoCtx &c, int lb, int le, int depth,
+                                                   int alloc, int limit, int *s_wave,
+                                                   int *overflow, const double *src, int base,
+                                                   const StagedIndex &ix) {
+    const int tid = threadIdx.x;
+    int used = 0;
+    for (int vb = lb; vb < le; vb += THREADS * kTopoK) {
+        const int v0 = vb + tid * kTopoK;
+        int a[kTopoK], b[kTopoK], split[kTopoK];
+        bool need[kTopoK];
+        double mid[kTopoK];
+#pragma unroll
+        for (int k = 0; k < kTopoK; ++k) {
+            const int v = min(v0 + k, le - 1);
+            a[k] = c.b_lo[v];
+            b[k] = c.b_hi[v];
+        }
+#pragma unroll
+        for (int k = 0; k < kTopoK; ++k) {
+            need[k] = (v0 + k < le) && (b[k] - a[k] > c.leaf_cap) && (depth < kMaxDepth);
+            const double xa = need[k] ? src[a[k] - base] : 0.0;
+            const double xb = need[k] ? src[b[k] - 1 - base] : 0.0;
+            mid[k] = xa + 0.5 * (xb - xa);
+        }
+        staged_splits<kTopoK>(src, base, ix, a, b, need, mid, split);
+        int mine = 0;
+#pragma unroll
+        for (int k = 0; k < kTopoK; ++k) mine += need[k] ? 2 : 0;
+        int added;
+        const int off = block_exclusive_scan<THREADS>(mine, s_wave, &added);
+        if (alloc + used + added > limit) {
+            *overflow = 1;
+            return used;
+        }
+        int ch = alloc + used + off;
+#pragma unroll
+        for (int k = 0; k < kTopoK; ++k) {
+            const int v = v0 + k;
+            if (v < le) {
+                if (need[k]) {
+                    c.b_left[v] = ch;
+                    c.b_lo[ch] = a[k];
+                    c.b_hi[ch] = split[k];
+                    c.b_lo[ch + 1] = split[k];
+                    c.b_hi[ch + 1] = b[k];
+                    ch += 2;
+                } else {
+                    c.b_left[v] = -1;
+                }
+            }
+        }
+        used += added;
+    }
+    return used;
+}
+
+// One breadth-first level of at most 64 nodes on wavefront 0 alone (its
+// lanes only), G = 64 / 2^ceil(log2 cnt) lanes per node.  With G >= 4 the
+// binary search stops once [l, r) holds at most G - 2 entries; the node's
+// lanes then load the window sorted[l - 1 .. r] together, the count of its
+// entries <= mid completes the partition point (the search is over sorted
+// values, so the predicate holds on a prefix), and the window already holds
+// the children's end values sorted[split - 1] and sorted[split].  A level
+// then costs the hint load and one window load.  The splits, slots and LDS
+// arrays are lds_level's, and the children of a level of at most 64 nodes
+// always fit the next level's LDS arrays.  Returns the children count
+// (uniform), or sets *overflow when they would pass `limit`.  Ends with the
+// wavefront's LDS writes visible to its own lanes.
+__device__ int narrow_level(const TopoCtx &c, int lb, int cnt, int depth, int alloc, int limit,
+                            const int *ca, const int *cb, const double *cxa, const double *cxb,
+                            int *na, int *nb, double *nxa, double *nxb, int *overflow) {
+    const int lane = threadIdx.x & 63;
+    int lg = 0;
+    while ((1 << lg) < cnt) ++lg;  // cnt <= 64
+    const int G = 64 >> lg, i = lane >> (6 - lg), g = lane & (G - 1), gb = i * G;
+    const bool valid = i < cnt;
+    const int a = valid ? ca[i] : 0, b = valid ? cb[i] : 0;
+    const double xa = valid ? cxa[i] : 0.0, xb = valid ? cxb[i] : 0.0;
+    const bool need = valid && (b - a > c.leaf_cap) && (depth < kMaxDepth);
+    const double mid = xa + 0.5 * (xb - xa);
+    const int kb = hint_bucket(mid, c.h);  // in [0, R-1]
+    int hl = need ? c.bstart[kb] : 0, hr = need ? c.bstart[kb + 1] : 0;
+    if (need && (hl < 0 || hr < 0)) {  // a bucket in a long gap
+        const int4 *gt = reinterpret_cast<const int4 *>(c.gap + 4);
+        for (int e = 0; e < c.ng; ++e) {
+            const int4 t = gt[e];
+            if (hl < 0 && t.x <= kb && kb <= t.y) hl = t.z;
+            if (hr < 0 && t.x <= kb + 1 && kb + 1 <= t.y) hr = t.z;
+        }
+    }
+    const bool ok = hl >= 0 && hr >= 0;
+    int l = need ? (ok ? hl : a) : 0, r = need ? (ok ? hr : b) : 0;
+    const int W = (G >= 4) ? G - 2 : 0;  // window capacity
+    while (__any(r - l > W)) {
+        if (r - l > W) {
+            const int m = l + (r - l) / 2;
+            const double xm = c.sorted[min(m, c.p - 1)];
+            if (xm <= mid) l = m + 1;
+            else r = m;
+        }
+    }
+    double wv = 0.0;
+    int u = l;
+    if (W > 0) {
+        const bool in = need && g <= r - l + 1;  // the window is [l - 1, r]
+        wv = in ? c.sorted[min(max(l - 1 + g, 0), c.p - 1)] : 0.0;
+        const bool pred = need && g >= 1 && g <= r - l && wv <= mid;
+        const uint64_t gm = (G == 64) ? ~0ull : (((1ull << G) - 1) << gb);
+        u += __popcll(__ballot(pred) & gm);
+    }
+    int split = 0;
+    bool from_window = false;
+    if (need) {
+        const int uu = min(max(u, a), b);
+        split = (uu == a || uu == b) ? (a + b) / 2 : uu;
+        from_window = W > 0 && split == u;
+    }
+    const int g0 = from_window ? split - l : 0;  // sorted[split - 1] sits at window entry g0
+    const double w0 = __shfl(wv, gb + g0, 64), w1 = __shfl(wv, gb + min(g0 + 1, G - 1), 64);
+    double xs0 = 0.0, xs1 = 0.0;
+    if (need) {
+        if (from_window) {
+            xs0 = w0;
+            xs1 = w1;
+        } else {
+            xs0 = c.sorted[split - 1];  // a < split < b
+            xs1 = c.sorted[split];
+        }
+    }
+    const bool lead = g == 0;
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    const uint64_t mneed = __ballot(lead && need);
+    const int added = 2 * __popcll(mneed);
+    if (alloc + added > limit) {
+        *overflow = 1;  // uniform
+        return 0;
+    }
+    if (lead && valid) {
+        const int o = 2 * __popcll(mneed & below);
+        c.b_left[lb + i] = need ? alloc + o : -1;
+        if (need) {
+            const int ch = alloc + o;
+            c.b_lo[ch] = a;
+            c.b_hi[ch] = split;
+            c.b_lo[ch + 1] = split;
+            c.b_hi[ch + 1] = b;
+            na[o] = a;
+            nb[o] = split;
+            nxa[o] = xa;
+            nxb[o] = xs0;
+            na[o + 1] = split;
+            nb[o + 1] = b;
+            nxa[o + 1] = xs1;
+            nxb[o + 1] = xb;
+        }
+    }
+    wave_sync();
+    return added;
+}
+
 // A: breadth-first from the root while levels are smaller than kFrontier,
-// one node per thread, each level in LDS (lds_level), double-buffered.
+// each level in LDS, double-buffered: levels of at most 64 nodes on
+// wavefront 0 (narrow_level), then one node per thread (lds_level).
 constexpr int kTopCap = 2 * kFrontier;  // children of a level below kFrontier
 
 __global__ __launch_bounds__(kTopoThreads) void topo_top_kernel(TopoCtx c, int cap_top,
@@ -604,7 +910,34 @@ __global__ __launch_bounds__(kTopoThreads) void topo_top_kernel(TopoCtx c, int c
         info[kInfoTopLevel + 1] = 1;
     }
     lds_barrier();
+    __shared__ int s_state[4];
     int lb = 0, le = 1, depth = 0, overflow = 0, cur = 0;
+    if (tid < 64) {
+        while (lb < le && le - lb <= 64 && !overflow) {
+            const int nxt = cur ^ 1;
+            const int added = narrow_level(c, lb, le - lb, depth, le, cap_top, s_a[cur], s_b[cur],
+                                           s_xa[cur], s_xb[cur], s_a[nxt], s_b[nxt], s_xa[nxt],
+                                           s_xb[nxt], &overflow);
+            if (overflow) break;
+            lb = le;
+            le += added;
+            ++depth;
+            cur = nxt;
+            if (tid == 0) info[kInfoTopLevel + depth + 1] = le;
+        }
+        if (tid == 0) {
+            s_state[0] = lb;
+            s_state[1] = le;
+            s_state[2] = depth;
+            s_state[3] = overflow | (cur << 1);
+        }
+    }
+    lds_barrier();
+    lb = s_state[0];
+    le = s_state[1];
+    depth = s_state[2];
+    overflow = s_state[3] & 1;
+    cur = s_state[3] >> 1;
     bool fits = true;
     while (lb < le && le - lb < kFrontier && !overflow) {
         const int nxt = cur ^ 1;
@@ -637,6 +970,8 @@ __global__ __launch_bounds__(kSubThreads) void topo_sub_kernel(TopoCtx c,
     __shared__ int s_lb[kLv], s_le[kLv];
     __shared__ int s_a[2][kSubCap], s_b[2][kSubCap];
     __shared__ double s_xa[2][kSubCap], s_xb[2][kSubCap];
+    __shared__ double s_src[kSubStaged];
+    __shared__ int s_first[kSubBuckets + 1];
     const int f = blockIdx.x;
     if (f >= info[kInfoF]) return;
     c.h = hint_of(c.sorted, c.p, c.R);
@@ -646,6 +981,47 @@ __global__ __launch_bounds__(kSubThreads) void topo_sub_kernel(TopoCtx c,
     const int a = c.b_lo[v], b = c.b_hi[v];
     int lb = v, le = v + 1, alloc = T + 2 * a, lev = 0, overflow = 0, cur = 0;
     const int limit = T + 2 * b;
+    // a subtree of kSubStageMin to kSubStaged sources searches its staged copy
+    const bool staged = b - a >= kSubStageMin && b - a <= kSubStaged;
+    StagedIndex ix{s_first, 0.0, 0.0};
+    if (staged) {
+        for (int i = threadIdx.x; i < b - a; i += kSubThreads) s_src[i] = c.sorted[a + i];
+        lds_barrier();
+        ix.x0 = s_src[0];
+        const double span = s_src[b - a - 1] - ix.x0;
+        ix.scale = (span > 0.0) ? (double)kSubBuckets / span : 0.0;
+        // first[k]: the first index of each non-empty bucket, then a suffix
+        // minimum carries it into the empty buckets below (and n past the last)
+        const int n = b - a;
+        for (int k = threadIdx.x; k <= kSubBuckets; k += kSubThreads) s_first[k] = n;
+        lds_barrier();
+        for (int i = threadIdx.x; i < n; i += kSubThreads) {
+            const int bi = staged_bucket(s_src[i], ix);
+            if (i == 0 || staged_bucket(s_src[i - 1], ix) < bi) s_first[bi] = i;
+        }
+        lds_barrier();
+        constexpr int C = (kSubBuckets + kSubThreads) / kSubThreads;  // entries per thread
+        const int k0 = threadIdx.x * C, k1 = min(k0 + C, kSubBuckets + 1);
+        const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+        int m = n;
+        for (int k = k0; k < k1; ++k) m = min(m, s_first[k]);
+        int x = m;  // becomes the minimum over this wavefront's lanes >= lane
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_down(x, o, 64);
+            if (lane + o < 64) x = min(x, y);
+        }
+        if (lane == 0) s_wave[wid] = x;
+        const int next = __shfl_down(x, 1, 64);
+        lds_barrier();
+        int run = (lane < 63) ? next : n;  // minimum over the chunks after this thread's
+        for (int w = wid + 1; w < kSubThreads / 64; ++w) run = min(run, s_wave[w]);
+        for (int k = k1 - 1; k >= k0; --k) {
+            run = min(run, s_first[k]);
+            s_first[k] = run;
+        }
+        lds_barrier();  // s_wave is reused by the levels' scans
+    }
     if (threadIdx.x == 0) {
         s_lb[0] = lb;
         s_le[0] = le;
@@ -662,17 +1038,27 @@ __global__ __launch_bounds__(kSubThreads) void topo_sub_kernel(TopoCtx c,
         if (in_lds) {
             const int nxt = cur ^ 1;
             bool fits = true;
-            used = lds_level<kSubThreads, kSubCap / kSubThreads>(
-                c, lb, le - lb, D + lev, alloc, limit, kSubCap, s_a[cur], s_b[cur], s_xa[cur],
-                s_xb[cur], s_a[nxt], s_b[nxt], s_xa[nxt], s_xb[nxt], s_wave, &overflow, &fits);
+            if (staged)
+                used = lds_level_staged<kSubThreads, kSubCap / kSubThreads>(
+                    c, lb, le - lb, D + lev, alloc, limit, kSubCap, s_a[cur], s_b[cur], s_xa[cur],
+                    s_xb[cur], s_a[nxt], s_b[nxt], s_xa[nxt], s_xb[nxt], s_wave, &overflow, &fits,
+                    s_src, a, ix);
+            else
+                used = lds_level<kSubThreads, kSubCap / kSubThreads>(
+                    c, lb, le - lb, D + lev, alloc, limit, kSubCap, s_a[cur], s_b[cur], s_xa[cur],
+                    s_xb[cur], s_a[nxt], s_b[nxt], s_xa[nxt], s_xb[nxt], s_wave, &overflow, &fits);
             cur = nxt;
             if (!fits) {
                 in_lds = false;
                 __syncthreads();  // the next level reads the slot arrays
             }
         } else {
-            used = expand_level<kSubThreads>(c, lb, le, D + lev, alloc, limit, s_wave,
-                                             &overflow);
+            if (staged)
+                used = expand_level_staged<kSubThreads>(c, lb, le, D + lev, alloc, limit, s_wave,
+                                                        &overflow, s_src, a, ix);
+            else
+                used = expand_level<kSubThreads>(c, lb, le, D + lev, alloc, limit, s_wave,
+                                                 &overflow);
             __syncthreads();
         }
         if (used == 0 || overflow) break;
@@ -685,11 +1071,33 @@ __global__ __launch_bounds__(kSubThreads) void topo_sub_kernel(TopoCtx c,
             s_le[lev] = le;
         }
     }
-    __syncthreads();
+    __syncthreads();  // also orders this workgroup's slot-array stores before the loads below
     const int n_lv = lev + 1;
-    for (int l = n_lv - 1; l >= 0; --l) {
-        sizes_level<kSubThreads>(c, s_lb[l], s_le[l]);
-        __syncthreads();
+    // Sizes, deepest level first.  The levels below the root fill the slots
+    // [s0, alloc) contiguously; when they fit, their left-child slots and
+    // sizes sit in LDS (in the staged sources' space, no longer needed).
+    const int s0 = T + 2 * a, ns = alloc - s0;
+    if (ns <= kSubStaged) {
+        int *l_left = reinterpret_cast<int *>(s_src), *l_size = l_left + kSubStaged;
+        for (int i = threadIdx.x; i < ns; i += kSubThreads) l_left[i] = c.b_left[s0 + i];
+        lds_barrier();
+        for (int l = n_lv - 1; l >= 1; --l) {
+            for (int u = s_lb[l] + (int)threadIdx.x; u < s_le[l]; u += kSubThreads) {
+                const int lf = l_left[u - s0];
+                l_size[u - s0] = (lf < 0) ? 1 : 1 + l_size[lf - s0] + l_size[lf + 1 - s0];
+            }
+            lds_barrier();
+        }
+        for (int i = threadIdx.x; i < ns; i += kSubThreads) c.b_size[s0 + i] = l_size[i];
+        if (threadIdx.x == 0) {
+            const int lf = c.b_left[v];
+            c.b_size[v] = (lf < 0) ? 1 : 1 + l_size[lf - s0] + l_size[lf + 1 - s0];
+        }
+    } else {
+        for (int l = n_lv - 1; l >= 0; --l) {
+            sizes_level<kSubThreads>(c, s_lb[l], s_le[l]);
+            __syncthreads();
+        }
     }
     for (int l = threadIdx.x; l < kLv; l += kSubThreads) {
         lv_start[f * kLv + l] = (l < n_lv) ? s_lb[l] : 0;
@@ -722,16 +1130,43 @@ __global__ __launch_bounds__(kTopoThreads) void topo_mid_kernel(
             s_size[v] = (v >= front) ? c.b_size[v] : 0;
         }
         __syncthreads();
-        for (int d = D - 1; d >= 0; --d) {
+        // The top levels [0, dn) hold at most 64 slots each: wavefront 0
+        // sweeps them alone, one slot per lane, ordered by wavefront-scope
+        // LDS fences instead of workgroup barriers.
+        int dn = 0;
+        while (dn < D && lvl[dn + 1] - lvl[dn] <= 64) ++dn;
+        for (int d = D - 1; d >= dn; --d) {
             for (int v = lvl[d] + tid; v < lvl[d + 1]; v += kTopoThreads) {
                 const int l = s_left[v];
                 s_size[v] = (l < 0) ? 1 : 1 + s_size[l] + s_size[l + 1];
             }
             __syncthreads();
         }
-        if (tid == 0) s_pre[0] = 0;
+        if (tid < 64) {
+            for (int d = dn - 1; d >= 0; --d) {
+                const int v = lvl[d] + tid;
+                if (v < lvl[d + 1]) {
+                    const int l = s_left[v];
+                    s_size[v] = (l < 0) ? 1 : 1 + s_size[l] + s_size[l + 1];
+                }
+                wave_sync();
+            }
+            if (tid == 0) s_pre[0] = 0;
+            wave_sync();
+            for (int d = 0; d < dn; ++d) {
+                const int v = lvl[d] + tid;
+                if (v < lvl[d + 1]) {
+                    const int l = s_left[v];
+                    if (l >= 0) {
+                        s_pre[l] = s_pre[v] + 1;
+                        s_pre[l + 1] = s_pre[v] + 1 + s_size[l];
+                    }
+                }
+                wave_sync();
+            }
+        }
         __syncthreads();
-        for (int d = 0; d < D; ++d) {
+        for (int d = dn; d < D; ++d) {
             for (int v = lvl[d] + tid; v < lvl[d + 1]; v += kTopoThreads) {
                 const int l = s_left[v];
                 if (l >= 0) {
@@ -757,7 +1192,6 @@ __global__ __launch_bounds__(kTopoThreads) void topo_mid_kernel(
             __syncthreads();
         }
     }
-    for (int i = tid; i < T; i += kTopoThreads) list[i] = i;
     // subtree f's nodes below its root take list[lv_off[f * kLv], + sub_nodes[f])
     const int mine = (tid < F) ? sub_nodes[tid] : 0;
     int tot;
@@ -770,6 +1204,8 @@ __global__ __launch_bounds__(kTopoThreads) void topo_mid_kernel(
     __threadfence_system();
     __syncthreads();
     if (tid == 0) __hip_atomic_store(mail, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    // the top slots' node-list entries (read by later kernels, not the host)
+    for (int i = tid; i < T; i += kTopoThreads) list[i] = i;
 }
 
 // D: pre-order numbers inside the subtree and its entries of the node list.
@@ -798,14 +1234,6 @@ struct TreeArrays {
     double *lo, *hi, *acc, *nodes, *w;
     int32_t *lo_idx, *hi_idx, *right, *esc;
 };
-
-// Makes a wavefront's LDS writes visible to its own lanes' reads: the build
-// kernels below give each wavefront its own LDS region.
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Barycentric accumulation of `nrows` rows (x_r, mass_r) into one panel's
 // weights w[0..n) (chebcode.rs:309-337), one wavefront per panel; w32
@@ -1315,14 +1743,28 @@ __global__ __launch_bounds__(kMergeMaxThreads) void merge_top_kernel(
 // each level then reads and writes LDS only (weights also go to the tree and
 // the f32 panel) behind an LDS-only barrier.  Per node the arithmetic and
 // its order are merge_group's, so the weights are the same bit for bit.
+//
+// A merge evaluates, for each of its 2n rows (a child's Chebyshev point x_r,
+// with that child's weight as mass), the quotients q_rj = lam_j / (x_r - t_j)
+// against the node's positions t_j, their sum s_r and the first exact hit,
+// then adds the rows into the weights in row order: w_j = fma(mass_r *
+// (1 / s_r), q_rj, w_j), or w_hit += mass_r, skipping rows of zero mass.
+// Only the row sums depend on the level below.  The narrow top levels (the
+// first kQNarrow slots, one or a few nodes per level, where the levels are a
+// chain of latencies on one CU) therefore get their quotients, 1 / s_r and
+// hits as tables built by all wavefronts at once, and each of their levels
+// is then the 2n-long fused multiply-add chain alone.
 constexpr int kTopLdsWaves = 16;          // the top-level merge workgroup
 constexpr size_t kTopLdsWgMax = 163840;   // LDS of the one top-level merge workgroup
+constexpr int kQNarrow = 40;              // top slots with quotient tables
 
 __host__ __device__ inline size_t top_lds_bytes(int slots, int n, int waves) {
     return (size_t)slots * (2 * sizeof(int) + 2 * (size_t)n * sizeof(double)) +
            (size_t)waves * sizeof(double) * (size_t)bary_lds_doubles(n);
 }
 
+// One group of nodes merged from LDS (merge_sub_lds_kernel): merge_group's
+// arithmetic with positions and weights staged in LDS.
 template <int NT>
 __device__ __forceinline__ void merge_group_lds(int lb, int base, int cnt,
                                                 const int *__restrict__ s_left,
@@ -1404,19 +1846,77 @@ __global__ __launch_bounds__(kTopLdsWaves * kBaryLanes) void merge_top_lds_kerne
         s_w[e] = t.w[o];  // finished weights; merged nodes overwrite theirs
     }
     lds_barrier();
-    for (int d = levels - 1; d >= 0; --d) {
+    // levels [0, dq) hold the first lvl[dq] <= kQNarrow slots
+    int dq = 0;
+    while (dq < levels && lvl[dq + 1] <= kQNarrow) ++dq;
+    for (int d = levels - 1; d >= dq; --d) {
         const int lb = lvl[d], cnt = lvl[d + 1] - lvl[d];
         for (int c = wv * G; c < cnt; c += waves * G)
             merge_group_lds<NT>(lb, c, cnt, s_left, s_k, s_pos, s_w, lr, t, panel32,
                                 bary + (size_t)wv * bary_lds_doubles(NT));
         lds_barrier();
     }
+    if (dq == 0) return;
+    // quotient tables of slots [0, P), in the per-wavefront scratch (free now):
+    // q at ((u * 2n + r) * n + j), 1 / s_r and the hit at (u * 2n + r)
+    constexpr int R = 2 * NT;
+    static_assert(kQNarrow * (R * NT + R + R / 2 + 1) <=
+                      kTopLdsWaves * (kBaryLanes * NT + kBaryLanes + kBaryLanes / 2),
+                  "the narrow quotient tables fit the merge scratch");
+    const int P = lvl[dq];
+    double *tq = bary, *ti = tq + (size_t)P * R * NT;
+    int *th = reinterpret_cast<int *>(ti + (size_t)P * R);
+    for (int e = threadIdx.x; e < P * R; e += blockDim.x) {
+        const int u = e / R, r = e - u * R;
+        const int left = s_left[u];
+        if (left < 0) continue;
+        const double x = s_pos[(left + (r < NT ? 0 : 1)) * NT + (r < NT ? r : r - NT)];
+        double sum = 0.0;
+        int hit = -1;
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+            const double d = x - s_pos[u * NT + j];
+            const double q = lr[j] / d;
+            tq[(size_t)e * NT + j] = q;
+            sum += q;
+            hit = (hit < 0 && d == 0.0) ? j : hit;
+        }
+        ti[e] = 1.0 / sum;
+        th[e] = hit;
+    }
+    lds_barrier();
+    // n lanes per node (lane j owns w_j), kBaryLanes / n nodes per wavefront
+    constexpr int PG = kBaryLanes / NT;
+    const int lane = threadIdx.x & (kBaryLanes - 1), pg = lane / NT, j = lane - pg * NT;
+    for (int d = dq - 1; d >= 0; --d) {
+        const int lb = lvl[d], cnt = lvl[d + 1] - lvl[d];
+        const int i = wv * PG + pg;  // at most kQNarrow nodes: one pass
+        const int u = lb + i;
+        const int left = (pg < PG && i < cnt) ? s_left[u] : -1;
+        if (left >= 0) {
+            double acc = 0.0;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const double mass = s_w[(left + (r < NT ? 0 : 1)) * NT + (r < NT ? r : r - NT)];
+                const int h = th[u * R + r];
+                const double f = __builtin_fma(mass * ti[u * R + r], tq[((size_t)u * R + r) * NT + j],
+                                               acc),
+                             a = acc + mass;
+                acc = (mass == 0.0) ? acc : ((h == -1) ? f : ((h == j) ? a : acc));
+            }
+            s_w[u * NT + j] = acc;
+            const int k = s_k[u];
+            t.w[(size_t)k * NT + j] = acc;
+            panel32[(size_t)k * 2 * NT + NT + j] = (float)acc;
+        }
+        lds_barrier();
+    }
 }
 
 // LDS form of the subtree merges for the presets' orders: subtree f (its root
-// and the contiguous slots of its levels below) staged in LDS as in
-// merge_top_lds_kernel, local node 0 being the root and slot v >= base1 the
-// local node v - base1 + 1.  A subtree with more than `cap` nodes takes the
+// and the contiguous slots of its levels below) staged in LDS with positions,
+// weights, left-child slots and pre-order numbers, local node 0 being the
+// root and slot v >= base1 the local node v - base1 + 1.  A subtree with more than `cap` nodes takes the
 // global path in the same workgroup.
 template <int NT>
 __global__ __launch_bounds__(kMergeSubWaves * kBaryLanes) void merge_sub_lds_kernel(
