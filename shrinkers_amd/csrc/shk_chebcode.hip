@@ -2585,7 +2585,7 @@ __global__ __launch_bounds__(256) void eval_x2_kernel(EvalView v, int nq, double
 // by about rho^-n, rho = 8 + sqrt(65) (5e-14 at n = 11).  A target whose
 // near list overflows falls back to the per-query walk.
 constexpr double kLocalWidth = 0.25;
-constexpr int kLocalNear = 64;
+constexpr int kLocalNear = 16;
 
 // The reference traversal for one query (the per-lane walk of
 // eval_f64_kernel<false>, same arithmetic and order).
@@ -2642,21 +2642,32 @@ __global__ void local_zero_kernel(int *count) {
 // its parent is wider (or T is the root); a leaf wider than wmax under such
 // parents is a fallback target (code ~T).  Every source lies in exactly one
 // target.  The order of the list does not matter: targets are independent.
-// Appends this lane's n entries (codes c0, c1) to a list with one atomic per
-// wavefront.
-__device__ __forceinline__ void wave_append(int n, int c0, int c1, int *__restrict__ list,
-                                            int *__restrict__ counter, int lane) {
+// Appends this thread's n (0..2) entries c0, c1 to a list with one atomic
+// per workgroup (THREADS threads): ballots give each wavefront's offsets, a
+// workgroup scan the wavefronts', and thread 0 reserves the whole block.
+// Many wavefronts adding to one counter serialise at the L2.
+template <int THREADS>
+__device__ __forceinline__ void block_append(int n, int c0, int c1, int *__restrict__ list,
+                                             int *__restrict__ counter, int *s_part) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const uint64_t m1 = __ballot(n >= 1), m2 = __ballot(n >= 2);
-    const int tot = __popcll(m1) + __popcll(m2);
-    int base = 0;
-    if (tot > 0) {
-        if (lane == 0) base = atomicAdd(counter, tot);
-        base = __shfl(base, 0, 64);
+    if (lane == 0) s_part[wid] = __popcll(m1) + __popcll(m2);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int w = 0; w < THREADS / 64; ++w) {
+            const int c = s_part[w];
+            s_part[w] = tot;  // the wavefront's offset within the block
+            tot += c;
+        }
+        s_part[THREADS / 64] = (tot > 0) ? atomicAdd(counter, tot) : 0;
     }
-    const int at = base + __popcll(m1 & below) + __popcll(m2 & below);
+    __syncthreads();
+    const int at = s_part[THREADS / 64] + s_part[wid] + __popcll(m1 & below) + __popcll(m2 & below);
     if (n >= 1) list[at] = c0;
     if (n >= 2) list[at + 1] = c1;
+    __syncthreads();  // s_part is reused by the next call
 }
 
 // Target lists: node T is a target (list, count[0]) when it is no wider than
@@ -2664,12 +2675,14 @@ __device__ __forceinline__ void wave_append(int n, int c0, int c1, int *__restri
 // under such parents is a wide target (wide_list, count[2]).  Every source
 // lies in exactly one target.  List order does not matter: targets are
 // independent.
-__global__ __launch_bounds__(256) void local_plan_kernel(EvalView v, double wmax,
-                                                        int *__restrict__ list,
-                                                        int *__restrict__ wide_list,
-                                                        int *__restrict__ count) {
+constexpr int kPlanThreads = 1024;
+
+__global__ __launch_bounds__(kPlanThreads) void local_plan_kernel(EvalView v, double wmax,
+                                                                 int *__restrict__ list,
+                                                                 int *__restrict__ wide_list,
+                                                                 int *__restrict__ count) {
+    __shared__ int s_part[kPlanThreads / 64 + 1];
     const int node = blockIdx.x * blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & 63;
     int t[2] = {0, 0}, wd[2] = {0, 0}, nt = 0, nw = 0;
     if (node < v.N) {
         const int4 lk = v.link[node];
@@ -2688,102 +2701,145 @@ __global__ __launch_bounds__(256) void local_plan_kernel(EvalView v, double wmax
             }
         }
     }
-    wave_append(nt, t[0], t[1], list, count, lane);
-    wave_append(nw, wd[0], wd[1], wide_list, count + 2, lane);
+    block_append<kPlanThreads>(nt, t[0], t[1], list, count, s_part);
+    block_append<kPlanThreads>(nw, wd[0], wd[1], wide_list, count + 2, s_part);
 }
 
-// One wavefront per target (grid-stride over the list).  A leaf wider than
-// the target width sums the walk's accepted panels at each of its queries
-// instead (M2P; it has at most leaf_cap queries).  The walk is
-// breadth-first: each level's candidate nodes are spread over the lanes,
-// and ballots compact the outcomes -- accepted panels, reached leaves, the
-// children of rejected nodes (the next level) -- into LDS lists, so a target
-// costs one round of loads per tree level instead of one per visit.  A
-// target whose lists overflow falls back to the per-query walk.  Then lane l
+// Two kernels per evaluation, one wavefront per target (grid-stride over
+// the targets, the wide leaves first).
+//
+// local_walk_kernel walks the tree for the target breadth-first: each
+// level's candidate nodes are spread over the lanes, and ballots compact the
+// outcomes (accepted panels, reached leaves, the children of rejected nodes:
+// the next level) into lists, so a target costs one round of loads per tree
+// level instead of one per visit.  The accepted panels and reached leaves go
+// to the target's slots in global memory (kLocalPanels and kLocalNear of
+// them); a target whose lists overflow goes to the fallback list instead.
+// The walk is a chain of round trips with little arithmetic, so it is kept
+// to few registers and little LDS, and many wavefronts share a SIMD.
+//
+// local_f64_kernel then evaluates each target from its lists.  Lane l
 // evaluates the M2L terms u = l + 64 s (s < kS) of every accepted panel:
 // target point k = u / NT, panel point j = u % NT; the per-term sums are
-// added per k in j order.
+// added per k in j order.  A leaf wider than the target width instead sums
+// the accepted panels at each of its queries (M2P; it has at most leaf_cap
+// queries).
 constexpr int kLocalFront = 512;
-constexpr int kLocalPanels = 512;
+constexpr int kLocalPanels = 64;
+constexpr int kLocalStage = 32;   // panels staged in LDS at a time for M2L
 
-template <int NT>
-__global__ __launch_bounds__(64) void local_f64_kernel(EvalView v, const int *__restrict__ list,
-                                                       const int *__restrict__ wide_list,
-                                                       int *__restrict__ count,
-                                                       int *__restrict__ fb_list, double eta,
-                                                       double scale, double *__restrict__ out_re,
-                                                       double *__restrict__ out_im) {
-    constexpr int kTerms = NT * NT;
-    constexpr int kS = (kTerms + 63) / 64;
-    __shared__ double s_fr[64 * kS], s_fi[64 * kS];
-    __shared__ double s_cr[NT], s_ci[NT], s_y[NT];
-    __shared__ int2 s_near[kLocalNear];
+__global__ __launch_bounds__(64) void local_walk_kernel(EvalView v, const int *__restrict__ list,
+                                                        const int *__restrict__ wide_list,
+                                                        int *__restrict__ count,
+                                                        int *__restrict__ fb_list, double eta,
+                                                        int *__restrict__ acc_n,
+                                                        int *__restrict__ acc_buf,
+                                                        int2 *__restrict__ near_buf) {
     __shared__ int s_front[2][kLocalFront];
-    __shared__ int s_acc[kLocalPanels];
     const int lane = threadIdx.x;
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    // wide leaves first: each walks as long as a narrow target but is rarer,
-    // so starting them early keeps them off the tail of the launch
     const int n_wide = count[2];
     const int total = count[0] + n_wide;
     const double eta2 = eta * eta;
     for (int e = blockIdx.x; e < total; e += gridDim.x) {
-        const bool wide = e < n_wide;
-        const int T = wide ? wide_list[e] : list[e - n_wide];
-        const int4 lkT = v.link[T];
-        const int q0 = lkT.z, q1 = lkT.w;
+        const int T = (e < n_wide) ? wide_list[e] : list[e - n_wide];
+        const double tlo = v.lo[T], thi = v.hi[T];
+        int *acc_out = acc_buf + (size_t)e * kLocalPanels;
+        int2 *near_out = near_buf + (size_t)e * kLocalNear;
         bool local_ok = true;
         int n_near = 0, n_acc = 0;
-        {
-            const double tlo = v.lo[T], thi = v.hi[T];
-            // breadth-first walk
-            if (lane == 0) s_front[0][0] = 0;
-            __syncthreads();
-            int fs = 1, cur = 0;
-            while (fs > 0 && local_ok) {
-                int nfs = 0;
-                for (int base = 0; base < fs; base += 64) {
-                    const int idx = base + lane;
-                    const bool valid = idx < fs;
-                    const int node = valid ? s_front[cur][idx] : 0;
-                    const int4 lk = v.link[node];
-                    const double plo = v.lo[node], phi = v.hi[node], pacc = v.acc[node];
-                    const bool leaf = valid && lk.x < 0;
-                    const bool acc =
-                        valid && !leaf &&
-                        accept_f64_of(plo, phi, pacc, fmin(fmax(0.5 * (plo + phi), tlo), thi), eta2);
-                    const bool desc = valid && !leaf && !acc;
-                    const uint64_t ml = __ballot(leaf), ma = __ballot(acc), md = __ballot(desc);
-                    if (leaf) {
-                        const int at = n_near + __popcll(ml & below);
-                        if (at < kLocalNear) s_near[at] = make_int2(lk.z, lk.w);
-                    }
-                    if (acc) {
-                        const int at = n_acc + __popcll(ma & below);
-                        if (at < kLocalPanels) s_acc[at] = node;
-                    }
-                    if (desc) {
-                        const int at = nfs + 2 * __popcll(md & below);
-                        if (at + 1 < kLocalFront) {
-                            s_front[cur ^ 1][at] = node + 1;
-                            s_front[cur ^ 1][at + 1] = lk.x;
-                        }
-                    }
-                    n_near += __popcll(ml);
-                    n_acc += __popcll(ma);
-                    nfs += 2 * __popcll(md);
+        if (lane == 0) s_front[0][0] = 0;
+        __syncthreads();
+        int fs = 1, cur = 0;
+        while (fs > 0 && local_ok) {
+            int nfs = 0;
+            for (int base = 0; base < fs; base += 64) {
+                const int idx = base + lane;
+                const bool valid = idx < fs;
+                const int node = valid ? s_front[cur][idx] : 0;
+                const int4 lk = v.link[node];
+                const double plo = v.lo[node], phi = v.hi[node], pacc = v.acc[node];
+                const bool leaf = valid && lk.x < 0;
+                // the reference's test at the target's point nearest the panel centre
+                const bool acc =
+                    valid && !leaf &&
+                    accept_f64_of(plo, phi, pacc, fmin(fmax(0.5 * (plo + phi), tlo), thi), eta2);
+                const bool desc = valid && !leaf && !acc;
+                const uint64_t ml = __ballot(leaf), ma = __ballot(acc), md = __ballot(desc);
+                if (leaf) {
+                    const int at = n_near + __popcll(ml & below);
+                    if (at < kLocalNear) near_out[at] = make_int2(lk.z, lk.w);
                 }
-                if (n_near > kLocalNear || n_acc > kLocalPanels || nfs > kLocalFront)
-                    local_ok = false;
-                __syncthreads();
-                cur ^= 1;
-                fs = nfs;
+                if (acc) {
+                    const int at = n_acc + __popcll(ma & below);
+                    if (at < kLocalPanels) acc_out[at] = node;
+                }
+                if (desc) {
+                    const int at = nfs + 2 * __popcll(md & below);
+                    if (at + 1 < kLocalFront) {
+                        s_front[cur ^ 1][at] = node + 1;
+                        s_front[cur ^ 1][at + 1] = lk.x;
+                    }
+                }
+                n_near += __popcll(ml);
+                n_acc += __popcll(ma);
+                nfs += 2 * __popcll(md);
             }
+            if (n_near > kLocalNear || n_acc > kLocalPanels || nfs > kLocalFront) local_ok = false;
+            __syncthreads();
+            cur ^= 1;
+            fs = nfs;
         }
+        if (lane == 0) {
+            // panels, then leaves (the near count in the upper half)
+            acc_n[e] = local_ok ? (n_acc | (n_near << 16)) : -1;
+            if (!local_ok) fb_list[atomicAdd(count + 1, 1)] = T;
+        }
+    }
+}
+
+template <int NT>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void local_f64_kernel(
+    EvalView v, const int *__restrict__ list,
+                                                       const int *__restrict__ wide_list,
+                                                       const int *__restrict__ count, double eta,
+                                                       double scale,
+                                                       const int *__restrict__ acc_n,
+                                                       const int *__restrict__ acc_buf,
+                                                       const int2 *__restrict__ near_buf,
+                                                       double *__restrict__ out_re,
+                                                       double *__restrict__ out_im) {
+    constexpr int kTerms = NT * NT;
+    constexpr int kS = (kTerms + 63) / 64;
+    // One LDS region, reused in turn: the staged panels (positions, then
+    // weights), the M2L term sums, the reached leaves' sources.
+    constexpr int kRegion = 2 * kLocalStage * NT;
+    static_assert(2 * 64 * kS <= kRegion, "the term sums fit the staging region");
+    constexpr int kNearStage = kRegion;
+    __shared__ double s_region[kRegion];
+    __shared__ double s_cr[NT], s_ci[NT], s_y[NT];
+    __shared__ int2 s_near[kLocalNear];
+    __shared__ int s_acc[kLocalPanels];
+    double *s_pt = s_region, *s_pw = s_region + kLocalStage * NT;
+    double *s_fr = s_region, *s_fi = s_region + 64 * kS, *s_ns = s_region;
+    const int lane = threadIdx.x;
+    const int n_wide = count[2];
+    const int total = count[0] + n_wide;
+    const double eta2 = eta * eta;
+    for (int e = blockIdx.x; e < total; e += gridDim.x) {
+        const int code = acc_n[e];
+        if (code < 0) continue;  // the fallback kernel evaluates it
+        const bool wide = e < n_wide;
+        const int T = wide ? wide_list[e] : list[e - n_wide];
+        const int n_acc = code & 0xffff, n_near = code >> 16;
+        if (lane < n_acc) s_acc[lane] = acc_buf[(size_t)e * kLocalPanels + lane];
+        if (lane < n_near) s_near[lane] = near_buf[(size_t)e * kLocalNear + lane];
+        const int4 lkT = v.link[T];
+        const int q0 = lkT.z, q1 = lkT.w;
+        __syncthreads();
         if (wide) {
-          // a leaf wider than the target width: no local expansion
-          if (local_ok) {
-            // the accepted panels at each query (M2P), the walk shared
+            // a leaf wider than the target width: the accepted panels at each
+            // query (M2P), the walk shared
             for (int c0 = q0; c0 < q1; c0 += 64) {
                 const int i = c0 + lane;
                 if (i >= q1) continue;
@@ -2813,8 +2869,7 @@ __global__ __launch_bounds__(64) void local_f64_kernel(EvalView v, const int *__
                 out_re[o] = fr * scale;
                 out_im[o] = (eta * fi) * scale;
             }
-          }
-        } else if (local_ok) {
+        } else {
             // M2L over the accepted panels
             double yk[kS], ar[kS], ai[kS];
             int jk[kS];
@@ -2826,31 +2881,31 @@ __global__ __launch_bounds__(64) void local_f64_kernel(EvalView v, const int *__
                 yk[s] = v.nodes[(size_t)T * NT + k];
                 ar[s] = ai[s] = 0.0;
             }
-            // four panels per step, their loads issued together
-            for (int a0 = 0; a0 < n_acc; a0 += 4) {
-                double tj[4][kS], wj[4][kS];
-#pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    const int node = s_acc[min(a0 + b, n_acc - 1)];
-#pragma unroll
-                    for (int s = 0; s < kS; ++s) {
-                        tj[b][s] = v.nodes[(size_t)node * NT + jk[s]];
-                        wj[b][s] = v.w[(size_t)node * NT + jk[s]];
-                    }
+            // kLocalStage panels at a time: their positions and weights are
+            // staged in LDS with one round of loads, then each lane runs its
+            // terms over them in list order
+            for (int a0 = 0; a0 < n_acc; a0 += kLocalStage) {
+                const int cnt = min(kLocalStage, n_acc - a0);
+                for (int e = lane; e < cnt * NT; e += 64) {
+                    const int b = e / NT, j = e - b * NT;
+                    const size_t o = (size_t)s_acc[a0 + b] * NT + j;
+                    s_pt[e] = v.nodes[o];
+                    s_pw[e] = v.w[o];
                 }
-#pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    if (a0 + b >= n_acc) break;
+                __syncthreads();
+#pragma unroll 4
+                for (int b = 0; b < cnt; ++b) {
 #pragma unroll
                     for (int s = 0; s < kS; ++s) {
                         if (lane + 64 * s < kTerms) {
-                            const double d = yk[s] - tj[b][s];
-                            const double q = wj[b][s] * rcp_f64(__builtin_fma(d, d, eta2));
+                            const double d = yk[s] - s_pt[b * NT + jk[s]];
+                            const double q = s_pw[b * NT + jk[s]] * rcp_f64(__builtin_fma(d, d, eta2));
                             ar[s] = __builtin_fma(q, d, ar[s]);
                             ai[s] += q;
                         }
                     }
                 }
+                __syncthreads();  // the next chunk overwrites the stage
             }
 #pragma unroll
             for (int s = 0; s < kS; ++s) {
@@ -2868,6 +2923,19 @@ __global__ __launch_bounds__(64) void local_f64_kernel(EvalView v, const int *__
                 }
                 s_cr[lane] = cr;
                 s_ci[lane] = ci;
+            }
+            __syncthreads();  // the term sums are consumed
+            // the reached leaves' sources, in leaf order, staged when they fit
+            int n_ns = 0;
+            for (int m = 0; m < n_near; ++m) n_ns += s_near[m].y - s_near[m].x;
+            const bool near_staged = n_ns <= kNearStage;
+            if (near_staged) {
+                int off = 0;
+                for (int m = 0; m < n_near; ++m) {
+                    const int2 lf = s_near[m];
+                    for (int jj = lane; jj < lf.y - lf.x; jj += 64) s_ns[off + jj] = v.sorted[lf.x + jj];
+                    off += lf.y - lf.x;
+                }
             }
             __syncthreads();
             for (int c0 = q0; c0 < q1; c0 += 64) {
@@ -2897,15 +2965,24 @@ __global__ __launch_bounds__(64) void local_f64_kernel(EvalView v, const int *__
                     fr = num_r * id;
                     fi = num_i * id;
                 }
-                // the reached leaves, exactly
+                // the reached leaves, exactly (in leaf and source order)
                 double nr = 0.0, ni = 0.0;
-                for (int m = 0; m < n_near; ++m) {
-                    const int2 lf = s_near[m];
-                    for (int j = lf.x; j < lf.y; ++j) {
-                        const double d = x - v.sorted[j];
+                if (near_staged) {
+                    for (int j = 0; j < n_ns; ++j) {
+                        const double d = x - s_ns[j];
                         const double inv = rcp_f64(__builtin_fma(d, d, eta2));
                         nr = __builtin_fma(d, inv, nr);
                         ni += inv;
+                    }
+                } else {
+                    for (int m = 0; m < n_near; ++m) {
+                        const int2 lf = s_near[m];
+                        for (int j = lf.x; j < lf.y; ++j) {
+                            const double d = x - v.sorted[j];
+                            const double inv = rcp_f64(__builtin_fma(d, d, eta2));
+                            nr = __builtin_fma(d, inv, nr);
+                            ni += inv;
+                        }
                     }
                 }
                 const int o = v.perm ? v.perm[i] : i;
@@ -2914,7 +2991,6 @@ __global__ __launch_bounds__(64) void local_f64_kernel(EvalView v, const int *__
             }
         }
         __syncthreads();  // LDS is reused by the next target
-        if (!local_ok && lane == 0) fb_list[atomicAdd(count + 1, 1)] = T;
     }
 }
 
@@ -3481,23 +3557,32 @@ int cheb_eval_local(const ChebTree *t, double eta, double scale, double *d_out_r
     if (int st = wait_tree(t, stream)) return st;
     EvalView v = view_of(t);
     const int N = (int)t->N;
-    // counts (targets, fallbacks, wide leaves) and the three lists
+    // counts (targets, fallbacks, wide leaves), the three lists, and per
+    // target (at most N) its list counts, panels and reached leaves
     int *buf = nullptr;
     CachedBuf cache(CACHE_CHEB_LOCAL, stream);
-    if (int e = cache.get(3 * (size_t)N + 4, &buf)) return e;
+    const size_t per_target = 1 + kLocalPanels + 2 * kLocalNear;
+    if (int e = cache.get(4 + (3 + per_target) * (size_t)N, &buf)) return e;
     int *count = buf, *list = buf + 4, *fb_list = list + N, *wide_list = fb_list + N;
+    int *acc_n = wide_list + N, *acc_buf = acc_n + N;
+    int2 *near_buf = reinterpret_cast<int2 *>(acc_buf + (size_t)N * kLocalPanels);
     hipLaunchKernelGGL(local_zero_kernel, dim3(1), dim3(64), 0, stream, count);
-    hipLaunchKernelGGL(local_plan_kernel, dim3(blocks_for((size_t)N, 256)), dim3(256), 0, stream,
-                       v, kLocalWidth * eta, list, wide_list, count);
+    hipLaunchKernelGGL(local_plan_kernel, dim3(blocks_for((size_t)N, kPlanThreads)),
+                       dim3(kPlanThreads), 0, stream, v, kLocalWidth * eta, list, wide_list, count);
     SHK_HIP(hipGetLastError());
     const unsigned grid = (unsigned)std::min(N, 32768);
     ProfileMark m = profile_begin(stream);
+    hipLaunchKernelGGL(local_walk_kernel, dim3(grid), dim3(64), 0, stream, v, list, wide_list,
+                       count, fb_list, eta, acc_n, acc_buf, near_buf);
+    SHK_HIP(hipGetLastError());
     if (t->n == 11)
         hipLaunchKernelGGL(local_f64_kernel<11>, dim3(grid), dim3(64), 0, stream, v, list,
-                           wide_list, count, fb_list, eta, scale, d_out_re, d_out_im);
+                           wide_list, count, eta, scale, acc_n, acc_buf, near_buf, d_out_re,
+                           d_out_im);
     else
         hipLaunchKernelGGL(local_f64_kernel<12>, dim3(grid), dim3(64), 0, stream, v, list,
-                           wide_list, count, fb_list, eta, scale, d_out_re, d_out_im);
+                           wide_list, count, eta, scale, acc_n, acc_buf, near_buf, d_out_re,
+                           d_out_im);
     profile_end(m, stream);
     SHK_HIP(hipGetLastError());
     hipLaunchKernelGGL(local_fallback_kernel, dim3((unsigned)std::min(N, 8192)), dim3(64), 0,
