@@ -339,14 +339,15 @@ shk_status shk_chebcode_evaluate_many_dev(const shk_chebcode *h, const double *e
  * interpolates them (L2P) and adds the exact sums of the leaves the walk
  * reached.  A leaf wider than eta/4 shares one such walk among its queries
  * and sums the accepted panels at each query; a target whose walk outgrows
- * its lists (64 leaves, 512 panels, a 512-node front) keeps the reference's
+ * its lists (16 leaves, 64 panels, a 256-node front) keeps the reference's
  * per-query walk.  Trees with an f64 far field and n = 11 or 12
  * (SHK_ERR_BAD_ARG otherwise).  Error: that of the reference traversal or
  * smaller, plus the interpolation error ~ (8 + sqrt 65)^-n (5e-14 at
- * n = 11).  It pays off at about a million eigenvalues; below that the
- * latency of one target's walk bounds the launch and the per-query
- * evaluation is as fast or faster.  Enqueued on `stream` without
- * synchronising. */
+ * n = 11).  Faster than shk_chebcode_evaluate_dev from about 2e5
+ * eigenvalues (on MP spectra at eta = 1/sqrt(p): 1.1-1.2x at 2e5, 1.6-2.4x
+ * at 1e6; 0.8x at 5e4, where the latency of one target's walk bounds the
+ * launch).  Scratch from the calling thread's cached buffers.  Enqueued on
+ * `stream` without synchronising. */
 shk_status shk_chebcode_evaluate_local_dev(const shk_chebcode *h, double eta, double scale,
                                            double *d_out_re, double *d_out_im, void *stream);
 

@@ -2724,39 +2724,48 @@ __global__ __launch_bounds__(kPlanThreads) void local_plan_kernel(EvalView v, do
 // added per k in j order.  A leaf wider than the target width instead sums
 // the accepted panels at each of its queries (M2P; it has at most leaf_cap
 // queries).
-constexpr int kLocalFront = 512;
 constexpr int kLocalPanels = 64;
 constexpr int kLocalStage = 32;   // panels staged in LDS at a time for M2L
 
+// Two targets per wavefront, one per 32-lane half, so that twice as many
+// walks are in flight per SIMD: each half spreads its own front over its
+// lanes, and its ballots are masked to its half.  Both halves step the
+// levels together; a half whose walk has ended idles.
+constexpr int kWalkFront = 256;  // front capacity per target
+
 __global__ __launch_bounds__(64) void local_walk_kernel(EvalView v, const int *__restrict__ list,
                                                         const int *__restrict__ wide_list,
-                                                        int *__restrict__ count,
-                                                        int *__restrict__ fb_list, double eta,
+                                                        const int *__restrict__ count, double eta,
                                                         int *__restrict__ acc_n,
                                                         int *__restrict__ acc_buf,
                                                         int2 *__restrict__ near_buf) {
-    __shared__ int s_front[2][kLocalFront];
-    const int lane = threadIdx.x;
-    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    __shared__ int s_front[2][2][kWalkFront];  // [half][buffer][entry]
+    const int lane = threadIdx.x, h = lane >> 5, hl = lane & 31;
+    const uint64_t hmask = h ? 0xffffffff00000000ull : 0x00000000ffffffffull;
+    const uint64_t below = ((lane == 0) ? 0ull : (~0ull >> (64 - lane))) & hmask;
     const int n_wide = count[2];
     const int total = count[0] + n_wide;
     const double eta2 = eta * eta;
-    for (int e = blockIdx.x; e < total; e += gridDim.x) {
-        const int T = (e < n_wide) ? wide_list[e] : list[e - n_wide];
+    for (int e0 = 2 * blockIdx.x; e0 < total; e0 += 2 * gridDim.x) {
+        const int e = e0 + h;
+        const bool has = e < total;
+        const int T = has ? ((e < n_wide) ? wide_list[e] : list[e - n_wide]) : 0;
         const double tlo = v.lo[T], thi = v.hi[T];
         int *acc_out = acc_buf + (size_t)e * kLocalPanels;
         int2 *near_out = near_buf + (size_t)e * kLocalNear;
-        bool local_ok = true;
-        int n_near = 0, n_acc = 0;
-        if (lane == 0) s_front[0][0] = 0;
+        bool ok = has;
+        int n_near = 0, n_acc = 0, fs = has ? 1 : 0, cur = 0;
+        if (hl == 0) s_front[h][0][0] = 0;
         __syncthreads();
-        int fs = 1, cur = 0;
-        while (fs > 0 && local_ok) {
+        while (__any(fs > 0 && ok)) {
+            const bool act = fs > 0 && ok;  // uniform within the half
+            const int chunks = act ? (fs + 31) / 32 : 0;
+            const int steps = max(chunks, __shfl_xor(chunks, 32, 64));
             int nfs = 0;
-            for (int base = 0; base < fs; base += 64) {
-                const int idx = base + lane;
-                const bool valid = idx < fs;
-                const int node = valid ? s_front[cur][idx] : 0;
+            for (int c = 0; c < steps; ++c) {
+                const int idx = c * 32 + hl;
+                const bool valid = c < chunks && idx < fs;
+                const int node = valid ? s_front[h][cur][idx] : 0;
                 const int4 lk = v.link[node];
                 const double plo = v.lo[node], phi = v.hi[node], pacc = v.acc[node];
                 const bool leaf = valid && lk.x < 0;
@@ -2765,7 +2774,8 @@ __global__ __launch_bounds__(64) void local_walk_kernel(EvalView v, const int *_
                     valid && !leaf &&
                     accept_f64_of(plo, phi, pacc, fmin(fmax(0.5 * (plo + phi), tlo), thi), eta2);
                 const bool desc = valid && !leaf && !acc;
-                const uint64_t ml = __ballot(leaf), ma = __ballot(acc), md = __ballot(desc);
+                const uint64_t ml = __ballot(leaf) & hmask, ma = __ballot(acc) & hmask,
+                               md = __ballot(desc) & hmask;
                 if (leaf) {
                     const int at = n_near + __popcll(ml & below);
                     if (at < kLocalNear) near_out[at] = make_int2(lk.z, lk.w);
@@ -2776,25 +2786,24 @@ __global__ __launch_bounds__(64) void local_walk_kernel(EvalView v, const int *_
                 }
                 if (desc) {
                     const int at = nfs + 2 * __popcll(md & below);
-                    if (at + 1 < kLocalFront) {
-                        s_front[cur ^ 1][at] = node + 1;
-                        s_front[cur ^ 1][at + 1] = lk.x;
+                    if (at + 1 < kWalkFront) {
+                        s_front[h][cur ^ 1][at] = node + 1;
+                        s_front[h][cur ^ 1][at + 1] = lk.x;
                     }
                 }
                 n_near += __popcll(ml);
                 n_acc += __popcll(ma);
                 nfs += 2 * __popcll(md);
             }
-            if (n_near > kLocalNear || n_acc > kLocalPanels || nfs > kLocalFront) local_ok = false;
+            if (act && (n_near > kLocalNear || n_acc > kLocalPanels || nfs > kWalkFront)) ok = false;
             __syncthreads();
-            cur ^= 1;
-            fs = nfs;
+            if (act) {
+                cur ^= 1;
+                fs = nfs;
+            }
         }
-        if (lane == 0) {
-            // panels, then leaves (the near count in the upper half)
-            acc_n[e] = local_ok ? (n_acc | (n_near << 16)) : -1;
-            if (!local_ok) fb_list[atomicAdd(count + 1, 1)] = T;
-        }
+        // panels, then leaves (the near count in the upper half); -1: overflow
+        if (has && hl == 0) acc_n[e] = ok ? (n_acc | (n_near << 16)) : -1;
     }
 }
 
@@ -2828,18 +2837,92 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void lo
     const double eta2 = eta * eta;
     for (int e = blockIdx.x; e < total; e += gridDim.x) {
         const int code = acc_n[e];
-        if (code < 0) continue;  // the fallback kernel evaluates it
         const bool wide = e < n_wide;
         const int T = wide ? wide_list[e] : list[e - n_wide];
+        if (code < 0) {
+            // the walk's lists overflowed: the reference's per-query walk
+            const int4 lkT = v.link[T];
+            for (int i = lkT.z + lane; i < lkT.w; i += 64) {
+                double re, im;
+                walk_one_f64(v, v.sorted[i], eta, eta2, re, im);
+                const int o = v.perm ? v.perm[i] : i;
+                out_re[o] = re * scale;
+                out_im[o] = im * scale;
+            }
+            continue;
+        }
         const int n_acc = code & 0xffff, n_near = code >> 16;
         if (lane < n_acc) s_acc[lane] = acc_buf[(size_t)e * kLocalPanels + lane];
         if (lane < n_near) s_near[lane] = near_buf[(size_t)e * kLocalNear + lane];
         const int4 lkT = v.link[T];
         const int q0 = lkT.z, q1 = lkT.w;
         __syncthreads();
-        if (wide) {
+        if (wide && q1 - q0 <= 64) {
             // a leaf wider than the target width: the accepted panels at each
-            // query (M2P), the walk shared
+            // query (M2P), the walk shared; one query per lane, the panels
+            // and then the reached leaves' sources staged in LDS as below
+            const int i = q0 + lane;
+            const bool q_ok = i < q1;
+            const double x = q_ok ? v.sorted[i] : 0.0;
+            double fr = 0.0, fi = 0.0;
+            for (int a0 = 0; a0 < n_acc; a0 += kLocalStage) {
+                const int cnt = min(kLocalStage, n_acc - a0);
+                for (int e2 = lane; e2 < cnt * NT; e2 += 64) {
+                    const int b = e2 / NT, j = e2 - b * NT;
+                    const size_t o = (size_t)s_acc[a0 + b] * NT + j;
+                    s_pt[e2] = v.nodes[o];
+                    s_pw[e2] = v.w[o];
+                }
+                __syncthreads();
+                if (q_ok) {
+                    for (int b = 0; b < cnt; ++b) {
+#pragma unroll
+                        for (int j = 0; j < NT; ++j) {
+                            const double d = x - s_pt[b * NT + j];
+                            const double q = s_pw[b * NT + j] * rcp_f64(__builtin_fma(d, d, eta2));
+                            fr = __builtin_fma(q, d, fr);
+                            fi += q;
+                        }
+                    }
+                }
+                __syncthreads();  // the next chunk overwrites the stage
+            }
+            int n_ns = 0;
+            for (int m = 0; m < n_near; ++m) n_ns += s_near[m].y - s_near[m].x;
+            if (n_ns <= kNearStage) {
+                int off = 0;
+                for (int m = 0; m < n_near; ++m) {
+                    const int2 lf = s_near[m];
+                    for (int jj = lane; jj < lf.y - lf.x; jj += 64) s_ns[off + jj] = v.sorted[lf.x + jj];
+                    off += lf.y - lf.x;
+                }
+                __syncthreads();
+                if (q_ok)
+                    for (int j = 0; j < n_ns; ++j) {
+                        const double d = x - s_ns[j];
+                        const double inv = rcp_f64(__builtin_fma(d, d, eta2));
+                        fr = __builtin_fma(d, inv, fr);
+                        fi += inv;
+                    }
+            } else if (q_ok) {
+                for (int m = 0; m < n_near; ++m) {
+                    const int2 lf = s_near[m];
+                    for (int j = lf.x; j < lf.y; ++j) {
+                        const double d = x - v.sorted[j];
+                        const double inv = rcp_f64(__builtin_fma(d, d, eta2));
+                        fr = __builtin_fma(d, inv, fr);
+                        fi += inv;
+                    }
+                }
+            }
+            if (q_ok) {
+                const int o = v.perm ? v.perm[i] : i;
+                out_re[o] = fr * scale;
+                out_im[o] = (eta * fi) * scale;
+            }
+        } else if (wide) {
+            // the same with more than 64 queries (a leaf at the depth cap):
+            // the panels read from global memory per query
             for (int c0 = q0; c0 < q1; c0 += 64) {
                 const int i = c0 + lane;
                 if (i >= q1) continue;
@@ -2991,29 +3074,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) void lo
             }
         }
         __syncthreads();  // LDS is reused by the next target
-    }
-}
-
-// The targets that keep the reference's per-query walk (leaves wider than
-// the target width, targets whose near list overflowed): one wavefront per
-// target, one query per lane.
-__global__ __launch_bounds__(64) void local_fallback_kernel(EvalView v,
-                                                            const int *__restrict__ fb_list,
-                                                            const int *__restrict__ count,
-                                                            double eta, double scale,
-                                                            double *__restrict__ out_re,
-                                                            double *__restrict__ out_im) {
-    const int total = count[1];
-    const double eta2 = eta * eta;
-    for (int e = blockIdx.x; e < total; e += gridDim.x) {
-        const int4 lkT = v.link[fb_list[e]];
-        for (int i = lkT.z + (int)threadIdx.x; i < lkT.w; i += 64) {
-            double re, im;
-            walk_one_f64(v, v.sorted[i], eta, eta2, re, im);
-            const int o = v.perm ? v.perm[i] : i;
-            out_re[o] = re * scale;
-            out_im[o] = im * scale;
-        }
     }
 }
 
@@ -3557,13 +3617,13 @@ int cheb_eval_local(const ChebTree *t, double eta, double scale, double *d_out_r
     if (int st = wait_tree(t, stream)) return st;
     EvalView v = view_of(t);
     const int N = (int)t->N;
-    // counts (targets, fallbacks, wide leaves), the three lists, and per
+    // counts (targets, unused, wide leaves), the two target lists, and per
     // target (at most N) its list counts, panels and reached leaves
     int *buf = nullptr;
     CachedBuf cache(CACHE_CHEB_LOCAL, stream);
     const size_t per_target = 1 + kLocalPanels + 2 * kLocalNear;
-    if (int e = cache.get(4 + (3 + per_target) * (size_t)N, &buf)) return e;
-    int *count = buf, *list = buf + 4, *fb_list = list + N, *wide_list = fb_list + N;
+    if (int e = cache.get(4 + (2 + per_target) * (size_t)N, &buf)) return e;
+    int *count = buf, *list = buf + 4, *wide_list = list + N;
     int *acc_n = wide_list + N, *acc_buf = acc_n + N;
     int2 *near_buf = reinterpret_cast<int2 *>(acc_buf + (size_t)N * kLocalPanels);
     hipLaunchKernelGGL(local_zero_kernel, dim3(1), dim3(64), 0, stream, count);
@@ -3572,8 +3632,8 @@ int cheb_eval_local(const ChebTree *t, double eta, double scale, double *d_out_r
     SHK_HIP(hipGetLastError());
     const unsigned grid = (unsigned)std::min(N, 32768);
     ProfileMark m = profile_begin(stream);
-    hipLaunchKernelGGL(local_walk_kernel, dim3(grid), dim3(64), 0, stream, v, list, wide_list,
-                       count, fb_list, eta, acc_n, acc_buf, near_buf);
+    hipLaunchKernelGGL(local_walk_kernel, dim3((grid + 1) / 2), dim3(64), 0, stream, v, list,
+                       wide_list, count, eta, acc_n, acc_buf, near_buf);
     SHK_HIP(hipGetLastError());
     if (t->n == 11)
         hipLaunchKernelGGL(local_f64_kernel<11>, dim3(grid), dim3(64), 0, stream, v, list,
@@ -3584,9 +3644,6 @@ int cheb_eval_local(const ChebTree *t, double eta, double scale, double *d_out_r
                            wide_list, count, eta, scale, acc_n, acc_buf, near_buf, d_out_re,
                            d_out_im);
     profile_end(m, stream);
-    SHK_HIP(hipGetLastError());
-    hipLaunchKernelGGL(local_fallback_kernel, dim3((unsigned)std::min(N, 8192)), dim3(64), 0,
-                       stream, v, fb_list, count, eta, scale, d_out_re, d_out_im);
     SHK_HIP(hipGetLastError());
     return 0;
 }
