@@ -420,18 +420,29 @@ def test_local_expansion_within_reference_error(shk, preset, p, c):
     assert e_diff <= 2.0 * e_ref + 1e-13
 
 
-@pytest.mark.parametrize("case", ["unsorted", "small_eta", "duplicates", "tiny"])
+@pytest.mark.parametrize("case", ["unsorted", "small_eta", "duplicates", "tiny", "geometric"])
 def test_local_expansion_edge_cases(shk, case):
     """Unsorted input (scatter through the permutation); eta = 1e-3/sqrt(p),
-    where leaves are wider than eta/4 and most targets fall back to the
-    per-query walk; heavy duplicates (zero-width targets: the interpolant is
-    the constant at the node); p = 1, 2, 3 (the root is the target)."""
+    where leaves are wider than eta/4 (wide targets, M2P at each query);
+    heavy duplicates (zero-width targets: the interpolant is the constant at
+    the node); p = 1, 2, 3 (the root is the target); a geometric cluster
+    1 + 2^-k (k < 52) beside a bulk at eta = 1e-9, where the tree is a chain
+    of one-value leaves and the walks of the targets near 1 reach more than
+    16 leaves: they overflow their lists and take the per-query walk (five
+    targets in a CPU model of the walk on the oracle tree)."""
     from shrinkers_amd import spectra as S
     theta, n, leaf, _ = shk.CHEB_PRESETS["chebcode"]
     if case == "tiny":
         for ev in (np.array([1.5]), np.array([0.5, 0.9]), np.array([2.0, 0.1, 0.7])):
             e_loc, e_ref, _ = _local_vs_reference(shk, ev, 0.3, theta, n, leaf)
             assert e_loc <= 1e-13, ev
+        return
+    if case == "geometric":
+        ev = np.concatenate([1.0 + 0.5 ** np.arange(52), np.linspace(2.5, 3.5, 3000)])
+        e_loc, e_ref, e_diff = _local_vs_reference(shk, ev, 1e-9, theta, n, leaf)
+        print(f"local geometric: {e_loc:.3e} (reference {e_ref:.3e}, difference {e_diff:.3e})",
+              flush=True)
+        assert e_loc <= 1.05 * e_ref + 1e-13
         return
     p = 30_000
     if case == "unsorted":
