@@ -1689,7 +1689,7 @@ __device__ __forceinline__ void merge_level(int n, int lb, int cnt, int wv, int 
 constexpr int kMergeMaxThreads = 1024;
 constexpr int kMergeSubWaves = 8;   // wavefronts per subtree workgroup
 constexpr size_t kSubLdsMax = 81920;  // LDS of a subtree workgroup (two per CU)
-constexpr int kMergeTopRounds = 2;  // a top level of at most this many groups per wavefront
+constexpr int kMergeTopRounds = 1;  // a top level of at most this many groups per wavefront
                                     // merges in the one-workgroup kernel
 
 // One top level (slots [lb, lb + cnt)), one wavefront per group of nodes.
