@@ -1846,60 +1846,59 @@ __global__ __launch_bounds__(kTopLdsWaves * kBaryLanes) void merge_top_lds_kerne
         s_w[e] = t.w[o];  // finished weights; merged nodes overwrite theirs
     }
     lds_barrier();
-    // levels [0, dq) hold the first lvl[dq] <= kQNarrow slots
-    int dq = 0;
-    while (dq < levels && lvl[dq + 1] <= kQNarrow) ++dq;
-    for (int d = levels - 1; d >= dq; --d) {
-        const int lb = lvl[d], cnt = lvl[d + 1] - lvl[d];
-        for (int c = wv * G; c < cnt; c += waves * G)
-            merge_group_lds<NT>(lb, c, cnt, s_left, s_k, s_pos, s_w, lr, t, panel32,
-                                bary + (size_t)wv * bary_lds_doubles(NT));
-        lds_barrier();
-    }
-    if (dq == 0) return;
-    // quotient tables of slots [0, P), in the per-wavefront scratch (free now):
-    // q at ((u * 2n + r) * n + j), 1 / s_r and the hit at (u * 2n + r)
-    constexpr int R = 2 * NT;
-    static_assert(kQNarrow * (R * NT + R + R / 2 + 1) <=
-                      kTopLdsWaves * (kBaryLanes * NT + kBaryLanes + kBaryLanes / 2),
-                  "the narrow quotient tables fit the merge scratch");
-    const int P = lvl[dq];
-    double *tq = bary, *ti = tq + (size_t)P * R * NT;
-    int *th = reinterpret_cast<int *>(ti + (size_t)P * R);
-    for (int e = threadIdx.x; e < P * R; e += blockDim.x) {
-        const int u = e / R, r = e - u * R;
-        const int left = s_left[u];
-        if (left < 0) continue;
-        const double x = s_pos[(left + (r < NT ? 0 : 1)) * NT + (r < NT ? r : r - NT)];
-        double sum = 0.0;
-        int hit = -1;
+    // Quotient tables in the per-wavefront scratch: for the rows of slots
+    // [u0, u0 + P), q at (((u - u0) * 2n + r) * n + j), 1 / s_r and the hit
+    // at ((u - u0) * 2n + r); one row per thread.  A level's nodes then each
+    // take n lanes (lane j owns w_j), kBaryLanes / n nodes per wavefront.
+    constexpr int R = 2 * NT, PG = kBaryLanes / NT;
+    constexpr int kScratch = kTopLdsWaves * (kBaryLanes * NT + kBaryLanes + kBaryLanes / 2);
+    // P parents' tables: P * 2n rows of n quotients, 1 / s_r (double), hit (int)
+    static_assert(kQNarrow * R * NT + kQNarrow * R + (kQNarrow * R + 1) / 2 <= kScratch,
+                  "the narrow levels' quotient tables fit the merge scratch");
+    static_assert(kTopLdsWaves * G * R * NT + kTopLdsWaves * G * R +
+                          (kTopLdsWaves * G * R + 1) / 2 <=
+                      kScratch,
+                  "one round of a level's quotient tables fits the merge scratch");
+    static_assert(kTopLdsWaves * G <= kTopLdsWaves * PG, "a round of nodes takes one pass");
+    static_assert(kMergeTopRounds == 1, "a level's quotient tables hold one round of nodes");
+    double *tq = bary, *ti = nullptr;
+    int *th = nullptr;
+    auto tables = [&](int u0, int P) {
+        ti = tq + (size_t)P * R * NT;
+        th = reinterpret_cast<int *>(ti + (size_t)P * R);
+        for (int e = threadIdx.x; e < P * R; e += blockDim.x) {
+            const int u = u0 + e / R, r = e % R;
+            const int left = s_left[u];
+            if (left < 0) continue;
+            const double x = s_pos[(left + (r < NT ? 0 : 1)) * NT + (r < NT ? r : r - NT)];
+            double sum = 0.0;
+            int hit = -1;
 #pragma unroll
-        for (int j = 0; j < NT; ++j) {
-            const double d = x - s_pos[u * NT + j];
-            const double q = lr[j] / d;
-            tq[(size_t)e * NT + j] = q;
-            sum += q;
-            hit = (hit < 0 && d == 0.0) ? j : hit;
+            for (int j = 0; j < NT; ++j) {
+                const double d = x - s_pos[u * NT + j];
+                const double q = lr[j] / d;
+                tq[(size_t)e * NT + j] = q;
+                sum += q;
+                hit = (hit < 0 && d == 0.0) ? j : hit;
+            }
+            ti[e] = 1.0 / sum;
+            th[e] = hit;
         }
-        ti[e] = 1.0 / sum;
-        th[e] = hit;
-    }
-    lds_barrier();
-    // n lanes per node (lane j owns w_j), kBaryLanes / n nodes per wavefront
-    constexpr int PG = kBaryLanes / NT;
+        lds_barrier();
+    };
     const int lane = threadIdx.x & (kBaryLanes - 1), pg = lane / NT, j = lane - pg * NT;
-    for (int d = dq - 1; d >= 0; --d) {
-        const int lb = lvl[d], cnt = lvl[d + 1] - lvl[d];
-        const int i = wv * PG + pg;  // at most kQNarrow nodes: one pass
+    auto chains = [&](int lb, int cnt, int u0) {
+        const int i = wv * PG + pg;
         const int u = lb + i;
         const int left = (pg < PG && i < cnt) ? s_left[u] : -1;
         if (left >= 0) {
+            const int e0 = (u - u0) * R;
             double acc = 0.0;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const double mass = s_w[(left + (r < NT ? 0 : 1)) * NT + (r < NT ? r : r - NT)];
-                const int h = th[u * R + r];
-                const double f = __builtin_fma(mass * ti[u * R + r], tq[((size_t)u * R + r) * NT + j],
+                const int h = th[e0 + r];
+                const double f = __builtin_fma(mass * ti[e0 + r], tq[(size_t)(e0 + r) * NT + j],
                                                acc),
                              a = acc + mass;
                 acc = (mass == 0.0) ? acc : ((h == -1) ? f : ((h == j) ? a : acc));
@@ -1910,7 +1909,19 @@ __global__ __launch_bounds__(kTopLdsWaves * kBaryLanes) void merge_top_lds_kerne
             panel32[(size_t)k * 2 * NT + NT + j] = (float)acc;
         }
         lds_barrier();
+    };
+    // levels [0, dq) hold the first lvl[dq] <= kQNarrow slots: their tables
+    // are built at once; each wider level builds its own
+    int dq = 0;
+    while (dq < levels && lvl[dq + 1] <= kQNarrow) ++dq;
+    for (int d = levels - 1; d >= dq; --d) {
+        const int lb = lvl[d], cnt = lvl[d + 1] - lvl[d];
+        tables(lb, cnt);
+        chains(lb, cnt, lb);
     }
+    if (dq == 0) return;
+    tables(0, lvl[dq]);
+    for (int d = dq - 1; d >= 0; --d) chains(lvl[d], lvl[d + 1] - lvl[d], 0);
 }
 
 // LDS form of the subtree merges for the presets' orders: subtree f (its root
