@@ -1737,23 +1737,115 @@ __global__ __launch_bounds__(kMergeMaxThreads) void merge_top_kernel(
     }
 }
 
+// Quotient tables of a merge level (used by the LDS forms of the merges).  A
+// merge evaluates, for each of its 2n rows (a child's Chebyshev point x_r,
+// with that child's weight as mass), the quotients q_rj = lam_j / (x_r - t_j)
+// against the node's positions t_j, their sum s_r and the first exact hit,
+// then adds the rows into the weights in row order: w_j = fma(mass_r *
+// (1 / s_r), q_rj, w_j), or w_hit += mass_r, skipping rows of zero mass
+// (merge_group).  Only the row sums depend on the level below.  So the whole
+// workgroup first builds the tables of P nodes (slots [u0, u0 + P), one row
+// per thread): q at ((u - u0) * 2n + r) * n + j, 1 / s_r and the hit at
+// (u - u0) * 2n + r.  Each node's weights are then the 2n-long fused
+// multiply-add chain alone, n lanes per node (lane j owns w_j).  The
+// operations and their order are merge_group's: the weights are the same
+// bit for bit.
+__host__ __device__ constexpr int quot_table_doubles(int P, int n) {
+    return P * 2 * n * n + P * 2 * n + (P * 2 * n + 1) / 2;
+}
+// Nodes whose tables fit `doubles` and whose chains take one pass of `waves`.
+__host__ __device__ constexpr int quot_table_cap(int doubles, int n, int waves) {
+    int P = 1;
+    while (quot_table_doubles(P + 1, n) <= doubles && P + 1 <= waves * (kBaryLanes / n)) ++P;
+    return P;
+}
+
+template <int NT>
+struct QuotTables {
+    double *q, *inv;
+    int *hit;
+    int u0;
+};
+
+template <int NT>
+__device__ __forceinline__ QuotTables<NT> quot_tables(double *scratch, int u0, int P,
+                                                      const int *s_left, const double *s_pos,
+                                                      const double (&lr)[NT]) {
+    constexpr int R = 2 * NT;
+    QuotTables<NT> tb{scratch, scratch + (size_t)P * R * NT, nullptr, u0};
+    tb.hit = reinterpret_cast<int *>(tb.inv + (size_t)P * R);
+    for (int e = threadIdx.x; e < P * R; e += blockDim.x) {
+        const int u = u0 + e / R, r = e % R;
+        const int left = s_left[u];
+        if (left < 0) continue;
+        const double x = s_pos[(left + (r < NT ? 0 : 1)) * NT + (r < NT ? r : r - NT)];
+        double sum = 0.0;
+        int hit = -1;
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+            const double d = x - s_pos[u * NT + j];
+            const double q = lr[j] / d;
+            tb.q[(size_t)e * NT + j] = q;
+            sum += q;
+            hit = (hit < 0 && d == 0.0) ? j : hit;
+        }
+        tb.inv[e] = 1.0 / sum;
+        tb.hit[e] = hit;
+    }
+    return tb;
+}
+
+// The row-sum chains of the nodes in slots [lb, lb + cnt) (tables built),
+// one pass: cnt <= wavefronts * (kBaryLanes / n).
+template <int NT>
+__device__ __forceinline__ void quot_chains(const QuotTables<NT> &tb, int lb, int cnt,
+                                            const int *s_left, const int *s_k, double *s_w,
+                                            const TreeArrays &t, float *__restrict__ panel32) {
+    constexpr int R = 2 * NT, PG = kBaryLanes / NT;
+    const int lane = threadIdx.x & (kBaryLanes - 1), pg = lane / NT, j = lane - pg * NT;
+    const int i = (int)(threadIdx.x / kBaryLanes) * PG + pg;
+    const int u = lb + i;
+    const int left = (pg < PG && i < cnt) ? s_left[u] : -1;
+    if (left < 0) return;
+    const int e0 = (u - tb.u0) * R;
+    double acc = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const double mass = s_w[(left + (r < NT ? 0 : 1)) * NT + (r < NT ? r : r - NT)];
+        const int h = tb.hit[e0 + r];
+        const double f = __builtin_fma(mass * tb.inv[e0 + r], tb.q[(size_t)(e0 + r) * NT + j], acc),
+                     a = acc + mass;
+        acc = (mass == 0.0) ? acc : ((h == -1) ? f : ((h == j) ? a : acc));
+    }
+    s_w[u * NT + j] = acc;
+    const int k = s_k[u];
+    t.w[(size_t)k * NT + j] = acc;
+    panel32[(size_t)k * 2 * NT + NT + j] = (float)acc;
+}
+
+// One level (slots [lb, lb + cnt)), pcap nodes per round of tables and chains.
+template <int NT>
+__device__ __forceinline__ void merge_level_quot(int lb, int cnt, int pcap, const int *s_left,
+                                                 const int *s_k, const double *s_pos, double *s_w,
+                                                 const double (&lr)[NT], const TreeArrays &t,
+                                                 float *__restrict__ panel32, double *scratch) {
+    for (int c0 = 0; c0 < cnt; c0 += pcap) {
+        const int P = min(pcap, cnt - c0);
+        const QuotTables<NT> tb = quot_tables<NT>(scratch, lb + c0, P, s_left, s_pos, lr);
+        lds_barrier();
+        quot_chains<NT>(tb, lb + c0, P, s_left, s_k, s_w, t, panel32);
+        lds_barrier();  // the next round overwrites the tables
+    }
+}
+
 // LDS form of the top merges for the presets' orders.  The slots of levels
 // [0, levels] (level `levels` holds finished children) are staged in LDS once
 // with their left-child slots, pre-order numbers, positions and weights;
 // each level then reads and writes LDS only (weights also go to the tree and
-// the f32 panel) behind an LDS-only barrier.  Per node the arithmetic and
-// its order are merge_group's, so the weights are the same bit for bit.
-//
-// A merge evaluates, for each of its 2n rows (a child's Chebyshev point x_r,
-// with that child's weight as mass), the quotients q_rj = lam_j / (x_r - t_j)
-// against the node's positions t_j, their sum s_r and the first exact hit,
-// then adds the rows into the weights in row order: w_j = fma(mass_r *
-// (1 / s_r), q_rj, w_j), or w_hit += mass_r, skipping rows of zero mass.
-// Only the row sums depend on the level below.  The narrow top levels (the
-// first kQNarrow slots, one or a few nodes per level, where the levels are a
-// chain of latencies on one CU) therefore get their quotients, 1 / s_r and
-// hits as tables built by all wavefronts at once, and each of their levels
-// is then the 2n-long fused multiply-add chain alone.
+// the f32 panel) behind an LDS-only barrier, merged through quotient tables
+// (merge_level_quot).  The narrow top levels (the first kQNarrow slots, one
+// or a few nodes per level, where the levels are a chain of latencies on one
+// CU) get their tables in one pass.
 constexpr int kTopLdsWaves = 16;          // the top-level merge workgroup
 constexpr size_t kTopLdsWgMax = 163840;   // LDS of the one top-level merge workgroup
 constexpr int kQNarrow = 40;              // top slots with quotient tables
@@ -1821,10 +1913,9 @@ __global__ __launch_bounds__(kTopLdsWaves * kBaryLanes) void merge_top_lds_kerne
     const int2 *__restrict__ mrec, const double *__restrict__ lam, TreeArrays t,
     float *__restrict__ panel32) {
     extern __shared__ double lds[];
-    constexpr int G = kBaryLanes / (2 * NT);
     const int *lvl = info + kInfoTopLevel;
     const int S = lvl[levels + 1], fin = lvl[levels];
-    const int waves = blockDim.x / kBaryLanes, wv = threadIdx.x / kBaryLanes;
+    const int waves = blockDim.x / kBaryLanes;
     double *s_pos = lds;
     double *s_w = s_pos + (size_t)S * NT;
     double *bary = s_w + (size_t)S * NT;
@@ -1846,82 +1937,24 @@ __global__ __launch_bounds__(kTopLdsWaves * kBaryLanes) void merge_top_lds_kerne
         s_w[e] = t.w[o];  // finished weights; merged nodes overwrite theirs
     }
     lds_barrier();
-    // Quotient tables in the per-wavefront scratch: for the rows of slots
-    // [u0, u0 + P), q at (((u - u0) * 2n + r) * n + j), 1 / s_r and the hit
-    // at ((u - u0) * 2n + r); one row per thread.  A level's nodes then each
-    // take n lanes (lane j owns w_j), kBaryLanes / n nodes per wavefront.
-    constexpr int R = 2 * NT, PG = kBaryLanes / NT;
     constexpr int kScratch = kTopLdsWaves * (kBaryLanes * NT + kBaryLanes + kBaryLanes / 2);
-    // P parents' tables: P * 2n rows of n quotients, 1 / s_r (double), hit (int)
-    static_assert(kQNarrow * R * NT + kQNarrow * R + (kQNarrow * R + 1) / 2 <= kScratch,
+    static_assert(quot_table_doubles(kQNarrow, NT) <= kScratch,
                   "the narrow levels' quotient tables fit the merge scratch");
-    static_assert(kTopLdsWaves * G * R * NT + kTopLdsWaves * G * R +
-                          (kTopLdsWaves * G * R + 1) / 2 <=
-                      kScratch,
-                  "one round of a level's quotient tables fits the merge scratch");
-    static_assert(kTopLdsWaves * G <= kTopLdsWaves * PG, "a round of nodes takes one pass");
-    static_assert(kMergeTopRounds == 1, "a level's quotient tables hold one round of nodes");
-    double *tq = bary, *ti = nullptr;
-    int *th = nullptr;
-    auto tables = [&](int u0, int P) {
-        ti = tq + (size_t)P * R * NT;
-        th = reinterpret_cast<int *>(ti + (size_t)P * R);
-        for (int e = threadIdx.x; e < P * R; e += blockDim.x) {
-            const int u = u0 + e / R, r = e % R;
-            const int left = s_left[u];
-            if (left < 0) continue;
-            const double x = s_pos[(left + (r < NT ? 0 : 1)) * NT + (r < NT ? r : r - NT)];
-            double sum = 0.0;
-            int hit = -1;
-#pragma unroll
-            for (int j = 0; j < NT; ++j) {
-                const double d = x - s_pos[u * NT + j];
-                const double q = lr[j] / d;
-                tq[(size_t)e * NT + j] = q;
-                sum += q;
-                hit = (hit < 0 && d == 0.0) ? j : hit;
-            }
-            ti[e] = 1.0 / sum;
-            th[e] = hit;
-        }
-        lds_barrier();
-    };
-    const int lane = threadIdx.x & (kBaryLanes - 1), pg = lane / NT, j = lane - pg * NT;
-    auto chains = [&](int lb, int cnt, int u0) {
-        const int i = wv * PG + pg;
-        const int u = lb + i;
-        const int left = (pg < PG && i < cnt) ? s_left[u] : -1;
-        if (left >= 0) {
-            const int e0 = (u - u0) * R;
-            double acc = 0.0;
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const double mass = s_w[(left + (r < NT ? 0 : 1)) * NT + (r < NT ? r : r - NT)];
-                const int h = th[e0 + r];
-                const double f = __builtin_fma(mass * ti[e0 + r], tq[(size_t)(e0 + r) * NT + j],
-                                               acc),
-                             a = acc + mass;
-                acc = (mass == 0.0) ? acc : ((h == -1) ? f : ((h == j) ? a : acc));
-            }
-            s_w[u * NT + j] = acc;
-            const int k = s_k[u];
-            t.w[(size_t)k * NT + j] = acc;
-            panel32[(size_t)k * 2 * NT + NT + j] = (float)acc;
-        }
-        lds_barrier();
-    };
+    constexpr int pcap = quot_table_cap(kScratch, NT, kTopLdsWaves);
     // levels [0, dq) hold the first lvl[dq] <= kQNarrow slots: their tables
-    // are built at once; each wider level builds its own
+    // are built at once; each wider level builds its own, pcap nodes at a time
     int dq = 0;
     while (dq < levels && lvl[dq + 1] <= kQNarrow) ++dq;
-    for (int d = levels - 1; d >= dq; --d) {
-        const int lb = lvl[d], cnt = lvl[d + 1] - lvl[d];
-        tables(lb, cnt);
-        chains(lb, cnt, lb);
-    }
+    for (int d = levels - 1; d >= dq; --d)
+        merge_level_quot<NT>(lvl[d], lvl[d + 1] - lvl[d], pcap, s_left, s_k, s_pos, s_w, lr, t,
+                             panel32, bary);
     if (dq == 0) return;
-    tables(0, lvl[dq]);
-    for (int d = dq - 1; d >= 0; --d) chains(lvl[d], lvl[d + 1] - lvl[d], 0);
+    const QuotTables<NT> tb = quot_tables<NT>(bary, 0, lvl[dq], s_left, s_pos, lr);
+    lds_barrier();
+    for (int d = dq - 1; d >= 0; --d) {
+        quot_chains<NT>(tb, lvl[d], lvl[d + 1] - lvl[d], s_left, s_k, s_w, t, panel32);
+        lds_barrier();
+    }
 }
 
 // LDS form of the subtree merges for the presets' orders: subtree f (its root
@@ -1974,6 +2007,9 @@ __global__ __launch_bounds__(kMergeSubWaves * kBaryLanes) void merge_sub_lds_ker
         s_w[e] = t.w[o];  // leaf weights; merged nodes overwrite theirs
     }
     lds_barrier();
+    // Levels here are wide enough that one group's divisions and column
+    // sums, wave-synchronous, beat quotient tables built between workgroup
+    // barriers (36.0 against 31.6 us at 1e6, FAST).
     for (int l = n_lv - 2; l >= 0; --l) {
         const int lb = (l == 0) ? 0 : lv_start[f * kLv + l] - base1 + 1;
         const int cnt = lv_cnt[f * kLv + l];
