@@ -44,7 +44,7 @@ def test_tree_identical_to_oracle(shk, orc, preset, p):
 
 @pytest.mark.parametrize("preset", ["chebcode_fast", "chebcode"])
 @pytest.mark.parametrize("case", ["mp200k_c05", "mp1m_c01", "geometric_depth_cap", "duplicates",
-                                  "clusters"])
+                                  "clusters", "mp300k_tiny", "mp300k_huge", "staged_outlier"])
 def test_tree_topology_large_and_degenerate(shk, orc, preset, case):
     """Topology, intervals and positions equal the oracle's tree on large MP
     spectra, on a geometric spectrum whose midpoint splits peel one value per
@@ -53,10 +53,25 @@ def test_tree_topology_large_and_degenerate(shk, orc, preset, case):
     clusters of geometric runs: 199 long gaps (more than the split hint's gap
     table holds), a tree that ends inside the breadth-first top walk with
     ~9 600 top slots (the top sizes and pre-order numbers outside LDS), and
-    merge levels past the LDS budget."""
+    merge levels past the LDS budget.  The subtree walks that search staged
+    sources through a local bucket index (subtrees of 256 to 5 120 sources)
+    are also run at extreme magnitudes (an MP spectrum scaled by 1e-300 and
+    by 1e300) and on subtrees whose value span is stretched by a close
+    outlier (most of a subtree's buckets empty)."""
     from shrinkers_amd import spectra as S
     if case == "mp200k_c05":
         ev = S.mp_inverse_cdf_spectrum(200_000, c=0.5)
+    elif case == "mp300k_tiny":
+        ev = 1e-300 * S.mp_inverse_cdf_spectrum(300_000, c=0.5)
+    elif case == "mp300k_huge":
+        ev = 1e300 * S.mp_inverse_cdf_spectrum(300_000, c=0.5)
+    elif case == "staged_outlier":
+        # 600 groups of 500 values packed within 1e-9, each with one outlier
+        # 0.004 above it: a frontier subtree holds a group and its outlier
+        rng = np.random.default_rng(5)
+        base = np.linspace(1.0, 7.0, 600)
+        ev = np.sort(np.concatenate([np.concatenate([b + 1e-9 * rng.random(500), [b + 0.004]])
+                                     for b in base]))
     elif case == "mp1m_c01":
         ev = S.mp_inverse_cdf_spectrum(1_000_000, c=0.1)
     elif case == "geometric_depth_cap":
@@ -72,8 +87,13 @@ def test_tree_topology_large_and_degenerate(shk, orc, preset, case):
     assert g["right"].size == o["right"].size
     for k in ("lo_idx", "hi_idx", "right", "lo", "hi", "nodes"):
         np.testing.assert_array_equal(g[k], o[k])
-    wscale = np.max(np.abs(o["w"]))
-    assert np.max(np.abs(g["w"] - o["w"])) <= 1e-13 * wscale
+    # at 1e-300 the reference's own barycentric sums overflow (quotients
+    # lam_j / (x - t_j) near 1e305): the weights must then fail at the same
+    # entries, and agree elsewhere
+    finite = np.isfinite(o["w"])
+    np.testing.assert_array_equal(np.isfinite(g["w"]), finite)
+    wscale = np.max(np.abs(o["w"][finite]))
+    assert np.max(np.abs(g["w"][finite] - o["w"][finite])) <= 1e-13 * wscale
 
 
 @pytest.mark.parametrize("p", [1000, 2000, 5000, 10000])
