@@ -2616,11 +2616,12 @@ __global__ __launch_bounds__(256) void eval_x2_kernel(EvalView v, int nq, double
 // No counterpart in the reference: its far field is the per-query panel sum
 // (M2P, chebcode.rs:607-711), and its notes leave a Chebyshev M2L to future
 // work (docs/internals.md:580-599).  Targets are the topmost nodes no wider
-// than kLocalWidth * eta, plus the leaves wider than that (which keep the
-// per-query walk).  One wavefront per target T walks the tree once, with the
-// reference's acceptance test (accept_f64_of) at T's gap to the panel, i.e.
-// at the query of T nearest to it: a panel accepted for T is accepted by the
-// reference for every query of T, so the M2P part is never less accurate.
+// than kLocalWidth * eta, plus the leaves wider than that (which share one
+// walk and sum the accepted panels at each query).  Each target T walks the
+// tree once, with the reference's acceptance test (accept_f64_of) at T's gap
+// to the panel, i.e. at the query of T nearest to it: a panel accepted for T
+// is accepted by the reference for every query of T, so the far field is
+// never coarser than the reference's.
 // Accepted panels are evaluated at T's n Chebyshev points y_k (M2L: n^2
 // terms per panel, shared by all of T's queries, instead of n per query);
 // leaves the walk reaches are listed and summed exactly per query, as the
@@ -2630,7 +2631,7 @@ __global__ __launch_bounds__(256) void eval_x2_kernel(EvalView v, int nq, double
 // field sum_j w_j / (x - t_j - i eta) is analytic within eta of the real
 // axis, so on an interval of width <= eta/4 its degree n-1 interpolant errs
 // by about rho^-n, rho = 8 + sqrt(65) (5e-14 at n = 11).  A target whose
-// near list overflows falls back to the per-query walk.
+// lists overflow gets the per-query walk.
 constexpr double kLocalWidth = 0.25;
 constexpr int kLocalNear = 16;
 
