@@ -72,6 +72,44 @@ __device__ __forceinline__ void sym_handover() {
 #endif
 }
 
+// One source against this lane's kSR queries: the FP32 i-side runs (ar, ai)
+// and, when JSIDE, the source's lane sums over the kSR pairs (bre, bim).
+// FIRST starts the runs with the batch's first source: a run that starts
+// from its first term equals one that starts from 0 and adds it.
+template <int FORM, bool JSIDE, bool FIRST>
+__device__ __forceinline__ void sym_source(double sv, double c, const float (&qh)[kSR],
+                                           const float (&ql)[kSR], float eta2f,
+                                           float (&ar)[kSR], float (&ai)[kSR], float &bre,
+                                           float &bim) {
+    const double t = sv - c;
+    const float sh = (float)t;
+    const float sl = (FORM == 2) ? (float)(t - (double)sh) : 0.0f;
+#pragma unroll
+    for (int k = 0; k < kSR; ++k) {
+        const float d = pair_diff<FORM>(qh[k], ql[k], sh, sl);
+        const float r = __builtin_amdgcn_rcpf(__builtin_fmaf(d, d, eta2f));
+        if (FIRST) {
+            ar[k] = d * r;
+            ai[k] = r;
+        } else {
+            ar[k] = __builtin_fmaf(d, r, ar[k]);
+            ai[k] += r;
+        }
+        if (JSIDE) {
+            if (k == 0) {
+                bre = d * r;
+                bim = r;
+            } else {
+                bre = __builtin_fmaf(d, r, bre);
+                bim += r;
+            }
+        }
+    }
+}
+
+// The wave's sources in batches of kSB.  n_src is wave-uniform (a scalar), so
+// a full batch runs with no per-source test; only the last tile's sources can
+// end inside a batch.
 template <int FORM, bool JSIDE>
 __device__ __forceinline__ void sym_sweep(const double *__restrict__ s_src, int n_src, double c,
                                           const float (&qh)[kSR], const float (&ql)[kSR],
@@ -79,37 +117,39 @@ __device__ __forceinline__ void sym_sweep(const double *__restrict__ s_src, int 
                                           float *__restrict__ s_jb, double *__restrict__ j_re,
                                           double *__restrict__ j_im, int lane) {
     for (int j0 = 0; j0 < kSQ; j0 += kSB) {  // same trip count in every wave
-        float ar[kSR], ai[kSR];
+        const int cnt = min(kSB, n_src - j0);
+        if (cnt > 0) {
+            float ar[kSR], ai[kSR], bre, bim;
+            sym_source<FORM, JSIDE, true>(s_src[j0], c, qh, ql, eta2f, ar, ai, bre, bim);
+            if (JSIDE) {
+                s_jb[0 * kJRow + lane] = bre;
+                s_jb[1 * kJRow + lane] = bim;
+            }
+            if (cnt == kSB) {
 #pragma unroll
-        for (int k = 0; k < kSR; ++k) ar[k] = ai[k] = 0.0f;
-        for (int u = 0; u < kSB; ++u) {
-            const int j = j0 + u;
-            if (j < n_src) {  // wave-uniform
-                const double t = s_src[j] - c;
-                const float sh = (float)t;
-                const float sl = (FORM == 2) ? (float)(t - (double)sh) : 0.0f;
-                float bre = 0.0f, bim = 0.0f;
-#pragma unroll
-                for (int k = 0; k < kSR; ++k) {
-                    const float d = pair_diff<FORM>(qh[k], ql[k], sh, sl);
-                    const float r = __builtin_amdgcn_rcpf(__builtin_fmaf(d, d, eta2f));
-                    ar[k] = __builtin_fmaf(d, r, ar[k]);
-                    ai[k] += r;
+                for (int u = 1; u < kSB; ++u) {
+                    sym_source<FORM, JSIDE, false>(s_src[j0 + u], c, qh, ql, eta2f, ar, ai, bre,
+                                                   bim);
                     if (JSIDE) {
-                        bre = __builtin_fmaf(d, r, bre);
-                        bim += r;
+                        s_jb[(2 * u + 0) * kJRow + lane] = bre;
+                        s_jb[(2 * u + 1) * kJRow + lane] = bim;
                     }
                 }
-                if (JSIDE) {
-                    s_jb[(2 * u + 0) * kJRow + lane] = bre;
-                    s_jb[(2 * u + 1) * kJRow + lane] = bim;
+            } else {
+                for (int u = 1; u < cnt; ++u) {
+                    sym_source<FORM, JSIDE, false>(s_src[j0 + u], c, qh, ql, eta2f, ar, ai, bre,
+                                                   bim);
+                    if (JSIDE) {
+                        s_jb[(2 * u + 0) * kJRow + lane] = bre;
+                        s_jb[(2 * u + 1) * kJRow + lane] = bim;
+                    }
                 }
             }
-        }
 #pragma unroll
-        for (int k = 0; k < kSR; ++k) {
-            re[k] += (double)ar[k];
-            iv[k] += (double)ai[k];
+            for (int k = 0; k < kSR; ++k) {
+                re[k] += (double)ar[k];
+                iv[k] += (double)ai[k];
+            }
         }
         if (JSIDE) {
             sym_handover();
@@ -121,16 +161,16 @@ __device__ __forceinline__ void sym_sweep(const double *__restrict__ s_src, int 
                 const int row_id = lane >> 1, h = lane & 1;
                 const int u = row_id >> 1, comp = row_id & 1;
                 const float *row = s_jb + (2 * u + comp) * kJRow + 32 * h;
-                float ca = 0.0f, cb = 0.0f;
+                float ca = row[0], cb = row[16];
 #pragma unroll
-                for (int l = 0; l < 16; ++l) {
+                for (int l = 1; l < 16; ++l) {
                     ca += row[l];
                     cb += row[16 + l];
                 }
                 const double mine = (double)ca + (double)cb;
                 const double other = __shfl_xor(mine, 1, 64);
                 const double tot = h ? other + mine : mine + other;
-                if (h == 0 && j0 + u < n_src) {
+                if (h == 0 && u < cnt) {
                     if (comp) j_im[j0 + u] = tot;
                     else j_re[j0 + u] = -tot;
                 }
@@ -190,7 +230,7 @@ __device__ __forceinline__ void sym_sweep_f64(const double *__restrict__ s_src, 
     for (int j0 = 0; j0 < kSQ; j0 += kSB64) {  // same trip count in every wave
         for (int u = 0; u < kSB64; ++u) {
             const int j = j0 + u;
-            if (j < n_src) {  // wave-uniform
+            if (j < n_src) {  // wave-uniform (a scalar test)
                 const double sv = s_src[j];
                 double d[kSR], t[kSR], r[kSR];
 #pragma unroll
@@ -199,9 +239,12 @@ __device__ __forceinline__ void sym_sweep_f64(const double *__restrict__ s_src, 
                     t[k] = __builtin_fma(d[k], d[k], e2);
                 }
                 rcp_pairs_f64(t, r);
-                double bre = 0.0, bim = 0.0;
+                // the lane sums start from the first pair's term
+                double bre = d[0] * r[0], bim = r[0];
+                re[0] = __builtin_fma(d[0], r[0], re[0]);
+                iv[0] += r[0];
 #pragma unroll
-                for (int k = 0; k < kSR; ++k) {
+                for (int k = 1; k < kSR; ++k) {
                     re[k] = __builtin_fma(d[k], r[k], re[k]);
                     iv[k] += r[k];
                     if (JSIDE) {
@@ -223,9 +266,9 @@ __device__ __forceinline__ void sym_sweep_f64(const double *__restrict__ s_src, 
                 const int row_id = lane >> 2, q = lane & 3;
                 const int u = row_id >> 1, comp = row_id & 1;
                 const double *row = s_jb + (2 * u + comp) * kJRow + 16 * q;
-                double cq = 0.0;
+                double cq = row[0];
 #pragma unroll
-                for (int l = 0; l < 16; ++l) cq += row[l];
+                for (int l = 1; l < 16; ++l) cq += row[l];
                 const double o1 = __shfl_xor(cq, 1, 64);
                 const double pair = (q & 1) ? o1 + cq : cq + o1;   // c0+c1 or c2+c3
                 const double o2 = __shfl_xor(pair, 2, 64);
@@ -246,7 +289,9 @@ __global__ __launch_bounds__(kSL * kSW) void sym_f64_kernel(const double *__rest
     tile_pair_of((int)blockIdx.x + id0, bd.nt, a, b);
     __shared__ double s_src[kSW][kSQ];
     __shared__ double s_jbuf[kSW * 2 * kSB64 * kJRow];  // j-side lane sums / i-side combine
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // w is wave-uniform; in a scalar register the source counts and batch
+    // tests below stay scalar
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int q0 = a * kST + lane * kSR;
     double x[kSR], re[kSR], iv[kSR];
 #pragma unroll
@@ -278,7 +323,9 @@ __global__ __launch_bounds__(kSL * kSW) void sym_f32c_kernel(const double *__res
     tile_pair_of((int)blockIdx.x + id0, bd.nt, a, b);
     __shared__ double s_src[kSW][kSQ];                      // 8 KB
     __shared__ double s_jbuf[kSW * 2 * kSB * kJRow / 2];    // j-side lane sums / i-side combine
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // w is wave-uniform; in a scalar register the source counts and batch
+    // tests below stay scalar
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = threadIdx.x & 63;
     const bool jside = a < b;
 
     // queries: kSR contiguous per lane; only the last tile can be partial, and

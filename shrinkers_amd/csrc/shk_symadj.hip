@@ -75,6 +75,63 @@ __device__ __forceinline__ void jside_store(double Q, double Ap, double Bp, doub
     *eta_t = g_i * Ap + (two_eta * g_r) * Bp;
 }
 
+// One source against this lane's kJR queries: the FP32 i-side runs (rp, ra,
+// rb) and, when JSIDE, the source's lane sums (bq, ba, bb).  FIRST starts the
+// runs with the batch's first source: a run that starts from its first term
+// equals one that starts from 0 and adds it.
+template <int FORM, bool JSIDE, bool FIRST>
+__device__ __forceinline__ void adj_source_f32c(double sv, float grj, float gij, int j, double c,
+                                                const float (&qh)[kJR], const float (&ql)[kJR],
+                                                const float (&gr)[kJR], const float (&gi)[kJR],
+                                                float eta2f, float m2eta2f, float (&rp)[kJR],
+                                                float (&ra)[kJR], float (&rb)[kJR], float &bq,
+                                                float &ba, float &bb, int qoff) {
+    const double t = sv - c;
+    const float sh = (float)t;
+    const float sl = (FORM == 2) ? (float)(t - (double)sh) : 0.0f;
+#pragma unroll
+    for (int k = 0; k < kJR; ++k) {
+        const float d = pair_diff<FORM>(qh[k], ql[k], sh, sl);
+        const float r = __builtin_amdgcn_rcpf(__builtin_fmaf(d, d, eta2f));
+        const float r2 = r * r;
+        float al = __builtin_fmaf(m2eta2f, r2, r);  // (d^2 - eta^2) / D^2
+        float be = d * r2;
+        if (!JSIDE) {
+            // the pair (i, i): T = 0, so it adds nothing.  Left in, its
+            // alpha = -1/eta^2 would enter P and A and cancel only in
+            // P - Gr_i A, to the rounding of 1/eta^2.
+            const bool self = qoff + k == j;
+            al = self ? 0.0f : al;
+            be = self ? 0.0f : be;
+        }
+        if (FIRST) {
+            rp[k] = __builtin_fmaf(-be, gij, al * grj);
+            ra[k] = al;
+            rb[k] = be;
+        } else {
+            rp[k] = __builtin_fmaf(al, grj, rp[k]);
+            rp[k] = __builtin_fmaf(-be, gij, rp[k]);
+            ra[k] += al;
+            rb[k] += be;
+        }
+        if (JSIDE) {
+            if (k == 0) {
+                bq = __builtin_fmaf(be, gi[k], al * gr[k]);
+                ba = al;
+                bb = be;
+            } else {
+                bq = __builtin_fmaf(al, gr[k], bq);
+                bq = __builtin_fmaf(be, gi[k], bq);
+                ba += al;
+                bb += be;
+            }
+        }
+    }
+}
+
+// The wave's sources in batches of kAB.  n_src is wave-uniform (a scalar), so
+// a full batch runs with no per-source test; only the last tile's sources can
+// end inside a batch.
 template <int FORM, bool JSIDE>
 __device__ __forceinline__ void adj_sweep_f32c(
     const double *__restrict__ s_src, const float *__restrict__ s_gr,
@@ -84,55 +141,43 @@ __device__ __forceinline__ void adj_sweep_f32c(
     const double *__restrict__ g_re, const double *__restrict__ g_im, double two_eta,
     double *__restrict__ j_grad, double *__restrict__ j_eta, int lane, int qoff) {
     for (int j0 = 0; j0 < kJQ; j0 += kAB) {  // same trip count in every wave
-        float rp[kJR], ra[kJR], rb[kJR];
-#pragma unroll
-        for (int k = 0; k < kJR; ++k) rp[k] = ra[k] = rb[k] = 0.0f;
-        for (int u = 0; u < kAB; ++u) {
-            const int j = j0 + u;
-            if (j < n_src) {  // wave-uniform
-                const double t = s_src[j] - c;
-                const float sh = (float)t;
-                const float sl = (FORM == 2) ? (float)(t - (double)sh) : 0.0f;
-                const float grj = s_gr[j], gij = s_gi[j];
-                float bq = 0.0f, ba = 0.0f, bb = 0.0f;
-#pragma unroll
-                for (int k = 0; k < kJR; ++k) {
-                    const float d = pair_diff<FORM>(qh[k], ql[k], sh, sl);
-                    const float r = __builtin_amdgcn_rcpf(__builtin_fmaf(d, d, eta2f));
-                    const float r2 = r * r;
-                    float al = __builtin_fmaf(m2eta2f, r2, r);  // (d^2 - eta^2) / D^2
-                    float be = d * r2;
-                    if (!JSIDE) {
-                        // the pair (i, i): T = 0, so it adds nothing.  Left in,
-                        // its alpha = -1/eta^2 would enter P and A and cancel
-                        // only in P - Gr_i A, to the rounding of 1/eta^2.
-                        const bool self = qoff + k == j;
-                        al = self ? 0.0f : al;
-                        be = self ? 0.0f : be;
-                    }
-                    rp[k] = __builtin_fmaf(al, grj, rp[k]);
-                    rp[k] = __builtin_fmaf(-be, gij, rp[k]);
-                    ra[k] += al;
-                    rb[k] += be;
-                    if (JSIDE) {
-                        bq = __builtin_fmaf(al, gr[k], bq);
-                        bq = __builtin_fmaf(be, gi[k], bq);
-                        ba += al;
-                        bb += be;
-                    }
-                }
+        const int cnt = min(kAB, n_src - j0);
+        if (cnt > 0) {
+            float rp[kJR], ra[kJR], rb[kJR], bq, ba, bb;
+            auto store = [&](int u) {
                 if (JSIDE) {
                     s_jb[(kComp * u + 0) * kARow + lane] = bq;
                     s_jb[(kComp * u + 1) * kARow + lane] = ba;
                     s_jb[(kComp * u + 2) * kARow + lane] = bb;
                 }
-            }
-        }
+            };
+            adj_source_f32c<FORM, JSIDE, true>(s_src[j0], s_gr[j0], s_gi[j0], j0, c, qh, ql, gr,
+                                               gi, eta2f, m2eta2f, rp, ra, rb, bq, ba, bb, qoff);
+            store(0);
+            if (cnt == kAB) {
 #pragma unroll
-        for (int k = 0; k < kJR; ++k) {
-            P[k] += (double)rp[k];
-            A[k] += (double)ra[k];
-            B[k] += (double)rb[k];
+                for (int u = 1; u < kAB; ++u) {
+                    const int j = j0 + u;
+                    adj_source_f32c<FORM, JSIDE, false>(s_src[j], s_gr[j], s_gi[j], j, c, qh, ql,
+                                                        gr, gi, eta2f, m2eta2f, rp, ra, rb, bq,
+                                                        ba, bb, qoff);
+                    store(u);
+                }
+            } else {
+                for (int u = 1; u < cnt; ++u) {
+                    const int j = j0 + u;
+                    adj_source_f32c<FORM, JSIDE, false>(s_src[j], s_gr[j], s_gi[j], j, c, qh, ql,
+                                                        gr, gi, eta2f, m2eta2f, rp, ra, rb, bq,
+                                                        ba, bb, qoff);
+                    store(u);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kJR; ++k) {
+                P[k] += (double)rp[k];
+                A[k] += (double)ra[k];
+                B[k] += (double)rb[k];
+            }
         }
         if (JSIDE) {
             wave_handover();
@@ -143,9 +188,9 @@ __device__ __forceinline__ void adj_sweep_f32c(
             double mine = 0.0;
             if (row_id < kComp * kAB) {
                 const float *row = s_jb + row_id * kARow + 32 * h;
-                float ca = 0.0f, cb = 0.0f;
+                float ca = row[0], cb = row[16];
 #pragma unroll
-                for (int l = 0; l < 16; ++l) {
+                for (int l = 1; l < 16; ++l) {
                     ca += row[l];
                     cb += row[16 + l];
                 }
@@ -156,9 +201,10 @@ __device__ __forceinline__ void adj_sweep_f32c(
             // source u's sums sit on lanes 6u, 6u + 2, 6u + 4
             const double t1 = __shfl_down(tot, 2, 64), t2 = __shfl_down(tot, 4, 64);
             if (lane < 2 * kComp * kAB && lane % (2 * kComp) == 0) {
-                const int j = j0 + lane / (2 * kComp);
-                if (j < n_src)
-                    jside_store(tot, t1, t2, g_re[j], g_im[j], two_eta, j_grad + j, j_eta + j);
+                const int u = lane / (2 * kComp);
+                if (u < cnt)
+                    jside_store(tot, t1, t2, g_re[j0 + u], g_im[j0 + u], two_eta,
+                                j_grad + j0 + u, j_eta + j0 + u);
             }
             wave_handover();  // the next batch overwrites s_jb
         }
@@ -201,10 +247,16 @@ __device__ __forceinline__ void adj_sweep_f64(
                     A[k] += al;
                     B[k] += be;
                     if (JSIDE) {
-                        bq = __builtin_fma(al, gr[k], bq);
-                        bq = __builtin_fma(be, gi[k], bq);
-                        ba += al;
-                        bb += be;
+                        if (k == 0) {  // the lane sums start from the first pair's term
+                            bq = __builtin_fma(be, gi[k], al * gr[k]);
+                            ba = al;
+                            bb = be;
+                        } else {
+                            bq = __builtin_fma(al, gr[k], bq);
+                            bq = __builtin_fma(be, gi[k], bq);
+                            ba += al;
+                            bb += be;
+                        }
                     }
                 }
                 if (JSIDE) {
@@ -222,8 +274,9 @@ __device__ __forceinline__ void adj_sweep_f64(
             double cq = 0.0;
             if (row_id < kComp * kAB64) {
                 const double *row = s_jb + row_id * kARow + 16 * q;
+                cq = row[0];
 #pragma unroll
-                for (int l = 0; l < 16; ++l) cq += row[l];
+                for (int l = 1; l < 16; ++l) cq += row[l];
             }
             const double o1 = __shfl_xor(cq, 1, 64);
             const double pair = (q & 1) ? o1 + cq : cq + o1;
@@ -294,7 +347,7 @@ static_assert(kComp * kAB * 4 == kComp * kAB64 * 8, "both forms use the same row
 static_assert(kJbufBytes >= 3 * kJT * 8, "i-side combine fits in the row buffer");
 static_assert(kJQ % kAB == 0 && kJQ % kAB64 == 0, "batches tile the wave's sources");
 
-__global__ __launch_bounds__(kJL * kJW) void symadj_f32c_kernel(
+__global__ __launch_bounds__(kJL * kJW) __attribute__((amdgpu_waves_per_eu(3))) void symadj_f32c_kernel(
     const double *__restrict__ ev, const double *__restrict__ g_re,
     const double *__restrict__ g_im, int p, double eta, SymBand bd, int id0) {
     int a, b;  // query tile a (in the band), source tile b >= a
@@ -302,7 +355,9 @@ __global__ __launch_bounds__(kJL * kJW) void symadj_f32c_kernel(
     __shared__ double s_src[kJW][kJQ];
     __shared__ float s_gr[kJW][kJQ], s_gi[kJW][kJQ];
     __shared__ double s_jbuf[kJbufBytes / 8];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // w is wave-uniform; in a scalar register the source counts and batch
+    // tests below stay scalar
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = threadIdx.x & 63;
     const double two_eta = 2.0 * eta;
 
     // queries: kJR contiguous per lane (only the last tile, which meets only
@@ -378,7 +433,9 @@ __global__ __launch_bounds__(kJL * kJW) void symadj_f64_kernel(
     __shared__ double s_src[kJW][kJQ];
     __shared__ double s_gr[kJW][kJQ], s_gi[kJW][kJQ];
     __shared__ double s_jbuf[kJbufBytes / 8];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // w is wave-uniform; in a scalar register the source counts and batch
+    // tests below stay scalar
+    const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = threadIdx.x & 63;
     const double two_eta = 2.0 * eta;
     const int q0 = a * kJT + lane * kJR;
     double x[kJR], gr[kJR], gi[kJR], P[kJR], A[kJR], B[kJR];
