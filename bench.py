@@ -182,7 +182,7 @@ def many_leg(shk, sync) -> dict:
     path (the single-spectrum kernels, largest spectra first, over 1 or 4
     library streams); then the same spectra as a prepared batch
     (shk_batch_*: device table, launch captured in a hipGraph and replayed,
-    tile height tuned).  Per-spectrum times are device events (stream pool)
+    configuration tuned: split-tile heights 1, 2, 4 or the symmetric kernel).  Per-spectrum times are device events (stream pool)
     or per-workgroup device wall-clock stamps (segmented launch)."""
     import torch
     from shrinkers_amd import spectra
@@ -221,20 +221,30 @@ def many_leg(shk, sync) -> dict:
     # captured once in a hipGraph, tile height tuned on this batch
     from shrinkers_amd.batch import BatchedStieltjes
     gb = BatchedStieltjes(sp, etas)
+
+    def replay_ms(reps=10):
+        for _ in range(2):
+            gb.run()
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            gb.run()
+        sync()
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    # every configuration: split-tile heights 1, 2, 4 and the symmetric kernel
+    per_config = {}
+    for r in (1, 2, 4, 8):
+        gb.set_config(r)
+        per_config[str(r)] = replay_ms(5)
     r_best, ms_tune = gb.tune()
-    for _ in range(2):
-        gb.run()
-    sync()
-    reps = 10
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        gb.run()
-    sync()
-    ms = (time.perf_counter() - t0) / reps * 1e3
+    ms = replay_ms()
     s0, s1 = gb.spectrum_times_us()
     dur = (s1 - s0) * 1e-3
+    gres = gb.results()
     out["hipgraph_replay"] = {
         "precision": "f32c", "tuned_queries_per_lane": r_best, "tune_best_ms": ms_tune,
+        "ms_per_batch_by_config": per_config,
         "ms_per_batch": ms, "spectra_per_s": 512 / (ms * 1e-3),
         "pair_evals_per_s": pairs / (ms * 1e-3),
         "spectrum_ms_p50": float(np.percentile(dur, 50)),
@@ -254,6 +264,15 @@ def many_leg(shk, sync) -> dict:
         num = np.sum((res[k][0] - er) ** 2 + (res[k][1] - ei) ** 2)
         worst = max(worst, float(np.sqrt(num / np.sum(er ** 2 + ei ** 2))))
     out["worst_rel_l2_vs_fp64_sample8"] = worst
+    worst = 0.0  # the tuned prepared batch, same sample
+    for k in np.random.default_rng(1).choice(len(sp), 8, replace=False):
+        d = torch.from_numpy(sp[k]).cuda()
+        er, ei = shk.all_points_dev(d, etas[k], "blocked", precision="f64")
+        sync()
+        er, ei = er.cpu().numpy(), ei.cpu().numpy()
+        num = np.sum((gres[k][0] - er) ** 2 + (gres[k][1] - ei) ** 2)
+        worst = max(worst, float(np.sqrt(num / np.sum(er ** 2 + ei ** 2))))
+    out["hipgraph_replay"]["worst_rel_l2_vs_fp64_sample8"] = worst
     return out
 
 

@@ -384,20 +384,29 @@ shk_status shk_chebcode_count_work(const shk_chebcode *h, const double *d_query,
  * every result equals the split-tile single-spectrum kernel's bit for bit
  * (shk_all_points_dev's outside its symmetric range, see above);
  * r = 1 or 4 (64- or 256-query tiles) agree within the FP32-compensated
- * bound.  A batch is used from one thread at a time; its runs share the
- * stamp buffer, so a run's times are read before the next run. */
+ * bound.  r = 8 runs the symmetric kernel instead (tiles of 512, each
+ * unordered pair once): the tile pairs of all spectra, in groups of spectra
+ * whose partial sums fit the symmetric scratch limit
+ * (shk_set_symmetric_scratch_limit, read when r = 8 is first prepared), each
+ * group's launch followed by an in-order fold; every result then equals the
+ * single-spectrum symmetric transform's bit for bit (shk_all_points_dev's
+ * inside its symmetric range).  r = 8 is refused when one spectrum's partials
+ * (16 * ceil(p/512) * p bytes) alone exceed that limit.  A batch is used from
+ * one thread at a time; its runs share the stamp buffer, so a run's times
+ * are read before the next run. */
 typedef struct shk_batch shk_batch;
 
 shk_status shk_batch_create(const double *d_eigenvalues, const int64_t *offsets, size_t n_spectra,
                             const double *etas, int scaled, shk_batch **out);
 /* Enqueue the graph on `stream`; returns without synchronising. */
 shk_status shk_batch_run(shk_batch *b, double *d_out_re, double *d_out_im, void *stream);
-/* Time each tile height r in {1, 2, 4} on this batch (three replays each,
- * synchronising `stream`) and keep the fastest; the chosen r and its time
- * are returned. */
+/* Time each configuration r in {1, 2, 4, 8} on this batch (three replays
+ * each, synchronising `stream`; r = 8 only where it is accepted) and keep
+ * the fastest; the chosen r and its time are returned. */
 shk_status shk_batch_tune(shk_batch *b, double *d_out_re, double *d_out_im, void *stream,
                           int *best_r, double *best_ms);
-/* Set the tile height r (1, 2 or 4 queries per lane). */
+/* Set the configuration r: 1, 2 or 4 queries per lane of the split-tile
+ * kernel, or 8 for the symmetric kernel. */
 shk_status shk_batch_set_config(shk_batch *b, int r);
 /* Per-spectrum start/end of the last run in microseconds from that run's
  * first workgroup start (device wall clock); the caller synchronises first.

@@ -10,7 +10,8 @@ the range in which it takes the symmetric kernel, ``symmetric_range()``).
 
 ``BatchedStieltjes``: a fixed set of spectra kept on the device with its
 spectrum table; the segmented launch is captured in a hipGraph and replayed
-by each ``run``, with per-spectrum device times and a tile-height tuner.
+by each ``run``, with per-spectrum device times and a tuner that picks the
+split-tile kernel's tile height or the symmetric kernel.
 """
 from __future__ import annotations
 
@@ -119,7 +120,8 @@ class BatchedStieltjes:
             self._h = None
 
     def tune(self, stream=None):
-        """Time the tile heights (1, 2, 4 queries per lane) on this batch and
+        """Time the configurations on this batch (1, 2, 4 queries per lane of
+        the split-tile kernel; 8, the symmetric kernel, where accepted) and
         keep the fastest; returns ``(queries_per_lane, ms)``."""
         r, ms = ctypes.c_int(), ctypes.c_double()
         check(_lib.load().shk_batch_tune(self._h, self.re.data_ptr(), self.im.data_ptr(),
@@ -128,8 +130,11 @@ class BatchedStieltjes:
         return r.value, ms.value
 
     def set_config(self, queries_per_lane: int) -> None:
-        """Tile height: 1, 2 (default: results equal the split-tile
-        single-spectrum kernel's bit for bit) or 4 queries per lane."""
+        """Configuration: 1, 2 (default: results equal the split-tile
+        single-spectrum kernel's bit for bit) or 4 queries per lane of the
+        split-tile kernel, or 8 for the symmetric kernel (each unordered pair
+        once; results equal the single-spectrum symmetric transform's bit for
+        bit)."""
         check(_lib.load().shk_batch_set_config(self._h, int(queries_per_lane)))
 
     def run(self, stream=None):

@@ -5,19 +5,27 @@
 // all-points transform (compute_all_stieltjes with the exact family,
 // src/stieltjes/mod.rs:709-723) in FP32-compensated arithmetic.
 //
-// The launch is the segmented all-points kernel of shk_exact.hip: one
+// The launch is the segmented all-points kernel of shk_exact.hip (one
 // one-wavefront workgroup per (spectrum, query tile), the tiles of all
-// spectra in one grid, largest spectra first.  A batch keeps its spectrum
-// table on the device and captures the launch (a single kernel node) in a
-// hipGraph on first use; later runs replay the graph.  Every workgroup stamps
+// spectra in one grid, largest spectra first), or with r = 8 the segmented
+// symmetric kernel of shk_sym.hip and its folds.  A batch keeps its spectrum
+// table on the device and captures the launch (a single kernel node; a
+// launch and a fold per group for r = 8) in a hipGraph on first use; later
+// runs replay the graph.  Every workgroup stamps
 // the device wall clock at its start and end, from which the per-spectrum
 // start and end times are derived.
 //
-// Tuning: the tile height (r = 1, 2 or 4 queries per lane, tiles of 64, 128
-// or 256 queries) is timed on the actual batch and the fastest kept.  With
-// r = 2 the tiles are the single-spectrum kernel's, so every result equals
-// that kernel's bit for bit; r = 1 and 4 use other tiles (their own centres
-// and pair forms) and agree within the FP32-compensated bound.
+// Tuning: the configuration is timed on the actual batch and the fastest
+// kept.  r = 1, 2 or 4 queries per lane are the split-tile kernel's tiles of
+// 64, 128 or 256 queries: with r = 2 the tiles are the single-spectrum
+// kernel's, so every result equals that kernel's bit for bit; r = 1 and 4 use
+// other tiles (their own centres and pair forms) and agree within the
+// FP32-compensated bound.  r = 8 is the symmetric kernel (tiles of 512, 8
+// queries per lane), which evaluates each unordered pair once: its tile pairs
+// of all spectra run in groups whose partial sums fit the symmetric scratch
+// limit, each group's launch followed by an in-order fold, and every result
+// equals the single-spectrum symmetric transform's bit for bit (the
+// all-points entry's own result inside its symmetric range).
 #include "shk_internal.hpp"
 #include "shrinkers_stieltjes.h"
 
@@ -31,9 +39,10 @@ using namespace shk;
 
 namespace {
 
-constexpr int kRs[3] = {1, 2, 4};
+constexpr int kRs[4] = {1, 2, 4, 8};
+constexpr int kSymR = 8;  // the symmetric kernel's queries per lane
 
-int r_index(int r) { return r == 1 ? 0 : (r == 2 ? 1 : 2); }
+int r_index(int r) { return r == 1 ? 0 : (r == 2 ? 1 : (r == 4 ? 2 : 3)); }
 
 }  // namespace
 
@@ -45,12 +54,18 @@ struct shk_batch {
     int scaled = 1;
     std::vector<size_t> order;   // spectra by descending size (the grid order)
     int r = 2;
-    // per tile height: device spectrum table and its tile count
+    // per tile height: device spectrum table and its workgroup count (query
+    // tiles; tile pairs for r = 8)
     SegSpectrum *d_table[3] = {nullptr, nullptr, nullptr};
     int n_table = 0;
-    int n_tiles[3] = {0, 0, 0};
-    std::vector<int> tile0[3];   // host copy of each table's first tiles
-    long long *d_stamps = nullptr;  // 2 per tile of the largest tile count
+    int64_t n_tiles[4] = {0, 0, 0, 0};
+    std::vector<int64_t> tile0[4];  // host copy of each table's first workgroups
+    long long *d_stamps = nullptr;  // 2 per tile of the largest tile count (r = 1)
+    // r = 8: symmetric table, groups, partial-sum scratch and stamps
+    SymSegSpectrum *d_sym = nullptr;
+    std::vector<SymSegGroup> sym_groups;
+    double *d_sym_part = nullptr;
+    long long *d_sym_stamps = nullptr;
     // captured graph
     hipGraphExec_t exec = nullptr;
     hipStream_t cap = nullptr;
@@ -76,7 +91,48 @@ shk_status bfail(shk_status s, const std::string &m) {
     } while (0)
 
 // Device table for tile height r (built once per r).
+shk_status prepare_sym(shk_batch *b) {
+    if (b->d_sym) return SHK_OK;
+    std::vector<SymSegSpectrum> table(b->n);
+    std::vector<SymSegGroup> groups;
+    int64_t pairs = 0;
+    const int st = sym_many_plan(b->off.data(), b->order.data(), b->n, b->eta.data(), b->scaled,
+                                 sym_scratch_budget() / sizeof(double), table.data(), groups,
+                                 &pairs);
+    if (st) return (shk_status)st;
+    size_t part = 0;
+    for (const SymSegGroup &g : groups) part = std::max(part, g.part_doubles);
+    hipError_t e = hipMalloc(&b->d_sym, sizeof(SymSegSpectrum) * b->n);
+    if (e == hipSuccess)
+        e = hipMemcpy(b->d_sym, table.data(), sizeof(SymSegSpectrum) * b->n,
+                      hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&b->d_sym_part, sizeof(double) * std::max<size_t>(1, part));
+    if (e == hipSuccess)
+        e = hipMalloc(&b->d_sym_stamps, sizeof(long long) * 2 * (size_t)std::max<int64_t>(1, pairs));
+    if (e != hipSuccess) {
+        set_error(std::string("batch (r = 8): ") + hipGetErrorString(e));
+        (void)hipFree(b->d_sym);
+        (void)hipFree(b->d_sym_part);
+        (void)hipFree(b->d_sym_stamps);
+        b->d_sym = nullptr;
+        b->d_sym_part = nullptr;
+        b->d_sym_stamps = nullptr;
+        return SHK_ERR_HIP;
+    }
+    b->sym_groups = groups;
+    b->n_tiles[3] = pairs;
+    b->tile0[3].resize(b->n);
+    for (size_t k = 0; k < b->n; ++k) {
+        const SymSegGroup *g = nullptr;
+        for (const SymSegGroup &h : groups)
+            if ((int)k >= h.first && (int)k < h.first + h.count) g = &h;
+        b->tile0[3][k] = g->pair_base + table[k].pair0;
+    }
+    return SHK_OK;
+}
+
 shk_status prepare(shk_batch *b, int r) {
+    if (r == kSymR) return prepare_sym(b);
     const int ri = r_index(r);
     if (b->d_table[ri]) return SHK_OK;
     std::vector<SegSpectrum> table(b->n);
@@ -103,8 +159,11 @@ shk_status capture(shk_batch *b, double *d_re, double *d_im) {
     const int ri = r_index(b->r);
     hipGraph_t g = nullptr;
     SHK_BHIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
-    const int st = exact_many(b->d_ev, b->d_table[ri], b->n_table, b->n_tiles[ri], 1, d_re, d_im,
-                              b->d_stamps, b->cap, b->r);
+    const int st = (b->r == kSymR)
+                       ? sym_many(b->d_ev, b->d_sym, b->sym_groups, b->d_sym_part, d_re, d_im,
+                                  b->d_sym_stamps, b->cap)
+                       : exact_many(b->d_ev, b->d_table[ri], b->n_table, (int)b->n_tiles[ri], 1,
+                                    d_re, d_im, b->d_stamps, b->cap, b->r);
     const hipError_t ce = hipStreamEndCapture(b->cap, &g);
     if (st || ce != hipSuccess) {
         if (g) (void)hipGraphDestroy(g);
@@ -205,7 +264,15 @@ shk_status shk_batch_tune(shk_batch *b, double *d_out_re, double *d_out_im, void
     SHK_BHIP(hipEventCreate(&e1));
     shk_status st = SHK_OK;
     for (int R : kRs) {
-        if ((st = prepare(b, R)) != SHK_OK) break;
+        if ((st = prepare(b, R)) != SHK_OK) {
+            // the symmetric kernel is refused for spectra whose partial sums
+            // alone exceed the symmetric scratch limit: the split-tile
+            // heights remain
+            if (R != kSymR) break;
+            st = SHK_OK;
+            set_error("");
+            continue;
+        }
         b->r = R;
         if ((st = launch(b, d_out_re, d_out_im, s)) != SHK_OK) break;  // capture + warm-up
         float ms = 0.f, tot = 1e30f;
@@ -237,7 +304,8 @@ shk_status shk_batch_tune(shk_batch *b, double *d_out_re, double *d_out_im, void
 
 shk_status shk_batch_set_config(shk_batch *b, int r) {
     if (!b) return bfail(SHK_ERR_BAD_ARG, "NULL handle");
-    if (r != 1 && r != 2 && r != 4) return bfail(SHK_ERR_BAD_ARG, "r must be 1, 2 or 4");
+    if (r != 1 && r != 2 && r != 4 && r != kSymR)
+        return bfail(SHK_ERR_BAD_ARG, "r must be 1, 2, 4 or 8");
     const shk_status st = prepare(b, r);
     if (st != SHK_OK) return st;
     b->r = r;
@@ -248,22 +316,22 @@ shk_status shk_batch_spectrum_times(const shk_batch *b, double *start_us, double
     if (!b || !start_us || !end_us) return bfail(SHK_ERR_BAD_ARG, "NULL pointer");
     if (!b->ran || !b->exec) return bfail(SHK_ERR_BAD_ARG, "the batch has not run yet");
     const int ri = r_index(b->g_r);
-    const int nt = b->n_tiles[ri];
-    std::vector<long long> t(2 * (size_t)std::max(1, nt));
-    SHK_BHIP(hipMemcpy(t.data(), b->d_stamps, sizeof(long long) * 2 * (size_t)nt,
-                       hipMemcpyDeviceToHost));
+    const int64_t nt = b->n_tiles[ri];
+    std::vector<long long> t(2 * (size_t)std::max<int64_t>(1, nt));
+    SHK_BHIP(hipMemcpy(t.data(), b->g_r == kSymR ? b->d_sym_stamps : b->d_stamps,
+                       sizeof(long long) * 2 * (size_t)nt, hipMemcpyDeviceToHost));
     int dev = 0, khz = 0;
     SHK_BHIP(hipGetDevice(&dev));
     SHK_BHIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
     if (khz <= 0) return bfail(SHK_ERR_HIP, "device wall-clock rate unavailable");
     const double us_per_tick = 1e3 / (double)khz;
     long long origin = t[0];
-    for (int k = 0; k < nt; ++k) origin = std::min(origin, t[2 * (size_t)k]);
-    const std::vector<int> &t0 = b->tile0[ri];
+    for (int64_t k = 0; k < nt; ++k) origin = std::min(origin, t[2 * (size_t)k]);
+    const std::vector<int64_t> &t0 = b->tile0[ri];
     for (int e = 0; e < b->n_table; ++e) {
-        const int a = t0[e], z = (e + 1 < b->n_table) ? t0[e + 1] : nt;
+        const int64_t a = t0[e], z = (e + 1 < b->n_table) ? t0[e + 1] : nt;
         long long lo = t[2 * (size_t)a], hi = t[2 * (size_t)a + 1];
-        for (int k = a; k < z; ++k) {
+        for (int64_t k = a; k < z; ++k) {
             lo = std::min(lo, t[2 * (size_t)k]);
             hi = std::max(hi, t[2 * (size_t)k + 1]);
         }
@@ -282,6 +350,9 @@ shk_status shk_batch_destroy(shk_batch *b) {
     for (SegSpectrum *t : b->d_table)
         if (t) (void)hipFree(t);
     if (b->d_stamps) (void)hipFree(b->d_stamps);
+    if (b->d_sym) (void)hipFree(b->d_sym);
+    if (b->d_sym_part) (void)hipFree(b->d_sym_part);
+    if (b->d_sym_stamps) (void)hipFree(b->d_sym_stamps);
     delete b;
     return SHK_OK;
 }

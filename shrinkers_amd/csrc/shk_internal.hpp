@@ -7,6 +7,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 namespace shk {
 
@@ -245,6 +246,42 @@ int exact_many_plan(const int64_t *offsets, const size_t *order, size_t n, const
 int exact_many(const double *ev, const SegSpectrum *d_table, int n_table, int n_tiles,
                int f32c, double *out_re, double *out_im, long long *d_stamps,
                hipStream_t stream, int r = 2);
+
+// Segmented symmetric launch (prepared batches): the FP32-compensated
+// symmetric kernel over the tile pairs of many spectra, in groups of
+// consecutive table entries whose partials fit the scratch.  Per spectrum:
+struct SymSegSpectrum {
+    int64_t off;    // index of its first eigenvalue (and first output)
+    int64_t part;   // its first partial (a double) in the group's scratch
+    int32_t p, nt;  // eigenvalues, tiles of kSymTile
+    int32_t pair0;  // its first workgroup in the group's launch
+    int32_t fold0;  // its first workgroup in the group's fold
+    double eta;
+    double scale;   // 1/p or 1
+};
+// Per group: table entries [first, first + count), `pairs` workgroups (tile
+// pairs) from global pair index pair_base on, `folds` fold workgroups, and
+// the scratch doubles its partials take.
+struct SymSegGroup {
+    int first, count, pairs, folds;
+    int64_t pair_base;
+    size_t part_doubles;
+};
+// Fills the table (spectra in `order`, every one non-empty) and cuts it into
+// groups whose partials (2 * nt * p doubles per spectrum) fit budget_doubles.
+// Fails when one spectrum's partials alone exceed the budget.  *n_pairs
+// receives the total tile pairs (the stamp count is twice that).
+int sym_many_plan(const int64_t *offsets, const size_t *order, size_t n, const double *etas,
+                  int scaled, size_t budget_doubles, SymSegSpectrum *table,
+                  std::vector<SymSegGroup> &groups, int64_t *n_pairs);
+// Enqueues each group's kernel and in-order fold on `stream`: every result
+// equals the single-spectrum symmetric FP32-compensated transform (exact_sym)
+// of that spectrum bit for bit.  d_table holds the whole table; scratch holds
+// the largest group's partials; d_stamps, if not NULL, receives 2 * n_pairs
+// device wall-clock ticks (each workgroup's start and end, by global pair).
+int sym_many(const double *ev, const SymSegSpectrum *d_table,
+             const std::vector<SymSegGroup> &groups, double *scratch, double *out_re,
+             double *out_im, long long *d_stamps, hipStream_t stream);
 
 // float -> double widening on device (for the f32 entry points).
 int widen_f32(const float *in, double *out, size_t n, hipStream_t stream);

@@ -317,10 +317,10 @@ __global__ __launch_bounds__(kSL * kSW) void sym_f64_kernel(const double *__rest
 
 static_assert(kSW * 2 * kSB64 * kJRow >= 2 * kST, "FP64 combine buffer fits in s_jbuf");
 
-__global__ __launch_bounds__(kSL * kSW) void sym_f32c_kernel(const double *__restrict__ ev, int p,
-                                                             double eta, SymBand bd, int id0) {
-    int a, b;  // query tile a (in the band), source tile b >= a
-    tile_pair_of((int)blockIdx.x + id0, bd.nt, a, b);
+// Tile pair (a, b), a <= b, of one spectrum (ev, p) into the band bd: the
+// body of the single-spectrum and the segmented FP32-compensated kernels.
+__device__ __forceinline__ void sym_f32c_pair(const double *__restrict__ ev, int p, double eta,
+                                              const SymBand &bd, int a, int b) {
     __shared__ double s_src[kSW][kSQ];                      // 8 KB
     __shared__ double s_jbuf[kSW * 2 * kSB * kJRow / 2];    // j-side lane sums / i-side combine
     // w is wave-uniform; in a scalar register the source counts and batch
@@ -386,6 +386,13 @@ __global__ __launch_bounds__(kSL * kSW) void sym_f32c_kernel(const double *__res
     sym_finish(re, iv, s_jbuf, w, lane, bd.slot(0, a, b), bd.slot(1, a, b), p, q0);
 }
 
+__global__ __launch_bounds__(kSL * kSW) void sym_f32c_kernel(const double *__restrict__ ev, int p,
+                                                             double eta, SymBand bd, int id0) {
+    int a, b;  // query tile a (in the band), source tile b >= a
+    tile_pair_of((int)blockIdx.x + id0, bd.nt, a, b);
+    sym_f32c_pair(ev, p, eta, bd, a, b);
+}
+
 static_assert(kSW * 2 * kSB * kJRow / 2 >= 2 * kST, "i-side combine buffer fits in s_jbuf");
 static_assert(kSQ % kSB == 0, "batches tile the wave's sources");
 static_assert(kST == kSymTile, "the band layout's tile");
@@ -398,6 +405,31 @@ static_assert(kST == kSymTile, "the band layout's tile");
 // additions.  A group of 64 points never straddles a tile.
 constexpr int kFoldWG = 64;
 constexpr int kFoldAhead = 8;
+
+// Adds slots 0..n-1 (stride apart) of both components to s0 and s1 in slot
+// order, loading kFoldAhead slots ahead of the additions.
+__device__ __forceinline__ void fold_slots(const double *__restrict__ c0,
+                                           const double *__restrict__ c1, int n, size_t stride,
+                                           double &s0, double &s1) {
+    int m = 0;
+    for (; m + kFoldAhead <= n; m += kFoldAhead) {
+        double v0[kFoldAhead], v1[kFoldAhead];
+#pragma unroll
+        for (int u = 0; u < kFoldAhead; ++u) {
+            v0[u] = c0[(size_t)(m + u) * stride];
+            v1[u] = c1[(size_t)(m + u) * stride];
+        }
+#pragma unroll
+        for (int u = 0; u < kFoldAhead; ++u) {
+            s0 += v0[u];
+            s1 += v1[u];
+        }
+    }
+    for (; m < n; ++m) {
+        s0 += c0[(size_t)m * stride];
+        s1 += c1[(size_t)m * stride];
+    }
+}
 
 __global__ __launch_bounds__(kFoldWG) void sym_band_reduce_kernel(SymBand bd, int p, double mul,
                                                                  double scale,
@@ -417,24 +449,7 @@ __global__ __launch_bounds__(kFoldWG) void sym_band_reduce_kernel(SymBand bd, in
         s0 = acc[i];
         s1 = acc[(size_t)p + i];
     }
-    int m = 0;
-    for (; m + kFoldAhead <= n; m += kFoldAhead) {
-        double v0[kFoldAhead], v1[kFoldAhead];
-#pragma unroll
-        for (int u = 0; u < kFoldAhead; ++u) {
-            v0[u] = c0[(size_t)(m + u) * stride];
-            v1[u] = c1[(size_t)(m + u) * stride];
-        }
-#pragma unroll
-        for (int u = 0; u < kFoldAhead; ++u) {
-            s0 += v0[u];
-            s1 += v1[u];
-        }
-    }
-    for (; m < n; ++m) {
-        s0 += c0[(size_t)m * stride];
-        s1 += c1[(size_t)m * stride];
-    }
+    fold_slots(c0, c1, n, stride, s0, s1);
     if (band) {
         o0[i] = s0 * scale;
         o1[i] = (mul * s1) * scale;
@@ -442,6 +457,63 @@ __global__ __launch_bounds__(kFoldWG) void sym_band_reduce_kernel(SymBand bd, in
         acc[i] = s0;
         acc[(size_t)p + i] = s1;
     }
+}
+
+// ---- segmented launch: many spectra, one group of them per launch ---------
+//
+// The workgroups of a group are its spectra's tile pairs, spectrum by
+// spectrum (largest first); each spectrum's partials take one band of the
+// band layout (r0 = 0, r1 = nt) at its own offset in the scratch.  The fold
+// adds each point's slots in order and scales as sym_band_reduce_kernel does
+// for a single band, so every result equals exact_sym's bit for bit.
+
+// The last entry of specs[0, n) whose member `first` is <= id.
+template <int32_t SymSegSpectrum::*FIRST>
+__device__ __forceinline__ int sym_seg_find(const SymSegSpectrum *__restrict__ specs, int n,
+                                            int id) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (specs[mid].*FIRST <= id) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ SymBand sym_seg_band(const SymSegSpectrum &sp, double *scratch) {
+    return SymBand{scratch + sp.part, nullptr, 0, sp.nt, sp.nt, sp.p, 0};
+}
+
+template <bool STAMP>
+__global__ __launch_bounds__(kSL * kSW) void sym_many_f32c_kernel(
+    const double *__restrict__ ev, const SymSegSpectrum *__restrict__ specs, int n_specs,
+    double *__restrict__ scratch, long long *__restrict__ stamps) {
+    if (STAMP && threadIdx.x == 0)
+        stamps[2 * (size_t)blockIdx.x] = (long long)wall_clock64();
+    const SymSegSpectrum sp =
+        specs[sym_seg_find<&SymSegSpectrum::pair0>(specs, n_specs, (int)blockIdx.x)];
+    int a, b;
+    tile_pair_of((int)blockIdx.x - sp.pair0, sp.nt, a, b);
+    sym_f32c_pair(ev + sp.off, sp.p, sp.eta, sym_seg_band(sp, scratch), a, b);
+    if (STAMP) {
+        __syncthreads();  // the end stamp follows every lane's stores
+        if (threadIdx.x == 0) stamps[2 * (size_t)blockIdx.x + 1] = (long long)wall_clock64();
+    }
+}
+
+__global__ __launch_bounds__(kFoldWG) void sym_many_fold_kernel(
+    const SymSegSpectrum *__restrict__ specs, int n_specs, const double *__restrict__ scratch,
+    double *__restrict__ out_re, double *__restrict__ out_im) {
+    const SymSegSpectrum sp =
+        specs[sym_seg_find<&SymSegSpectrum::fold0>(specs, n_specs, (int)blockIdx.x)];
+    const int i = ((int)blockIdx.x - sp.fold0) * kFoldWG + (int)threadIdx.x;
+    if (i >= sp.p) return;
+    const int t = i / kSymTile, loc = i - t * kSymTile;
+    const SymBand bd = sym_seg_band(sp, const_cast<double *>(scratch));
+    double s0 = 0.0, s1 = 0.0;
+    fold_slots(bd.slot(0, t, 0) + loc, bd.slot(1, t, 0) + loc, sp.nt, (size_t)sp.p, s0, s1);
+    out_re[sp.off + i] = s0 * sp.scale;
+    out_im[sp.off + i] = (sp.eta * s1) * sp.scale;
 }
 
 // index of row x's first tile pair in the row-major triangle of nt tiles
@@ -541,6 +613,77 @@ int exact_sym_on(const double *ev, size_t p, double eta, double scale, int f64, 
     SymCtx ctx{ev, (int)p, eta, f64, stream};
     return sym_run_bands(p, scratch, scratch_doubles, eta, scale, out_re, out_im, sym_launch,
                          &ctx, stream);
+}
+
+int sym_many_plan(const int64_t *offsets, const size_t *order, size_t n, const double *etas,
+                  int scaled, size_t budget_doubles, SymSegSpectrum *table,
+                  std::vector<SymSegGroup> &groups, int64_t *n_pairs) {
+    groups.clear();
+    int64_t pair_base = 0;
+    SymSegGroup g{0, 0, 0, 0, 0, 0};
+    auto close = [&]() {
+        if (g.count > 0) {
+            groups.push_back(g);
+            pair_base += g.pairs;
+        }
+        g = SymSegGroup{g.first + g.count, 0, 0, 0, pair_base, 0};
+    };
+    for (size_t idx = 0; idx < n; ++idx) {
+        const size_t k = order[idx];
+        const int64_t p = offsets[k + 1] - offsets[k];
+        if (p <= 0 || p > (int64_t)kSymMaxP) {
+            set_error("segmented symmetric launch: every spectrum needs 1 <= p <= 4194304");
+            return 6;
+        }
+        const int64_t nt = (p + kSymTile - 1) / kSymTile;
+        const size_t part = 2 * (size_t)nt * (size_t)p;
+        if (part > budget_doubles) {
+            set_error("segmented symmetric launch: the partial sums of a spectrum of " +
+                      std::to_string(p) + " eigenvalues exceed the symmetric scratch limit");
+            return 6;
+        }
+        const int64_t pairs = nt * (nt + 1) / 2, folds = (p + kFoldWG - 1) / kFoldWG;
+        if (g.part_doubles + part > budget_doubles || g.pairs + pairs > (int64_t)INT32_MAX ||
+            g.folds + folds > (int64_t)INT32_MAX)
+            close();
+        SymSegSpectrum &e = table[idx];
+        e.off = offsets[k];
+        e.part = (int64_t)g.part_doubles;
+        e.p = (int32_t)p;
+        e.nt = (int32_t)nt;
+        e.pair0 = g.pairs;
+        e.fold0 = g.folds;
+        e.eta = etas[k];
+        e.scale = scaled ? 1.0 / (double)p : 1.0;
+        g.count += 1;
+        g.pairs += (int)pairs;
+        g.folds += (int)folds;
+        g.part_doubles += part;
+    }
+    close();
+    *n_pairs = pair_base;
+    return 0;
+}
+
+int sym_many(const double *ev, const SymSegSpectrum *d_table,
+             const std::vector<SymSegGroup> &groups, double *scratch, double *out_re,
+             double *out_im, long long *d_stamps, hipStream_t stream) {
+    for (const SymSegGroup &g : groups) {
+        ProfileMark pm = profile_begin(stream);
+        const SymSegSpectrum *tab = d_table + g.first;
+        if (d_stamps)
+            hipLaunchKernelGGL(sym_many_f32c_kernel<true>, dim3(g.pairs), dim3(kSL * kSW), 0,
+                               stream, ev, tab, g.count, scratch, d_stamps + 2 * g.pair_base);
+        else
+            hipLaunchKernelGGL(sym_many_f32c_kernel<false>, dim3(g.pairs), dim3(kSL * kSW), 0,
+                               stream, ev, tab, g.count, scratch, nullptr);
+        profile_end(pm, stream);
+        SHK_HIP(hipGetLastError());
+        hipLaunchKernelGGL(sym_many_fold_kernel, dim3(g.folds), dim3(kFoldWG), 0, stream, tab,
+                           g.count, scratch, out_re, out_im);
+        SHK_HIP(hipGetLastError());
+    }
+    return 0;
 }
 
 }  // namespace shk
