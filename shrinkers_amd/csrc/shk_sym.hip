@@ -44,19 +44,25 @@ namespace {
 // Defaults measured on MI355X at p = 2e4 / 5e4 / 2e5 (ms per transform):
 // 8 per lane with the wave-local handover 0.088 / 0.352 / 4.68; 16 per lane
 // 0.159 / 0.370 / 4.68 (too few tile pairs at 5e4 to fill the chip evenly);
-// a workgroup barrier instead costs 1-4 %.
+// a workgroup barrier instead costs 1-4 %.  SHK_SYM_RUN is the length of the
+// FP32 i-side runs: 32 instead of 16 gains under 1 % at 5e4 and raises the
+// error against FP64 from 1.8e-8 to 2.3e-8.
 #ifndef SHK_SYM_R
 #define SHK_SYM_R 8
 #endif
 #ifndef SHK_SYM_WAVE_SYNC
 #define SHK_SYM_WAVE_SYNC 1
 #endif
+#ifndef SHK_SYM_RUN
+#define SHK_SYM_RUN 16
+#endif
 constexpr int kSR = SHK_SYM_R;      // queries per lane (contiguous)
 constexpr int kSL = 64;             // lanes per wavefront
 constexpr int kSW = 4;              // wavefronts per workgroup
 constexpr int kST = kSL * kSR;      // tile: 512 eigenvalues at kSR = 8
 constexpr int kSQ = kST / kSW;      // sources per wavefront
-constexpr int kSB = 16;             // sources per batch (FP32 run and j-side reduction)
+constexpr int kSB = 16;             // sources per batch (j-side reduction)
+constexpr int kSRun = SHK_SYM_RUN;  // sources per FP32 i-side run (a multiple of kSB)
 constexpr int kJRow = kSL + 1;      // padded LDS row of 64 lane sums (no bank conflicts)
 
 // Makes this wave's LDS writes visible to its own lanes' reads (each wave
@@ -107,75 +113,97 @@ __device__ __forceinline__ void sym_source(double sv, double c, const float (&qh
     }
 }
 
-// The wave's sources in batches of kSB.  n_src is wave-uniform (a scalar), so
-// a full batch runs with no per-source test; only the last tile's sources can
-// end inside a batch.
+// One batch of kSB sources from j0 (cnt of them, cnt <= 0 past the end):
+// the i-side runs continue (START: begin with this batch), and with JSIDE the
+// lane sums of each source go through LDS and are reduced per (source,
+// component).  n_src is wave-uniform (a scalar), so a full batch runs with
+// no per-source test; only the last tile's sources can end inside a batch.
+template <int FORM, bool JSIDE, bool START>
+__device__ __forceinline__ void sym_batch(const double *__restrict__ s_src, int j0, int cnt,
+                                          double c, const float (&qh)[kSR],
+                                          const float (&ql)[kSR], float eta2f, float (&ar)[kSR],
+                                          float (&ai)[kSR], float *__restrict__ s_jb,
+                                          double *__restrict__ j_re, double *__restrict__ j_im,
+                                          int lane) {
+    if (cnt > 0) {
+        float bre, bim;
+        sym_source<FORM, JSIDE, START>(s_src[j0], c, qh, ql, eta2f, ar, ai, bre, bim);
+        if (JSIDE) {
+            s_jb[0 * kJRow + lane] = bre;
+            s_jb[1 * kJRow + lane] = bim;
+        }
+        if (cnt == kSB) {
+#pragma unroll
+            for (int u = 1; u < kSB; ++u) {
+                sym_source<FORM, JSIDE, false>(s_src[j0 + u], c, qh, ql, eta2f, ar, ai, bre, bim);
+                if (JSIDE) {
+                    s_jb[(2 * u + 0) * kJRow + lane] = bre;
+                    s_jb[(2 * u + 1) * kJRow + lane] = bim;
+                }
+            }
+        } else {
+            for (int u = 1; u < cnt; ++u) {
+                sym_source<FORM, JSIDE, false>(s_src[j0 + u], c, qh, ql, eta2f, ar, ai, bre, bim);
+                if (JSIDE) {
+                    s_jb[(2 * u + 0) * kJRow + lane] = bre;
+                    s_jb[(2 * u + 1) * kJRow + lane] = bim;
+                }
+            }
+        }
+    }
+    if (JSIDE) {
+        sym_handover();
+        {
+            // two lanes per (source, component) row: lane h = 0 adds
+            // row[0, 32) and h = 1 row[32, 64), each as two FP32 chains
+            // of 16 joined in FP64; the row total is (h0 + h1) in that
+            // order on both lanes, ((c0 + c1) + (c2 + c3)) overall
+            const int row_id = lane >> 1, h = lane & 1;
+            const int u = row_id >> 1, comp = row_id & 1;
+            const float *row = s_jb + (2 * u + comp) * kJRow + 32 * h;
+            float ca = row[0], cb = row[16];
+#pragma unroll
+            for (int l = 1; l < 16; ++l) {
+                ca += row[l];
+                cb += row[16 + l];
+            }
+            const double mine = (double)ca + (double)cb;
+            const double other = __shfl_xor(mine, 1, 64);
+            const double tot = h ? other + mine : mine + other;
+            if (h == 0 && u < cnt) {
+                if (comp) j_im[j0 + u] = tot;
+                else j_re[j0 + u] = -tot;
+            }
+        }
+        sym_handover();  // the next batch overwrites s_jb
+    }
+}
+
+// The wave's sources in i-side runs of kSRun (FP32, folded into FP64 after
+// each run), each run in batches of kSB.
 template <int FORM, bool JSIDE>
 __device__ __forceinline__ void sym_sweep(const double *__restrict__ s_src, int n_src, double c,
                                           const float (&qh)[kSR], const float (&ql)[kSR],
                                           float eta2f, double (&re)[kSR], double (&iv)[kSR],
                                           float *__restrict__ s_jb, double *__restrict__ j_re,
                                           double *__restrict__ j_im, int lane) {
-    for (int j0 = 0; j0 < kSQ; j0 += kSB) {  // same trip count in every wave
-        const int cnt = min(kSB, n_src - j0);
-        if (cnt > 0) {
-            float ar[kSR], ai[kSR], bre, bim;
-            sym_source<FORM, JSIDE, true>(s_src[j0], c, qh, ql, eta2f, ar, ai, bre, bim);
-            if (JSIDE) {
-                s_jb[0 * kJRow + lane] = bre;
-                s_jb[1 * kJRow + lane] = bim;
-            }
-            if (cnt == kSB) {
+    for (int j0 = 0; j0 < kSQ; j0 += kSRun) {  // same trip count in every wave
+        float ar[kSR], ai[kSR];
+        const int cnt0 = min(kSB, n_src - j0);
+        sym_batch<FORM, JSIDE, true>(s_src, j0, cnt0, c, qh, ql, eta2f, ar, ai, s_jb, j_re, j_im,
+                                     lane);
 #pragma unroll
-                for (int u = 1; u < kSB; ++u) {
-                    sym_source<FORM, JSIDE, false>(s_src[j0 + u], c, qh, ql, eta2f, ar, ai, bre,
-                                                   bim);
-                    if (JSIDE) {
-                        s_jb[(2 * u + 0) * kJRow + lane] = bre;
-                        s_jb[(2 * u + 1) * kJRow + lane] = bim;
-                    }
-                }
-            } else {
-                for (int u = 1; u < cnt; ++u) {
-                    sym_source<FORM, JSIDE, false>(s_src[j0 + u], c, qh, ql, eta2f, ar, ai, bre,
-                                                   bim);
-                    if (JSIDE) {
-                        s_jb[(2 * u + 0) * kJRow + lane] = bre;
-                        s_jb[(2 * u + 1) * kJRow + lane] = bim;
-                    }
-                }
-            }
+        for (int q = 1; q < kSRun / kSB; ++q) {
+            const int jq = j0 + q * kSB;
+            sym_batch<FORM, JSIDE, false>(s_src, jq, min(kSB, n_src - jq), c, qh, ql, eta2f, ar,
+                                          ai, s_jb, j_re, j_im, lane);
+        }
+        if (cnt0 > 0) {
 #pragma unroll
             for (int k = 0; k < kSR; ++k) {
                 re[k] += (double)ar[k];
                 iv[k] += (double)ai[k];
             }
-        }
-        if (JSIDE) {
-            sym_handover();
-            {
-                // two lanes per (source, component) row: lane h = 0 adds
-                // row[0, 32) and h = 1 row[32, 64), each as two FP32 chains
-                // of 16 joined in FP64; the row total is (h0 + h1) in that
-                // order on both lanes, ((c0 + c1) + (c2 + c3)) overall
-                const int row_id = lane >> 1, h = lane & 1;
-                const int u = row_id >> 1, comp = row_id & 1;
-                const float *row = s_jb + (2 * u + comp) * kJRow + 32 * h;
-                float ca = row[0], cb = row[16];
-#pragma unroll
-                for (int l = 1; l < 16; ++l) {
-                    ca += row[l];
-                    cb += row[16 + l];
-                }
-                const double mine = (double)ca + (double)cb;
-                const double other = __shfl_xor(mine, 1, 64);
-                const double tot = h ? other + mine : mine + other;
-                if (h == 0 && u < cnt) {
-                    if (comp) j_im[j0 + u] = tot;
-                    else j_re[j0 + u] = -tot;
-                }
-            }
-            sym_handover();  // the next batch overwrites s_jb
         }
     }
 }
@@ -394,7 +422,7 @@ __global__ __launch_bounds__(kSL * kSW) void sym_f32c_kernel(const double *__res
 }
 
 static_assert(kSW * 2 * kSB * kJRow / 2 >= 2 * kST, "i-side combine buffer fits in s_jbuf");
-static_assert(kSQ % kSB == 0, "batches tile the wave's sources");
+static_assert(kSQ % kSRun == 0 && kSRun % kSB == 0, "runs and batches tile the wave's sources");
 static_assert(kST == kSymTile, "the band layout's tile");
 
 // The in-order fold of one band's partials, one thread per point of
