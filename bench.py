@@ -40,9 +40,9 @@ TRAFFIC_P50000 = (1681.0 * 2.000 + 76562.5 * 1.000) * 1024
 # Fraction of the kernel's time the vector ALUs issue (rocprofv3 derived counter VALUBusy
 # = SQ_ACTIVE_INST_VALU / (CU_NUM * GRBM_GUI_ACTIVE)) for sym_f32c_kernel at p = 50 000,
 # from `rocprofv3 --pmc VALUBusy VALUUtilization -- python3 bench.py --kernels`; its
-# instruction mix is 4.22 VALU instructions per ordered pair (SQ_INSTS_VALU).
-VALU_BUSY_P50000 = 0.989
-VALU_INSTS_PER_PAIR_P50000 = 4.22
+# instruction mix is 3.99 VALU instructions per ordered pair (SQ_INSTS_VALU).
+VALU_BUSY_P50000 = 0.990
+VALU_INSTS_PER_PAIR_P50000 = 3.99
 
 
 def log(msg: str) -> None:
