@@ -28,9 +28,18 @@
  * shk_all_points_dev, shk_at_points_dev and shk_all_points_many_dev): the
  * tree's node count sizes the rest of its build, so the call waits once for
  * its topology kernel, and with it for the work queued before it on the
- * stream; a build cannot be captured in a HIP graph.  Evaluations of a
- * built tree (shk_chebcode_evaluate_dev, shk_chebcode_evaluate_many_dev) can
- * be, once its build has finished (SHK_ERR_BAD_ARG if it has not).
+ * stream.
+ *
+ * HIP graph capture: entry points whose work is captured whole are
+ * shk_all_points_noalloc_dev, the evaluations of a built tree
+ * (shk_chebcode_evaluate_dev, shk_chebcode_evaluate_many_dev; SHK_ERR_BAD_ARG
+ * if the tree's build has not finished) and the prepared batch
+ * (shk_batch_run, which captures its own graph).  On a stream under capture,
+ * a ChebCode build (including the ChebCode methods of shk_all_points_dev and
+ * shk_at_points_dev), shk_all_points_many_dev, and every other _dev entry
+ * point that needs scratch memory (the cached buffers below cannot be shared
+ * with a graph) return SHK_ERR_BAD_ARG before enqueuing any work, so the
+ * capture stays valid.
  *
  * Thread safety: calls on distinct streams may run concurrently; scratch
  * memory is stream-ordered per call (pool allocations, or the calling

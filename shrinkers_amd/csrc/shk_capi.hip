@@ -504,6 +504,18 @@ int cache_acquire(int slot, size_t bytes, hipStream_t s, void **out) {
         set_error("buffer cache: no current device");
         return SHK_ERR_HIP;
     }
+    // A cached buffer cannot serve a graph: replays would share it with
+    // later eager calls, and its event would be recorded inside the capture.
+    // Calls that need scratch are refused under capture (before their first
+    // kernel); shk_all_points_noalloc_dev serves captured callers.
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    SHK_HIP(hipStreamIsCapturing(s, &cs));
+    if (cs != hipStreamCaptureStatusNone) {
+        set_error("this call needs scratch memory, which the library does not allocate under "
+                  "graph capture (a captured call cannot share the cached buffers); "
+                  "shk_all_points_noalloc_dev runs the all-points transform captured");
+        return SHK_ERR_BAD_ARG;
+    }
     if (!c->last) SHK_HIP(hipEventCreateWithFlags(&c->last, hipEventDisableTiming));
     if (c->used && c->last_stream != s) SHK_HIP(hipStreamWaitEvent(s, c->last, 0));
     if (c->cap < bytes) {
@@ -954,6 +966,17 @@ shk_status shk_all_points_many_dev(const double *d_eigenvalues, const int64_t *o
         return offsets[a + 1] - offsets[a] > offsets[b + 1] - offsets[b];
     });
     hipStream_t caller = (hipStream_t)stream;
+    {
+        // The spectrum table travels through host staging memory that the
+        // next call overwrites, and ChebCode spectra wait on the host once
+        // per build: refused under capture, before anything is enqueued.
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        SHK_HIP_S(hipStreamIsCapturing(caller, &cs));
+        if (cs != hipStreamCaptureStatusNone)
+            return fail(SHK_ERR_BAD_ARG,
+                        "the multi-spectrum call cannot be captured in a graph; a prepared batch "
+                        "(shk_batch_create, shk_batch_run) replays its launch from a graph");
+    }
     if (n_streams == 0 && exact && kind == EXACT_PLAIN) {
         // One segmented launch on the caller's stream: no library streams,
         // no partial sums.

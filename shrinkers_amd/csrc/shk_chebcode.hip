@@ -897,6 +897,20 @@ __global__ __launch_bounds__(kTopoThreads) void topo_top_kernel(TopoCtx c, int c
     __shared__ int s_a[2][kTopCap], s_b[2][kTopCap];
     __shared__ double s_xa[2][kTopCap], s_xb[2][kTopCap];
     const int tid = threadIdx.x;
+    // An input that ingest_kernel found not ascending is sorted and its
+    // topology built again; this pass then publishes an empty top walk (no
+    // frontier: the subtree kernels return at once) instead of walking the
+    // unsorted copy.
+    if (info[kInfoUnsorted] != 0) {
+        if (tid == 0) {
+            info[kInfoOverflow] = 0;
+            info[kInfoT] = 0;
+            info[kInfoF] = 0;
+            info[kInfoD] = 0;
+            info[kInfoFront] = 0;
+        }
+        return;
+    }
     c.h = hint_of(c.sorted, c.p, c.R);
     c.ng = min(c.gap[0], kMaxGaps);
     if (tid == 0) {
@@ -1119,6 +1133,18 @@ __global__ __launch_bounds__(kTopoThreads) void topo_mid_kernel(
     __shared__ int s_wave[kTopoThreads / 64];
     __shared__ int s_left[kMidLds], s_size[kMidLds], s_pre[kMidLds];
     const int tid = threadIdx.x;
+    // the info words to the host mailbox, then its sequence number
+    auto publish = [&]() {
+        __syncthreads();
+        for (int i = tid; i < kInfoInts; i += kTopoThreads) mail[kMailData + i] = info[i];
+        __threadfence_system();
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(mail, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    };
+    if (info[kInfoUnsorted] != 0) {  // the host sorts and builds again (topo_top_kernel)
+        publish();
+        return;
+    }
     const int T = info[kInfoT], F = info[kInfoF], D = info[kInfoD];
     const int *lvl = info + kInfoTopLevel;  // top level d = BFS slots [lvl[d], lvl[d+1])
     // Expanded top levels are 0..D-1; with a frontier (F > 0) level D holds
@@ -1198,12 +1224,7 @@ __global__ __launch_bounds__(kTopoThreads) void topo_mid_kernel(
     const int off = block_exclusive_scan<kTopoThreads>(mine, s_wave, &tot);
     if (tid < F) lv_off[tid * kLv] = T + off;
     if (tid == 0) info[kInfoN] = T + tot;
-    // the info words to the host mailbox, then its sequence number
-    __syncthreads();
-    for (int i = tid; i < kInfoInts; i += kTopoThreads) mail[kMailData + i] = info[i];
-    __threadfence_system();
-    __syncthreads();
-    if (tid == 0) __hip_atomic_store(mail, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    publish();
     // the top slots' node-list entries (read by later kernels, not the host)
     for (int i = tid; i < T; i += kTopoThreads) list[i] = i;
 }
@@ -3207,6 +3228,19 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
     if (!(prm.theta > 0.0) || prm.leaf_cap < 1) {
         set_error("chebcode build: theta must be > 0 and leaf_cap >= 1");
         return 6;
+    }
+    // The build waits once on the host for its topology (below), which a
+    // stream under graph capture cannot do: refuse before enqueuing anything,
+    // so the capture stays valid.
+    {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        SHK_HIP(hipStreamIsCapturing(stream, &cs));
+        if (cs != hipStreamCaptureStatusNone) {
+            set_error("chebcode build: a ChebCode build waits on the host for its topology and "
+                      "cannot be captured in a graph; build the tree outside the capture "
+                      "(shk_chebcode_build_dev) and capture its evaluations");
+            return 6;
+        }
     }
     std::unique_ptr<ChebTree, void (*)(ChebTree *)> t(new ChebTree(), &cheb_destroy);
     t->p = p;

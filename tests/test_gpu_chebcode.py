@@ -547,3 +547,52 @@ def test_evaluations_of_a_built_tree_captured_in_a_graph(shk, preset):
         torch.cuda.synchronize()
         for got, want in zip(out + mout, (er, ei, mr, mi)):
             assert torch.equal(got, want), (preset, rep)
+
+
+def test_builds_refused_under_capture(shk):
+    """A ChebCode build waits on the host once for its topology, so one-shot
+    ChebCode transforms (all points, at points) and the multi-spectrum call
+    under graph capture are refused with SHK_ERR_BAD_ARG before anything is
+    enqueued: the capture stays valid, and the evaluation of a tree built
+    beforehand, captured beside the refused calls, replays bit-equal to its
+    eager result."""
+    import torch
+    from shrinkers_amd import _lib
+    from shrinkers_amd import spectra as S
+    from shrinkers_amd.batch import all_points_many_dev
+    theta, n, leaf, f32 = shk.CHEB_PRESETS["chebcode_fast"]
+    p = 20_000
+    ev = torch.from_numpy(np.ascontiguousarray(S.mp_inverse_cdf_spectrum(p, c=0.3))).cuda()
+    eta = 1.0 / np.sqrt(p)
+    b = shk.ChebCodeBatch.build_dev(ev, theta, n, leaf, bool(f32))
+    er, ei = b.evaluate_dev(eta, 1.0 / p)
+    torch.cuda.synchronize()
+    out = (torch.empty_like(er), torch.empty_like(ei))
+    q = ev[:100].contiguous()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    refused = [
+        lambda: shk.all_points_dev(ev, eta, "chebcode_fast", stream=s),
+        lambda: shk.at_points_dev(q, ev, eta, "chebcode", stream=s),
+        lambda: all_points_many_dev(ev, [0, p // 2, p], [eta, eta], "chebcode_fast", stream=s),
+        lambda: all_points_many_dev(ev, [0, p // 2, p], [eta, eta], "blocked", stream=s),
+    ]
+    with torch.cuda.graph(g, stream=s):
+        for call in refused:
+            with pytest.raises(_lib.ShkError) as err:
+                call()
+            assert err.value.status == _lib.SHK_ERR_BAD_ARG
+            assert "captured" in err.value.message
+        b.evaluate_dev(eta, 1.0 / p, out=out, stream=s)
+    for rep in range(2):
+        out[0].fill_(float("nan"))
+        out[1].fill_(float("nan"))
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out[0], er) and torch.equal(out[1], ei), rep
+    # eager one-shot calls still build and evaluate after the capture
+    r, i = shk.all_points_dev(ev, eta, "chebcode_fast")
+    torch.cuda.synchronize()
+    assert torch.isfinite(r).all() and torch.isfinite(i).all()

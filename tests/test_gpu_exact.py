@@ -521,3 +521,94 @@ def test_symmetric_above_old_maximum(shk):
     r64, i64, r32, i32, sr, si = (t.cpu().numpy() for t in (r64, i64, r32, i32, sr, si))
     assert_close(r64, i64, sr, si, 1e-13, "banded symmetric f64 vs split f64 at 3e5")
     assert rel_l2(r32, i32, r64, i64) <= 1e-6
+
+
+
+def test_exact_entries_under_capture(shk):
+    """Under HIP graph capture (torch.cuda.graph on a side stream) the entry
+    points that need scratch memory (the all-points and at-points
+    transforms, the all-points adjoint) are refused with SHK_ERR_BAD_ARG
+    before any work is enqueued, since a captured call cannot share the
+    thread's cached buffers; the capture stays valid.  The no-allocation
+    entry captured beside them (f32c and FP64, one band and several) replays
+    bit-equal to its eager calls, three times, with eager calls of every
+    entry and synchronisations between the replays; the eager calls keep
+    their results bit for bit."""
+    import torch
+    from shrinkers_amd import _lib
+    from shrinkers_amd import spectra as S
+    p1, p2 = 20_000, 200_000
+    ev1 = torch.from_numpy(np.ascontiguousarray(S.mp_inverse_cdf_spectrum(p1, c=0.25))).cuda()
+    ev2 = torch.from_numpy(np.ascontiguousarray(S.mp_inverse_cdf_spectrum(p2, c=0.3))).cuda()
+    e1, e2 = 1.0 / np.sqrt(p1), 1.0 / np.sqrt(p2)
+    q = ev1[::97].contiguous()
+    rng = np.random.default_rng(5)
+    g_re = torch.from_numpy(rng.standard_normal(p1)).cuda()
+    g_im = torch.from_numpy(rng.standard_normal(p1)).cuda()
+    noalloc = _lib.load().shk_all_points_noalloc_dev
+
+    def na(ev, eta, prec):
+        def call(out, st):
+            _lib.check(noalloc(ev.data_ptr(), ev.numel(), eta, prec, 1, out[0].data_ptr(),
+                               out[1].data_ptr(), st.cuda_stream))
+        return call, (ev.numel(), ev.numel())
+
+    def adjoint(out, st):
+        _lib.check(_lib.load().shk_exact_adjoint_all_points_dev(
+            ev1.data_ptr(), p1, e1, g_re.data_ptr(), g_im.data_ptr(), _lib.PREC_F32C,
+            out[0].data_ptr(), out[1].data_ptr(), st.cuda_stream))
+
+    captured = [na(ev1, e1, _lib.PREC_F32C), na(ev1, e1, _lib.PREC_F64),
+                na(ev2, e2, _lib.PREC_F32C)]
+    refused = [
+        (lambda out, st: shk.all_points_dev(ev1, e1, "blocked", precision="f32c", out=out,
+                                            stream=st), (p1, p1)),
+        (lambda out, st: shk.all_points_dev(ev1, e1, "blocked", precision="f64", out=out,
+                                            stream=st), (p1, p1)),
+        (lambda out, st: shk.at_points_dev(q, ev1, e1, "blocked", out=out, stream=st),
+         (q.numel(), q.numel())),
+        (adjoint, (p1, 1)),
+    ]
+    main = torch.cuda.current_stream()
+
+    def run_all(calls):
+        res = []
+        for fn, (n0, n1) in calls:
+            out = (torch.empty(n0, dtype=torch.float64, device="cuda"),
+                   torch.empty(n1, dtype=torch.float64, device="cuda"))
+            fn(out, main)
+            res.append(out)
+        torch.cuda.synchronize()
+        return res
+
+    want_cap = run_all(captured)
+    want_ref = run_all(refused)
+    outs = [(torch.empty_like(a), torch.empty_like(b)) for a, b in want_cap]
+    s = torch.cuda.Stream()
+    s.wait_stream(main)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        for fn, (n0, n1) in refused:
+            out = (torch.empty(n0, dtype=torch.float64, device="cuda"),
+                   torch.empty(n1, dtype=torch.float64, device="cuda"))
+            with pytest.raises(_lib.ShkError) as err:
+                fn(out, s)
+            assert err.value.status == _lib.SHK_ERR_BAD_ARG
+            assert "graph capture" in err.value.message
+        for (fn, _), out in zip(captured, outs):
+            fn(out, s)
+
+    def same(got, want):
+        return all(torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
+                   for a, b in zip(got, want))
+
+    for rep in range(3):
+        for a, b in outs:
+            a.fill_(float("nan"))
+            b.fill_(float("nan"))
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert same(outs, want_cap), rep
+        assert same(run_all(captured), want_cap), rep
+        assert same(run_all(refused), want_ref), rep
