@@ -1502,13 +1502,25 @@ __global__ __launch_bounds__(kBaryLanes) void node_group_kernel(
     const int slot = blockIdx.x * kNodeGroup + g;
     const BaryLds bl = bary_lds(lds, NT);
     double *pos = lds + bary_lds_doubles(NT);  // kNodeGroup x NT positions
+    // The loads that do not depend on the node go out first, and a leaf's
+    // rows go out with its range ends: three dependent round trips per
+    // wavefront (list, slot arrays, sources).
+    double lr[NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) lr[j] = tab[NT + j];
+    const double tab_r = (r < NT) ? tab[r] : 0.0;
     int k = 0, a = 0, cnt = 0;
     bool leaf = false;
+    double xrow0 = 0.0, xrow1 = 0.0;  // this lane's rows of a leaf: a + r, a + kGroupLanes + r
     if (slot < count) {
         const int v = list[slot];
         k = b_pre[v];
         a = b_lo[v];
         const int b = b_hi[v], l = b_left[v], size = b_size[v];
+        if (l < 0) {
+            if (r < b - a) xrow0 = sorted[a + r];
+            if (kGroupLanes + r < b - a) xrow1 = sorted[a + kGroupLanes + r];
+        }
         const int right = (l < 0) ? -1 : b_pre[l + 1];
         const double lo = sorted[a], hi = sorted[b - 1];
         const double hw = 0.5 * (hi - lo);
@@ -1527,7 +1539,7 @@ __global__ __launch_bounds__(kBaryLanes) void node_group_kernel(
         }
         const double c = 0.5 * (lo + hi), hl = 0.5 * (hi - lo);
         if (r < NT) {
-            const double x = c + hl * tab[r];
+            const double x = c + hl * tab_r;
             t.nodes[(size_t)k * NT + r] = x;
             panel32[(size_t)k * 2 * NT + r] = (float)x;
             pos[g * NT + r] = x;
@@ -1540,9 +1552,6 @@ __global__ __launch_bounds__(kBaryLanes) void node_group_kernel(
     const unsigned long long big = __ballot(leaf && !small);
     if (!any_small && !big) return;
     wave_sync();
-    double lr[NT];
-#pragma unroll
-    for (int j = 0; j < NT; ++j) lr[j] = tab[NT + j];
     if (any_small) {
         double tr[NT];
 #pragma unroll
@@ -1554,7 +1563,7 @@ __global__ __launch_bounds__(kBaryLanes) void node_group_kernel(
             int hit = -2;  // as in bary_accumulate
             double m = 0.0;
             if (small && rr < cnt) {
-                const double x = sorted[a + rr];
+                const double x = (ch == 0) ? xrow0 : xrow1;
                 const double mass = 1.0;
                 double s = 0.0;
                 hit = -1;
