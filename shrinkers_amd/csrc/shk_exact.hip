@@ -556,6 +556,103 @@ int launch_reduce(const double *part, int n_split, size_t nq, double eta, double
     return 0;
 }
 
+// ---- few query points against many sources -----------------------------------
+//
+// A grid of query points (the El Karoui consumer's 200, the bulk fit's 512)
+// against a whole spectrum: with few query tiles the split plan above gives
+// each query hundreds of splits, and the in-order reduction of those
+// partials is a chain of dependent loads (about 60 us for 200 points against
+// 5e4 sources).  Here one workgroup takes one query point: thread t sums the
+// sources t, t + kGridWG, ... in order (coalesced loads), then the thread
+// sums are added in a fixed tree through LDS, and eta and the scale are
+// applied once (as reduce_kernel does).  Deterministic; only the summation
+// order differs from the split plan (both within the family's tolerances).
+// FP32-compensated: the pair difference is the FP64 difference rounded to
+// f32 (one rounding, exact for the near pairs by Sterbenz), the reciprocal
+// v_rcp_f32, and each thread's FP32 sums over kGridUnroll sources are folded
+// into FP64.  FP64: reciprocals two per v_rcp_f64 (rcp_pairs_f64) and FP64
+// sums.
+constexpr int kGridWG = 512;
+constexpr int kGridUnroll = 8;  // sources per thread and step: their loads go out together
+
+template <bool F32C>
+__global__ __launch_bounds__(kGridWG) void grid_query_kernel(const double *__restrict__ q,
+                                                             const double *__restrict__ src,
+                                                             int p, double eta, double scale,
+                                                             double *__restrict__ out_re,
+                                                             double *__restrict__ out_im) {
+    constexpr int U = kGridUnroll;
+    __shared__ double s_re[kGridWG], s_im[kGridWG];
+    const int t = threadIdx.x;
+    const double x = q[blockIdx.x];
+    const double e2 = eta * eta;
+    const float e2f = (float)e2;
+    double re = 0.0, im = 0.0;
+    int j = t;
+    // steps of U sources (j, j + kGridWG, ...): FP32-compensated sums each
+    // step's terms in FP32 and folds them into FP64
+    for (; j + (U - 1) * kGridWG < p; j += U * kGridWG) {
+        double d[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) d[u] = x - src[j + u * kGridWG];
+        if constexpr (F32C) {
+            float ar = 0.f, ai = 0.f;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const float df = (float)d[u];
+                const float r = __builtin_amdgcn_rcpf(__builtin_fmaf(df, df, e2f));
+                ar = __builtin_fmaf(df, r, ar);
+                ai += r;
+            }
+            re += (double)ar;
+            im += (double)ai;
+        } else {
+            double tt[U], rr[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) tt[u] = __builtin_fma(d[u], d[u], e2);
+            rcp_pairs_f64<U>(tt, rr);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                re = __builtin_fma(d[u], rr[u], re);
+                im += rr[u];
+            }
+        }
+    }
+    for (; j < p; j += kGridWG) {  // the last few sources, one at a time
+        const double d = x - src[j];
+        if constexpr (F32C) {
+            const float df = (float)d;
+            const float r = __builtin_amdgcn_rcpf(__builtin_fmaf(df, df, e2f));
+            re += (double)(df * r);
+            im += (double)r;
+        } else {
+            const double r = rcp_f64(__builtin_fma(d, d, e2));
+            re = __builtin_fma(d, r, re);
+            im += r;
+        }
+    }
+    s_re[t] = re;
+    s_im[t] = im;
+    __syncthreads();
+#pragma unroll
+    for (int w = kGridWG / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            s_re[t] += s_re[t + w];
+            s_im[t] += s_im[t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        out_re[blockIdx.x] = s_re[0] * scale;
+        out_im[blockIdx.x] = (eta * s_im[0]) * scale;
+    }
+}
+
+// A few query points against many sources: the grid kernel instead of the
+// split plan (at most kGridMaxQ points, at least 8 sources per point).
+constexpr size_t kGridMaxQ = 4096;
+bool grid_queries(size_t nq, size_t p) { return nq <= kGridMaxQ && 8 * nq <= p; }
+
 }  // namespace
 
 int exact_reduce_plain(const double *part, int n_split, size_t nq, double eta, double scale,
@@ -578,6 +675,12 @@ int exact_f64(const double *q, size_t nq, const double *src, size_t p, double et
     if (nq > (size_t)INT32_MAX || p > (size_t)INT32_MAX) {
         set_error("exact kernel: sizes above 2^31-1 are not supported");
         return 6;
+    }
+    if (kind == EXACT_PLAIN && grid_queries(nq, p)) {
+        hipLaunchKernelGGL(grid_query_kernel<false>, dim3((unsigned)nq), dim3(kGridWG), 0, stream,
+                           q, src, (int)p, eta, scale, out[0], out[1]);
+        SHK_HIP(hipGetLastError());
+        return 0;
     }
     int qt, ns, per;
     plan_splits(nq, p, &qt, &ns, &per);
@@ -622,6 +725,12 @@ int exact_f32c(const double *q, size_t nq, const double *src, size_t p, double e
     if (nq > (size_t)INT32_MAX || p > (size_t)INT32_MAX) {
         set_error("exact kernel: sizes above 2^31-1 are not supported");
         return 6;
+    }
+    if (!out_f32 && grid_queries(nq, p)) {
+        hipLaunchKernelGGL(grid_query_kernel<true>, dim3((unsigned)nq), dim3(kGridWG), 0, stream,
+                           q, src, (int)p, eta, scale, (double *)out_re, (double *)out_im);
+        SHK_HIP(hipGetLastError());
+        return 0;
     }
     int qt, ns, per;
     plan_splits(nq, p, &qt, &ns, &per);

@@ -526,14 +526,15 @@ def test_symmetric_above_old_maximum(shk):
 
 def test_exact_entries_under_capture(shk):
     """Under HIP graph capture (torch.cuda.graph on a side stream) the entry
-    points that need scratch memory (the all-points and at-points
-    transforms, the all-points adjoint) are refused with SHK_ERR_BAD_ARG
-    before any work is enqueued, since a captured call cannot share the
-    thread's cached buffers; the capture stays valid.  The no-allocation
-    entry captured beside them (f32c and FP64, one band and several) replays
-    bit-equal to its eager calls, three times, with eager calls of every
-    entry and synchronisations between the replays; the eager calls keep
-    their results bit for bit."""
+    points that need scratch memory (the all-points transform, the at-points
+    transform at many points, the all-points adjoint) are refused with
+    SHK_ERR_BAD_ARG before any work is enqueued, since a captured call
+    cannot share the thread's cached buffers; the capture stays valid.  The
+    no-allocation entry (f32c and FP64, one band and several) and the
+    at-points transform at a grid of points (no scratch), captured beside
+    them, replay bit-equal to their eager calls, three times, with eager
+    calls of every entry and synchronisations between the replays; the eager
+    calls keep their results bit for bit."""
     import torch
     from shrinkers_amd import _lib
     from shrinkers_amd import spectra as S
@@ -541,7 +542,8 @@ def test_exact_entries_under_capture(shk):
     ev1 = torch.from_numpy(np.ascontiguousarray(S.mp_inverse_cdf_spectrum(p1, c=0.25))).cuda()
     ev2 = torch.from_numpy(np.ascontiguousarray(S.mp_inverse_cdf_spectrum(p2, c=0.3))).cuda()
     e1, e2 = 1.0 / np.sqrt(p1), 1.0 / np.sqrt(p2)
-    q = ev1[::97].contiguous()
+    q = ev1[::97].contiguous()          # 207 points: the grid kernel, no scratch
+    q_many = ev1[:5000].contiguous()    # 5 000 points: the split plan and its partials
     rng = np.random.default_rng(5)
     g_re = torch.from_numpy(rng.standard_normal(p1)).cuda()
     g_im = torch.from_numpy(rng.standard_normal(p1)).cuda()
@@ -558,15 +560,24 @@ def test_exact_entries_under_capture(shk):
             ev1.data_ptr(), p1, e1, g_re.data_ptr(), g_im.data_ptr(), _lib.PREC_F32C,
             out[0].data_ptr(), out[1].data_ptr(), st.cuda_stream))
 
+    def grid(prec):
+        def call(out, st):
+            _lib.check(_lib.load().shk_exact_at_points_dev(
+                q.data_ptr(), q.numel(), ev1.data_ptr(), p1, e1, prec, out[0].data_ptr(),
+                out[1].data_ptr(), st.cuda_stream))
+        return call, (q.numel(), q.numel())
+
     captured = [na(ev1, e1, _lib.PREC_F32C), na(ev1, e1, _lib.PREC_F64),
-                na(ev2, e2, _lib.PREC_F32C)]
+                na(ev2, e2, _lib.PREC_F32C), grid(_lib.PREC_F64), grid(_lib.PREC_F32C),
+                (lambda out, st: shk.at_points_dev(q, ev1, e1, "blocked", out=out, stream=st),
+                 (q.numel(), q.numel()))]
     refused = [
         (lambda out, st: shk.all_points_dev(ev1, e1, "blocked", precision="f32c", out=out,
                                             stream=st), (p1, p1)),
         (lambda out, st: shk.all_points_dev(ev1, e1, "blocked", precision="f64", out=out,
                                             stream=st), (p1, p1)),
-        (lambda out, st: shk.at_points_dev(q, ev1, e1, "blocked", out=out, stream=st),
-         (q.numel(), q.numel())),
+        (lambda out, st: shk.at_points_dev(q_many, ev1, e1, "blocked", out=out, stream=st),
+         (q_many.numel(), q_many.numel())),
         (adjoint, (p1, 1)),
     ]
     main = torch.cuda.current_stream()
@@ -612,3 +623,36 @@ def test_exact_entries_under_capture(shk):
         assert same(outs, want_cap), rep
         assert same(run_all(captured), want_cap), rep
         assert same(run_all(refused), want_ref), rep
+
+
+@pytest.mark.parametrize("nq,p", [(1, 8), (1, 50_000), (20, 160), (200, 50_000), (513, 10_000),
+                                  (4096, 32_768), (4096, 40_001)])
+def test_grid_query_path(shk, orc, nq, p):
+    """A few query points against many sources (at most 4 096 points, at
+    least 8 sources per point) take the one-workgroup-per-point kernel
+    instead of the split plan: FP64 within 1e-13 of the oracle's at-points
+    sums, FP32-compensated within 1e-6, through shk_exact_at_points_dev and
+    (FP64) the method API.  Half of the points are eigenvalues themselves (a
+    zero pair difference), the rest spread beyond both ends of the
+    spectrum."""
+    import torch
+    from shrinkers_amd import _lib
+    from shrinkers_amd import spectra as S
+    from shrinkers_amd.stieltjes import _PRECISIONS
+    rng = np.random.default_rng(nq + p)
+    ev = (S.mp_inverse_cdf_spectrum(p, c=0.3, seed=1) if p >= 64
+          else np.sort(rng.uniform(0.2, 3.0, p)))
+    q = np.sort(np.concatenate([rng.uniform(ev[0] - 0.5, ev[-1] + 0.5, nq - nq // 2),
+                                rng.choice(ev, nq // 2)]))
+    eta = 1.0 / np.sqrt(p)
+    rr, ri = orc.at_points(q, ev, eta)
+    dq, de = torch.from_numpy(q).cuda(), torch.from_numpy(ev).cuda()
+    for prec, tol in (("f64", 1e-13), ("f32c", 1e-6)):
+        re, im = torch.full_like(dq, np.nan), torch.full_like(dq, np.nan)
+        assert _lib.load().shk_exact_at_points_dev(
+            dq.data_ptr(), nq, de.data_ptr(), p, eta, _PRECISIONS[prec], re.data_ptr(),
+            im.data_ptr(), None) == _lib.SHK_OK
+        torch.cuda.synchronize()
+        assert_close(re.cpu().numpy(), im.cpu().numpy(), rr, ri, tol, f"grid {prec} {nq}x{p}")
+    r, i = shk.compute_stieltjes_at_points(q, ev, eta, "blocked")
+    assert_close(r, i, rr, ri, 1e-13, f"grid method API {nq}x{p}")
