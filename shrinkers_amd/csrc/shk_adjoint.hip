@@ -470,6 +470,89 @@ __global__ __launch_bounds__(kAWG) void adj_qry_f64_kernel(const double *__restr
     }
 }
 
+// The query sweep at a few points against many sources (grid_queries): one
+// workgroup per point, thread t sums the sources t, t + kGridThreads, ...,
+// eight per step with their loads issued together (FP32-compensated: the
+// FP64 difference rounded to f32, each step's FP32 sums folded into FP64),
+// the thread sums are added in a fixed tree, and grad_x and the eta term
+// follow as in adj_qry_reduce_kernel.  No partial sums.
+constexpr int kGridStep = 8;
+
+template <bool F32C>
+__global__ __launch_bounds__(kGridThreads) void adj_grid_qry_kernel(
+    const double *__restrict__ q, const double *__restrict__ ev, int p, double eta,
+    const double *__restrict__ gre, const double *__restrict__ gim, double *__restrict__ grad_q,
+    double *__restrict__ eta_term) {
+    constexpr int U = kGridStep;
+    __shared__ double s_p[kGridThreads], s_q[kGridThreads];
+    const int t = threadIdx.x, i = blockIdx.x;
+    const double x = q[i];
+    const double e2 = eta * eta;
+    const float e2f = (float)e2;
+    double P = 0.0, Q = 0.0;
+    int j = t;
+    for (; j + (U - 1) * kGridThreads < p; j += U * kGridThreads) {
+        double d[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) d[u] = x - ev[j + u * kGridThreads];
+        if constexpr (F32C) {
+            float a = 0.f, b = 0.f;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const float df = (float)d[u];
+                const float r = __builtin_amdgcn_rcpf(__builtin_fmaf(df, df, e2f));
+                const float e = __builtin_fmaf(-df, df, e2f);
+                a = __builtin_fmaf(r, r * e, a);
+                b = __builtin_fmaf(r, r * df, b);
+            }
+            P += (double)a;
+            Q += (double)b;
+        } else {
+            double tt[U], rr[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) tt[u] = __builtin_fma(d[u], d[u], e2);
+            rcp_pairs_f64<U>(tt, rr);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const double e = __builtin_fma(-d[u], d[u], e2);
+                P = __builtin_fma(rr[u], rr[u] * e, P);
+                Q = __builtin_fma(rr[u], rr[u] * d[u], Q);
+            }
+        }
+    }
+    for (; j < p; j += kGridThreads) {  // the last few sources, one at a time
+        const double d = x - ev[j];
+        if constexpr (F32C) {
+            const float df = (float)d;
+            const float r = __builtin_amdgcn_rcpf(__builtin_fmaf(df, df, e2f));
+            const float e = __builtin_fmaf(-df, df, e2f);
+            P += (double)(r * (r * e));
+            Q += (double)(r * (r * df));
+        } else {
+            const double r = rcp_f64(__builtin_fma(d, d, e2));
+            P = __builtin_fma(r, r * __builtin_fma(-d, d, e2), P);
+            Q = __builtin_fma(r, r * d, Q);
+        }
+    }
+    s_p[t] = P;
+    s_q[t] = Q;
+    __syncthreads();
+#pragma unroll
+    for (int w = kGridThreads / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            s_p[t] += s_p[t + w];
+            s_q[t] += s_q[t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        const double dre = s_p[0], dim = -2.0 * eta * s_q[0];
+        const double a = gre[i], b = gim[i];
+        if (grad_q) grad_q[i] = a * dre + b * dim;
+        if (eta_term) eta_term[i] = a * dim - b * dre;
+    }
+}
+
 // Split reduction of (P, Q), then grad_x and the per-query eta term.
 __global__ __launch_bounds__(kRedWG) void adj_qry_reduce_kernel(
     const double *__restrict__ part, int n_split, int nq, double eta,
@@ -684,7 +767,29 @@ int exact_adjoint(const double *q, size_t nq, const double *ev, size_t p, double
             return 7;
         }
     }
-    if (grad_q || grad_eta) {
+    if ((grad_q || grad_eta) && grid_queries(nq, p)) {
+        double *eta_term = nullptr;
+        CachedBuf cache(CACHE_ADJ_PART, stream);
+        if (grad_eta)
+            if (int e = cache.get(nq, &eta_term)) return e;
+        if (f32c)
+            hipLaunchKernelGGL(adj_grid_qry_kernel<true>, dim3((unsigned)nq), dim3(kGridThreads),
+                               0, stream, q, ev, (int)p, eta, gre, gim, grad_q, eta_term);
+        else
+            hipLaunchKernelGGL(adj_grid_qry_kernel<false>, dim3((unsigned)nq), dim3(kGridThreads),
+                               0, stream, q, ev, (int)p, eta, gre, gim, grad_q, eta_term);
+        hipError_t le = hipGetLastError();
+        if (le == hipSuccess && grad_eta) {
+            hipLaunchKernelGGL(ordered_sum_kernel, dim3(1), dim3(kSumThreads), 0, stream,
+                               eta_term, (int)nq, 1.0, grad_eta, accumulate_eta);
+            le = hipGetLastError();
+        }
+        if (le != hipSuccess) {
+            set_error(std::string("adjoint (query sweep at grid points): ") +
+                      hipGetErrorString(le));
+            return 7;
+        }
+    } else if (grad_q || grad_eta) {
         int tiles, ns, per;
         plan(nq, p, &tiles, &ns, &per);
         double *part = nullptr;

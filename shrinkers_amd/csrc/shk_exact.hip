@@ -572,7 +572,7 @@ int launch_reduce(const double *part, int n_split, size_t nq, double eta, double
 // v_rcp_f32, and each thread's FP32 sums over kGridUnroll sources are folded
 // into FP64.  FP64: reciprocals two per v_rcp_f64 (rcp_pairs_f64) and FP64
 // sums.
-constexpr int kGridWG = 512;
+constexpr int kGridWG = kGridThreads;
 constexpr int kGridUnroll = 8;  // sources per thread and step: their loads go out together
 
 template <bool F32C>
@@ -647,11 +647,6 @@ __global__ __launch_bounds__(kGridWG) void grid_query_kernel(const double *__res
         out_im[blockIdx.x] = (eta * s_im[0]) * scale;
     }
 }
-
-// A few query points against many sources: the grid kernel instead of the
-// split plan (at most kGridMaxQ points, at least 8 sources per point).
-constexpr size_t kGridMaxQ = 4096;
-bool grid_queries(size_t nq, size_t p) { return nq <= kGridMaxQ && 8 * nq <= p; }
 
 }  // namespace
 

@@ -76,6 +76,26 @@ def test_adjoint_matches_oracle(shk, orc, precision, tol):
 
 
 @pytest.mark.parametrize("precision,tol", [("f64", 1e-12), ("f32c", 5e-6)])
+@pytest.mark.parametrize("p,nq", [(5000, 200), (10_000, 512), (40_000, 4096), (9, 1)])
+def test_adjoint_at_grid_points(shk, orc, precision, tol, p, nq):
+    """A grid of query points (at most 4 096, at least 8 eigenvalues per
+    point): the forward sums and the query sweep of the adjoint (dL/dx,
+    dL/deta) run one workgroup per point.  Against the oracle's FP64 adjoint,
+    with a quarter of the points on eigenvalues."""
+    ev = _spectrum(p) if p >= 64 else np.sort(np.random.default_rng(p).uniform(0.2, 3.0, p))
+    rng = np.random.default_rng(p + nq)
+    grid = np.sort(np.concatenate([rng.uniform(ev[0] - 0.3, ev[-1] + 0.3, nq - nq // 4),
+                                   rng.choice(ev, nq // 4, replace=False)]))
+    eta = 1.0 / np.sqrt(p)
+    gq, gev, ge, gre, gim = _adjoint_case(precision, grid, ev, eta, seed=p)
+    rev, rq, re_ = orc.adjoint_at_points(grid, ev, eta, gre, gim)
+    e_ev, e_q, e_eta = _rel(gev, rev), _rel(gq, rq), abs(ge - re_) / abs(re_)
+    print(f"grid adjoint {precision} {nq}x{p}: lam {e_ev:.3e} x {e_q:.3e} eta {e_eta:.3e}",
+          flush=True)
+    assert e_ev <= tol and e_q <= tol and e_eta <= tol
+
+
+@pytest.mark.parametrize("precision,tol", [("f64", 1e-12), ("f32c", 5e-6)])
 def test_adjoint_all_points(shk, orc, precision, tol):
     """Queries = eigenvalues: autograd adds the query and eigenvalue parts."""
     import torch
