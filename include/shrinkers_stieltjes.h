@@ -176,9 +176,13 @@ shk_status shk_symmetric_range(int precision, size_t *min_p, size_t *max_p);
  * counterpart in the reference (its kernels keep no partial sums). */
 shk_status shk_set_symmetric_scratch_limit(size_t bytes);
 
-/* Resolve the auto presets (src/config.rs:543-555) for this library:
- * Auto/SpeedAuto -> CHEBCODE_FAST, AccuracyAuto -> BLOCKED; a concrete
- * method is returned unchanged. */
+/* Resolve the auto presets (src/config.rs:543-555) for this library, by
+ * size as the reference's measured table does (src/config/pareto_autogen.rs):
+ * Auto/SpeedAuto -> BLOCKED for p <= 20 000 (the exact transform is the
+ * faster one there on MI355X), CHEBCODE_FAST above; AccuracyAuto -> BLOCKED.
+ * The at-points entries resolve Auto/SpeedAuto by the pair count instead
+ * (BLOCKED up to 2e8 query-eigenvalue pairs), and shk_all_points_many_dev by
+ * its largest spectrum.  A concrete method is returned unchanged. */
 int shk_resolve_auto_method(int method, int parallel, size_t p);
 
 /* ---- exact and treecode all-points transform ---------------------------- */

@@ -678,6 +678,28 @@ struct shk_chebcode {
     ChebTree *tree = nullptr;
 };
 
+namespace {
+// Speed presets on the GPU, from measured one-shot times on MI355X (MP
+// spectra, eta = 1/sqrt(p), `all_points_dev` / `at_points_dev`): the exact
+// all-points transform is faster than ChebCode FAST up to about p = 20 000
+// (FP64 0.128 vs 0.130 ms there, FP32-compensated 0.081; 0.015 vs 0.084 ms
+// at 1 000), and the exact at-points transform up to about 2e8 pairs (200
+// points x 5e4: 0.012 vs 0.125 ms; 2 000 x 5e4: 0.056 vs 0.137 ms; 4 096 x
+// 2e5: 0.42 vs 0.16 ms).  Both are also the more accurate.  The
+// reference's presets come from its own CPU measurements the same way
+// (src/config/pareto_autogen.rs).
+constexpr size_t kSpeedExactMaxP = 20000;
+constexpr double kSpeedExactMaxPairs = 2e8;
+
+int resolve_auto_at_points(int method, size_t nq, size_t p) {
+    if (method == SHK_AUTO || method == SHK_SPEED_AUTO)
+        return ((double)nq * (double)p <= kSpeedExactMaxPairs) ? SHK_BLOCKED
+                                                                 : SHK_CHEBCODE_FAST;
+    return shk_resolve_auto_method(method, 0, p);
+}
+
+}  // namespace
+
 extern "C" {
 
 const char *shk_last_error(void) { return last_error(); }
@@ -720,11 +742,10 @@ shk_status shk_symmetric_range(int precision, size_t *min_p, size_t *max_p) {
 
 int shk_resolve_auto_method(int method, int parallel, size_t p) {
     (void)parallel;
-    (void)p;
     switch (method) {
         case SHK_AUTO:
         case SHK_SPEED_AUTO:
-            return SHK_CHEBCODE_FAST;
+            return (p <= kSpeedExactMaxP) ? SHK_BLOCKED : SHK_CHEBCODE_FAST;
         case SHK_ACCURACY_AUTO:
             return SHK_BLOCKED;
         default:
@@ -837,7 +858,7 @@ shk_status shk_compute_stieltjes_at_points(const double *query_points, size_t nq
     if (!all_finite(eigenvalues, p) || !all_finite(query_points, nq))
         return fail(SHK_ERR_NONFINITE, "eigenvalues must be finite (found NaN or infinity)");
     SHK_CHECK(check_eta(eta));
-    method = shk_resolve_auto_method(method, parallel, p);
+    method = resolve_auto_at_points(method, nq, p);
     SHK_CHECK(check_method(method));
     SHK_CHECK(ensure_device());
     hipStream_t s = host_stream();
@@ -953,7 +974,14 @@ shk_status shk_all_points_many_dev(const double *d_eigenvalues, const int64_t *o
     if (precision == SHK_PREC_F32C)
         for (size_t k = 0; k < n_spectra; ++k)
             if (!f32c_eta_ok(etas[k])) precision = SHK_PREC_F64;
-    method = shk_resolve_auto_method(method, 0, (size_t)offsets[n_spectra]);
+    {
+        // the auto presets by the largest spectrum: the segmented exact launch
+        // while every spectrum is in the exact family's fast range
+        size_t p_max = 0;
+        for (size_t k = 0; k < n_spectra; ++k)
+            p_max = std::max(p_max, (size_t)(offsets[k + 1] - offsets[k]));
+        method = shk_resolve_auto_method(method, 0, p_max);
+    }
     SHK_CHECK(check_method(method));
     SHK_CHECK(ensure_device());
     const bool exact = family_of(method) == FAM_EXACT;
@@ -1116,7 +1144,7 @@ shk_status shk_at_points_dev(const double *d_query, size_t nq, const double *d_e
     if (!d_query || !d_eigenvalues || !d_out_re || !d_out_im)
         return fail(SHK_ERR_BAD_ARG, "NULL pointer");
     SHK_CHECK(check_eta(eta));
-    method = shk_resolve_auto_method(method, 0, p);
+    method = resolve_auto_at_points(method, nq, p);
     SHK_CHECK(check_method(method));
     SHK_CHECK(ensure_device());
     return (shk_status)run_atpoints(d_query, nq, d_eigenvalues, p, eta, method, cutoff_ratio,

@@ -79,7 +79,9 @@ def test_auto_resolution(shk):
     from shrinkers_amd import _lib
     lib = _lib.load()
     assert lib.shk_resolve_auto_method(19, 0, 50000) == 14   # auto -> chebcode_fast
-    assert lib.shk_resolve_auto_method(21, 1, 10) == 14      # speed_auto
+    assert lib.shk_resolve_auto_method(21, 1, 20001) == 14   # speed_auto above 20 000
+    assert lib.shk_resolve_auto_method(21, 1, 20000) == 2    # speed_auto: exact up to 20 000
+    assert lib.shk_resolve_auto_method(19, 0, 10) == 2
     assert lib.shk_resolve_auto_method(22, 0, 10) == 2       # accuracy_auto -> blocked
     assert lib.shk_resolve_auto_method(4, 0, 10) == 4
     assert abs(lib.shk_default_eta(400) - 0.005) < 1e-15
