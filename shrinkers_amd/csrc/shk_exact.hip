@@ -58,11 +58,21 @@ namespace {
 #ifndef SHK_EXACT_TARGET_WG
 #define SHK_EXACT_TARGET_WG 16384
 #endif
+#ifndef SHK_EXACT_MIN_SRC
+#define SHK_EXACT_MIN_SRC 64
+#endif
 constexpr int kWG = SHK_EXACT_WG;          // threads per workgroup
 constexpr int kR = SHK_EXACT_R;            // queries per thread
 constexpr int kQT = kWG * kR;              // queries per tile
 constexpr int kPiece = SHK_EXACT_PIECE;    // sources staged in LDS per pass
 constexpr int kTargetWG = SHK_EXACT_TARGET_WG;  // workgroups to aim for
+// Sources per split at least.  Measured (all-points, ms, f64 / f32c): 64 ->
+// p = 1 000: 0.014 / 0.015, 5 000: 0.029 / 0.023, 8 000: 0.045 / 0.035;
+// 128 -> 0.017 / 0.013, 0.025 / 0.018, 0.040 / 0.029; 256 -> 0.027 / 0.019,
+// 0.031 / 0.021, 0.040 / 0.026; grids of 8 000 and 20 000 points against
+// 5e4 sources and the 512-spectrum batch unchanged.  64 keeps the FP64
+// transform at p ~ 1 000 (the spiked-Wishart workload) fastest.
+constexpr int kMinSrc = SHK_EXACT_MIN_SRC;
 
 template <int KIND>
 struct NAcc {
@@ -524,7 +534,7 @@ __global__ __launch_bounds__(kWG) void many_f64_kernel(const double *__restrict_
 void plan_splits(size_t nq, size_t p, int *n_qtiles, int *n_split, int *src_per_split) {
     const int qt = (int)((nq + kQT - 1) / kQT);
     int ns = std::max(1, (kTargetWG + qt - 1) / qt);
-    const int max_split = (int)std::max<size_t>(1, (p + 63) / 64);  // >= 64 sources each
+    const int max_split = (int)std::max<size_t>(1, (p + kMinSrc - 1) / kMinSrc);
     ns = std::min(ns, max_split);
     int per = (int)((p + ns - 1) / ns);
     per = (per + 31) / 32 * 32;
