@@ -596,3 +596,29 @@ def test_builds_refused_under_capture(shk):
     r, i = shk.all_points_dev(ev, eta, "chebcode_fast")
     torch.cuda.synchronize()
     assert torch.isfinite(r).all() and torch.isfinite(i).all()
+
+
+def test_fast_wave_walk_is_bitwise_the_per_lane_walk(shk):
+    """The FAST all-points evaluation at 6e5 eigenvalues walks the tree once
+    per wavefront (the union of its 64 queries' paths; above 2^19 queries,
+    without the held loads).  Each query still accumulates its own terms in
+    the reference's visit order, so the values equal the per-lane walk (the
+    tree's evaluate_points at fewer points than sources) bit for bit, on
+    sorted and on shuffled input."""
+    import torch
+    from shrinkers_amd import spectra as S
+    theta, n, leaf, f32 = shk.CHEB_PRESETS["chebcode_fast"]
+    p = 600_000
+    ev = S.mp_inverse_cdf_spectrum(p, c=0.3, seed=2)
+    eta = 1.0 / np.sqrt(p)
+    rng = np.random.default_rng(3)
+    for name, arr in (("sorted", ev), ("shuffled", ev[rng.permutation(p)])):
+        d = torch.from_numpy(np.ascontiguousarray(arr)).cuda()
+        b = shk.ChebCodeBatch.build_dev(d, theta, n, leaf, bool(f32))
+        r2, i2 = b.evaluate_dev(eta, 1.0)
+        torch.cuda.synchronize()
+        r2, i2 = r2.cpu().numpy(), i2.cpu().numpy()
+        k = 500_000  # per-lane walk: fewer query points than sources
+        r1, i1 = b.evaluate_points(arr[:k], eta)
+        np.testing.assert_array_equal(r2[:k], r1, err_msg=name)
+        np.testing.assert_array_equal(i2[:k], i1, err_msg=name)
