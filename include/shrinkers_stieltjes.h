@@ -32,9 +32,10 @@
  *
  * HIP graph capture: entry points whose work is captured whole are
  * shk_all_points_noalloc_dev, the exact at-points transforms at a grid of
- * points (shk_at_points_dev with an exact method and shk_exact_at_points_dev,
- * for at most 4 096 points and at least 8 eigenvalues per point: one
- * workgroup per point, no scratch), the evaluations of a built tree
+ * points (shk_at_points_dev with an exact method and no cutoff, and
+ * shk_exact_at_points_dev, for at most 4 096 points and at least 8
+ * eigenvalues per point: one workgroup per point, no scratch), the
+ * evaluations of a built tree
  * (shk_chebcode_evaluate_dev, shk_chebcode_evaluate_many_dev; SHK_ERR_BAD_ARG
  * if the tree's build has not finished) and the prepared batch
  * (shk_batch_run, which captures its own graph).  On a stream under capture,
