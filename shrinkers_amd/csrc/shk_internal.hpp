@@ -141,12 +141,17 @@ int exact_f32c(const double *q, size_t nq, const double *src, size_t p, double e
 
 // Scratch doubles the exact kernels need for nq queries against p sources.
 size_t exact_workspace_doubles(size_t nq, size_t p, ExactKind kind);
+// Threads of a grid-point workgroup (below); each sums every kGridThreads-th
+// source.
+constexpr int kGridThreads = 512;
 // A few query points against many sources (at most 4 096 points, at least 8
 // sources per point): the forward and adjoint sums at such points run one
-// workgroup per point instead of the split plan, with no partial sums.
-inline bool grid_queries(size_t nq, size_t p) { return nq <= 4096 && 8 * nq <= p; }
-// Threads of such a workgroup; each sums every kGridThreads-th source.
-constexpr int kGridThreads = 512;
+// workgroup per point instead of the split plan, with no partial sums.  The
+// kernels index sources in 32 bits and step up to 16 kGridThreads past the
+// last one, so p stays that far below 2^31.
+inline bool grid_queries(size_t nq, size_t p) {
+    return nq <= 4096 && 8 * nq <= p && p <= (size_t)INT32_MAX - 16 * (size_t)kGridThreads;
+}
 // In-order sum of n_split FP64 partials laid out [re/im][split][query], times
 // scale, with eta applied to the imaginary part (kind EXACT_PLAIN).
 int exact_reduce_plain(const double *part, int n_split, size_t nq, double eta, double scale,

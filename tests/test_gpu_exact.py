@@ -656,3 +656,25 @@ def test_grid_query_path(shk, orc, nq, p):
         assert_close(re.cpu().numpy(), im.cpu().numpy(), rr, ri, tol, f"grid {prec} {nq}x{p}")
     r, i = shk.compute_stieltjes_at_points(q, ev, eta, "blocked")
     assert_close(r, i, rr, ri, 1e-13, f"grid method API {nq}x{p}")
+
+
+def test_speed_auto_follows_the_measured_table(shk, orc):
+    """SpeedAuto resolves by size: the exact transform up to p = 20 000 (FP64
+    within 1e-13 of the oracle), ChebCode FAST above (the same bits as asking
+    for chebcode_fast); at points by the pair count (exact up to 2e8 pairs)."""
+    from shrinkers_amd import spectra as S
+    ev = S.mp_inverse_cdf_spectrum(1000, c=0.25)
+    eta = 1.0 / np.sqrt(ev.size)
+    r, i = shk.compute_all_stieltjes(ev, eta, "speed_auto")
+    rr, ri = orc.tiled_parallel(ev, eta)
+    assert_close(r, i, rr / ev.size, ri / ev.size, 1e-13, "speed_auto p=1000")
+    big = S.mp_inverse_cdf_spectrum(30_000, c=0.25)
+    eta = 1.0 / np.sqrt(big.size)
+    a = shk.compute_all_stieltjes(big, eta, "speed_auto")
+    b = shk.compute_all_stieltjes(big, eta, "chebcode_fast")
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+    q = np.linspace(big[0] - 0.2, big[-1] + 0.2, 200)
+    r, i = shk.compute_stieltjes_at_points(q, big, eta, "speed_auto")
+    rr, ri = orc.at_points(q, big, eta)
+    assert_close(r, i, rr, ri, 1e-13, "speed_auto at 200 points")
