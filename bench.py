@@ -276,6 +276,40 @@ def many_leg(shk, sync) -> dict:
     return out
 
 
+def workspace_times(shk, d, eta, method, sync, reps=10) -> dict:
+    """The ChebCode one-shot transform built in a caller's workspace (no
+    host wait, no allocation): eager calls, and the call captured in a HIP
+    graph and replayed."""
+    import torch
+    nbytes = shk.chebcode_workspace_bytes(d.numel(), method)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    out = (torch.empty_like(d), torch.empty_like(d))
+    for _ in range(3):
+        shk.chebcode_all_points_ws_dev(d, eta, method, ws, out=out)
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        shk.chebcode_all_points_ws_dev(d, eta, method, ws, out=out)
+    sync()
+    eager = (time.perf_counter() - t0) / reps * 1e3
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        shk.chebcode_all_points_ws_dev(d, eta, method, ws, out=out, stream=s)
+    for _ in range(3):
+        g.replay()
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.replay()
+    sync()
+    graph = (time.perf_counter() - t0) / reps * 1e3
+    del g
+    return {"ms_workspace_one_shot": eager, "ms_workspace_graph_replay": graph,
+            "workspace_bytes": nbytes}
+
+
 def chebcode_leg(shk, sync) -> dict:
     """ChebCode at p = 2e5 and 1e6 (FAST, DEFAULT and the f64 n = 12 point),
     error against the GPU FP64 exact kernel, build+evaluate and evaluate-only times."""
@@ -340,6 +374,7 @@ def chebcode_leg(shk, sync) -> dict:
                     shk.all_points_dev(d, eta, name)
                 sync()
                 row[name]["ms_transform_one_shot"] = (time.perf_counter() - t0) / 10 * 1e3
+                row[name].update(workspace_times(shk, d, eta, name, sync))
             if name == "chebcode":
                 # evaluate_many: four etas, two per traversal (host outputs)
                 etas = [eta * f for f in (0.5, 1.0, 2.0, 4.0)]
@@ -669,7 +704,8 @@ def main() -> int:
         "rel_l2_f32c_vs_fp64_gpu": rel(r32, r64),
         "rel_l2_f32c_vs_fp64_oracle": err_oracle_32,
         "rel_l2_fp64_gpu_vs_fp64_oracle": err_oracle_64,
-        "chebcode_fast": {"ms_end_to_end": tcf * 1e3, "eval_kernel_ms": kcf},
+        "chebcode_fast": {"ms_end_to_end": tcf * 1e3, "eval_kernel_ms": kcf,
+                          **workspace_times(shk, d_ev, eta, "chebcode_fast", sync)},
     }
     if cpu:
         line["cpu_baseline"]["gpu_over_cpu"] = value / cpu["value"]

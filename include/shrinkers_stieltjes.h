@@ -38,7 +38,9 @@
  * evaluations of a built tree
  * (shk_chebcode_evaluate_dev, shk_chebcode_evaluate_many_dev; SHK_ERR_BAD_ARG
  * if the tree's build has not finished) and the prepared batch
- * (shk_batch_run, which captures its own graph).  On a stream under capture,
+ * (shk_batch_run, which captures its own graph), and the ChebCode one-shot
+ * transforms in a caller's workspace (shk_chebcode_all_points_ws_dev,
+ * shk_chebcode_at_points_ws_dev).  On a stream under capture,
  * a ChebCode build (including the ChebCode methods of shk_all_points_dev and
  * shk_at_points_dev), shk_all_points_many_dev, and every other _dev entry
  * point that needs scratch memory (the cached buffers below cannot be shared
@@ -284,6 +286,50 @@ shk_status shk_all_points_many_dev(const double *d_eigenvalues, const int64_t *o
 shk_status shk_at_points_dev(const double *d_query, size_t nq, const double *d_eigenvalues,
                              size_t p, double eta, int method, double cutoff_ratio,
                              double *d_out_re, double *d_out_im, void *stream);
+
+/* ---- ChebCode one-shot transforms in a caller's workspace ------------------
+ *
+ * The ChebCode methods of shk_all_points_dev / shk_at_points_dev
+ * (compute_all_stieltjes_chebcode_preset, src/stieltjes/chebcode.rs:1015-1120,
+ * and the at-points arm, src/stieltjes/mod.rs:461-479) with every size kept
+ * on the device: no host wait and no allocation, so the whole call can be
+ * captured in a HIP graph and replayed on new eigenvalues of the same count.
+ * The reference has no such form; these entries are for callers that
+ * capture or cannot let a call wait on the host.
+ *
+ * The eigenvalues must be ascending (as symmetric eigensolvers return them).
+ * On ascending input the results equal shk_all_points_dev's /
+ * shk_at_points_dev's for the same method bit for bit.  Input that is not
+ * ascending (or holds NaN) gives NaN outputs; shk_chebcode_workspace_status
+ * reports it after the fact.  method: a ChebCode preset (13..16); the auto
+ * methods are refused (SHK_ERR_BAD_METHOD).
+ *
+ * The workspace (device memory, 256-byte aligned, at least
+ * shk_chebcode_workspace_bytes(p, method) bytes) holds the tree at its largest
+ * possible node count (2p plus the top walk's slots), so it is larger than
+ * the tree the call builds: about 620 bytes per eigenvalue at the FAST order
+ * and 770 at n = 11, plus 0.6 MB (p = 1e6: 0.62 / 0.77 GB).  Calls that share
+ * a workspace must not overlap (the same stream orders them). */
+shk_status shk_chebcode_workspace_bytes(size_t p, int method, size_t *bytes);
+
+/* Scaled (1/p) when scaled != 0, else raw sums, at every eigenvalue. */
+shk_status shk_chebcode_all_points_ws_dev(const double *d_eigenvalues, size_t p, double eta,
+                                          int method, int scaled, void *d_workspace,
+                                          size_t workspace_bytes, double *d_out_re,
+                                          double *d_out_im, void *stream);
+
+/* Raw sums at nq device query points (the tree at grid_leaf_cap's leaf size,
+ * as shk_at_points_dev builds it). */
+shk_status shk_chebcode_at_points_ws_dev(const double *d_query, size_t nq,
+                                         const double *d_eigenvalues, size_t p, double eta,
+                                         int method, void *d_workspace, size_t workspace_bytes,
+                                         double *d_out_re, double *d_out_im, void *stream);
+
+/* *ascending = 1 if the last call on this workspace found its eigenvalues
+ * ascending (its outputs are the transform), 0 if not (its outputs are NaN).
+ * Waits for the stream. */
+shk_status shk_chebcode_workspace_status(const void *d_workspace, void *stream,
+                                         int *ascending);
 
 /* ---- ChebCodeBatch (src/stieltjes/chebcode.rs:1148-1362) ----------------- */
 

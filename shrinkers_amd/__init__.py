@@ -13,6 +13,10 @@ from .stieltjes import (  # noqa: F401
     ChebCodeBatch,
     all_points_dev,
     at_points_dev,
+    chebcode_all_points_ws_dev,
+    chebcode_at_points_ws_dev,
+    chebcode_workspace_bytes,
+    chebcode_workspace_status,
     compute_all_stieltjes,
     compute_all_stieltjes_f32,
     compute_all_stieltjes_with_deriv,

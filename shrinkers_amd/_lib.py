@@ -61,6 +61,11 @@ SIGNATURES = {
     "shk_all_points_dev": (_i, [_vp, _sz, _d, _i, _d, _i, _i, _vp, _vp, _vp]),
     "shk_all_points_noalloc_dev": (_i, [_vp, _sz, _d, _i, _i, _vp, _vp, _vp]),
     "shk_at_points_dev": (_i, [_vp, _sz, _vp, _sz, _d, _i, _d, _vp, _vp, _vp]),
+    "shk_chebcode_workspace_bytes": (_i, [_sz, _i, _szp]),
+    "shk_chebcode_all_points_ws_dev": (_i, [_vp, _sz, _d, _i, _i, _vp, _sz, _vp, _vp, _vp]),
+    "shk_chebcode_at_points_ws_dev": (_i, [_vp, _sz, _vp, _sz, _d, _i, _vp, _sz, _vp, _vp,
+                                           _vp]),
+    "shk_chebcode_workspace_status": (_i, [_vp, _vp, _ip]),
     "shk_all_points_many_dev": (_i, [_vp, _i64p, _sz, _dp, _i, _i, _i, _vp, _vp, _i,
                                      ctypes.POINTER(ctypes.c_float), _vp]),
     "shk_chebcode_build": (_i, [_dp, _sz, _d, _i, _sz, _i, ctypes.POINTER(_vp)]),

@@ -1151,6 +1151,69 @@ shk_status shk_at_points_dev(const double *d_query, size_t nq, const double *d_e
                                     d_out_re, d_out_im, (hipStream_t)stream);
 }
 
+// ---- ChebCode one-shot transforms in a caller's workspace --------------------
+
+namespace {
+// The workspace entries take the ChebCode presets only (no auto method: its
+// family depends on the size).
+shk_status ws_method(int method) {
+    if (family_of(method) != FAM_CHEB)
+        return fail(SHK_ERR_BAD_METHOD,
+                    "the workspace entries take a ChebCode preset (chebcode, chebcode_fast, "
+                    "chebcode_xtreme, chebcode_balanced)");
+    return SHK_OK;
+}
+}  // namespace
+
+shk_status shk_chebcode_workspace_bytes(size_t p, int method, size_t *bytes) {
+    if (!bytes) return fail(SHK_ERR_BAD_ARG, "NULL pointer");
+    *bytes = 0;
+    if (p == 0) return SHK_OK;
+    SHK_CHECK(ws_method(method));
+    if (p > (size_t)INT32_MAX / 2) return fail(SHK_ERR_BAD_ARG, "p must be below 2^30");
+    *bytes = cheb_ws_bytes(p, preset_of(method).n);
+    return SHK_OK;
+}
+
+shk_status shk_chebcode_all_points_ws_dev(const double *d_eigenvalues, size_t p, double eta,
+                                          int method, int scaled, void *d_workspace,
+                                          size_t workspace_bytes, double *d_out_re,
+                                          double *d_out_im, void *stream) {
+    if (p == 0) return SHK_OK;
+    if (!d_eigenvalues || !d_workspace || !d_out_re || !d_out_im)
+        return fail(SHK_ERR_BAD_ARG, "NULL pointer");
+    SHK_CHECK(check_eta(eta));
+    SHK_CHECK(ws_method(method));
+    SHK_CHECK(ensure_device());
+    const double scale = scaled ? 1.0 / (double)p : 1.0;
+    return (shk_status)cheb_oneshot_ws(d_eigenvalues, p, preset_of(method), nullptr, p, eta, scale,
+                                       1, d_workspace, workspace_bytes, d_out_re, d_out_im,
+                                       (hipStream_t)stream);
+}
+
+shk_status shk_chebcode_at_points_ws_dev(const double *d_query, size_t nq,
+                                         const double *d_eigenvalues, size_t p, double eta,
+                                         int method, void *d_workspace, size_t workspace_bytes,
+                                         double *d_out_re, double *d_out_im, void *stream) {
+    if (nq == 0 || p == 0) return SHK_OK;
+    if (!d_query || !d_eigenvalues || !d_workspace || !d_out_re || !d_out_im)
+        return fail(SHK_ERR_BAD_ARG, "NULL pointer");
+    SHK_CHECK(check_eta(eta));
+    SHK_CHECK(ws_method(method));
+    SHK_CHECK(ensure_device());
+    ChebParams prm = preset_of(method);
+    prm.leaf_cap = grid_leaf_cap(prm.n, (int64_t)p, (int64_t)nq, prm.leaf_cap);
+    return (shk_status)cheb_oneshot_ws(d_eigenvalues, p, prm, d_query, nq, eta, 1.0, 0,
+                                       d_workspace, workspace_bytes, d_out_re, d_out_im,
+                                       (hipStream_t)stream);
+}
+
+shk_status shk_chebcode_workspace_status(const void *d_workspace, void *stream, int *ascending) {
+    if (!d_workspace || !ascending) return fail(SHK_ERR_BAD_ARG, "NULL pointer");
+    SHK_CHECK(ensure_device());
+    return (shk_status)cheb_ws_status(d_workspace, (hipStream_t)stream, ascending);
+}
+
 shk_status shk_exact_at_points_dev(const double *d_query, size_t nq, const double *d_eigenvalues,
                                    size_t p, double eta, int precision, double *d_out_re,
                                    double *d_out_im, void *stream) {

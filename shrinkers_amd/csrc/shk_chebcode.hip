@@ -76,6 +76,9 @@ struct ChebTree {
     bool cached_src = false, cached_nodes = false;
     hipStream_t stream = nullptr;
     hipEvent_t ready = nullptr;  // recorded after the build's last kernel (none if transient)
+    // a tree built in a caller's workspace with its sizes on the device
+    // (cheb_oneshot_ws): the build's info words, which the walks read
+    const int *info_dev = nullptr;
 };
 
 namespace {
@@ -295,6 +298,16 @@ constexpr int kInfoT = 4, kInfoF = 5, kInfoD = 6, kInfoFront = 7;
 constexpr int kInfoTopLevel = 8;                          // kMaxDepth + 3 entries
 constexpr int kInfoInts = kInfoTopLevel + kMaxDepth + 3;
 static_assert(kMailData + kInfoInts <= kMailInts, "the info words fit in the host mailbox");
+
+// A build whose later kernels read its sizes from the info words: it failed
+// when its input was not ascending or its node slots ran out, and then has
+// no nodes.
+__device__ __forceinline__ bool dev_build_failed(const int *info) {
+    return info[kInfoUnsorted] != 0 || info[kInfoOverflow] != 0;
+}
+__device__ __forceinline__ int dev_node_count(const int *info) {
+    return dev_build_failed(info) ? 0 : info[kInfoN];
+}
 
 struct TopoCtx {
     const double *sorted;
@@ -1133,8 +1146,10 @@ __global__ __launch_bounds__(kTopoThreads) void topo_mid_kernel(
     __shared__ int s_wave[kTopoThreads / 64];
     __shared__ int s_left[kMidLds], s_size[kMidLds], s_pre[kMidLds];
     const int tid = threadIdx.x;
-    // the info words to the host mailbox, then its sequence number
+    // the info words to the host mailbox, then its sequence number (a build
+    // whose later kernels read the info words themselves passes no mailbox)
     auto publish = [&]() {
+        if (!mail) return;
         __syncthreads();
         for (int i = tid; i < kInfoInts; i += kTopoThreads) mail[kMailData + i] = info[i];
         __threadfence_system();
@@ -1426,15 +1441,15 @@ __device__ __forceinline__ BaryLds bary_lds(double *base, int n) {
 // (pre-order number, right child) and the f32 mirrors (chebcode.rs:551-570);
 // a leaf also gets its weights from its sources (chebcode.rs:401-460).
 // Internal nodes get theirs from the merge kernels.
-__global__ __launch_bounds__(kBaryLanes) void node_kernel(
-    const double *__restrict__ sorted, int n, double theta_sq, const int *__restrict__ list,
-    const int *__restrict__ b_lo, const int *__restrict__ b_hi, const int *__restrict__ b_left,
-    const int *__restrict__ b_size, const int *__restrict__ b_pre, const double *__restrict__ tab,
-    TreeArrays t, int2 *__restrict__ mrec, float4 *__restrict__ rec32, int4 *__restrict__ link,
-    float *__restrict__ panel32) {
-    extern __shared__ double lds[];
+__device__ __forceinline__ void node_one(
+    int slot, const double *__restrict__ sorted, int n, double theta_sq,
+    const int *__restrict__ list, const int *__restrict__ b_lo, const int *__restrict__ b_hi,
+    const int *__restrict__ b_left, const int *__restrict__ b_size, const int *__restrict__ b_pre,
+    const double *__restrict__ tab, const TreeArrays &t, int2 *__restrict__ mrec,
+    float4 *__restrict__ rec32, int4 *__restrict__ link, float *__restrict__ panel32,
+    double *lds) {
     const int lane = threadIdx.x;
-    const int v = list[blockIdx.x];
+    const int v = list[slot];
     const int k = b_pre[v], a = b_lo[v], b = b_hi[v], l = b_left[v], size = b_size[v];
     const int right = (l < 0) ? -1 : b_pre[l + 1];
     const double lo = sorted[a], hi = sorted[b - 1];
@@ -1472,6 +1487,29 @@ __global__ __launch_bounds__(kBaryLanes) void node_kernel(
                     panel32 + (size_t)k * 2 * n + n, n, bl.sq, bl.sm, bl.shit);
 }
 
+// The node count is the grid, or the info words' count when `info` is set
+// (a build whose sizes stay on the device): the workgroups then take slots
+// blockIdx.x, blockIdx.x + gridDim.x, ...
+__global__ __launch_bounds__(kBaryLanes) void node_kernel(
+    const double *__restrict__ sorted, int n, double theta_sq, const int *__restrict__ list,
+    const int *__restrict__ b_lo, const int *__restrict__ b_hi, const int *__restrict__ b_left,
+    const int *__restrict__ b_size, const int *__restrict__ b_pre, const double *__restrict__ tab,
+    TreeArrays t, int2 *__restrict__ mrec, float4 *__restrict__ rec32, int4 *__restrict__ link,
+    float *__restrict__ panel32, const int *__restrict__ info) {
+    extern __shared__ double lds[];
+    if (!info) {
+        node_one(blockIdx.x, sorted, n, theta_sq, list, b_lo, b_hi, b_left, b_size, b_pre, tab, t,
+                 mrec, rec32, link, panel32, lds);
+        return;
+    }
+    const int count = dev_node_count(info);
+    for (int s = blockIdx.x; s < count; s += gridDim.x) {
+        wave_sync();  // the previous node's LDS reads are done
+        node_one(s, sorted, n, theta_sq, list, b_lo, b_hi, b_left, b_size, b_pre, tab, t, mrec,
+                 rec32, link, panel32, lds);
+    }
+}
+
 // node_kernel for the presets' orders with kNodeGroup nodes per wavefront:
 // lane group g (kGroupLanes lanes) writes node list[G*blockIdx + g]'s
 // records and positions and, for a leaf of up to kGroupRowsMax sources,
@@ -1489,17 +1527,17 @@ __host__ __device__ inline int node_group_lds_doubles(int n) {
 }
 
 template <int NT>
-__global__ __launch_bounds__(kBaryLanes) void node_group_kernel(
-    const double *__restrict__ sorted, int count, double theta_sq, const int *__restrict__ list,
-    const int *__restrict__ b_lo, const int *__restrict__ b_hi, const int *__restrict__ b_left,
-    const int *__restrict__ b_size, const int *__restrict__ b_pre, const double *__restrict__ tab,
-    TreeArrays t, int2 *__restrict__ mrec, float4 *__restrict__ rec32, int4 *__restrict__ link,
-    float *__restrict__ panel32) {
+__device__ __forceinline__ void node_group_one(
+    int grp, const double *__restrict__ sorted, int count, double theta_sq,
+    const int *__restrict__ list, const int *__restrict__ b_lo, const int *__restrict__ b_hi,
+    const int *__restrict__ b_left, const int *__restrict__ b_size, const int *__restrict__ b_pre,
+    const double *__restrict__ tab, const TreeArrays &t, int2 *__restrict__ mrec,
+    float4 *__restrict__ rec32, int4 *__restrict__ link, float *__restrict__ panel32,
+    double *lds) {
     static_assert(NT <= kGroupLanes, "one lane per weight in a lane group");
-    extern __shared__ double lds[];
     const int lane = threadIdx.x;
     const int g = lane / kGroupLanes, r = lane - g * kGroupLanes;
-    const int slot = blockIdx.x * kNodeGroup + g;
+    const int slot = grp * kNodeGroup + g;
     const BaryLds bl = bary_lds(lds, NT);
     double *pos = lds + bary_lds_doubles(NT);  // kNodeGroup x NT positions
     // The loads that do not depend on the node go out first, and a leaf's
@@ -1604,6 +1642,30 @@ __global__ __launch_bounds__(kBaryLanes) void node_group_kernel(
         bary_accumulate<NT>(row, cg, pos + gg * NT, tab + NT, t.w + (size_t)kg * NT,
                             panel32 + (size_t)kg * 2 * NT + NT, NT, bl.sq, bl.sm, bl.shit);
         wave_sync();  // the next leaf overwrites sq, sm, shit
+    }
+}
+
+// The node count is `count`, or the info words' count when `info` is set
+// (a build whose sizes stay on the device): the workgroups then take groups
+// blockIdx.x, blockIdx.x + gridDim.x, ...
+template <int NT>
+__global__ __launch_bounds__(kBaryLanes) void node_group_kernel(
+    const double *__restrict__ sorted, int count, double theta_sq, const int *__restrict__ list,
+    const int *__restrict__ b_lo, const int *__restrict__ b_hi, const int *__restrict__ b_left,
+    const int *__restrict__ b_size, const int *__restrict__ b_pre, const double *__restrict__ tab,
+    TreeArrays t, int2 *__restrict__ mrec, float4 *__restrict__ rec32, int4 *__restrict__ link,
+    float *__restrict__ panel32, const int *__restrict__ info) {
+    extern __shared__ double lds[];
+    if (!info) {
+        node_group_one<NT>(blockIdx.x, sorted, count, theta_sq, list, b_lo, b_hi, b_left, b_size,
+                           b_pre, tab, t, mrec, rec32, link, panel32, lds);
+        return;
+    }
+    const int n_nodes = dev_node_count(info);
+    for (int grp = blockIdx.x; grp * kNodeGroup < n_nodes; grp += gridDim.x) {
+        wave_sync();  // the previous group's LDS reads are done
+        node_group_one<NT>(grp, sorted, n_nodes, theta_sq, list, b_lo, b_hi, b_left, b_size,
+                           b_pre, tab, t, mrec, rec32, link, panel32, lds);
     }
 }
 
@@ -1937,6 +1999,49 @@ __device__ __forceinline__ void merge_group_lds(int lb, int base, int cnt,
     wave_sync();  // the next group overwrites sq, sm, shit
 }
 
+// The top levels with internal nodes (top) and how many leading ones the
+// one-workgroup LDS merge takes (small), from the info words: cheb_build's
+// host rule, for a build whose sizes stay on the device.  Levels [small, top)
+// are the wide ones, merged one level per launch.
+__device__ __forceinline__ void top_split(const int *info, int n, int &top, int &small) {
+    const int F = info[kInfoF], D = info[kInfoD];
+    top = dev_build_failed(info) ? 0 : ((F > 0) ? D : max(D - 1, 0));
+    const int *lvl = info + kInfoTopLevel;
+    const int G = merge_group_size(n);
+    small = 0;
+    while (small < top && top_lds_bytes(lvl[small + 2], n, kTopLdsWaves) <= kTopLdsWgMax &&
+           (lvl[small + 1] - lvl[small] + G - 1) / G <= kMergeTopRounds * kTopLdsWaves)
+        ++small;
+}
+
+// Wide top levels of a build whose sizes stay on the device: launch k merges
+// level top - 1 - k if it is a wide one (launches k = 0, 1, ... in stream
+// order take the deepest first), kWideWaves wavefronts per workgroup striding
+// over its groups as merge_range_kernel's workgroups do.  kWideLaunches such
+// launches run; wide levels beyond them go to the one-workgroup kernel's
+// global form.
+constexpr int kWideWaves = 4;
+constexpr int kWideLaunches = 5;
+
+__global__ __launch_bounds__(kWideWaves * kBaryLanes) void merge_wide_kernel(
+    int n, int k, const int *__restrict__ info, const int2 *__restrict__ mrec,
+    const double *__restrict__ lam, TreeArrays t, float *__restrict__ panel32) {
+    extern __shared__ double lds[];
+    int top, small;
+    top_split(info, n, top, small);
+    const int d = top - 1 - k;
+    if (d < small) return;  // uniform
+    const int *lvl = info + kInfoTopLevel;
+    const int waves = blockDim.x / kBaryLanes, wv = threadIdx.x / kBaryLanes;
+    merge_level(n, lvl[d], lvl[d + 1] - lvl[d], blockIdx.x * waves + wv, gridDim.x * waves, mrec,
+                lam, t, panel32, lds + (size_t)wv * bary_lds_doubles(n));
+}
+
+// levels < 0: a build whose sizes stay on the device (launched with
+// kTopLdsWgMax bytes of LDS): the levels come from top_split, after the
+// wide-level launches; when the wide levels outnumber those launches, the
+// remaining levels are merged here in the global form (merge_level per level,
+// the arithmetic of every form being merge_group's).
 template <int NT>
 __global__ __launch_bounds__(kTopLdsWaves * kBaryLanes) void merge_top_lds_kernel(
     int levels, const int *__restrict__ info, const int *__restrict__ b_left,
@@ -1944,6 +2049,22 @@ __global__ __launch_bounds__(kTopLdsWaves * kBaryLanes) void merge_top_lds_kerne
     float *__restrict__ panel32) {
     extern __shared__ double lds[];
     const int *lvl = info + kInfoTopLevel;
+    if (levels < 0) {
+        int top, small;
+        top_split(info, NT, top, small);
+        if (top - small > kWideLaunches) {
+            const int waves = blockDim.x / kBaryLanes, wv = threadIdx.x / kBaryLanes;
+            double *mine = lds + (size_t)wv * bary_lds_doubles(NT);
+            for (int d = top - kWideLaunches - 1; d >= 0; --d) {
+                merge_level(NT, lvl[d], lvl[d + 1] - lvl[d], wv, waves, mrec, lam, t, panel32,
+                            mine);
+                __syncthreads();
+            }
+            return;
+        }
+        levels = small;
+        if (levels == 0) return;
+    }
     const int S = lvl[levels + 1], fin = lvl[levels];
     const int waves = blockDim.x / kBaryLanes;
     double *s_pos = lds;
@@ -1997,10 +2118,13 @@ __global__ __launch_bounds__(kMergeSubWaves * kBaryLanes) void merge_sub_lds_ker
     const int *__restrict__ lv_start, const int *__restrict__ lv_cnt,
     const int *__restrict__ sub_nodes, const int *__restrict__ b_left,
     const int2 *__restrict__ mrec, const double *__restrict__ lam, TreeArrays t,
-    float *__restrict__ panel32, int cap) {
+    float *__restrict__ panel32, int cap, const int *__restrict__ info) {
     extern __shared__ double lds[];
     constexpr int G = kBaryLanes / (2 * NT);
     const int f = blockIdx.x;
+    // a grid of kMaxFrontier workgroups when the frontier size stays on the
+    // device (info set): those past it exit
+    if (info && (f >= info[kInfoF] || dev_build_failed(info))) return;
     const int waves = blockDim.x / kBaryLanes, wv = threadIdx.x / kBaryLanes;
     int n_lv = 1;
     while (n_lv < kLv && lv_cnt[f * kLv + n_lv] > 0) ++n_lv;
@@ -2071,7 +2195,22 @@ struct EvalView {
     const float *panel32;
     const int32_t *perm;
     int N, n;
+    // set for a tree whose build kept its sizes on the device: the walks
+    // read the node count from these info words
+    const int *info = nullptr;
 };
+
+// The node count of a walk: v.N, or the build's info words for a tree whose
+// sizes stayed on the device.  `bad` marks such a build that found its input
+// not ascending (or its node slots exhausted): the outputs are then NaN.
+__device__ __forceinline__ int walk_nodes(const EvalView &v, bool &bad) {
+    if (!v.info) {
+        bad = false;
+        return v.N;
+    }
+    bad = dev_build_failed(v.info);
+    return dev_node_count(v.info);
+}
 
 // Exact f64 leaf sum in the reference's two-lane form (chebcode.rs:622-647).
 __device__ __forceinline__ void leaf_f64(const double *__restrict__ xs, int len, double x,
@@ -2172,10 +2311,12 @@ __global__ __launch_bounds__(256) void eval_f64_kernel(EvalView v, const double 
     const double eta2 = eta * eta;
     const int n = v.n;
     double re = 0.0, im = 0.0;
-    int skip = valid ? 0 : v.N;
+    bool bad;
+    const int N = walk_nodes(v, bad);
+    int skip = valid ? 0 : N;
     int node = 0;
     constexpr int kP = (NT > 0) ? NT : 1;
-    while (node < v.N) {
+    while (node < N) {
         if constexpr (kWave) node = __builtin_amdgcn_readfirstlane(node);
         int4 lk = v.link[node];  // right, esc, lo_idx, hi_idx
         double rlo = 0.0, rhi = 0.0, racc = 0.0, pts[kP], pws[kP];
@@ -2271,8 +2412,8 @@ __global__ __launch_bounds__(256) void eval_f64_kernel(EvalView v, const double 
     }
     if (!valid) return;
     const int o = (queries_are_sources && v.perm) ? v.perm[i] : i;
-    out_re[o] = re * scale;
-    out_im[o] = im * scale;
+    out_re[o] = bad ? __builtin_nan("") : re * scale;
+    out_im[o] = bad ? __builtin_nan("") : im * scale;
 }
 
 // NT > 0 as in eval_f64_kernel: record and f32 panel loaded with the link.
@@ -2293,10 +2434,12 @@ __global__ __launch_bounds__(256) void eval_fast_kernel(EvalView v, const double
     const int n = v.n;
     float far_re[4] = {0.f, 0.f, 0.f, 0.f}, far_im[4] = {0.f, 0.f, 0.f, 0.f};
     double leaf_re = 0.0, leaf_inv = 0.0;
-    int skip = valid ? 0 : v.N;
+    bool bad;
+    const int N = walk_nodes(v, bad);
+    int skip = valid ? 0 : N;
     int node = 0;
     constexpr int kP = (NT > 0) ? NT : 1;
-    while (node < v.N) {
+    while (node < N) {
         if constexpr (kWave) node = __builtin_amdgcn_readfirstlane(node);
         int4 lk = v.link[node];
         float4 rec = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -2404,8 +2547,8 @@ __global__ __launch_bounds__(256) void eval_fast_kernel(EvalView v, const double
     const double re = leaf_re + (double)hr;
     const double im = eta * leaf_inv + eta * (double)hi;
     const int o = (queries_are_sources && v.perm) ? v.perm[i] : i;
-    out_re[o] = re * scale;
-    out_im[o] = im * scale;
+    out_re[o] = bad ? __builtin_nan("") : re * scale;
+    out_im[o] = bad ? __builtin_nan("") : im * scale;
 }
 
 // Sparse queries (a grid across the spectrum, fewer than kSparseMax points):
@@ -2439,8 +2582,10 @@ __global__ __launch_bounds__(kSparseWaves * 64) void eval_f64_sparse_kernel(
     const double eta2 = eta * eta;
     const int n = v.n;
     double lr = 0.0, li = 0.0, fr = 0.0, fi = 0.0;
+    bool bad;
+    const int N = walk_nodes(v, bad);
     int node = 0;
-    while (node < v.N) {
+    while (node < N) {
         node = __builtin_amdgcn_readfirstlane(node);
         const int4 lk = v.link[node];
         if (lk.x < 0) {
@@ -2467,8 +2612,8 @@ __global__ __launch_bounds__(kSparseWaves * 64) void eval_f64_sparse_kernel(
     }
     const double re = wave_sum_fixed(lr + fr), im = wave_sum_fixed(li + fi);
     if (lane == 0) {
-        out_re[q] = re * scale;
-        out_im[q] = (eta * im) * scale;
+        out_re[q] = bad ? __builtin_nan("") : re * scale;
+        out_im[q] = bad ? __builtin_nan("") : (eta * im) * scale;
     }
 }
 
@@ -2487,8 +2632,10 @@ __global__ __launch_bounds__(kSparseWaves * 64) void eval_fast_sparse_kernel(
     const int n = v.n;
     double leaf_re = 0.0, leaf_inv = 0.0;
     float far_re = 0.f, far_im = 0.f;
+    bool bad;
+    const int N = walk_nodes(v, bad);
     int node = 0;
-    while (node < v.N) {
+    while (node < N) {
         node = __builtin_amdgcn_readfirstlane(node);
         const int4 lk = v.link[node];
         if (lk.x < 0) {
@@ -2521,8 +2668,8 @@ __global__ __launch_bounds__(kSparseWaves * 64) void eval_fast_sparse_kernel(
     const double re = wave_sum_fixed(leaf_re + (double)far_re);
     const double li = wave_sum_fixed(leaf_inv), fi = wave_sum_fixed((double)far_im);
     if (lane == 0) {
-        out_re[q] = re * scale;
-        out_im[q] = (eta * li + eta * fi) * scale;
+        out_re[q] = bad ? __builtin_nan("") : re * scale;
+        out_im[q] = bad ? __builtin_nan("") : (eta * li + eta * fi) * scale;
     }
 }
 
@@ -3171,6 +3318,7 @@ EvalView view_of(const ChebTree *t) {
     v.perm = t->perm;
     v.N = (int)t->N;
     v.n = t->n;
+    v.info = t->info_dev;
     return v;
 }
 
@@ -3451,11 +3599,11 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
                              : (n == 11) ? node_group_kernel<11> : node_group_kernel<12>;
         hipLaunchKernelGGL(kern, grid, dim3(kBaryLanes), bytes, stream, t->sorted, N,
                            t->theta_sq, d_list, b_lo, b_hi, b_left, b_size, b_pre, d_tab, ta,
-                           d_mrec, t->rec32, t->link, t->panel32);
+                           d_mrec, t->rec32, t->link, t->panel32, nullptr);
     } else {
         hipLaunchKernelGGL(node_kernel, dim3(N), dim3(kBaryLanes), lds + sizeof(double) * n,
                            stream, t->sorted, n, t->theta_sq, d_list, b_lo, b_hi, b_left, b_size,
-                           b_pre, d_tab, ta, d_mrec, t->rec32, t->link, t->panel32);
+                           b_pre, d_tab, ta, d_mrec, t->rec32, t->link, t->panel32, nullptr);
     }
     SHK_HIP(hipGetLastError());
     // 4. parents, deepest first.  The subtree workgroups keep their LDS
@@ -3471,13 +3619,16 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
         const dim3 blk(kMergeSubWaves * kBaryLanes);
         if (n == 8)
             hipLaunchKernelGGL(merge_sub_lds_kernel<8>, dim3(F), blk, bytes, stream, lv_start,
-                               lv_cnt, sub_nodes, b_left, d_mrec, d_tab + n, ta, t->panel32, cap);
+                               lv_cnt, sub_nodes, b_left, d_mrec, d_tab + n, ta, t->panel32, cap,
+                               nullptr);
         else if (n == 11)
             hipLaunchKernelGGL(merge_sub_lds_kernel<11>, dim3(F), blk, bytes, stream, lv_start,
-                               lv_cnt, sub_nodes, b_left, d_mrec, d_tab + n, ta, t->panel32, cap);
+                               lv_cnt, sub_nodes, b_left, d_mrec, d_tab + n, ta, t->panel32, cap,
+                               nullptr);
         else
             hipLaunchKernelGGL(merge_sub_lds_kernel<12>, dim3(F), blk, bytes, stream, lv_start,
-                               lv_cnt, sub_nodes, b_left, d_mrec, d_tab + n, ta, t->panel32, cap);
+                               lv_cnt, sub_nodes, b_left, d_mrec, d_tab + n, ta, t->panel32, cap,
+                               nullptr);
         SHK_HIP(hipGetLastError());
     } else if (F > 0) {
         const int waves = std::min(lds_waves, kMergeSubWaves);
@@ -3533,6 +3684,228 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
         SHK_HIP(hipEventRecord(t->ready, stream));
     }
     *out = t.release();
+    return 0;
+}
+
+// ---- one-shot transform in a caller's workspace -----------------------------
+//
+// cheb_build waits once on the host: the node count sizes the node arena and
+// the top merges' launches, and the sortedness check decides the sort.  The
+// workspace form takes ascending input and keeps every size on the device:
+//   - the node arena is carved at its bound (every node slot, cap_top + 2p),
+//     so no count is needed to place it;
+//   - the node kernels run a fixed grid that strides over the node count
+//     read from the info words, the subtree merges a grid of kMaxFrontier
+//     workgroups past the frontier size of which exit, and the top merges
+//     one workgroup that reads its levels there;
+//   - the walks read the node count there too.
+// Nothing waits on the host and nothing is allocated, so a captured graph can
+// hold the whole transform.  The kernels and their arithmetic are those of
+// cheb_build and cheb_eval (the top merges take merge_top_kernel's form for
+// every top level), so on ascending input the results equal the transient
+// build's bit for bit.  Input that is not ascending (ingest_kernel's check)
+// gets an empty topology and NaN outputs; info word kInfoUnsorted reports it.
+
+namespace {
+
+struct WsParts {
+    int *info = nullptr, *b_buf = nullptr, *lv_buf = nullptr, *hint = nullptr, *gap = nullptr;
+    int2 *mrec = nullptr;
+    double *tab = nullptr, *sorted = nullptr;
+    float *s_hi = nullptr, *s_lo = nullptr;
+    double *lo = nullptr, *hi = nullptr, *acc = nullptr, *nodes = nullptr, *w = nullptr;
+    int32_t *lo_idx = nullptr, *hi_idx = nullptr, *right = nullptr, *esc = nullptr;
+    float4 *rec32 = nullptr;
+    int4 *link = nullptr;
+    float *panel32 = nullptr;
+    size_t slots = 0;
+    int cap_top = 0, R = 0;
+};
+
+// Carves the parts from base (sizes only when base is null); returns the
+// bytes used.  The info words come first (offset 0).
+size_t carve_ws(char *base, size_t p, int n, WsParts &w) {
+    Carver c;
+    c.base = base;
+    w.cap_top = (int)std::min<size_t>(2 * p - 1, (size_t)(kMaxDepth + 3) * kFrontier);
+    w.slots = (size_t)w.cap_top + 2 * p;
+    const size_t S = w.slots;
+    const int ip = (int)p;
+    w.R = (int)std::max<int64_t>(ip, std::min<int64_t>((int64_t)kHintPerSource * ip,
+                                                       kHintMaxBuckets));
+    w.info = c.take<int>(kInfoInts);
+    w.b_buf = c.take<int>(6 * S);
+    w.mrec = c.take<int2>(S);
+    w.lv_buf = c.take<int>(3 * (size_t)kMaxFrontier * kLv + kMaxFrontier);
+    w.tab = c.take<double>(2 * (size_t)n);
+    w.hint = c.take<int>((size_t)w.R + 1);
+    w.gap = c.take<int>(kGapInts);
+    w.sorted = c.take<double>(p);
+    w.s_hi = c.take<float>(p);
+    w.s_lo = c.take<float>(p);
+    w.lo = c.take<double>(S);
+    w.hi = c.take<double>(S);
+    w.acc = c.take<double>(S);
+    w.lo_idx = c.take<int32_t>(S);
+    w.hi_idx = c.take<int32_t>(S);
+    w.right = c.take<int32_t>(S);
+    w.esc = c.take<int32_t>(S);
+    w.nodes = c.take<double>(S * n);
+    w.w = c.take<double>(S * n);
+    w.rec32 = c.take<float4>(S);
+    w.link = c.take<int4>(S);
+    w.panel32 = c.take<float>(S * 2 * n);
+    return (c.off + 255) / 256 * 256;
+}
+
+constexpr int kWsNodeGrid = 16384;  // workgroups of the node kernels (stride over the nodes)
+
+}  // namespace
+
+size_t cheb_ws_bytes(size_t p, int n) {
+    if (p == 0 || n < 2 || n > kMaxN) return 0;
+    WsParts w;
+    return carve_ws(nullptr, p, n, w);
+}
+
+int cheb_oneshot_ws(const double *d_ev, size_t p, const ChebParams &prm, const double *d_query,
+                    size_t nq, double eta, double scale, int queries_are_sources, void *ws,
+                    size_t ws_bytes, double *d_out_re, double *d_out_im, hipStream_t stream) {
+    if (p == 0 || p > (size_t)INT32_MAX / 2) {
+        set_error("chebcode build: p must be in [1, 2^30)");
+        return 6;
+    }
+    const int n = prm.n;
+    if (!(n == 8 || n == 11 || n == 12) || !(prm.theta > 0.0) || prm.leaf_cap < 1) {
+        set_error("chebcode workspace build: the presets' orders (8, 11, 12), theta > 0 and "
+                  "leaf_cap >= 1");
+        return 6;
+    }
+    if (reinterpret_cast<uintptr_t>(ws) % 256 != 0) {
+        set_error("chebcode workspace: the workspace must be 256-byte aligned");
+        return 6;
+    }
+    const size_t need = cheb_ws_bytes(p, n);
+    if (!ws || ws_bytes < need) {
+        set_error("chebcode workspace: " + std::to_string(ws_bytes) + " bytes given, " +
+                  std::to_string(need) + " needed (shk_chebcode_workspace_bytes)");
+        return 6;
+    }
+    WsParts w;
+    (void)carve_ws(static_cast<char *>(ws), p, n, w);
+    const int ip = (int)p, R = w.R;
+    const int leaf_cap = (int)std::min<int64_t>(prm.leaf_cap, INT32_MAX);
+    const size_t S = w.slots;
+    int *b_lo = w.b_buf, *b_hi = w.b_buf + S, *b_left = w.b_buf + 2 * S, *b_size = w.b_buf + 3 * S,
+        *b_pre = w.b_buf + 4 * S, *d_list = w.b_buf + 5 * S;
+    int *lv_start = w.lv_buf, *lv_cnt = w.lv_buf + (size_t)kMaxFrontier * kLv,
+        *lv_off = w.lv_buf + 2 * (size_t)kMaxFrontier * kLv,
+        *sub_nodes = w.lv_buf + 3 * (size_t)kMaxFrontier * kLv;
+    ChebTab tab{};
+    for (int j = 0; j < n; ++j) {
+        tab.v[j] = std::cos((double)j * kPi / (double)(n - 1));
+        const double sign = (j % 2 == 0) ? 1.0 : -1.0;
+        tab.v[n + j] = (j == 0 || j == n - 1) ? 0.5 * sign : sign;
+    }
+    // 1. copy, ascending check, hint and split of the input, and the table
+    hipLaunchKernelGGL(hint_clear_kernel, dim3(blocks_for((size_t)R + 1, 256)), dim3(256), 0,
+                       stream, w.hint, R + 1, w.gap, w.info);
+    hipLaunchKernelGGL(ingest_kernel, dim3(blocks_for(std::max<size_t>(p, 2 * n), 256)), dim3(256),
+                       0, stream, d_ev, ip, R, w.sorted, w.hint, w.gap, w.s_hi, w.s_lo, w.info,
+                       tab, 2 * n, w.tab);
+    // 2. topology (no mailbox)
+    const TopoCtx c{w.sorted, ip,  leaf_cap, w.hint, R,      Hint{}, w.gap,
+                    0,        b_lo, b_hi,    b_left, b_size, b_pre};
+    hipLaunchKernelGGL(topo_top_kernel, dim3(1), dim3(kTopoThreads), 0, stream, c, w.cap_top,
+                       w.info);
+    hipLaunchKernelGGL(topo_sub_kernel, dim3(kMaxFrontier), dim3(kSubThreads), 0, stream, c,
+                       w.info, lv_start, lv_cnt, sub_nodes);
+    hipLaunchKernelGGL(topo_mid_kernel, dim3(1), dim3(kTopoThreads), 0, stream, c, w.info,
+                       sub_nodes, lv_off, d_list, nullptr, 0);
+    hipLaunchKernelGGL(topo_fin_kernel, dim3(kMaxFrontier), dim3(kSubThreads), 0, stream, c,
+                       w.info, lv_start, lv_cnt, lv_off, d_list);
+    SHK_HIP(hipGetLastError());
+    // 3. node records and leaf weights, striding over the node count
+    TreeArrays ta{w.lo, w.hi, w.acc, w.nodes, w.w, w.lo_idx, w.hi_idx, w.right, w.esc};
+    const double theta_sq = prm.theta * prm.theta;
+    if (prm.leaf_cap <= kGroupRowsMax) {
+        const dim3 grid((unsigned)std::min<size_t>((S + kNodeGroup - 1) / kNodeGroup, kWsNodeGrid));
+        const size_t bytes = sizeof(double) * (size_t)node_group_lds_doubles(n);
+        auto kern = (n == 8) ? node_group_kernel<8>
+                             : (n == 11) ? node_group_kernel<11> : node_group_kernel<12>;
+        hipLaunchKernelGGL(kern, grid, dim3(kBaryLanes), bytes, stream, w.sorted, 0, theta_sq,
+                           d_list, b_lo, b_hi, b_left, b_size, b_pre, w.tab, ta, w.mrec, w.rec32,
+                           w.link, w.panel32, w.info);
+    } else {
+        const dim3 grid((unsigned)std::min<size_t>(S, kWsNodeGrid));
+        hipLaunchKernelGGL(node_kernel, grid, dim3(kBaryLanes),
+                           bary_lds_bytes(n) + sizeof(double) * n, stream, w.sorted, n, theta_sq,
+                           d_list, b_lo, b_hi, b_left, b_size, b_pre, w.tab, ta, w.mrec, w.rec32,
+                           w.link, w.panel32, w.info);
+    }
+    SHK_HIP(hipGetLastError());
+    // 4. parents: the subtrees (workgroups past the frontier exit), then
+    // every top level in one workgroup
+    const size_t lds = bary_lds_bytes(n);
+    {
+        const size_t fixed = (size_t)kMergeSubWaves * lds;
+        const int cap = (int)((kSubLdsMax - fixed) / (2 * sizeof(int) + 2 * sizeof(double) * n));
+        const size_t bytes = fixed + (size_t)cap * (2 * sizeof(int) + 2 * sizeof(double) * n);
+        const dim3 blk(kMergeSubWaves * kBaryLanes);
+        auto kern = (n == 8) ? merge_sub_lds_kernel<8>
+                             : (n == 11) ? merge_sub_lds_kernel<11> : merge_sub_lds_kernel<12>;
+        hipLaunchKernelGGL(kern, dim3(kMaxFrontier), blk, bytes, stream, lv_start, lv_cnt,
+                           sub_nodes, b_left, w.mrec, w.tab + n, ta, w.panel32, cap, w.info);
+    }
+    {
+        // a top level holds fewer than 2 kFrontier slots
+        const int groups = (2 * kFrontier + merge_group_size(n) - 1) / merge_group_size(n);
+        const dim3 grid((unsigned)((groups + kWideWaves - 1) / kWideWaves));
+        for (int k = 0; k < kWideLaunches; ++k)
+            hipLaunchKernelGGL(merge_wide_kernel, grid, dim3(kWideWaves * kBaryLanes),
+                               kWideWaves * lds, stream, n, k, w.info, w.mrec, w.tab + n, ta,
+                               w.panel32);
+        const dim3 blk(kTopLdsWaves * kBaryLanes);
+        auto kern = (n == 8) ? merge_top_lds_kernel<8>
+                             : (n == 11) ? merge_top_lds_kernel<11> : merge_top_lds_kernel<12>;
+        hipLaunchKernelGGL(kern, dim3(1), blk, kTopLdsWgMax, stream, -1, w.info, b_left, w.mrec,
+                           w.tab + n, ta, w.panel32);
+    }
+    SHK_HIP(hipGetLastError());
+    // 5. the walks, with the node count from the info words
+    ChebTree t;
+    t.p = p;
+    t.n = n;
+    t.theta = prm.theta;
+    t.theta_sq = theta_sq;
+    t.leaf_cap = prm.leaf_cap;
+    t.f32_far = prm.f32_far;
+    t.sorted = w.sorted;
+    t.s_hi = w.s_hi;
+    t.s_lo = w.s_lo;
+    t.lo = w.lo;
+    t.hi = w.hi;
+    t.acc = w.acc;
+    t.lo_idx = w.lo_idx;
+    t.hi_idx = w.hi_idx;
+    t.right = w.right;
+    t.esc = w.esc;
+    t.nodes = w.nodes;
+    t.w = w.w;
+    t.rec32 = w.rec32;
+    t.link = w.link;
+    t.panel32 = w.panel32;
+    t.stream = stream;
+    t.info_dev = w.info;
+    return cheb_eval(&t, d_query, nq, eta, scale, queries_are_sources, d_out_re, d_out_im,
+                     stream);
+}
+
+int cheb_ws_status(const void *ws, hipStream_t stream, int *ascending) {
+    int words[kInfoUnsorted + 1] = {};
+    SHK_HIP(hipMemcpyAsync(words, ws, sizeof(words), hipMemcpyDeviceToHost, stream));
+    SHK_HIP(hipStreamSynchronize(stream));
+    *ascending = (words[kInfoUnsorted] == 0 && words[kInfoOverflow] == 0) ? 1 : 0;
     return 0;
 }
 

@@ -382,6 +382,18 @@ struct ChebParams {
 // synchronisation, and its evaluations no event.
 int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **out,
                hipStream_t stream, int transient = 0);
+// Build + evaluation in a caller's workspace with no host wait and no
+// allocation (every size read on the device), for ascending input of the
+// presets' orders: the transient build's kernels and results.  The sums
+// go to nq device query points, or to the sources themselves when
+// queries_are_sources (d_query unused), times scale.  Input that is not
+// ascending gets NaN outputs, which cheb_ws_status reports.
+size_t cheb_ws_bytes(size_t p, int n);
+int cheb_oneshot_ws(const double *d_ev, size_t p, const ChebParams &prm, const double *d_query,
+                    size_t nq, double eta, double scale, int queries_are_sources, void *ws,
+                    size_t ws_bytes, double *d_out_re, double *d_out_im, hipStream_t stream);
+// Whether the last workspace build found its input ascending.  Synchronises.
+int cheb_ws_status(const void *ws, hipStream_t stream, int *ascending);
 void cheb_destroy(ChebTree *t);
 size_t cheb_node_count(const ChebTree *t);
 size_t cheb_p(const ChebTree *t);

@@ -484,6 +484,65 @@ def at_points_dev(query, ev, eta: float, method: str = "blocked",
     return re, im
 
 
+# ---- ChebCode one-shot transforms in a caller's workspace -----------------------
+
+def chebcode_workspace_bytes(p: int, method: str = "chebcode_fast") -> int:
+    """Device bytes the workspace entries need for p eigenvalues."""
+    n = ctypes.c_size_t()
+    check(_lib.load().shk_chebcode_workspace_bytes(int(p), parse_method(method), ctypes.byref(n)))
+    return n.value
+
+
+def _workspace(ws):
+    import torch
+    if not (ws.is_cuda and ws.is_contiguous()):
+        raise ValueError("the workspace must be a contiguous CUDA tensor")
+    return ws.data_ptr(), ws.numel() * ws.element_size()
+
+
+def chebcode_all_points_ws_dev(ev, eta: float, method: str, workspace, scaled: bool = True,
+                               out=None, stream=None):
+    """ChebCode transform of ascending CUDA float64 eigenvalues, built and
+    evaluated in ``workspace`` (a CUDA tensor of at least
+    chebcode_workspace_bytes bytes) with no host wait and no allocation, so a
+    HIP graph can capture it.  Equal bit for bit to all_points_dev with the
+    same method; NaN outputs on input that is not ascending."""
+    import torch
+    if not (ev.is_cuda and ev.dtype == torch.float64 and ev.dim() == 1 and ev.is_contiguous()):
+        raise ValueError("ev must be a contiguous 1-D float64 CUDA tensor")
+    wp, wb = _workspace(workspace)
+    re, im = out if out is not None else (torch.empty_like(ev), torch.empty_like(ev))
+    check(_lib.load().shk_chebcode_all_points_ws_dev(
+        ev.data_ptr(), ev.numel(), float(eta), parse_method(method), int(bool(scaled)), wp, wb,
+        re.data_ptr(), im.data_ptr(), _stream_handle(stream)))
+    return re, im
+
+
+def chebcode_at_points_ws_dev(query, ev, eta: float, method: str, workspace, out=None,
+                              stream=None):
+    """Raw ChebCode sums at CUDA float64 query points (the at_points_dev tree),
+    built in ``workspace`` like chebcode_all_points_ws_dev."""
+    import torch
+    for t, nm in ((query, "query"), (ev, "ev")):
+        if not (t.is_cuda and t.dtype == torch.float64 and t.dim() == 1 and t.is_contiguous()):
+            raise ValueError(f"{nm} must be a contiguous 1-D float64 CUDA tensor")
+    wp, wb = _workspace(workspace)
+    re, im = out if out is not None else (torch.empty_like(query), torch.empty_like(query))
+    check(_lib.load().shk_chebcode_at_points_ws_dev(
+        query.data_ptr(), query.numel(), ev.data_ptr(), ev.numel(), float(eta),
+        parse_method(method), wp, wb, re.data_ptr(), im.data_ptr(), _stream_handle(stream)))
+    return re, im
+
+
+def chebcode_workspace_status(workspace, stream=None) -> bool:
+    """Whether the last workspace call found its eigenvalues ascending.
+    Waits for the stream."""
+    wp, _ = _workspace(workspace)
+    ok = ctypes.c_int()
+    check(_lib.load().shk_chebcode_workspace_status(wp, _stream_handle(stream), ctypes.byref(ok)))
+    return bool(ok.value)
+
+
 # ---- measurement hooks ----------------------------------------------------------
 
 def profile_enable(on: bool = True) -> None:
