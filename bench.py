@@ -610,6 +610,14 @@ def main() -> int:
                 for _ in range(3):
                     shk.all_points_dev(evc, 1.0 / np.sqrt(pc), m)
             sync()
+            # the same transforms built in a caller's workspace (no host wait)
+            for m in ("chebcode_fast", "chebcode"):
+                ws = torch.empty(shk.chebcode_workspace_bytes(pc, m), dtype=torch.uint8,
+                                 device=dev)
+                for _ in range(3):
+                    shk.chebcode_all_points_ws_dev(evc, 1.0 / np.sqrt(pc), m, ws)
+                sync()
+                del ws
         log("kernel workloads done")
         return 0
 
