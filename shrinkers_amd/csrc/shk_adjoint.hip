@@ -210,8 +210,8 @@ __global__ __launch_bounds__(kAWG) void adj_src_f64_kernel(
             s_c[u] = -e2 * a;
         }
         __syncthreads();
-        // two query points per step; the kAR reciprocals of each come in
-        // pairs from one v_rcp_f64 (rcp_pairs_f64)
+        // two query points per step; their 2 kAR reciprocals come four per
+        // v_rcp_f64 (rcp_many_f64)
         auto step = [&](int u, const double (&dd)[kAR], const double (&rr)[kAR]) {
             const double A = s_a[u], B = s_b[u], C = s_c[u];
 #pragma unroll
@@ -231,7 +231,7 @@ __global__ __launch_bounds__(kAWG) void adj_src_f64_kernel(
                 tt[k] = __builtin_fma(dd[k], dd[k], e2);
                 tt[kAR + k] = __builtin_fma(dd[kAR + k], dd[kAR + k], e2);
             }
-            rcp_pairs_f64(tt, rr);
+            rcp_many_f64(tt, rr);
             double d0[kAR], r0[kAR], d1[kAR], r1[kAR];
 #pragma unroll
             for (int k = 0; k < kAR; ++k) {
@@ -250,7 +250,7 @@ __global__ __launch_bounds__(kAWG) void adj_src_f64_kernel(
                 dd[k] = lam[k] - s_x[u];
                 tt[k] = __builtin_fma(dd[k], dd[k], e2);
             }
-            rcp_pairs_f64(tt, rr);
+            rcp_many_f64(tt, rr);
             step(u, dd, rr);
         }
     }
@@ -415,8 +415,8 @@ __global__ __launch_bounds__(kAWG) void adj_qry_f64_kernel(const double *__restr
         __syncthreads();
         for (int u = lane; u < cnt; u += kAWG) s_src[u] = ev[base + u];
         __syncthreads();
-        // two sources per step; the kAR reciprocals of each come in pairs
-        // from one v_rcp_f64 (rcp_pairs_f64)
+        // two sources per step; their 2 kAR reciprocals come four per
+        // v_rcp_f64 (rcp_many_f64)
         auto step = [&](const double (&dd)[kAR], const double (&rr)[kAR]) {
 #pragma unroll
             for (int k = 0; k < kAR; ++k) {
@@ -437,7 +437,7 @@ __global__ __launch_bounds__(kAWG) void adj_qry_f64_kernel(const double *__restr
                 tt[k] = __builtin_fma(dd[k], dd[k], e2);
                 tt[kAR + k] = __builtin_fma(dd[kAR + k], dd[kAR + k], e2);
             }
-            rcp_pairs_f64(tt, rr);
+            rcp_many_f64(tt, rr);
             double d0[kAR], r0[kAR], d1[kAR], r1[kAR];
 #pragma unroll
             for (int k = 0; k < kAR; ++k) {
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(kAWG) void adj_qry_f64_kernel(const double *__restr
                 dd[k] = x[k] - s_src[u];
                 tt[k] = __builtin_fma(dd[k], dd[k], e2);
             }
-            rcp_pairs_f64(tt, rr);
+            rcp_many_f64(tt, rr);
             step(dd, rr);
         }
     }
@@ -511,7 +511,7 @@ __global__ __launch_bounds__(kGridThreads) void adj_grid_qry_kernel(
             double tt[U], rr[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) tt[u] = __builtin_fma(d[u], d[u], e2);
-            rcp_pairs_f64<U>(tt, rr);
+            rcp_many_f64<U>(tt, rr);
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const double e = __builtin_fma(-d[u], d[u], e2);

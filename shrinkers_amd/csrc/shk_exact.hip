@@ -146,7 +146,7 @@ __global__ __launch_bounds__(kWG) void exact_f64_kernel(
                 tt[k] = __builtin_fma(dd[k], dd[k], e2[k]);
                 tt[kR + k] = __builtin_fma(dd[kR + k], dd[kR + k], e2[k]);
             }
-            rcp_pairs_f64(tt, rr);
+            rcp_many_f64(tt, rr);
             double d0[kR], r0[kR], d1[kR], r1[kR];
 #pragma unroll
             for (int k = 0; k < kR; ++k) {
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(kWG) void exact_f64_kernel(
                 dd[k] = x[k] - s0;
                 tt[k] = __builtin_fma(dd[k], dd[k], e2[k]);
             }
-            rcp_pairs_f64(tt, rr);
+            rcp_many_f64(tt, rr);
             accumulate(dd, rr);
         }
     }
@@ -490,7 +490,7 @@ __global__ __launch_bounds__(kWG) void many_f64_kernel(const double *__restrict_
                     tt[k] = __builtin_fma(dd[k], dd[k], e2[k]);
                     tt[kR + k] = __builtin_fma(dd[kR + k], dd[kR + k], e2[k]);
                 }
-                rcp_pairs_f64(tt, rr);
+                rcp_many_f64(tt, rr);
 #pragma unroll
                 for (int k = 0; k < kR; ++k) {
                     re[k] = __builtin_fma(dd[k], rr[k], re[k]);
@@ -510,7 +510,7 @@ __global__ __launch_bounds__(kWG) void many_f64_kernel(const double *__restrict_
                     dd[k] = x[k] - s0;
                     tt[k] = __builtin_fma(dd[k], dd[k], e2[k]);
                 }
-                rcp_pairs_f64(tt, rr);
+                rcp_many_f64(tt, rr);
 #pragma unroll
                 for (int k = 0; k < kR; ++k) {
                     re[k] = __builtin_fma(dd[k], rr[k], re[k]);
@@ -580,7 +580,7 @@ int launch_reduce(const double *part, int n_split, size_t nq, double eta, double
 // FP32-compensated: the pair difference is the FP64 difference rounded to
 // f32 (one rounding, exact for the near pairs by Sterbenz), the reciprocal
 // v_rcp_f32, and each thread's FP32 sums over kGridUnroll sources are folded
-// into FP64.  FP64: reciprocals two per v_rcp_f64 (rcp_pairs_f64) and FP64
+// into FP64.  FP64: reciprocals four per v_rcp_f64 (rcp_many_f64) and FP64
 // sums.
 constexpr int kGridWG = kGridThreads;
 constexpr int kGridUnroll = 8;  // sources per thread and step: their loads go out together
@@ -620,7 +620,7 @@ __global__ __launch_bounds__(kGridWG) void grid_query_kernel(const double *__res
             double tt[U], rr[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) tt[u] = __builtin_fma(d[u], d[u], e2);
-            rcp_pairs_f64<U>(tt, rr);
+            rcp_many_f64<U>(tt, rr);
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 re = __builtin_fma(d[u], rr[u], re);

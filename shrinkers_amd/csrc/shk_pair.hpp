@@ -64,6 +64,58 @@ __device__ __forceinline__ void rcp_pairs_f64(const double (&t)[R], double (&r)[
     }
 }
 
+// 1/t[k] for R (a multiple of 4) positive denominators, four per v_rcp_f64:
+// with A = t0 t1, B = t2 t3, P = A B and Q = 1/P (rcp_f64), 1/A = B Q and
+// 1/B = A Q, so 1/t0 = t1 (B Q), 1/t1 = t0 (B Q), 1/t2 = t3 (A Q) and
+// 1/t3 = t2 (A Q).  That holds while A, B and P are positive normal numbers;
+// a lane whose products are not takes rcp_f64 of each of its four
+// denominators instead.  Every lane's result depends on its own four
+// denominators only.  Relative error about 2.5 ulp (five roundings: A, P,
+// Q, B Q and the last product, against three for rcp_pairs_f64); per four
+// terms one quarter-rate v_rcp_f64, three FMAs and nine multiplies instead
+// of two reciprocals, six FMAs and six multiplies.
+#ifndef SHK_F64_RCP_QUADS
+#define SHK_F64_RCP_QUADS 1
+#endif
+template <int R>
+__device__ __forceinline__ void rcp_quads_f64(const double (&t)[R], double (&r)[R]) {
+    static_assert(R % 4 == 0, "denominators come in fours");
+    bool ok[R / 4], all = true;
+#pragma unroll
+    for (int k = 0; k < R; k += 4) {
+        const double a = t[k] * t[k + 1], b = t[k + 2] * t[k + 3];
+        const double p = a * b;
+        const double q = rcp_f64(p);
+        const double qa = b * q, qb = a * q;
+        r[k] = t[k + 1] * qa;
+        r[k + 1] = t[k] * qa;
+        r[k + 2] = t[k + 3] * qb;
+        r[k + 3] = t[k + 2] * qb;
+        ok[k / 4] = __builtin_amdgcn_class(a, 0x100) && __builtin_amdgcn_class(b, 0x100) &&
+                    __builtin_amdgcn_class(p, 0x100);
+        all = all && ok[k / 4];
+    }
+    if (!all) {  // one branch for all R/4 products, skipped unless a lane needs it
+#pragma unroll
+        for (int k = 0; k < R; k += 4) {
+            if (!ok[k / 4]) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) r[k + u] = rcp_f64(t[k + u]);
+            }
+        }
+    }
+}
+
+// The FP64 reciprocals of R denominators: four per v_rcp_f64 where R allows
+// (SHK_F64_RCP_QUADS), else two.
+template <int R>
+__device__ __forceinline__ void rcp_many_f64(const double (&t)[R], double (&r)[R]) {
+    if constexpr (SHK_F64_RCP_QUADS && R % 4 == 0)
+        rcp_quads_f64<R>(t, r);
+    else
+        rcp_pairs_f64<R>(t, r);
+}
+
 // Tile pair (a, b), a <= b, of the symmetric kernels' workgroup `id` in
 // [0, nt(nt+1)/2), in row-major order (row a holds b = a .. nt-1).  A
 // one-dimensional grid of exactly these pairs dispatches no workgroup that

@@ -266,7 +266,7 @@ __device__ __forceinline__ void sym_sweep_f64(const double *__restrict__ s_src, 
                     d[k] = x[k] - sv;
                     t[k] = __builtin_fma(d[k], d[k], e2);
                 }
-                rcp_pairs_f64(t, r);
+                rcp_many_f64(t, r);
                 // the lane sums start from the first pair's term
                 double bre = d[0] * r[0], bim = r[0];
                 re[0] = __builtin_fma(d[0], r[0], re[0]);

@@ -41,7 +41,7 @@
 // beta' = d r^2, then two fused multiply-adds and two additions at each end.
 // FP32-compensated form: the pair difference of the forward kernels
 // (per-lane FP64 centre, f32 hi/lo parts); runs of kAB sources summed in f32
-// and folded into FP64.  FP64 form: paired reciprocals (rcp_pairs_f64).
+// and folded into FP64.  FP64 form: reciprocals four per v_rcp_f64 (rcp_many_f64).
 #include "shk_internal.hpp"
 #include "shk_pair.hpp"
 
@@ -230,7 +230,7 @@ __device__ __forceinline__ void adj_sweep_f64(
                     d[k] = x[k] - sv;
                     t[k] = __builtin_fma(d[k], d[k], e2);
                 }
-                rcp_pairs_f64(t, r);
+                rcp_many_f64(t, r);
                 double bq = 0.0, ba = 0.0, bb = 0.0;
 #pragma unroll
                 for (int k = 0; k < kJR; ++k) {
