@@ -246,8 +246,15 @@ __device__ __forceinline__ void sym_finish(double (&re)[kSR], double (&iv)[kSR],
 }
 
 // FP64 variant: the j-side lane sums travel through LDS in FP64, kSB64
-// sources per batch (the same LDS bytes as the FP32 batches).
-constexpr int kSB64 = kSB / 2;
+// sources per batch (8: the same LDS bytes as the FP32 batches).  Measured
+// on MI355X (FP64 all-points, ms at p = 2e4 / 5e4 / 1e5): 8 per batch
+// 0.118 / 0.461 / 1.78; 4 (half the LDS, 7 instead of 4 workgroups per CU)
+// 0.121 / 0.477 / 1.82; 16: 0.123 / 0.504 / 1.94.
+#ifndef SHK_SYM_SB64
+#define SHK_SYM_SB64 8
+#endif
+constexpr int kSB64 = SHK_SYM_SB64;
+static_assert(kSB64 == 4 || kSB64 == 8 || kSB64 == 16, "a batch's rows split the wavefront evenly");
 
 template <bool JSIDE>
 __device__ __forceinline__ void sym_sweep_f64(const double *__restrict__ s_src, int n_src,
@@ -289,18 +296,21 @@ __device__ __forceinline__ void sym_sweep_f64(const double *__restrict__ s_src, 
         if (JSIDE) {
             sym_handover();
             {
-                // four lanes per (source, component) row, one chain of 16
-                // each; joined as (c0 + c1) + (c2 + c3) on every lane
-                const int row_id = lane >> 2, q = lane & 3;
+                // kL lanes per (source, component) row, one chain of kC
+                // each, joined in a fixed tree on every lane: at kSB64 = 8,
+                // four chains of 16 as (c0 + c1) + (c2 + c3)
+                constexpr int kL = kSL / (2 * kSB64), kC = kSL / kL;
+                const int row_id = lane / kL, q = lane % kL;
                 const int u = row_id >> 1, comp = row_id & 1;
-                const double *row = s_jb + (2 * u + comp) * kJRow + 16 * q;
-                double cq = row[0];
+                const double *row = s_jb + (2 * u + comp) * kJRow + kC * q;
+                double tot = row[0];
 #pragma unroll
-                for (int l = 1; l < 16; ++l) cq += row[l];
-                const double o1 = __shfl_xor(cq, 1, 64);
-                const double pair = (q & 1) ? o1 + cq : cq + o1;   // c0+c1 or c2+c3
-                const double o2 = __shfl_xor(pair, 2, 64);
-                const double tot = (q & 2) ? o2 + pair : pair + o2;
+                for (int l = 1; l < kC; ++l) tot += row[l];
+#pragma unroll
+                for (int o = 1; o < kL; o <<= 1) {
+                    const double other = __shfl_xor(tot, o, 64);
+                    tot = (q & o) ? other + tot : tot + other;
+                }
                 if (q == 0 && j0 + u < n_src) {
                     if (comp) j_im[j0 + u] = tot;
                     else j_re[j0 + u] = -tot;
