@@ -9,7 +9,8 @@ shuffled), a size and an eta, then checks, at the bars of the per-path tests:
   spectrum), FP64 through the host entry (1e-13) and FP32-compensated through
   the device entry (1e-6);
 - the ChebCode DEFAULT and FAST transforms against the oracle's ChebCode
-  (1e-13 / 1e-6) up to 20 000 eigenvalues;
+  (1e-13 / 1e-6) up to 20 000 eigenvalues, and the workspace build of the
+  sorted spectrum against the one-shot entry bit for bit;
 - the at-points adjoint (dL/dlambda, dL/dx, dL/deta) against the oracle's FP64
   adjoint (1e-12 FP64, 5e-6 FP32-compensated) up to 8 000 eigenvalues.
 The draws cover sizes that are not multiples of the kernels' tiles, both
@@ -108,10 +109,19 @@ def test_random_chebcode_cases(shk, orc, seed):
     tag = _tag(seed, kind, ev, eta, q)
     print(tag, flush=True)
     p = ev.size
+    import torch
+    d = torch.from_numpy(np.ascontiguousarray(np.sort(ev))).cuda()
     for preset, tol in (("chebcode", 1e-13), ("chebcode_fast", 1e-6)):
         rr, ri = orc.chebcode_preset(ev, eta, preset)
         r, i = shk.compute_all_stieltjes(ev, eta, preset)
         _close(r, i, rr / p, ri / p, tol, f"{tag} {preset}")
+        # the workspace build (ascending input) against the one-shot entry
+        ws = torch.empty(shk.chebcode_workspace_bytes(p, preset), dtype=torch.uint8,
+                         device="cuda")
+        got = shk.chebcode_all_points_ws_dev(d, eta, preset, ws)
+        want = shk.all_points_dev(d, eta, preset)
+        torch.cuda.synchronize()
+        assert torch.equal(got[0], want[0]) and torch.equal(got[1], want[1]), tag
 
 
 def _rel(a, b):
