@@ -2352,7 +2352,8 @@ __global__ __launch_bounds__(256) void eval_f64_kernel(EvalView v, const double 
                                      : accept_f64(v, node, x, eta2);
         if (active && accept && NT > 0) {
             double afr0 = 0.0, afr1 = 0.0, afi0 = 0.0, afi1 = 0.0;
-            // the terms' reciprocals two per v_rcp_f64 (rcp_pairs_f64)
+            // the terms' reciprocals four (and a last two) per v_rcp_f64
+            // (rcp_many_f64)
             constexpr int kPair = (kP >= 2) ? (kP & ~1) : 2;  // (NT = 0 never runs this)
             double dd[kPair], den[kPair], inv[kPair];
 #pragma unroll
@@ -2360,7 +2361,7 @@ __global__ __launch_bounds__(256) void eval_f64_kernel(EvalView v, const double 
                 dd[j] = x - pts[j];
                 den[j] = __builtin_fma(dd[j], dd[j], eta2);
             }
-            rcp_pairs_f64<kPair>(den, inv);
+            rcp_many_f64<kPair>(den, inv);
 #pragma unroll
             for (int j = 0; j + 2 <= kP; j += 2) {
                 const double q0 = pws[j] * inv[j];

@@ -64,56 +64,70 @@ __device__ __forceinline__ void rcp_pairs_f64(const double (&t)[R], double (&r)[
     }
 }
 
-// 1/t[k] for R (a multiple of 4) positive denominators, four per v_rcp_f64:
+// 1/t[k] for R (even) positive denominators, four per v_rcp_f64 for the
+// leading multiple of four and two per v_rcp_f64 (rcp_pairs_f64's
+// arithmetic) for a last pair, one fallback branch for all of them.  Four:
 // with A = t0 t1, B = t2 t3, P = A B and Q = 1/P (rcp_f64), 1/A = B Q and
 // 1/B = A Q, so 1/t0 = t1 (B Q), 1/t1 = t0 (B Q), 1/t2 = t3 (A Q) and
 // 1/t3 = t2 (A Q).  That holds while A, B and P are positive normal numbers;
-// a lane whose products are not takes rcp_f64 of each of its four
-// denominators instead.  Every lane's result depends on its own four
-// denominators only.  Relative error about 2.5 ulp (five roundings: A, P,
-// Q, B Q and the last product, against three for rcp_pairs_f64); per four
-// terms one quarter-rate v_rcp_f64, three FMAs and nine multiplies instead
-// of two reciprocals, six FMAs and six multiplies.
+// a lane whose products are not takes rcp_f64 of each of the four
+// denominators instead.  Every lane's result depends on its own denominators
+// only.  Relative error about 2.5 ulp (five roundings: A, P, Q, B Q and the
+// last product, against three for rcp_pairs_f64); per four terms one
+// quarter-rate v_rcp_f64, three FMAs and nine multiplies instead of two
+// reciprocals, six FMAs and six multiplies.  R = 2 gives rcp_pairs_f64's
+// results; SHK_F64_RCP_QUADS = 0 takes pairs throughout.
 #ifndef SHK_F64_RCP_QUADS
 #define SHK_F64_RCP_QUADS 1
 #endif
 template <int R>
-__device__ __forceinline__ void rcp_quads_f64(const double (&t)[R], double (&r)[R]) {
-    static_assert(R % 4 == 0, "denominators come in fours");
-    bool ok[R / 4], all = true;
+__device__ __forceinline__ void rcp_many_f64(const double (&t)[R], double (&r)[R]) {
+    static_assert(R % 2 == 0, "denominators are paired");
+    if constexpr (!SHK_F64_RCP_QUADS || R < 4) {
+        rcp_pairs_f64<R>(t, r);
+    } else {
+        constexpr int Q = R - R % 4;  // denominators taken four at a time
+        bool ok[R / 2], all = true;
 #pragma unroll
-    for (int k = 0; k < R; k += 4) {
-        const double a = t[k] * t[k + 1], b = t[k + 2] * t[k + 3];
-        const double p = a * b;
-        const double q = rcp_f64(p);
-        const double qa = b * q, qb = a * q;
-        r[k] = t[k + 1] * qa;
-        r[k + 1] = t[k] * qa;
-        r[k + 2] = t[k + 3] * qb;
-        r[k + 3] = t[k + 2] * qb;
-        ok[k / 4] = __builtin_amdgcn_class(a, 0x100) && __builtin_amdgcn_class(b, 0x100) &&
-                    __builtin_amdgcn_class(p, 0x100);
-        all = all && ok[k / 4];
-    }
-    if (!all) {  // one branch for all R/4 products, skipped unless a lane needs it
+        for (int k = 0; k < Q; k += 4) {
+            const double a = t[k] * t[k + 1], b = t[k + 2] * t[k + 3];
+            const double p = a * b;
+            const double q = rcp_f64(p);
+            const double qa = b * q, qb = a * q;
+            r[k] = t[k + 1] * qa;
+            r[k + 1] = t[k] * qa;
+            r[k + 2] = t[k + 3] * qb;
+            r[k + 3] = t[k + 2] * qb;
+            ok[k / 2] = __builtin_amdgcn_class(a, 0x100) && __builtin_amdgcn_class(b, 0x100) &&
+                        __builtin_amdgcn_class(p, 0x100);
+            all = all && ok[k / 2];
+        }
 #pragma unroll
-        for (int k = 0; k < R; k += 4) {
-            if (!ok[k / 4]) {
+        for (int k = Q; k < R; k += 2) {
+            const double p = t[k] * t[k + 1];
+            const double q = rcp_f64(p);
+            r[k] = t[k + 1] * q;
+            r[k + 1] = t[k] * q;
+            ok[k / 2] = __builtin_amdgcn_class(p, 0x100);
+            all = all && ok[k / 2];
+        }
+        if (!all) {  // one branch for all groups, skipped unless a lane needs it
 #pragma unroll
-                for (int u = 0; u < 4; ++u) r[k + u] = rcp_f64(t[k + u]);
+            for (int k = 0; k < Q; k += 4) {
+                if (!ok[k / 2]) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) r[k + u] = rcp_f64(t[k + u]);
+                }
+            }
+#pragma unroll
+            for (int k = Q; k < R; k += 2) {
+                if (!ok[k / 2]) {
+                    r[k] = rcp_f64(t[k]);
+                    r[k + 1] = rcp_f64(t[k + 1]);
+                }
             }
         }
     }
-}
-
-// The FP64 reciprocals of R denominators: four per v_rcp_f64 where R allows
-// (SHK_F64_RCP_QUADS), else two.
-template <int R>
-__device__ __forceinline__ void rcp_many_f64(const double (&t)[R], double (&r)[R]) {
-    if constexpr (SHK_F64_RCP_QUADS && R % 4 == 0)
-        rcp_quads_f64<R>(t, r);
-    else
-        rcp_pairs_f64<R>(t, r);
 }
 
 // Tile pair (a, b), a <= b, of the symmetric kernels' workgroup `id` in
