@@ -28,7 +28,9 @@
  * shk_all_points_dev, shk_at_points_dev and shk_all_points_many_dev): the
  * tree's node count sizes the rest of its build, so the call waits once for
  * its topology kernel, and with it for the work queued before it on the
- * stream.
+ * stream.  The ChebCode workspace entries (shk_chebcode_all_points_ws_dev,
+ * shk_chebcode_at_points_ws_dev) read every size on the device and do not
+ * wait.
  *
  * HIP graph capture: entry points whose work is captured whole are
  * shk_all_points_noalloc_dev, the exact at-points transforms at a grid of
