@@ -160,3 +160,31 @@ def test_all_points_adjoint_argument_checks(shk):
     assert f(dummy, 5, -0.1, dummy, dummy, _lib.PREC_F64, dummy, None, None) == _lib.SHK_ERR_BAD_ETA
     assert f(dummy, 5, 0.1, dummy, dummy, _lib.PREC_F32, dummy, None, None) == _lib.SHK_ERR_BAD_ARG
     assert "precision" in _lib.last_error()
+
+
+def test_chebcode_workspace_sizes(shk):
+    """The workspace bound (every node slot: 2p plus the top walk's slots)
+    grows with p and with the interpolation order, covers the tree arrays of
+    the largest tree a build can make, and the workspace entries take the
+    ChebCode presets only.  Sizes need no device."""
+    from shrinkers_amd import _lib
+    lib = _lib.load()
+    n = ctypes.c_size_t()
+
+    def size(p, m):
+        st = lib.shk_chebcode_workspace_bytes(p, m, ctypes.byref(n))
+        return st, n.value
+
+    assert size(0, 14) == (_lib.SHK_OK, 0)
+    prev = 0
+    for p in (1, 2, 33, 1000, 50_000, 1_000_000):
+        st, fast = size(p, 14)       # chebcode_fast, n = 8
+        st2, deflt = size(p, 13)     # chebcode, n = 11
+        assert st == st2 == _lib.SHK_OK
+        assert deflt > fast > prev and fast % 256 == 0
+        # node arrays alone at 2p - 1 nodes: 3 f64, 4 i32, 2n f64, float4, int4, 2n f32
+        assert fast >= (2 * p - 1) * (3 * 8 + 4 * 4 + 16 * 8 + 16 + 16 + 8 * 8)
+        prev = fast
+    for m in (2, 19, 21, 22, 999):   # blocked, auto, speed_auto, accuracy_auto, unknown
+        assert size(1000, m)[0] == _lib.SHK_ERR_BAD_METHOD, m
+    assert lib.shk_chebcode_workspace_bytes(1000, 14, None) == _lib.SHK_ERR_BAD_ARG
