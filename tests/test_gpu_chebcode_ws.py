@@ -182,3 +182,26 @@ def test_ws_argument_checks(shk):
     r, i = shk.chebcode_all_points_ws_dev(ev, 0.03, "chebcode", ws)
     torch.cuda.synchronize()
     assert torch.isfinite(r).all() and torch.isfinite(i).all()
+
+
+def test_ws_two_streams_at_once(shk):
+    """Two workspaces on two streams with no synchronisation between the
+    calls (different spectra, sizes and presets, several calls each): every
+    result equals the one-shot entry's on the same input bit for bit."""
+    import torch
+    cases = [("chebcode_fast", _spectrum("mp", 50_000)), ("chebcode", _spectrum("clusters", 30_000))]
+    evs = [_cuda(ev) for _, ev in cases]
+    wss = [_ws(shk, ev.size, m) for (m, ev) in cases]
+    wants = [shk.all_points_dev(d, 1.0 / np.sqrt(d.numel()), m) for (m, _), d in zip(cases, evs)]
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    outs = [[], []]
+    for rep in range(3):
+        for k, ((m, _), d) in enumerate(zip(cases, evs)):
+            with torch.cuda.stream(streams[k]):
+                outs[k].append(shk.chebcode_all_points_ws_dev(
+                    d, 1.0 / np.sqrt(d.numel()), m, wss[k], stream=streams[k]))
+    torch.cuda.synchronize()
+    for k in range(2):
+        for rep, got in enumerate(outs[k]):
+            _equal(got, wants[k], f"stream {k} call {rep}")
