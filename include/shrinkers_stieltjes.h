@@ -302,9 +302,10 @@ shk_status shk_at_points_dev(const double *d_query, size_t nq, const double *d_e
  * The eigenvalues must be ascending (as symmetric eigensolvers return them).
  * On ascending input the results equal shk_all_points_dev's /
  * shk_at_points_dev's for the same method bit for bit.  Input that is not
- * ascending (or holds NaN) gives NaN outputs; shk_chebcode_workspace_status
- * reports it after the fact.  method: a ChebCode preset (13..16); the auto
- * methods are refused (SHK_ERR_BAD_METHOD).
+ * ascending gives NaN outputs, and shk_chebcode_workspace_status reports it
+ * after the fact (NaN input gives NaN outputs too; the device entries do not
+ * scan for it).  method: a ChebCode preset (13..16); the auto methods are
+ * refused (SHK_ERR_BAD_METHOD).
  *
  * The workspace (device memory, 256-byte aligned, at least
  * shk_chebcode_workspace_bytes(p, method) bytes) holds the tree at its largest
