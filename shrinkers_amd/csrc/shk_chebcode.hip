@@ -173,11 +173,18 @@ constexpr int kInfoUnsortedFlag = 3;  // info[] word set by ingest_kernel
 constexpr int kGapInts = 4 + 4 * kMaxGaps;
 
 // Also clears the info words the checks below set (count, overflow flag,
-// unsortedness flag).
+// unsortedness flag).  Thread i clears bstart[4i, 4i + 4) with one 16-byte
+// store (bstart is 256-byte aligned), threads past them the last n % 4
+// entries: hint_clear_threads(n) threads in all.
+__host__ __device__ inline int hint_clear_threads(int n) { return n / 4 + (n & 3) + 4; }
 __global__ void hint_clear_kernel(int *__restrict__ bstart, int n, int *__restrict__ gap,
                                   int *__restrict__ info) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) bstart[i] = -1;
+    const int n4 = n / 4;
+    if (i < n4)
+        reinterpret_cast<int4 *>(bstart)[i] = make_int4(-1, -1, -1, -1);
+    else if (i < n4 + (n & 3))
+        bstart[4 * n4 + (i - n4)] = -1;
     if (i == 0) gap[0] = 0;
     if (i < 4) info[i] = 0;
 }
@@ -3506,8 +3513,8 @@ int cheb_build(const double *d_ev, size_t p, const ChebParams &prm, ChebTree **o
     // rebuilt; on unsorted values the first topology still stays in bounds
     // (every split lies strictly inside its node), it is only wasted work.
     auto topology = [&](bool first) -> int {
-        hipLaunchKernelGGL(hint_clear_kernel, dim3(blocks_for((size_t)R + 1, 256)), dim3(256), 0,
-                           stream, d_hint, R + 1, d_gap, d_info);
+        hipLaunchKernelGGL(hint_clear_kernel, dim3(blocks_for(hint_clear_threads(R + 1), 256)),
+                           dim3(256), 0, stream, d_hint, R + 1, d_gap, d_info);
         SHK_HIP(hipGetLastError());
         if (first)  // copy, check, hint and split of the input, and the table
             hipLaunchKernelGGL(ingest_kernel, dim3(blocks_for(std::max<size_t>(p, 2 * n), 256)),
@@ -3809,8 +3816,8 @@ int cheb_oneshot_ws(const double *d_ev, size_t p, const ChebParams &prm, const d
         tab.v[n + j] = (j == 0 || j == n - 1) ? 0.5 * sign : sign;
     }
     // 1. copy, ascending check, hint and split of the input, and the table
-    hipLaunchKernelGGL(hint_clear_kernel, dim3(blocks_for((size_t)R + 1, 256)), dim3(256), 0,
-                       stream, w.hint, R + 1, w.gap, w.info);
+    hipLaunchKernelGGL(hint_clear_kernel, dim3(blocks_for(hint_clear_threads(R + 1), 256)),
+                       dim3(256), 0, stream, w.hint, R + 1, w.gap, w.info);
     hipLaunchKernelGGL(ingest_kernel, dim3(blocks_for(std::max<size_t>(p, 2 * n), 256)), dim3(256),
                        0, stream, d_ev, ip, R, w.sorted, w.hint, w.gap, w.s_hi, w.s_lo, w.info,
                        tab, 2 * n, w.tab);
